@@ -1,0 +1,240 @@
+/*
+ * mtjgpu.h -- C ABI of the MI355X (gfx950) implementation of MTJ's
+ * unstructured-sparse hot path: CompRowMatrix CSR products, DenseVector
+ * operations, DiagonalPreconditioner / ILU(0), and the CG / BiCGstab /
+ * GMRES / IR solve loops, with all operands resident in device memory.
+ *
+ * Every entry point returns an int: 0 (MTJG_OK) on success, a positive
+ * MTJG_NOT_CONVERGED_* code when a solve ended the way MTJ ends it with an
+ * IterativeSolverNotConvergedException, or a negative MTJG_E_* code where MTJ
+ * throws a RuntimeException subclass.  No entry point throws across the ABI.
+ * mtjg_last_error() returns the calling thread's message for the last
+ * failure, worded like MTJ's exception messages.
+ *
+ * Threading: objects are not thread-safe (MTJ solvers hold scratch state,
+ * S/CG.java:40).  Use one mtjg_ctx per host thread; calls on one context
+ * are ordered on its HIP stream.  Calls that return host scalars (dot,
+ * norm, get, download, solve) synchronise the stream.
+ *
+ * Reference citations: S/ = src/main/java/no/uib/cipr/matrix/sparse/,
+ * M/ = src/main/java/no/uib/cipr/matrix/.
+ */
+#ifndef MTJGPU_H
+#define MTJGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mtjg_ctx_s *mtjg_ctx;
+typedef struct mtjg_vec_s *mtjg_vec;
+typedef struct mtjg_csr_s *mtjg_csr;
+typedef struct mtjg_prec_s *mtjg_prec;
+typedef struct mtjg_solver_s *mtjg_solver;
+
+/* ---- status codes ---------------------------------------------------------- */
+enum {
+    MTJG_OK = 0,
+    /* IterativeSolverNotConvergedException reasons
+     * (M/NotConvergedException.java:33-49) */
+    MTJG_NOT_CONVERGED_ITERATIONS = 1,
+    MTJG_NOT_CONVERGED_DIVERGENCE = 2,
+    MTJG_NOT_CONVERGED_BREAKDOWN = 3,
+    /* IllegalArgumentException (S/AbstractIterativeSolver.java:70-77,
+     * S/CompRowMatrix.java:192-203, S/GMRES.java:114-116) */
+    MTJG_E_INVALID_ARGUMENT = -1,
+    /* IndexOutOfBoundsException (M/AbstractMatrix.java:169-176,
+     * M/AbstractVector.java:174-178, S/CompRowMatrix.java:426-430) */
+    MTJG_E_INDEX_OUT_OF_BOUNDS = -2,
+    /* RuntimeException("Zero diagonal on row i+1")
+     * (S/DiagonalPreconditioner.java:71-72) */
+    MTJG_E_ZERO_DIAGONAL = -3,
+    /* RuntimeException("Missing diagonal entry on row k+1")
+     * (S/ILU.java:134-136) */
+    MTJG_E_MISSING_DIAGONAL = -4,
+    /* RuntimeException("Zero pivot encountered ...") (S/ILU.java:105-107) */
+    MTJG_E_ZERO_PIVOT = -5,
+    /* MatrixSingularException from the GMRES back-substitution
+     * (M/AbstractTriangDenseMatrix.java:251-252) */
+    MTJG_E_SINGULAR = -6,
+    /* UnsupportedOperationException / wrong operand kind */
+    MTJG_E_UNSUPPORTED = -7,
+    MTJG_E_OUT_OF_MEMORY = -8,
+    MTJG_E_DEVICE = -9,
+    MTJG_E_NULL = -10
+};
+
+/* Vector element types. */
+enum { MTJG_F64 = 0, MTJG_F32 = 1 };
+
+/* Vector.Norm (M/Vector.java:159-180). */
+enum { MTJG_NORM_ONE = 0, MTJG_NORM_TWO = 1, MTJG_NORM_TWO_ROBUST = 2,
+       MTJG_NORM_INFINITY = 3 };
+
+/* Solver kinds. */
+enum {
+    MTJG_CG = 0,        /* S/CG.java */
+    MTJG_BICGSTAB = 1,  /* S/BiCGstab.java */
+    MTJG_GMRES = 2,     /* S/GMRES.java */
+    MTJG_IR = 3,        /* S/IR.java */
+    MTJG_MIXED_CG = 4   /* FP32 inner Jacobi-CG under FP64 refinement; no
+                           MTJ counterpart (S/IR.java:57-70 is its outer
+                           loop's shape) */
+};
+
+const char *mtjg_strerror(int code);
+/* Message of the calling thread's last failure ("" if none). */
+const char *mtjg_last_error(void);
+
+/* ---- context --------------------------------------------------------------- */
+/* Binds a device and creates the context's HIP stream and workspace.  No MTJ
+ * counterpart (MTJ has no device); every object below belongs to a context. */
+int mtjg_ctx_create(int device, mtjg_ctx *out);
+int mtjg_ctx_destroy(mtjg_ctx ctx);
+int mtjg_ctx_synchronize(mtjg_ctx ctx);
+/* The context's stream (hipStream_t), for callers that interleave work. */
+void *mtjg_ctx_stream(mtjg_ctx ctx);
+
+/* ---- DenseVector (M/DenseVector.java) --------------------------------------- */
+/* new DenseVector(size): zero-filled (M/DenseVector.java:66-74). */
+int mtjg_vec_create(mtjg_ctx ctx, int64_t n, int dtype, mtjg_vec *out);
+/* Wraps caller-owned device memory of n elements (not freed on destroy). */
+int mtjg_vec_wrap(mtjg_ctx ctx, int64_t n, int dtype, void *device_ptr,
+                  mtjg_vec *out);
+int mtjg_vec_destroy(mtjg_vec v);
+int64_t mtjg_vec_size(mtjg_vec v);
+void *mtjg_vec_device_ptr(mtjg_vec v);
+/* Host <-> device copies of all n elements (element type of the vector). */
+int mtjg_vec_upload(mtjg_vec v, const void *host);
+int mtjg_vec_download(mtjg_vec v, void *host);
+/* zero() (M/DenseVector.java:189-193). */
+int mtjg_vec_zero(mtjg_vec x);
+/* scale(alpha): x *= alpha, no shortcut (M/DenseVector.java:195-200). */
+int mtjg_vec_scale(mtjg_vec x, double alpha);
+/* set(y): x = y (M/DenseVector.java:202-213). */
+int mtjg_vec_set(mtjg_vec x, mtjg_vec y);
+/* set(alpha, y): x = alpha*y, alpha == 0 zeroes (M/DenseVector.java:215-231). */
+int mtjg_vec_set_scaled(mtjg_vec x, double alpha, mtjg_vec y);
+/* add(y): x += y (M/DenseVector.java:233-246). */
+int mtjg_vec_add(mtjg_vec x, mtjg_vec y);
+/* add(alpha, y): x += alpha*y, alpha == 0 is a no-op
+ * (M/DenseVector.java:248-264). */
+int mtjg_vec_add_scaled(mtjg_vec x, double alpha, mtjg_vec y);
+/* dot(y) (M/DenseVector.java:266-279). */
+int mtjg_vec_dot(mtjg_vec x, mtjg_vec y, double *out);
+/* norm(type) (M/AbstractVector.java:180-190, M/DenseVector.java:281-318). */
+int mtjg_vec_norm(mtjg_vec x, int norm_type, double *out);
+
+/* ---- CompRowMatrix (S/CompRowMatrix.java) ----------------------------------- */
+/* CompRowMatrix(numRows, numColumns, nz) + set(i,j,v) for every entry
+ * (S/CompRowMatrix.java:178-208): copies the CSR to the device.  Columns
+ * within a row are sorted (values follow) and must be unique and in
+ * [0, n_cols); row pointers must be non-decreasing from 0 to nnz. */
+int mtjg_csr_create(mtjg_ctx ctx, int32_t n_rows, int32_t n_cols, int64_t nnz,
+                    const int32_t *row_pointer, const int32_t *column_index,
+                    const double *data, mtjg_csr *out);
+int mtjg_csr_destroy(mtjg_csr A);
+int mtjg_csr_shape(mtjg_csr A, int32_t *n_rows, int32_t *n_cols, int64_t *nnz);
+/* getRowPointers / getColumnIndices / getData (S/CompRowMatrix.java:255-271),
+ * copied to caller memory; any pointer may be NULL. */
+int mtjg_csr_download(mtjg_csr A, int32_t *row_pointer, int32_t *column_index,
+                      double *data);
+/* set(Matrix) with an identical pattern (S/CompRowMatrix.java:449-471). */
+int mtjg_csr_set_values(mtjg_csr A, const double *data);
+/* get(row, column): 0 outside the pattern (S/CompRowMatrix.java:406-417). */
+int mtjg_csr_get(mtjg_csr A, int32_t row, int32_t column, double *out);
+/* set(row, column, value): IndexOutOfBounds outside the pattern
+ * (S/CompRowMatrix.java:390-396, :422-431). */
+int mtjg_csr_set(mtjg_csr A, int32_t row, int32_t column, double value);
+/* mult(x, y): y = A x (S/CompRowMatrix.java:294-314). */
+int mtjg_csr_mult(mtjg_csr A, mtjg_vec x, mtjg_vec y);
+/* multAdd(alpha, x, y): y += alpha A x (S/CompRowMatrix.java:329-347). */
+int mtjg_csr_mult_add(mtjg_csr A, double alpha, mtjg_vec x, mtjg_vec y);
+/* mult(alpha, x, y) = multAdd(alpha, x, y.zero()) (M/AbstractMatrix.java:
+ * 148-150). */
+int mtjg_csr_mult_scaled(mtjg_csr A, double alpha, mtjg_vec x, mtjg_vec y);
+/* transMult(x, y): y = A' x (S/CompRowMatrix.java:349-366). */
+int mtjg_csr_trans_mult(mtjg_csr A, mtjg_vec x, mtjg_vec y);
+/* transMultAdd(alpha, x, y): y = alpha((y/alpha) + A' x)
+ * (S/CompRowMatrix.java:368-388). */
+int mtjg_csr_trans_mult_add(mtjg_csr A, double alpha, mtjg_vec x, mtjg_vec y);
+/* transMult(alpha, x, y) = transMultAdd(alpha, x, y.zero())
+ * (M/AbstractMatrix.java:182-184). */
+int mtjg_csr_trans_mult_scaled(mtjg_csr A, double alpha, mtjg_vec x,
+                               mtjg_vec y);
+/* Row-length histogram in power-of-two bins: bins[b] counts rows with length
+ * in [2^(b-1), 2^b) (bins[0] counts empty rows); nbins entries are written. */
+int mtjg_csr_row_histogram(mtjg_csr A, int64_t *bins, int nbins);
+
+/* ---- Preconditioners (S/Preconditioner.java) -------------------------------- */
+/* new DiagonalPreconditioner(n) (S/DiagonalPreconditioner.java:43-45). */
+int mtjg_prec_diagonal_create(mtjg_ctx ctx, int32_t n, mtjg_prec *out);
+/* new ILU(CompRowMatrix LU): LU supplies the pattern of the factor
+ * (S/ILU.java:54-62). */
+int mtjg_prec_ilu_create(mtjg_ctx ctx, mtjg_csr LU, mtjg_prec *out);
+/* setMatrix(A) (S/Preconditioner.java:63; S/DiagonalPreconditioner.java:
+ * 64-76; S/ILU.java:80-84).  On MTJG_E_ZERO_DIAGONAL, MTJG_E_MISSING_DIAGONAL
+ * or MTJG_E_ZERO_PIVOT, *bad_row (may be NULL) is the 0-based row. */
+int mtjg_prec_set_matrix(mtjg_prec M, mtjg_csr A, int32_t *bad_row);
+/* apply(b, x): x = M^-1 b (S/Preconditioner.java:42). */
+int mtjg_prec_apply(mtjg_prec M, mtjg_vec b, mtjg_vec x);
+/* transApply(b, x): x = M^-T b (S/Preconditioner.java:54). */
+int mtjg_prec_trans_apply(mtjg_prec M, mtjg_vec b, mtjg_vec x);
+int mtjg_prec_destroy(mtjg_prec M);
+
+/* ---- Iterative solvers (S/IterativeSolver.java) ----------------------------- */
+/* DefaultIterationMonitor(maxIter, rtol, atol, dtol)
+ * (S/DefaultIterationMonitor.java:72-90; defaults 100000, 1e-5, 1e-50, 1e5). */
+typedef struct {
+    int32_t max_iter;
+    double rtol;
+    double atol;
+    double dtol;
+} mtjg_monitor;
+
+/* IterationMonitor.iterations() / residual() and the exception reason
+ * (S/IterationMonitor.java:50-55, S/IterativeSolverNotConvergedException.java:
+ * 54-89). */
+typedef struct {
+    int32_t iterations;
+    double residual;
+    int32_t reason; /* 0 converged, else MTJG_NOT_CONVERGED_* or MTJG_E_* */
+} mtjg_result;
+
+/* IterationReporter.monitor(r, iter) (S/IterationReporter.java:28-52).
+ * Installing one makes the solver synchronise at every convergence check. */
+typedef void (*mtjg_reporter)(double residual, int32_t iteration, void *user);
+
+/* new CG(template) / new BiCGstab(template) / new GMRES(template) /
+ * new IR(template) (S/CG.java:51-56, S/BiCGstab.java:53-63,
+ * S/GMRES.java:84-104, S/IR.java:51-54): scratch vectors sized like the
+ * template.  Identity preconditioner and default monitor
+ * (S/AbstractIterativeSolver.java:45-48). */
+int mtjg_solver_create(mtjg_ctx ctx, int kind, mtjg_vec template_vec,
+                       mtjg_solver *out);
+int mtjg_solver_destroy(mtjg_solver S);
+/* setPreconditioner(M); NULL restores the identity
+ * (S/AbstractIterativeSolver.java:50-52). */
+int mtjg_solver_set_preconditioner(mtjg_solver S, mtjg_prec M);
+/* setIterationMonitor(new DefaultIterationMonitor(...))
+ * (S/AbstractIterativeSolver.java:62-64). */
+int mtjg_solver_set_monitor(mtjg_solver S, const mtjg_monitor *m);
+int mtjg_solver_set_reporter(mtjg_solver S, mtjg_reporter cb, void *user);
+/* GMRES.setRestart(restart) (S/GMRES.java:112-125). */
+int mtjg_gmres_set_restart(mtjg_solver S, int32_t restart);
+/* Inner solve of MTJG_MIXED_CG: FP32 Jacobi-CG to inner_rtol relative to its
+ * own initial residual, at most inner_max_iter iterations per refinement
+ * step.  No MTJ counterpart. */
+int mtjg_mixed_set_inner(mtjg_solver S, double inner_rtol,
+                         int32_t inner_max_iter);
+/* solve(A, b, x): x holds the initial guess and receives the solution
+ * (S/IterativeSolver.java:43).  result may be NULL. */
+int mtjg_solver_solve(mtjg_solver S, mtjg_csr A, mtjg_vec b, mtjg_vec x,
+                      mtjg_result *result);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

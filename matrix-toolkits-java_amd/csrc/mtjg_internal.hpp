@@ -1,0 +1,199 @@
+// Internal declarations shared by the HIP translation units of libmtjgpu.
+//
+// Device memory layout: every CompRowMatrix lives on the device as three
+// arrays (int32 row pointers, int32 column indices sorted within each row,
+// FP64 values; an FP32 copy of the values is added on demand for the mixed
+// precision path) plus a row-block schedule for the SpMV kernel.  Vectors are
+// plain contiguous device arrays.  Solver scalars (rho, alpha, omega, the
+// iteration monitor state) live in a small device struct so that a whole
+// Krylov iteration runs without returning scalars to the host.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "mtjgpu.h"
+
+namespace mtjg {
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string &msg);
+void hip_check(hipError_t e, const char *what, const char *file, int line);
+#define HIPCHK(x) ::mtjg::hip_check((x), #x, __FILE__, __LINE__)
+
+void set_last_error(const std::string &msg);
+
+// Workgroup size of every kernel in this library: four 64-wide wavefronts.
+constexpr int BS = 256;
+// Upper bound on the grid of any kernel that reduces across the grid; the
+// per-block partial sums of such a kernel fit in RED_MAX_BLOCKS slots.
+constexpr int RED_MAX_BLOCKS = 4096;
+// Number of independent sums one reduction kernel may carry.
+constexpr int RED_MAX_K = 4;
+
+// Solver status values kept in KState::status.
+enum : int {
+  ST_RUNNING = 0,
+  ST_CONVERGED = 1,
+  ST_ITERATIONS = 2,   // == 1 + MTJG_NOT_CONVERGED_ITERATIONS
+  ST_DIVERGENCE = 3,
+  ST_BREAKDOWN = 4,
+  ST_SINGULAR = 5,
+};
+
+// Device-resident scalar state of one Krylov solve.  All fields are written
+// only by the last workgroup of a reducing kernel (or by single-thread
+// kernels) and read by later kernels on the same stream.
+struct KState {
+  // monitor (DefaultIterationMonitor)
+  double rtol, atol, dtol;
+  double init_r;
+  double residual;
+  int32_t max_iter;
+  int32_t iter;
+  int32_t status;
+  int32_t flags;
+  // CG / BiCGstab scalars
+  double rho, rho_1, rho_2, alpha, beta, omega;
+  double s[8];  // scratch scalars (dot results)
+  // per-kernel grid tickets (self-resetting)
+  uint32_t ticket[8];
+  // GMRES small state
+  int32_t inner_i;     // index of the current Arnoldi step
+  int32_t restart;
+  int32_t cycle_done;  // inner loop of this cycle has ended
+  int32_t pad;
+  double normr;
+};
+
+// Tiny deterministic-reduction workspace shared by the context's BLAS-1
+// calls.
+struct RedWS {
+  double *partials;  // [RED_MAX_K * RED_MAX_BLOCKS]
+  uint32_t *ticket;  // [8]
+  double *out;       // [8]
+};
+
+// Grid size for an elementwise or reducing kernel over n items.
+inline int grid_for(int64_t n, int per_thread = 4, int cap = 2048) {
+  int64_t g = (n + (int64_t)BS * per_thread - 1) / ((int64_t)BS * per_thread);
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+}  // namespace mtjg
+
+struct mtjg_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  mtjg::RedWS red{};
+  double *h_out = nullptr;  // pinned host scalars [8]
+};
+
+struct mtjg_vec_s {
+  mtjg_ctx ctx = nullptr;
+  int64_t n = 0;
+  int dtype = MTJG_F64;
+  void *ptr = nullptr;
+  bool owned = true;
+};
+
+struct mtjg_csr_s {
+  mtjg_ctx ctx = nullptr;
+  int32_t nr = 0, nc = 0;
+  int64_t nnz = 0;
+  int32_t *rp = nullptr;
+  int32_t *ci = nullptr;
+  double *val = nullptr;
+  float *val32 = nullptr;  // FP32 copy of val (mixed precision), lazily built
+  // SpMV row-block schedule: row-block b covers rows [rb[b], rb[b+1]).
+  int32_t nrb = 0;
+  int32_t *rb = nullptr;
+  // Host copy of the row pointers (schedules, level analysis, validation).
+  std::vector<int32_t> h_rp;
+  // Transpose (CSR of A'), built on first transposed product.
+  mtjg_csr_s *trans = nullptr;
+};
+
+namespace mtjg {
+
+// ---- vector helpers (vector.hip) ----
+void vec_check_same(mtjg_vec x, mtjg_vec y);
+void launch_fill(hipStream_t s, int64_t n, double *x, double v);
+void launch_scale(hipStream_t s, int64_t n, double *x, double alpha);
+void launch_set_scaled(hipStream_t s, int64_t n, double *x, double alpha, const double *y);
+void launch_add_scaled(hipStream_t s, int64_t n, double *x, double alpha, const double *y);
+void launch_add(hipStream_t s, int64_t n, double *x, const double *y);
+// Deterministic two-stage dot / sum-of-squares; result left in red.out[slot].
+void launch_dot(hipStream_t s, int64_t n, const double *x, const double *y, const RedWS &red,
+                int slot);
+double read_scalar(mtjg_ctx ctx, const double *dev);
+
+// ---- SpMV (spmv.hip) ----
+enum SpmvMode : int {
+  SPMV_MULT = 0,      // y = A x (zero results stored as +0.0)
+  SPMV_MULTADD = 1,   // y = y0 + alpha * (A x)
+  SPMV_TMULTADD = 2,  // y = ((y0 * inv_alpha) + sum) * alpha (transposed storage)
+  SPMV_DOT = 3,       // y = A x, plus grid sums w.y and y.y, then fin()
+};
+
+// Solver epilogue selector for SPMV_DOT.
+enum Fin : int {
+  FIN_NONE = 0,
+  FIN_CG_ALPHA = 1,      // alpha = rho / (p.q); rho_1 = rho
+  FIN_BICG_ALPHA = 2,    // alpha = rho_1 / (rtilde.v)
+  FIN_BICG_OMEGA = 3,    // omega = (t.s) / (t.t)
+};
+
+struct SpmvArgs {
+  int mode = SPMV_MULT;
+  double alpha = 1.0;
+  double inv_alpha = 1.0;
+  const void *y0 = nullptr;
+  const void *w = nullptr;   // SPMV_DOT: first dot operand
+  KState *st = nullptr;      // SPMV_DOT: solver state (status checked on entry)
+  int fin = FIN_NONE;
+  double *partials = nullptr;
+  uint32_t *ticket = nullptr;
+};
+
+void spmv_f64(mtjg_csr A, const double *x, double *y, const SpmvArgs &a, hipStream_t s);
+void spmv_f32(mtjg_csr A, const float *x, float *y, const SpmvArgs &a, hipStream_t s);
+mtjg_csr_s *csr_transpose(mtjg_csr A);
+void csr_ensure_fp32(mtjg_csr A);
+
+// ---- preconditioners (precond.hip) ----
+enum PrecKind : int { PREC_IDENTITY = 0, PREC_DIAGONAL = 1, PREC_ILU = 2 };
+void prec_apply_dev(mtjg_prec M, const double *b, double *x, hipStream_t s, KState *st);
+void prec_trans_apply_dev(mtjg_prec M, const double *b, double *x, hipStream_t s);
+int prec_kind(mtjg_prec M);
+const double *prec_invdiag(mtjg_prec M);
+int32_t prec_n(mtjg_prec M);
+
+}  // namespace mtjg
+
+// C-ABI guard: converts internal failures into return codes.
+#define MTJG_API_BEGIN try {
+#define MTJG_API_END                                         \
+  }                                                          \
+  catch (const ::mtjg::Error &e) {                           \
+    ::mtjg::set_last_error(e.msg);                           \
+    return e.code;                                           \
+  }                                                          \
+  catch (const std::bad_alloc &) {                           \
+    ::mtjg::set_last_error("out of host memory");            \
+    return MTJG_E_OUT_OF_MEMORY;                             \
+  }                                                          \
+  catch (...) {                                              \
+    ::mtjg::set_last_error("unexpected internal error");     \
+    return MTJG_E_DEVICE;                                    \
+  }                                                          \
+  return MTJG_OK;

@@ -1,0 +1,514 @@
+// Preconditioners on the device: DiagonalPreconditioner (Jacobi) and ILU(0)
+// (S/DiagonalPreconditioner.java, S/ILU.java, S/UnitLowerCompRowMatrix.java,
+// S/UpperCompRowMatrix.java).
+//
+// ILU(0) is level scheduled.  Rows are grouped on the host into dependency
+// levels (a row's level is one more than the highest level among the rows its
+// strictly-lower entries reference), sorted by level, and each level is one
+// launch with one thread per row.  A thread does exactly MTJ's sequential
+// per-row work (the IKJ elimination of S/ILU.java:98-121, or the per-row sum
+// of the triangular solves), and every row it reads was finished by an
+// earlier launch, so factors and solves are bitwise identical to MTJ's.
+//
+// One deliberate difference: MTJ's merge walk (S/ILU.java:115-119) tests
+// colind[l] even after l has run past the end of row k, and can then update
+// the first entry of row k+1; this implementation stops at the end of row k.
+// The two agree on every pattern where row k+1 has a strictly-lower entry,
+// which includes all structurally symmetric patterns.
+#include <algorithm>
+#include <cstring>
+
+#include "mtjg_internal.hpp"
+#include "reduce.hpp"
+
+struct Levels {
+  int32_t nlev = 0;
+  std::vector<int32_t> ptr;  // host copy of level offsets [nlev + 1]
+  int32_t *rows = nullptr;   // device: rows sorted by level
+};
+
+struct mtjg_prec_s {
+  mtjg_ctx ctx = nullptr;
+  int kind = mtjg::PREC_IDENTITY;
+  int32_t n = 0;
+  // Jacobi
+  double *invdiag = nullptr;
+  int32_t *bad = nullptr;  // device scratch: first offending row
+  // ILU(0): factor storage in A's pattern
+  bool factored = false;
+  int64_t nnz = 0;
+  int32_t *rp = nullptr, *ci = nullptr, *diag = nullptr;
+  double *lu = nullptr;
+  double *y = nullptr;  // temporary of apply
+  std::vector<int32_t> h_rp, h_ci;
+  Levels lev_l, lev_u;
+  // transposed factor (transApply), built on first use
+  bool trans_ready = false;
+  int32_t *trp = nullptr, *tci = nullptr, *tdiag = nullptr, *tperm = nullptr;
+  double *tval = nullptr;
+  Levels lev_ut, lev_lt;
+};
+
+namespace mtjg {
+
+// --- Jacobi -----------------------------------------------------------------
+
+// invdiag[i] = 1 / A.get(i, i); the first zero or structurally missing
+// diagonal is recorded (S/DiagonalPreconditioner.java:69-75).
+__global__ __launch_bounds__(BS) void k_jacobi_setup(int32_t n, const int32_t *rp, const int32_t *ci,
+                                                     const double *val, double *invdiag,
+                                                     int32_t *bad) {
+  int32_t i = blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  int32_t lo = rp[i], hi = rp[i + 1] - 1;
+  double d = 0.0;
+  while (lo <= hi) {
+    int32_t mid = (lo + hi) >> 1;
+    int32_t c = ci[mid];
+    if (c < i) lo = mid + 1;
+    else if (c > i) hi = mid - 1;
+    else {
+      d = val[mid];
+      break;
+    }
+  }
+  if (d == 0) atomicMin(bad, i);
+  invdiag[i] = 1 / d;
+}
+
+// x_i = b_i * invdiag_i (S/DiagonalPreconditioner.java:54-55).
+__global__ __launch_bounds__(BS) void k_jacobi_apply(int64_t n, const double *b, const double *invdiag,
+                                                     double *x, const KState *st) {
+  if (st && st->status != ST_RUNNING) return;
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+    x[i] = b[i] * invdiag[i];
+}
+
+__global__ __launch_bounds__(BS) void k_copy_if_running(int64_t n, const double *b, double *x,
+                                                        const KState *st) {
+  if (st && st->status != ST_RUNNING) return;
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+    x[i] = b[i];
+}
+
+// --- ILU(0) -----------------------------------------------------------------
+
+// Diagonal positions (S/ILU.java:127-140); missing ones are recorded.
+__global__ __launch_bounds__(BS) void k_find_diag(int32_t n, const int32_t *rp, const int32_t *ci,
+                                                  int32_t *diag, int32_t *bad) {
+  int32_t i = blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  int32_t lo = rp[i], hi = rp[i + 1] - 1, pos = -1;
+  while (lo <= hi) {
+    int32_t mid = (lo + hi) >> 1;
+    int32_t c = ci[mid];
+    if (c < i) lo = mid + 1;
+    else if (c > i) hi = mid - 1;
+    else {
+      pos = mid;
+      break;
+    }
+  }
+  diag[i] = pos;
+  if (pos < 0) atomicMin(bad, i);
+}
+
+// IKJ elimination of the rows of one level (S/ILU.java:98-121).
+__global__ __launch_bounds__(BS) void k_ilu_factor_level(int32_t cnt, const int32_t *rows,
+                                                         const int32_t *rp, const int32_t *ci,
+                                                         const int32_t *diag, double *lu,
+                                                         int32_t *bad) {
+  int32_t t = blockIdx.x * BS + threadIdx.x;
+  if (t >= cnt) return;
+  const int32_t k = rows[t];
+  const int32_t kend = rp[k + 1];
+  for (int32_t i = rp[k]; i < diag[k]; ++i) {
+    const int32_t index = ci[i];
+    const double LUii = lu[diag[index]];
+    if (LUii == 0) {
+      atomicMin(bad, k);
+      return;
+    }
+    const double LUki = (lu[i] /= LUii);
+    int32_t l = rp[k] + 1;
+    for (int32_t j = diag[index] + 1; j < rp[index + 1]; ++j) {
+      const int32_t cj = ci[j];
+      while (l < kend && ci[l] < cj) l++;
+      if (l < kend && ci[l] == cj) lu[l] -= LUki * lu[j];
+    }
+  }
+}
+
+enum TrsvForm : int {
+  TRSV_ROW_UNIT = 0,     // x_i = b_i - sum_{beg..end} a x        (L.solve)
+  TRSV_ROW_DIV = 1,      // x_i = (b_i - sum) / d_i                (U.solve)
+  TRSV_COL_DIV = 2,      // s = b_i; s -= a x ascending; x_i = s/d  (U'.solve)
+  TRSV_COL_UNIT_REV = 3  // s = b_i; s -= a x descending; x_i = s   (L'.solve)
+};
+
+// One level of a triangular solve; one thread per row, MTJ's per-row order.
+template <int FORM>
+__global__ __launch_bounds__(BS) void k_trsv_level(int32_t cnt, const int32_t *rows,
+                                                   const int32_t *rp, const int32_t *ci,
+                                                   const int32_t *diag, const double *a,
+                                                   const double *b, double *x) {
+  int32_t t = blockIdx.x * BS + threadIdx.x;
+  if (t >= cnt) return;
+  const int32_t i = rows[t];
+  if (FORM == TRSV_ROW_UNIT) {
+    double sum = 0;
+    for (int32_t j = rp[i]; j < diag[i]; ++j) sum += a[j] * x[ci[j]];
+    x[i] = b[i] - sum;
+  } else if (FORM == TRSV_ROW_DIV) {
+    double sum = 0;
+    for (int32_t j = diag[i] + 1; j < rp[i + 1]; ++j) sum += a[j] * x[ci[j]];
+    x[i] = (b[i] - sum) / a[diag[i]];
+  } else if (FORM == TRSV_COL_DIV) {
+    double s = b[i];
+    for (int32_t j = rp[i]; j < diag[i]; ++j) s -= a[j] * x[ci[j]];
+    x[i] = s / a[diag[i]];
+  } else {
+    double s = b[i];
+    for (int32_t j = rp[i + 1] - 1; j > diag[i]; --j) s -= a[j] * x[ci[j]];
+    x[i] = s;
+  }
+}
+
+__global__ void k_gather(int64_t n, const int32_t *perm, const double *src, double *dst) {
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+    dst[i] = src[perm[i]];
+}
+
+// Level sets for a triangular dependency structure given, per row, the
+// index range of its dependency entries.  `forward` levels rows from the
+// first row down (dependencies have smaller indices); otherwise from the last
+// row up.
+static void build_levels(mtjg_ctx ctx, int32_t n, const std::vector<int32_t> &rp,
+                         const std::vector<int32_t> &ci, const std::vector<int32_t> &dbeg,
+                         const std::vector<int32_t> &dend, bool forward, Levels &L) {
+  std::vector<int32_t> lev((size_t)n, 0);
+  int32_t maxlev = 0;
+  for (int32_t t = 0; t < n; ++t) {
+    int32_t i = forward ? t : n - 1 - t;
+    int32_t l = 0;
+    for (int32_t j = dbeg[i]; j < dend[i]; ++j) l = std::max(l, lev[(size_t)ci[j]] + 1);
+    lev[(size_t)i] = l;
+    maxlev = std::max(maxlev, l);
+  }
+  (void)rp;
+  L.nlev = n > 0 ? maxlev + 1 : 0;
+  L.ptr.assign((size_t)L.nlev + 1, 0);
+  for (int32_t i = 0; i < n; ++i) L.ptr[(size_t)lev[(size_t)i] + 1]++;
+  for (int32_t l = 0; l < L.nlev; ++l) L.ptr[l + 1] += L.ptr[l];
+  std::vector<int32_t> pos(L.ptr.begin(), L.ptr.end() - (L.nlev > 0 ? 1 : 0));
+  std::vector<int32_t> rows((size_t)std::max(n, 1));
+  for (int32_t i = 0; i < n; ++i) rows[(size_t)pos[(size_t)lev[(size_t)i]]++] = i;
+  if (L.rows) (void)hipFree(L.rows);
+  HIPCHK(hipMalloc(&L.rows, sizeof(int32_t) * rows.size()));
+  HIPCHK(hipMemcpyAsync(L.rows, rows.data(), sizeof(int32_t) * rows.size(), hipMemcpyHostToDevice,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+}
+
+template <int FORM>
+static void run_levels(const Levels &L, hipStream_t s, const int32_t *rp, const int32_t *ci,
+                       const int32_t *diag, const double *a, const double *b, double *x) {
+  for (int32_t l = 0; l < L.nlev; ++l) {
+    int32_t cnt = L.ptr[l + 1] - L.ptr[l];
+    k_trsv_level<FORM><<<(cnt + BS - 1) / BS, BS, 0, s>>>(cnt, L.rows + L.ptr[l], rp, ci, diag, a, b, x);
+  }
+  HIPCHK(hipGetLastError());
+}
+
+static void free_levels(Levels &L) {
+  if (L.rows) (void)hipFree(L.rows);
+  L.rows = nullptr;
+  L.nlev = 0;
+  L.ptr.clear();
+}
+
+static void ilu_free_storage(mtjg_prec M) {
+  (void)hipFree(M->rp);
+  (void)hipFree(M->ci);
+  (void)hipFree(M->diag);
+  (void)hipFree(M->lu);
+  (void)hipFree(M->trp);
+  (void)hipFree(M->tci);
+  (void)hipFree(M->tdiag);
+  (void)hipFree(M->tperm);
+  (void)hipFree(M->tval);
+  M->rp = M->ci = M->diag = M->trp = M->tci = M->tdiag = M->tperm = nullptr;
+  M->lu = M->tval = nullptr;
+  free_levels(M->lev_l);
+  free_levels(M->lev_u);
+  free_levels(M->lev_ut);
+  free_levels(M->lev_lt);
+  M->trans_ready = false;
+  M->factored = false;
+}
+
+static int32_t read_bad(mtjg_prec M) {
+  int32_t h = INT32_MAX;
+  HIPCHK(hipMemcpyAsync(&h, M->bad, sizeof(int32_t), hipMemcpyDeviceToHost, M->ctx->stream));
+  HIPCHK(hipStreamSynchronize(M->ctx->stream));
+  return h;
+}
+
+static void reset_bad(mtjg_prec M) {
+  int32_t big = INT32_MAX;
+  HIPCHK(hipMemcpyAsync(M->bad, &big, sizeof(int32_t), hipMemcpyHostToDevice, M->ctx->stream));
+}
+
+// ILU.setMatrix: LU.set(A) takes A's pattern and values
+// (S/CompRowMatrix.java:449-471), then factor() (S/ILU.java:86-125).
+static void ilu_set_matrix(mtjg_prec M, mtjg_csr A, int32_t *bad_row) {
+  mtjg_ctx ctx = M->ctx;
+  hipStream_t s = ctx->stream;
+  const int32_t n = A->nr;
+  bool same_pattern = M->rp && M->nnz == A->nnz && M->h_rp == A->h_rp;
+  if (same_pattern) {
+    std::vector<int32_t> ci((size_t)A->nnz);
+    if (A->nnz)
+      HIPCHK(hipMemcpyAsync(ci.data(), A->ci, sizeof(int32_t) * (size_t)A->nnz, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    same_pattern = (ci == M->h_ci);
+  }
+  if (!same_pattern) {
+    ilu_free_storage(M);
+    M->nnz = A->nnz;
+    M->h_rp = A->h_rp;
+    M->h_ci.resize((size_t)A->nnz);
+    size_t nnz1 = (size_t)std::max<int64_t>(A->nnz, 1);
+    HIPCHK(hipMalloc(&M->rp, sizeof(int32_t) * (size_t)(n + 1)));
+    HIPCHK(hipMalloc(&M->ci, sizeof(int32_t) * nnz1));
+    HIPCHK(hipMalloc(&M->lu, sizeof(double) * nnz1));
+    HIPCHK(hipMalloc(&M->diag, sizeof(int32_t) * (size_t)std::max(n, 1)));
+    HIPCHK(hipMemcpyAsync(M->rp, A->rp, sizeof(int32_t) * (size_t)(n + 1), hipMemcpyDeviceToDevice, s));
+    if (A->nnz) {
+      HIPCHK(hipMemcpyAsync(M->ci, A->ci, sizeof(int32_t) * (size_t)A->nnz, hipMemcpyDeviceToDevice, s));
+      HIPCHK(hipMemcpyAsync(M->h_ci.data(), A->ci, sizeof(int32_t) * (size_t)A->nnz,
+                            hipMemcpyDeviceToHost, s));
+    }
+    reset_bad(M);
+    if (n > 0) k_find_diag<<<(n + BS - 1) / BS, BS, 0, s>>>(n, M->rp, M->ci, M->diag, M->bad);
+    HIPCHK(hipGetLastError());
+    int32_t b = read_bad(M);
+    if (b != INT32_MAX) {
+      if (bad_row) *bad_row = b;
+      ilu_free_storage(M);
+      M->h_rp.clear();
+      fail(MTJG_E_MISSING_DIAGONAL, "Missing diagonal entry on row " + std::to_string(b + 1));
+    }
+    std::vector<int32_t> hd((size_t)n);
+    if (n) HIPCHK(hipMemcpyAsync(hd.data(), M->diag, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    // L levels: dependencies are the strictly-lower entries [rp[i], diag[i])
+    std::vector<int32_t> beg((size_t)n), end((size_t)n);
+    for (int32_t i = 0; i < n; ++i) {
+      beg[(size_t)i] = M->h_rp[i];
+      end[(size_t)i] = hd[(size_t)i];
+    }
+    build_levels(ctx, n, M->h_rp, M->h_ci, beg, end, true, M->lev_l);
+    // U levels: strictly-upper entries (diag[i], rp[i+1])
+    for (int32_t i = 0; i < n; ++i) {
+      beg[(size_t)i] = hd[(size_t)i] + 1;
+      end[(size_t)i] = M->h_rp[i + 1];
+    }
+    build_levels(ctx, n, M->h_rp, M->h_ci, beg, end, false, M->lev_u);
+  }
+  if (A->nnz)
+    HIPCHK(hipMemcpyAsync(M->lu, A->val, sizeof(double) * (size_t)A->nnz, hipMemcpyDeviceToDevice, s));
+  reset_bad(M);
+  // rows of the first L level have no strictly-lower entries: nothing to do
+  for (int32_t l = 1; l < M->lev_l.nlev; ++l) {
+    int32_t cnt = M->lev_l.ptr[l + 1] - M->lev_l.ptr[l];
+    k_ilu_factor_level<<<(cnt + BS - 1) / BS, BS, 0, s>>>(cnt, M->lev_l.rows + M->lev_l.ptr[l], M->rp,
+                                                          M->ci, M->diag, M->lu, M->bad);
+  }
+  HIPCHK(hipGetLastError());
+  int32_t b = read_bad(M);
+  M->factored = false;
+  if (b != INT32_MAX) {
+    if (bad_row) *bad_row = b;
+    fail(MTJG_E_ZERO_PIVOT, "Zero pivot encountered on row " + std::to_string(b + 1) +
+                                " during ILU process");
+  }
+  M->factored = true;
+  if (M->trans_ready && M->tval)
+    k_gather<<<grid_for(M->nnz), BS, 0, s>>>(M->nnz, M->tperm, M->lu, M->tval);
+}
+
+// Transposed factor for transApply (S/ILU.java:72-78): CSR of (LU)' with
+// each transposed row in ascending original-row order, a gather map from the
+// factor storage, and the level sets of U' (forward) and L' (backward).
+static void ilu_build_transpose(mtjg_prec M) {
+  mtjg_ctx ctx = M->ctx;
+  hipStream_t s = ctx->stream;
+  const int32_t n = M->n;
+  const int64_t nnz = M->nnz;
+  std::vector<int32_t> trp((size_t)n + 1, 0), tci((size_t)nnz), tperm((size_t)nnz), tdiag((size_t)n, -1);
+  for (int64_t j = 0; j < nnz; ++j) trp[(size_t)M->h_ci[(size_t)j] + 1]++;
+  for (int32_t c = 0; c < n; ++c) trp[c + 1] += trp[c];
+  std::vector<int32_t> pos(trp.begin(), trp.end() - 1);
+  for (int32_t i = 0; i < n; ++i)
+    for (int32_t j = M->h_rp[i]; j < M->h_rp[i + 1]; ++j) {
+      int32_t c = M->h_ci[(size_t)j];
+      int32_t p = pos[(size_t)c]++;
+      tci[(size_t)p] = i;
+      tperm[(size_t)p] = j;
+      if (i == c) tdiag[(size_t)c] = p;
+    }
+  size_t nnz1 = (size_t)std::max<int64_t>(nnz, 1);
+  HIPCHK(hipMalloc(&M->trp, sizeof(int32_t) * (size_t)(n + 1)));
+  HIPCHK(hipMalloc(&M->tci, sizeof(int32_t) * nnz1));
+  HIPCHK(hipMalloc(&M->tperm, sizeof(int32_t) * nnz1));
+  HIPCHK(hipMalloc(&M->tval, sizeof(double) * nnz1));
+  HIPCHK(hipMalloc(&M->tdiag, sizeof(int32_t) * (size_t)std::max(n, 1)));
+  HIPCHK(hipMemcpyAsync(M->trp, trp.data(), sizeof(int32_t) * trp.size(), hipMemcpyHostToDevice, s));
+  if (nnz) {
+    HIPCHK(hipMemcpyAsync(M->tci, tci.data(), sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(M->tperm, tperm.data(), sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice, s));
+  }
+  if (n) HIPCHK(hipMemcpyAsync(M->tdiag, tdiag.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, s));
+  std::vector<int32_t> beg((size_t)n), end((size_t)n);
+  for (int32_t c = 0; c < n; ++c) {
+    beg[(size_t)c] = trp[c];
+    end[(size_t)c] = tdiag[(size_t)c];
+  }
+  build_levels(ctx, n, trp, tci, beg, end, true, M->lev_ut);
+  for (int32_t c = 0; c < n; ++c) {
+    beg[(size_t)c] = tdiag[(size_t)c] + 1;
+    end[(size_t)c] = trp[c + 1];
+  }
+  build_levels(ctx, n, trp, tci, beg, end, false, M->lev_lt);
+  if (nnz) k_gather<<<grid_for(nnz), BS, 0, s>>>(nnz, M->tperm, M->lu, M->tval);
+  HIPCHK(hipGetLastError());
+  M->trans_ready = true;
+}
+
+void prec_apply_dev(mtjg_prec M, const double *b, double *x, hipStream_t s, KState *st) {
+  const int32_t n = M->n;
+  if (M->kind == PREC_IDENTITY) {
+    // IdentityPreconditioner: x.set(b) (S/AbstractIterativeSolver.java:84-86)
+    if (b != x && n > 0) k_copy_if_running<<<grid_for(n), BS, 0, s>>>(n, b, x, st);
+  } else if (M->kind == PREC_DIAGONAL) {
+    if (n > 0) k_jacobi_apply<<<grid_for(n), BS, 0, s>>>(n, b, M->invdiag, x, st);
+  } else {
+    if (!M->factored) fail(MTJG_E_INVALID_ARGUMENT, "ILU.setMatrix must precede apply");
+    // L.solve(b, y); U.solve(y, x) (S/ILU.java:64-70)
+    run_levels<TRSV_ROW_UNIT>(M->lev_l, s, M->rp, M->ci, M->diag, M->lu, b, M->y);
+    run_levels<TRSV_ROW_DIV>(M->lev_u, s, M->rp, M->ci, M->diag, M->lu, M->y, x);
+  }
+  HIPCHK(hipGetLastError());
+}
+
+void prec_trans_apply_dev(mtjg_prec M, const double *b, double *x, hipStream_t s) {
+  if (M->kind != PREC_ILU) {
+    prec_apply_dev(M, b, x, s, nullptr);
+    return;
+  }
+  if (!M->factored) fail(MTJG_E_INVALID_ARGUMENT, "ILU.setMatrix must precede transApply");
+  if (!M->trans_ready) ilu_build_transpose(M);
+  // U.transSolve(b, y); L.transSolve(y, x) (S/ILU.java:72-78)
+  run_levels<TRSV_COL_DIV>(M->lev_ut, s, M->trp, M->tci, M->tdiag, M->tval, b, M->y);
+  run_levels<TRSV_COL_UNIT_REV>(M->lev_lt, s, M->trp, M->tci, M->tdiag, M->tval, M->y, x);
+}
+
+int prec_kind(mtjg_prec M) { return M ? M->kind : PREC_IDENTITY; }
+const double *prec_invdiag(mtjg_prec M) { return M ? M->invdiag : nullptr; }
+int32_t prec_n(mtjg_prec M) { return M ? M->n : 0; }
+
+}  // namespace mtjg
+
+using namespace mtjg;
+
+// ===========================================================================
+// C ABI
+
+extern "C" int mtjg_prec_diagonal_create(mtjg_ctx ctx, int32_t n, mtjg_prec *out) {
+  MTJG_API_BEGIN
+  if (!ctx || !out) fail(MTJG_E_NULL, "null argument");
+  if (n < 0) fail(MTJG_E_INVALID_ARGUMENT, "negative size");
+  auto *M = new mtjg_prec_s();
+  M->ctx = ctx;
+  M->kind = PREC_DIAGONAL;
+  M->n = n;
+  HIPCHK(hipMalloc(&M->invdiag, sizeof(double) * (size_t)std::max(n, 1)));
+  HIPCHK(hipMemsetAsync(M->invdiag, 0, sizeof(double) * (size_t)std::max(n, 1), ctx->stream));
+  HIPCHK(hipMalloc(&M->bad, sizeof(int32_t)));
+  *out = M;
+  MTJG_API_END
+}
+
+extern "C" int mtjg_prec_ilu_create(mtjg_ctx ctx, mtjg_csr LU, mtjg_prec *out) {
+  MTJG_API_BEGIN
+  if (!ctx || !LU || !out) fail(MTJG_E_NULL, "null argument");
+  if (LU->nr != LU->nc) fail(MTJG_E_INVALID_ARGUMENT, "ILU only applies to square matrices");
+  auto *M = new mtjg_prec_s();
+  M->ctx = ctx;
+  M->kind = PREC_ILU;
+  M->n = LU->nr;
+  HIPCHK(hipMalloc(&M->y, sizeof(double) * (size_t)std::max(M->n, 1)));
+  HIPCHK(hipMemsetAsync(M->y, 0, sizeof(double) * (size_t)std::max(M->n, 1), ctx->stream));
+  HIPCHK(hipMalloc(&M->bad, sizeof(int32_t)));
+  *out = M;
+  MTJG_API_END
+}
+
+extern "C" int mtjg_prec_set_matrix(mtjg_prec M, mtjg_csr A, int32_t *bad_row) {
+  MTJG_API_BEGIN
+  if (!M || !A) fail(MTJG_E_NULL, "null argument");
+  hipStream_t s = M->ctx->stream;
+  if (M->kind == PREC_DIAGONAL) {
+    if (A->nr != M->n) fail(MTJG_E_INVALID_ARGUMENT, "Matrix size differs from preconditioner size");
+    if (A->nc < A->nr) fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "matrix has fewer columns than rows");
+    reset_bad(M);
+    if (M->n > 0)
+      k_jacobi_setup<<<(M->n + BS - 1) / BS, BS, 0, s>>>(M->n, A->rp, A->ci, A->val, M->invdiag, M->bad);
+    HIPCHK(hipGetLastError());
+    int32_t b = read_bad(M);
+    if (b != INT32_MAX) {
+      if (bad_row) *bad_row = b;
+      fail(MTJG_E_ZERO_DIAGONAL, "Zero diagonal on row " + std::to_string(b + 1));
+    }
+  } else if (M->kind == PREC_ILU) {
+    if (A->nr != M->n || A->nc != M->n) fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "matrix size differs from ILU size");
+    ilu_set_matrix(M, A, bad_row);
+  }
+  MTJG_API_END
+}
+
+static void check_apply(mtjg_prec M, mtjg_vec b, mtjg_vec x) {
+  if (!M || !b || !x) fail(MTJG_E_NULL, "null argument");
+  if (b->dtype != MTJG_F64 || x->dtype != MTJG_F64)
+    fail(MTJG_E_INVALID_ARGUMENT, "Vector must be DenseVectors");
+  if (b->n != M->n || x->n != M->n) fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "vector size differs from preconditioner size");
+  if (M->kind == PREC_ILU && b->ptr == x->ptr && M->n > 0)
+    fail(MTJG_E_INVALID_ARGUMENT, "b and x must not alias");
+}
+
+extern "C" int mtjg_prec_apply(mtjg_prec M, mtjg_vec b, mtjg_vec x) {
+  MTJG_API_BEGIN
+  check_apply(M, b, x);
+  prec_apply_dev(M, (const double *)b->ptr, (double *)x->ptr, M->ctx->stream, nullptr);
+  MTJG_API_END
+}
+
+extern "C" int mtjg_prec_trans_apply(mtjg_prec M, mtjg_vec b, mtjg_vec x) {
+  MTJG_API_BEGIN
+  check_apply(M, b, x);
+  prec_trans_apply_dev(M, (const double *)b->ptr, (double *)x->ptr, M->ctx->stream);
+  MTJG_API_END
+}
+
+extern "C" int mtjg_prec_destroy(mtjg_prec M) {
+  MTJG_API_BEGIN
+  if (!M) return MTJG_OK;
+  HIPCHK(hipStreamSynchronize(M->ctx->stream));
+  (void)hipFree(M->invdiag);
+  (void)hipFree(M->bad);
+  (void)hipFree(M->y);
+  ilu_free_storage(M);
+  delete M;
+  MTJG_API_END
+}

@@ -1,0 +1,949 @@
+// Iterative solvers (S/CG.java, S/BiCGstab.java, S/GMRES.java, S/IR.java)
+// driving fused HIP kernels with every operand device resident.
+//
+// Each solver keeps its scalars (rho, alpha, omega, beta, the Hessenberg
+// matrix, the iteration-monitor state) in a device struct.  Reductions end in
+// the last workgroup of their kernel, which evaluates the next scalar and the
+// DefaultIterationMonitor predicate on the device and sets a status word.
+// Every kernel of an iteration returns at once when the status is no longer
+// "running", so the host can queue several iterations ahead and only reads
+// the status word between batches ("chunks") of iterations.
+//
+// Vector updates apply the same per-element operation as MTJ (no FMA:
+// compiled with -ffp-contract=off) and therefore equal MTJ bitwise given the
+// same scalars; dot products and norms differ only in summation order.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "mtjg_internal.hpp"
+#include "reduce.hpp"
+
+using namespace mtjg;
+
+struct mtjg_solver_s {
+  mtjg_ctx ctx = nullptr;
+  int kind = MTJG_CG;
+  int64_t n = 0;
+  mtjg_prec M = nullptr;
+  mtjg_monitor mon{100000, 1e-5, 1e-50, 1e5};
+  mtjg_reporter rep = nullptr;
+  void *rep_user = nullptr;
+  int32_t restart = 30;
+  double inner_rtol = 1e-4;
+  int32_t inner_max = 1000;
+  // device state
+  KState *st = nullptr;      // outer solver state
+  KState *st_in = nullptr;   // inner (FP32) solve of MTJG_MIXED_CG
+  KState *h_st = nullptr;    // pinned staging [2]
+  int32_t *h_slot = nullptr; // pinned status slots [2]
+  double *partials = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  std::vector<double *> v64;  // scratch FP64 vectors
+  std::vector<float *> v32;   // scratch FP32 vectors (mixed)
+  // GMRES
+  double *V = nullptr;   // (restart+1) x n basis, vector-major
+  double *H = nullptr;   // (restart+1) x restart, column-major
+  double *gs = nullptr;  // s, rotation c and s: 3 * (restart+1)
+  int32_t inner_total = 0;
+};
+
+namespace {
+
+constexpr int KSLOT = 0;  // ticket slots inside KState
+
+// ---------------------------------------------------------------------------
+// CG kernels
+
+enum PrecK : int { PK_IDENTITY = 0, PK_JACOBI = 1, PK_EXTERNAL = 2 };
+
+// Residual step of CG.  With UPDATE it first applies x += alpha p and
+// r += (-alpha) q (S/CG.java:80-81) and advances the iteration counter
+// (iter.next()); then z = M r for identity / Jacobi
+// (S/DiagonalPreconditioner.java:54-55), and the sums r.z and r.r.  The last
+// workgroup evaluates the monitor on ||r|| (S/CG.java:66) and, while
+// running, rho = r.z (S/CG.java:68) and beta = rho / rho_1 (S/CG.java:73).
+template <typename T, int PK, bool UPDATE>
+__global__ __launch_bounds__(BS) void k_cg_resid(int64_t n, T *x, const T *p, T *r, const T *q,
+                                                 const T *invdiag, T *z, KState *st,
+                                                 double *partials) {
+  if (st->status != ST_RUNNING) return;
+  const T alpha = (T)st->alpha;
+  double v[2] = {0.0, 0.0};
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+    T ri = r[i];
+    if (UPDATE && alpha != T(0)) {
+      x[i] = x[i] + alpha * p[i];
+      ri = ri + (-alpha) * q[i];
+      r[i] = ri;
+    }
+    if (PK == PK_JACOBI) {
+      T zi = ri * invdiag[i];
+      z[i] = zi;
+      v[0] += (double)ri * (double)zi;
+    } else if (PK == PK_IDENTITY) {
+      v[0] += (double)ri * (double)ri;
+    }
+    v[1] += (double)ri * (double)ri;
+  }
+  if (grid_sum<2>(v, partials, &st->ticket[KSLOT]) && threadIdx.x == 0) {
+    if (UPDATE) st->iter++;
+    int s = monitor_check(st, sqrt(v[1]));
+    if (s != ST_RUNNING) {
+      st->status = s;
+      return;
+    }
+    if (PK != PK_EXTERNAL) {
+      st->rho = v[0];
+      if (st->iter > 0) st->beta = st->rho / st->rho_1;
+    }
+  }
+}
+
+// rho = r.z for a preconditioner applied by separate launches (ILU).
+template <typename T>
+__global__ __launch_bounds__(BS) void k_cg_rho(int64_t n, const T *r, const T *z, KState *st,
+                                               double *partials) {
+  if (st->status != ST_RUNNING) return;
+  double v[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+    v[0] += (double)r[i] * (double)z[i];
+  if (grid_sum<1>(v, partials, &st->ticket[KSLOT + 1]) && threadIdx.x == 0) {
+    st->rho = v[0];
+    if (st->iter > 0) st->beta = st->rho / st->rho_1;
+  }
+}
+
+// p = z on the first iteration, else p.scale(beta).add(z) (S/CG.java:70-75):
+// two roundings, p*beta then + z.
+template <typename T>
+__global__ __launch_bounds__(BS) void k_cg_p(int64_t n, T *p, const T *z, const KState *st) {
+  if (st->status != ST_RUNNING) return;
+  const bool first = st->iter == 0;
+  const T beta = (T)st->beta;
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+    if (first) {
+      p[i] = z[i];
+    } else {
+      T pb = p[i] * beta;
+      p[i] = pb + z[i];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BiCGstab kernels (S/BiCGstab.java:74-113)
+
+// Top of an iteration.  With UPDATE: x += omega shat, r = s + (-omega) t
+// (S/BiCGstab.java:109-110), rho_2 = rho_1 and iter.next(); then the sums
+// r.r and rtilde.r.  The last workgroup checks the monitor on ||r||, sets
+// rho_1 = rtilde.r, raises breakdown when rho_1 == 0 or omega == 0
+// (:77-83) and forms beta (:88).
+template <bool UPDATE>
+__global__ __launch_bounds__(BS) void k_bicg_top(int64_t n, double *x, const double *shat, double *r,
+                                                 const double *s, const double *t,
+                                                 const double *rtilde, KState *st,
+                                                 double *partials) {
+  if (st->status != ST_RUNNING) return;
+  const double omega = st->omega;
+  double v[2] = {0.0, 0.0};
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+    double ri;
+    if (UPDATE) {
+      if (omega != 0) x[i] = x[i] + omega * shat[i];
+      ri = s[i];
+      if (omega != 0) ri = ri + (-omega) * t[i];
+      r[i] = ri;
+    } else {
+      ri = r[i];
+    }
+    v[0] += ri * ri;
+    v[1] += rtilde[i] * ri;
+  }
+  if (grid_sum<2>(v, partials, &st->ticket[KSLOT]) && threadIdx.x == 0) {
+    if (UPDATE) {
+      st->rho_2 = st->rho_1;
+      st->iter++;
+    }
+    int stt = monitor_check(st, sqrt(v[0]));
+    if (stt != ST_RUNNING) {
+      st->status = stt;
+      return;
+    }
+    st->rho_1 = v[1];
+    if (st->rho_1 == 0 || st->omega == 0) {
+      st->status = ST_BREAKDOWN;
+      return;
+    }
+    if (st->iter > 0) st->beta = (st->rho_1 / st->rho_2) * (st->alpha / st->omega);
+  }
+}
+
+// p = r (first) or p = r + beta * ((-omega) v + p) (S/BiCGstab.java:85-95).
+__global__ __launch_bounds__(BS) void k_bicg_p(int64_t n, double *p, const double *r, const double *v,
+                                               const KState *st) {
+  if (st->status != ST_RUNNING) return;
+  const bool first = st->iter == 0;
+  const double beta = st->beta, omega = st->omega;
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+    if (first) {
+      p[i] = r[i];
+    } else {
+      double temp = (-omega) * v[i];
+      temp = temp + p[i];
+      double pi = r[i];
+      if (beta != 0) pi = pi + beta * temp;
+      p[i] = pi;
+    }
+  }
+}
+
+// s = r + (-alpha) v; x += alpha phat; then the half-step check on ||s||
+// (S/BiCGstab.java:100-104).
+__global__ __launch_bounds__(BS) void k_bicg_s(int64_t n, double *x, const double *phat, const double *r,
+                                               const double *v, double *s, KState *st,
+                                               double *partials) {
+  if (st->status != ST_RUNNING) return;
+  const double alpha = st->alpha;
+  double acc[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+    double si = r[i];
+    if (alpha != 0) {
+      si = si + (-alpha) * v[i];
+      x[i] = x[i] + alpha * phat[i];
+    }
+    s[i] = si;
+    acc[0] += si * si;
+  }
+  if (grid_sum<1>(acc, partials, &st->ticket[KSLOT + 1]) && threadIdx.x == 0) {
+    int stt = monitor_check(st, sqrt(acc[0]));
+    if (stt != ST_RUNNING) st->status = stt;
+  }
+}
+
+__global__ void k_bicg_omega(KState *st) {
+  if (st->status != ST_RUNNING) return;
+  st->omega = st->s[0] / st->s[1];  // S/BiCGstab.java:108
+}
+
+// ---------------------------------------------------------------------------
+// GMRES kernels (S/GMRES.java:127-176)
+
+struct GmresDev {
+  double *H;   // column-major (m+1) x m
+  double *gs;  // s[m+1], c[m+1], sn[m+1]
+  int m;
+};
+
+// Start of an outer cycle.  ||r|| (r = M^-1 (b - A x)); unless first,
+// iter.next() of the outer loop; the outer check; then s = normr e1 and
+// the inner check at i = 0 (S/GMRES.java:137-144).
+__global__ __launch_bounds__(BS) void k_gmres_start(int64_t n, const double *r, KState *st,
+                                                    GmresDev g, int first, double *partials) {
+  if (st->status != ST_RUNNING) return;
+  double v[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+    v[0] += r[i] * r[i];
+  if (grid_sum<1>(v, partials, &st->ticket[KSLOT]) && threadIdx.x == 0) {
+    double normr = sqrt(v[0]);
+    if (!first) st->iter++;
+    int stt = monitor_check(st, normr);
+    if (stt != ST_RUNNING) {
+      st->status = stt;
+      return;
+    }
+    st->normr = normr;
+    for (int k = 0; k <= g.m; ++k) g.gs[k] = 0.0;
+    g.gs[0] = normr;
+    st->inner_i = 0;
+    st->cycle_done = 0;
+    stt = monitor_check(st, fabs(g.gs[0]));
+    if (stt == ST_CONVERGED) st->cycle_done = 1;
+    else if (stt != ST_RUNNING) st->status = stt;
+  }
+}
+
+// v0 = (1/normr) r, with DenseVector.set(alpha, y)'s alpha == 0 zeroing.
+__global__ __launch_bounds__(BS) void k_gmres_v0(int64_t n, const double *r, double *v0, const KState *st) {
+  if (st->status != ST_RUNNING) return;
+  const double a = 1 / st->normr;
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+    v0[i] = (a == 0) ? 0.0 : a * r[i];
+}
+
+// Modified Gram-Schmidt step k of Arnoldi column i: w += (-H(k-1,i)) v_{k-1}
+// (S/GMRES.java:151) for k > 0, then H(k,i) = w.v_k (:150).
+__global__ __launch_bounds__(BS) void k_gmres_mgs(int64_t n, double *w, const double *vprev,
+                                                  const double *vk, int k, int i, KState *st,
+                                                  GmresDev g, double *partials) {
+  if (st->status != ST_RUNNING) return;
+  const int ld = g.m + 1;
+  const double h = (k > 0) ? -g.H[(k - 1) + (size_t)i * ld] : 0.0;
+  double v[1] = {0.0};
+  for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
+    double we = w[e];
+    if (h != 0) {
+      we = we + h * vprev[e];
+      w[e] = we;
+    }
+    v[0] += we * vk[e];
+  }
+  if (grid_sum<1>(v, partials, &st->ticket[KSLOT + 1]) && threadIdx.x == 0) g.H[k + (size_t)i * ld] = v[0];
+}
+
+// GivensRotation constructor and apply (M/GivensRotation.java:42-92).
+__device__ void givens(double x, double y, double &c, double &s) {
+  double roe = fabs(x) > fabs(y) ? x : y;
+  double scale = fabs(x) + fabs(y);
+  if (scale != 0) {
+    double xs = x / scale;
+    double ys = y / scale;
+    double r = scale * sqrt(xs * xs + ys * ys);
+    if (roe < 0) r *= -1;
+    c = x / r;
+    s = y / r;
+  } else {
+    c = 1;
+    s = 0;
+  }
+}
+__device__ void givens_apply(double c, double s, double &a1, double &a2) {
+  double temp = c * a1 + s * a2;
+  a2 = -s * a1 + c * a2;
+  a1 = temp;
+}
+
+// Last MGS subtraction w += (-H(i,i)) v_i, then H(i+1,i) = ||w||
+// (S/GMRES.java:153); the last workgroup applies the previous rotations to
+// column i, forms rotation i, rotates H and s (:157-162), advances i and the
+// iteration counter, and runs the next inner check (:144).
+__global__ __launch_bounds__(BS) void k_gmres_arnoldi_end(int64_t n, double *w, const double *vi, int i,
+                                                          KState *st, GmresDev g, double *partials) {
+  if (st->status != ST_RUNNING) return;
+  const int ld = g.m + 1;
+  const double h = -g.H[i + (size_t)i * ld];
+  double v[1] = {0.0};
+  for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
+    double we = w[e];
+    if (h != 0) {
+      we = we + h * vi[e];
+      w[e] = we;
+    }
+    v[0] += we * we;
+  }
+  if (grid_sum<1>(v, partials, &st->ticket[KSLOT]) && threadIdx.x == 0) {
+    double *Hc = g.H + (size_t)i * ld;
+    double *s = g.gs, *rc = g.gs + ld, *rs = g.gs + 2 * ld;
+    Hc[i + 1] = sqrt(v[0]);
+    st->s[2] = Hc[i + 1];  // ||w||, used to scale v_{i+1} (:154)
+    for (int k = 0; k < i; ++k) givens_apply(rc[k], rs[k], Hc[k], Hc[k + 1]);
+    givens(Hc[i], Hc[i + 1], rc[i], rs[i]);
+    givens_apply(rc[i], rs[i], Hc[i], Hc[i + 1]);
+    givens_apply(rc[i], rs[i], s[i], s[i + 1]);
+    st->inner_i = i + 1;
+    st->iter++;
+    if (i + 1 < g.m) {
+      int stt = monitor_check(st, fabs(s[i + 1]));
+      if (stt == ST_CONVERGED) st->cycle_done = 1;
+      else if (stt != ST_RUNNING) st->status = stt;
+    } else {
+      st->cycle_done = 1;
+    }
+  }
+}
+
+// v_{i+1} = (1/H(i+1,i)) w (S/GMRES.java:154).
+__global__ __launch_bounds__(BS) void k_gmres_scale(int64_t n, const double *w, double *vnext, const KState *st) {
+  if (st->status != ST_RUNNING) return;
+  const double a = 1. / st->s[2];
+  for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS)
+    vnext[e] = (a == 0) ? 0.0 : a * w[e];
+}
+
+// new UpperTriangDenseMatrix(H, i, false).solve(s, s) (S/GMRES.java:166)
+// through dtrtrs: singular on an exactly zero diagonal
+// (M/AbstractTriangDenseMatrix.java:247-252), else reference-BLAS dtrsm
+// column-order back-substitution.
+__global__ void k_gmres_backsolve(KState *st, GmresDev g) {
+  if (st->status != ST_RUNNING) return;
+  const int k = st->inner_i, ld = g.m + 1;
+  double *s = g.gs;
+  for (int i = 0; i < k; ++i)
+    if (g.H[i + (size_t)i * ld] == 0) {
+      st->status = ST_SINGULAR;
+      return;
+    }
+  for (int kk = k - 1; kk >= 0; --kk) {
+    if (s[kk] != 0) {
+      s[kk] = s[kk] / g.H[kk + (size_t)kk * ld];
+      for (int i = 0; i < kk; ++i) s[i] = s[i] - s[kk] * g.H[i + (size_t)kk * ld];
+    }
+  }
+}
+
+// x.add(s_j, v_j) for j < i, in order (S/GMRES.java:167-168).
+__global__ __launch_bounds__(BS) void k_gmres_xupdate(int64_t n, double *x, const double *V, const KState *st,
+                                                      GmresDev g) {
+  if (st->status != ST_RUNNING) return;
+  const int k = st->inner_i;
+  const double *s = g.gs;
+  for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
+    double xe = x[e];
+    for (int j = 0; j < k; ++j) {
+      const double sj = s[j];
+      if (sj != 0) xe = xe + sj * V[(size_t)j * n + e];
+    }
+    x[e] = xe;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// IR (S/IR.java:57-70) and shared helpers
+
+// ||r|| and the monitor check; with advance, iter.next() first.
+__global__ __launch_bounds__(BS) void k_norm_check(int64_t n, const double *r, KState *st, int advance,
+                                                   double *partials) {
+  if (st->status != ST_RUNNING) return;
+  double v[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+    v[0] += r[i] * r[i];
+  if (grid_sum<1>(v, partials, &st->ticket[KSLOT]) && threadIdx.x == 0) {
+    if (advance) st->iter++;
+    int stt = monitor_check(st, sqrt(v[0]));
+    if (stt != ST_RUNNING) st->status = stt;
+    st->normr = sqrt(v[0]);
+  }
+}
+
+__global__ __launch_bounds__(BS) void k_add_if_running(int64_t n, double *x, const double *z, const KState *st) {
+  if (st->status != ST_RUNNING) return;
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+    x[i] = x[i] + z[i];
+}
+
+// Mixed precision: r32 = (float)(r / ||r||); x += ||r|| * (double)d32.
+__global__ __launch_bounds__(BS) void k_demote_scaled(int64_t n, const double *r, float *r32, const KState *st) {
+  if (st->status != ST_RUNNING) return;
+  const double sc = 1.0 / st->normr;
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+    r32[i] = (float)(r[i] * sc);
+}
+
+__global__ __launch_bounds__(BS) void k_promote_add(int64_t n, double *x, const float *d32, const KState *st) {
+  if (st->status != ST_RUNNING) return;
+  const double sc = st->normr;
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+    x[i] = x[i] + sc * (double)d32[i];
+}
+
+__global__ __launch_bounds__(BS) void k_invdiag32(int64_t n, const int32_t *rp, const int32_t *ci,
+                                                  const double *val, float *inv) {
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+    double d = 0;
+    for (int32_t j = rp[i]; j < rp[i + 1]; ++j)
+      if (ci[j] == (int32_t)i) d = val[j];
+    inv[i] = (float)(1 / d);
+  }
+}
+
+__global__ void k_copy_f32_zero(int64_t n, const float *a, float *b, float *z) {
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+    b[i] = a[i];
+    z[i] = 0.0f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host driver helpers
+
+void state_init(mtjg_solver S, KState *dst, int32_t max_iter, double rtol, double atol, double dtol) {
+  KState &h = S->h_st[dst == S->st ? 0 : 1];
+  memset(&h, 0, sizeof(KState));
+  h.max_iter = max_iter;
+  h.rtol = rtol;
+  h.atol = atol;
+  h.dtol = dtol;
+  h.alpha = 0;
+  h.rho_1 = 0;
+  h.status = ST_RUNNING;
+  HIPCHK(hipMemcpyAsync(dst, &h, sizeof(KState), hipMemcpyHostToDevice, S->ctx->stream));
+}
+
+void state_read(mtjg_solver S, KState *src, KState &out) {
+  KState *h = &S->h_st[src == S->st ? 0 : 1];
+  HIPCHK(hipMemcpyAsync(h, src, sizeof(KState), hipMemcpyDeviceToHost, S->ctx->stream));
+  HIPCHK(hipStreamSynchronize(S->ctx->stream));
+  out = *h;
+}
+
+// Queues status reads behind each chunk of iterations, keeping up to two
+// chunks in flight, until the status word leaves "running".
+template <typename Launch>
+void run_chunks(mtjg_solver S, KState *st, int chunk, Launch &&launch_iteration) {
+  hipStream_t s = S->ctx->stream;
+  if (S->rep) chunk = 1;
+  auto enqueue = [&](int slot) {
+    for (int c = 0; c < chunk; ++c) launch_iteration();
+    HIPCHK(hipMemcpyAsync(&S->h_slot[slot], &st->status, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(S->ev[slot], s));
+  };
+  if (S->rep) {
+    for (;;) {
+      enqueue(0);
+      HIPCHK(hipEventSynchronize(S->ev[0]));
+      KState k;
+      state_read(S, st, k);
+      S->rep(k.residual, k.iter, S->rep_user);
+      if (k.status != ST_RUNNING) return;
+    }
+  }
+  enqueue(0);
+  enqueue(1);
+  for (int k = 0;; ++k) {
+    int slot = k & 1;
+    HIPCHK(hipEventSynchronize(S->ev[slot]));
+    if (S->h_slot[slot] != ST_RUNNING) break;
+    enqueue(slot);
+  }
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+int status_to_code(int st) {
+  switch (st) {
+    case ST_CONVERGED: return MTJG_OK;
+    case ST_ITERATIONS: return MTJG_NOT_CONVERGED_ITERATIONS;
+    case ST_DIVERGENCE: return MTJG_NOT_CONVERGED_DIVERGENCE;
+    case ST_BREAKDOWN: return MTJG_NOT_CONVERGED_BREAKDOWN;
+    case ST_SINGULAR: return MTJG_E_SINGULAR;
+    default: return MTJG_E_DEVICE;
+  }
+}
+
+double *scratch(mtjg_solver S, size_t k) {
+  while (S->v64.size() <= k) {
+    double *p = nullptr;
+    HIPCHK(hipMalloc(&p, sizeof(double) * (size_t)std::max<int64_t>(S->n, 1)));
+    HIPCHK(hipMemsetAsync(p, 0, sizeof(double) * (size_t)std::max<int64_t>(S->n, 1), S->ctx->stream));
+    S->v64.push_back(p);
+  }
+  return S->v64[k];
+}
+
+float *scratch32(mtjg_solver S, size_t k) {
+  while (S->v32.size() <= k) {
+    float *p = nullptr;
+    HIPCHK(hipMalloc(&p, sizeof(float) * (size_t)std::max<int64_t>(S->n, 1)));
+    S->v32.push_back(p);
+  }
+  return S->v32[k];
+}
+
+// Iterations queued per status read: aims at a few hundred microseconds of
+// device work per chunk, fewer for large systems.
+int chunk_for(mtjg_csr A) {
+  double bytes = 12.0 * (double)A->nnz + 108.0 * (double)A->nr;
+  int c = (int)(2.0e8 / std::max(bytes, 1.0));
+  return std::max(1, std::min(c, 16));
+}
+
+int prec_k(mtjg_prec M) {
+  int k = prec_kind(M);
+  if (k == PREC_IDENTITY) return PK_IDENTITY;
+  if (k == PREC_DIAGONAL) return PK_JACOBI;
+  return PK_EXTERNAL;
+}
+
+// ---------------------------------------------------------------------------
+// CG (S/CG.java:58-87)
+
+void solve_cg(mtjg_solver S, mtjg_csr A, double *b, double *x) {
+  hipStream_t s = S->ctx->stream;
+  const int64_t n = S->n;
+  double *p = scratch(S, 0), *z = scratch(S, 1), *q = scratch(S, 2), *r = scratch(S, 3);
+  const int pk = prec_k(S->M);
+  const double *invd = prec_invdiag(S->M);
+  if (pk == PK_IDENTITY) z = r;  // identity: z = r bitwise, no copy needed
+  state_init(S, S->st, S->mon.max_iter, S->mon.rtol, S->mon.atol, S->mon.dtol);
+  // r = b - A x (S/CG.java:64)
+  SpmvArgs ra;
+  ra.mode = SPMV_MULTADD;
+  ra.alpha = -1;
+  ra.y0 = b;
+  spmv_f64(A, x, r, ra, s);
+  const int g = grid_for(n);
+  KState *st = S->st;
+  auto resid = [&](bool update) {
+    if (pk == PK_IDENTITY) {
+      if (update) k_cg_resid<double, PK_IDENTITY, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->partials);
+      else k_cg_resid<double, PK_IDENTITY, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->partials);
+    } else if (pk == PK_JACOBI) {
+      if (update) k_cg_resid<double, PK_JACOBI, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->partials);
+      else k_cg_resid<double, PK_JACOBI, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->partials);
+    } else {
+      if (update) k_cg_resid<double, PK_EXTERNAL, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->partials);
+      else k_cg_resid<double, PK_EXTERNAL, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->partials);
+      prec_apply_dev(S->M, r, z, s, st);
+      k_cg_rho<double><<<g, BS, 0, s>>>(n, r, z, st, S->partials);
+    }
+  };
+  resid(false);
+  HIPCHK(hipGetLastError());
+  SpmvArgs qa;
+  qa.mode = SPMV_DOT;
+  qa.w = p;
+  qa.st = st;
+  qa.fin = FIN_CG_ALPHA;
+  qa.partials = S->partials;
+  qa.ticket = &st->ticket[KSLOT + 2];
+  run_chunks(S, st, pk == PK_EXTERNAL ? 1 : chunk_for(A), [&]() {
+    k_cg_p<double><<<g, BS, 0, s>>>(n, p, z, st);
+    spmv_f64(A, p, q, qa, s);
+    resid(true);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+// ---------------------------------------------------------------------------
+// BiCGstab (S/BiCGstab.java:65-116)
+
+void solve_bicgstab(mtjg_solver S, mtjg_csr A, double *b, double *x) {
+  hipStream_t s = S->ctx->stream;
+  const int64_t n = S->n;
+  double *p = scratch(S, 0), *sv = scratch(S, 1), *phat = scratch(S, 2), *shat = scratch(S, 3),
+         *t = scratch(S, 4), *v = scratch(S, 5), *r = scratch(S, 6), *rtilde = scratch(S, 7);
+  const int pk = prec_k(S->M);
+  if (pk == PK_IDENTITY) {
+    phat = p;  // M.apply = x.set(b): the copy is exact
+    shat = sv;
+  }
+  KState *st = S->st;
+  state_init(S, st, S->mon.max_iter, S->mon.rtol, S->mon.atol, S->mon.dtol);
+  // omega = alpha = rho_1 = rho_2 = 1 (S/BiCGstab.java:69)
+  {
+    KState &h = S->h_st[0];
+    h.rho_1 = h.rho_2 = h.alpha = h.beta = h.omega = 1;
+    HIPCHK(hipMemcpyAsync(st, &h, sizeof(KState), hipMemcpyHostToDevice, s));
+  }
+  SpmvArgs ra;
+  ra.mode = SPMV_MULTADD;
+  ra.alpha = -1;
+  ra.y0 = b;
+  spmv_f64(A, x, r, ra, s);
+  if (n) HIPCHK(hipMemcpyAsync(rtilde, r, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
+  const int g = grid_for(n);
+  k_bicg_top<false><<<g, BS, 0, s>>>(n, x, shat, r, sv, t, rtilde, st, S->partials);
+  HIPCHK(hipGetLastError());
+  SpmvArgs va;
+  va.mode = SPMV_DOT;
+  va.w = rtilde;
+  va.st = st;
+  va.fin = FIN_BICG_ALPHA;
+  va.partials = S->partials;
+  va.ticket = &st->ticket[KSLOT + 2];
+  SpmvArgs ta = va;
+  ta.w = sv;
+  ta.fin = FIN_BICG_OMEGA;
+  ta.ticket = &st->ticket[KSLOT + 3];
+  run_chunks(S, st, pk == PK_EXTERNAL ? 1 : chunk_for(A), [&]() {
+    k_bicg_p<<<g, BS, 0, s>>>(n, p, r, v, st);
+    if (pk != PK_IDENTITY) prec_apply_dev(S->M, p, phat, s, st);
+    spmv_f64(A, phat, v, va, s);
+    k_bicg_s<<<g, BS, 0, s>>>(n, x, phat, r, v, sv, st, S->partials);
+    if (pk != PK_IDENTITY) prec_apply_dev(S->M, sv, shat, s, st);
+    spmv_f64(A, shat, t, ta, s);
+    k_bicg_omega<<<1, 1, 0, s>>>(st);
+    k_bicg_top<true><<<g, BS, 0, s>>>(n, x, shat, r, sv, t, rtilde, st, S->partials);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+// ---------------------------------------------------------------------------
+// GMRES (S/GMRES.java:127-176); the host reads the state after each inner
+// step to know whether the cycle has ended.
+
+void gmres_alloc(mtjg_solver S) {
+  if (S->V) return;
+  const int m = S->restart;
+  HIPCHK(hipMalloc(&S->V, sizeof(double) * (size_t)(m + 1) * (size_t)std::max<int64_t>(S->n, 1)));
+  HIPCHK(hipMemsetAsync(S->V, 0, sizeof(double) * (size_t)(m + 1) * (size_t)std::max<int64_t>(S->n, 1),
+                        S->ctx->stream));
+  HIPCHK(hipMalloc(&S->H, sizeof(double) * (size_t)(m + 1) * (size_t)m));
+  HIPCHK(hipMemsetAsync(S->H, 0, sizeof(double) * (size_t)(m + 1) * (size_t)m, S->ctx->stream));
+  HIPCHK(hipMalloc(&S->gs, sizeof(double) * 3 * (size_t)(m + 1)));
+  HIPCHK(hipMemsetAsync(S->gs, 0, sizeof(double) * 3 * (size_t)(m + 1), S->ctx->stream));
+}
+
+void gmres_free(mtjg_solver S) {
+  (void)hipFree(S->V);
+  (void)hipFree(S->H);
+  (void)hipFree(S->gs);
+  S->V = S->H = S->gs = nullptr;
+}
+
+void solve_gmres(mtjg_solver S, mtjg_csr A, double *b, double *x) {
+  hipStream_t s = S->ctx->stream;
+  const int64_t n = S->n;
+  gmres_alloc(S);
+  double *w = scratch(S, 0), *u = scratch(S, 1), *r = scratch(S, 2);
+  KState *st = S->st;
+  state_init(S, st, S->mon.max_iter, S->mon.rtol, S->mon.atol, S->mon.dtol);
+  GmresDev gd{S->H, S->gs, S->restart};
+  const int g = grid_for(n);
+  auto Vk = [&](int k) { return S->V + (size_t)k * (size_t)n; };
+  SpmvArgs ra;
+  ra.mode = SPMV_MULTADD;
+  ra.alpha = -1;
+  ra.y0 = b;
+  SpmvArgs ma;
+  ma.mode = SPMV_MULT;
+  auto residual = [&]() {
+    // u = b - A x; r = M^-1 u (S/GMRES.java:131-132, :170-171)
+    spmv_f64(A, x, u, ra, s);
+    prec_apply_dev(S->M, u, r, s, nullptr);
+  };
+  residual();
+  k_gmres_start<<<g, BS, 0, s>>>(n, r, st, gd, 1, S->partials);
+  HIPCHK(hipGetLastError());
+  KState k;
+  for (;;) {
+    state_read(S, st, k);
+    if (k.status != ST_RUNNING) break;
+    k_gmres_v0<<<g, BS, 0, s>>>(n, r, Vk(0), st);
+    for (int i = 0; i < S->restart && !k.cycle_done; ++i) {
+      spmv_f64(A, Vk(i), u, ma, s);
+      prec_apply_dev(S->M, u, w, s, nullptr);
+      for (int kk = 0; kk <= i; ++kk)
+        k_gmres_mgs<<<g, BS, 0, s>>>(n, w, kk > 0 ? Vk(kk - 1) : nullptr, Vk(kk), kk, i, st, gd, S->partials);
+      k_gmres_arnoldi_end<<<g, BS, 0, s>>>(n, w, Vk(i), i, st, gd, S->partials);
+      k_gmres_scale<<<g, BS, 0, s>>>(n, w, Vk(i + 1), st);
+      HIPCHK(hipGetLastError());
+      state_read(S, st, k);
+      if (k.status != ST_RUNNING) break;
+    }
+    if (k.status != ST_RUNNING) break;
+    k_gmres_backsolve<<<1, 1, 0, s>>>(st, gd);
+    k_gmres_xupdate<<<g, BS, 0, s>>>(n, x, S->V, st, gd);
+    residual();
+    k_gmres_start<<<g, BS, 0, s>>>(n, r, st, gd, 0, S->partials);
+    HIPCHK(hipGetLastError());
+  }
+}
+
+// ---------------------------------------------------------------------------
+// IR (S/IR.java:57-70)
+
+void solve_ir(mtjg_solver S, mtjg_csr A, double *b, double *x) {
+  hipStream_t s = S->ctx->stream;
+  const int64_t n = S->n;
+  double *z = scratch(S, 0), *r = scratch(S, 1);
+  KState *st = S->st;
+  state_init(S, st, S->mon.max_iter, S->mon.rtol, S->mon.atol, S->mon.dtol);
+  SpmvArgs ra;
+  ra.mode = SPMV_MULTADD;
+  ra.alpha = -1;
+  ra.y0 = b;
+  const int g = grid_for(n);
+  spmv_f64(A, x, r, ra, s);
+  k_norm_check<<<g, BS, 0, s>>>(n, r, st, 0, S->partials);
+  HIPCHK(hipGetLastError());
+  run_chunks(S, st, 1, [&]() {
+    prec_apply_dev(S->M, r, z, s, st);
+    k_add_if_running<<<g, BS, 0, s>>>(n, x, z, st);
+    spmv_f64(A, x, r, ra, s);
+    k_norm_check<<<g, BS, 0, s>>>(n, r, st, 1, S->partials);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision: FP64 refinement x += M32^-1 (b - A x) where M32^-1 is an
+// FP32 Jacobi-CG solve on the FP32 copy of A's values, run on the residual
+// scaled to unit norm.  The FP64 residual uses the FP64 values, so the outer
+// tolerance is met for A itself.
+
+void solve_mixed(mtjg_solver S, mtjg_csr A, double *b, double *x) {
+  hipStream_t s = S->ctx->stream;
+  const int64_t n = S->n;
+  double *r = scratch(S, 0);
+  float *r32 = scratch32(S, 0), *d32 = scratch32(S, 1), *p32 = scratch32(S, 2), *q32 = scratch32(S, 3),
+        *z32 = scratch32(S, 4), *rin = scratch32(S, 5), *inv32 = scratch32(S, 6);
+  csr_ensure_fp32(A);
+  const int g = grid_for(n);
+  k_invdiag32<<<g, BS, 0, s>>>(n, A->rp, A->ci, A->val, inv32);
+  KState *st = S->st, *si = S->st_in;
+  state_init(S, st, S->mon.max_iter, S->mon.rtol, S->mon.atol, S->mon.dtol);
+  SpmvArgs ra;
+  ra.mode = SPMV_MULTADD;
+  ra.alpha = -1;
+  ra.y0 = b;
+  spmv_f64(A, x, r, ra, s);
+  k_norm_check<<<g, BS, 0, s>>>(n, r, st, 0, S->partials);
+  HIPCHK(hipGetLastError());
+  SpmvArgs qa;
+  qa.mode = SPMV_DOT;
+  qa.w = p32;
+  qa.st = si;
+  qa.fin = FIN_CG_ALPHA;
+  qa.partials = S->partials;
+  qa.ticket = &si->ticket[KSLOT + 2];
+  S->inner_total = 0;
+  const int chunk = chunk_for(A);
+  for (;;) {
+    KState k;
+    state_read(S, st, k);
+    if (S->rep) S->rep(k.residual, k.iter, S->rep_user);
+    if (k.status != ST_RUNNING) break;
+    // inner FP32 Jacobi-CG on A32 d = r/||r||, from d = 0
+    k_demote_scaled<<<g, BS, 0, s>>>(n, r, r32, st);
+    k_copy_f32_zero<<<g, BS, 0, s>>>(n, r32, rin, d32);
+    state_init(S, si, S->inner_max, S->inner_rtol, 0.0, 1e300);
+    k_cg_resid<float, PK_JACOBI, false><<<g, BS, 0, s>>>(n, d32, p32, rin, q32, inv32, z32, si, S->partials);
+    HIPCHK(hipGetLastError());
+    run_chunks(S, si, chunk, [&]() {
+      k_cg_p<float><<<g, BS, 0, s>>>(n, p32, z32, si);
+      spmv_f32(A, p32, q32, qa, s);
+      k_cg_resid<float, PK_JACOBI, true><<<g, BS, 0, s>>>(n, d32, p32, rin, q32, inv32, z32, si, S->partials);
+      HIPCHK(hipGetLastError());
+    });
+    KState ki;
+    state_read(S, si, ki);
+    S->inner_total += ki.iter;
+    // x += ||r|| d; r = b - A x; outer check
+    k_promote_add<<<g, BS, 0, s>>>(n, x, d32, st);
+    spmv_f64(A, x, r, ra, s);
+    k_norm_check<<<g, BS, 0, s>>>(n, r, st, 1, S->partials);
+    HIPCHK(hipGetLastError());
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+
+extern "C" int mtjg_solver_create(mtjg_ctx ctx, int kind, mtjg_vec template_vec, mtjg_solver *out) {
+  MTJG_API_BEGIN
+  if (!ctx || !template_vec || !out) fail(MTJG_E_NULL, "null argument");
+  if (kind < MTJG_CG || kind > MTJG_MIXED_CG) fail(MTJG_E_INVALID_ARGUMENT, "unknown solver kind");
+  auto *S = new mtjg_solver_s();
+  S->ctx = ctx;
+  S->kind = kind;
+  S->n = template_vec->n;
+  HIPCHK(hipMalloc(&S->st, sizeof(KState)));
+  HIPCHK(hipMalloc(&S->st_in, sizeof(KState)));
+  HIPCHK(hipMemsetAsync(S->st, 0, sizeof(KState), ctx->stream));
+  HIPCHK(hipMemsetAsync(S->st_in, 0, sizeof(KState), ctx->stream));
+  HIPCHK(hipHostMalloc(&S->h_st, sizeof(KState) * 2, hipHostMallocDefault));
+  HIPCHK(hipHostMalloc(&S->h_slot, sizeof(int32_t) * 2, hipHostMallocDefault));
+  HIPCHK(hipMalloc(&S->partials, sizeof(double) * RED_MAX_K * RED_MAX_BLOCKS));
+  HIPCHK(hipEventCreateWithFlags(&S->ev[0], hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&S->ev[1], hipEventDisableTiming));
+  // scratch vectors come from template.copy() (S/CG.java:51-56)
+  int nvec = kind == MTJG_CG ? 4 : kind == MTJG_BICGSTAB ? 8 : kind == MTJG_GMRES ? 3 : kind == MTJG_IR ? 2 : 1;
+  for (int k = 0; k < nvec; ++k) scratch(S, (size_t)k);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  *out = S;
+  MTJG_API_END
+}
+
+extern "C" int mtjg_solver_destroy(mtjg_solver S) {
+  MTJG_API_BEGIN
+  if (!S) return MTJG_OK;
+  HIPCHK(hipStreamSynchronize(S->ctx->stream));
+  for (double *p : S->v64) (void)hipFree(p);
+  for (float *p : S->v32) (void)hipFree(p);
+  gmres_free(S);
+  (void)hipFree(S->st);
+  (void)hipFree(S->st_in);
+  (void)hipFree(S->partials);
+  (void)hipHostFree(S->h_st);
+  (void)hipHostFree(S->h_slot);
+  (void)hipEventDestroy(S->ev[0]);
+  (void)hipEventDestroy(S->ev[1]);
+  delete S;
+  MTJG_API_END
+}
+
+extern "C" int mtjg_solver_set_preconditioner(mtjg_solver S, mtjg_prec M) {
+  MTJG_API_BEGIN
+  if (!S) fail(MTJG_E_NULL, "null solver");
+  S->M = M;
+  MTJG_API_END
+}
+
+extern "C" int mtjg_solver_set_monitor(mtjg_solver S, const mtjg_monitor *m) {
+  MTJG_API_BEGIN
+  if (!S || !m) fail(MTJG_E_NULL, "null argument");
+  S->mon = *m;
+  MTJG_API_END
+}
+
+extern "C" int mtjg_solver_set_reporter(mtjg_solver S, mtjg_reporter cb, void *user) {
+  MTJG_API_BEGIN
+  if (!S) fail(MTJG_E_NULL, "null solver");
+  S->rep = cb;
+  S->rep_user = user;
+  MTJG_API_END
+}
+
+extern "C" int mtjg_gmres_set_restart(mtjg_solver S, int32_t restart) {
+  MTJG_API_BEGIN
+  if (!S) fail(MTJG_E_NULL, "null solver");
+  if (restart <= 0) fail(MTJG_E_INVALID_ARGUMENT, "restart must be a positive integer");
+  HIPCHK(hipStreamSynchronize(S->ctx->stream));
+  gmres_free(S);
+  S->restart = restart;
+  MTJG_API_END
+}
+
+extern "C" int mtjg_mixed_set_inner(mtjg_solver S, double inner_rtol, int32_t inner_max_iter) {
+  MTJG_API_BEGIN
+  if (!S) fail(MTJG_E_NULL, "null solver");
+  if (!(inner_rtol > 0) || inner_max_iter <= 0) fail(MTJG_E_INVALID_ARGUMENT, "bad inner settings");
+  S->inner_rtol = inner_rtol;
+  S->inner_max = inner_max_iter;
+  MTJG_API_END
+}
+
+extern "C" int mtjg_solver_solve(mtjg_solver S, mtjg_csr A, mtjg_vec b, mtjg_vec x, mtjg_result *result) {
+  MTJG_API_BEGIN
+  if (!S || !A || !b || !x) fail(MTJG_E_NULL, "null argument");
+  // checkSizes (S/AbstractIterativeSolver.java:70-77)
+  if (A->nr != A->nc) fail(MTJG_E_INVALID_ARGUMENT, "!A.isSquare()");
+  if (b->n != A->nr) fail(MTJG_E_INVALID_ARGUMENT, "b.size() != A.numRows()");
+  if (b->n != x->n) fail(MTJG_E_INVALID_ARGUMENT, "b.size() != x.size()");
+  if (b->n != S->n) fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "solver scratch size differs from the system size");
+  if (b->dtype != MTJG_F64 || x->dtype != MTJG_F64) fail(MTJG_E_UNSUPPORTED, "solvers take FP64 vectors");
+  if (b->ptr == x->ptr && b->n > 0) fail(MTJG_E_INVALID_ARGUMENT, "b and x must not alias");
+  if (S->M && prec_n(S->M) != S->n) fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "preconditioner size differs");
+  if (A->nr == 0) fail(MTJG_E_INVALID_ARGUMENT, "empty system");
+  HIPCHK(hipSetDevice(S->ctx->device));
+  double *bp = (double *)b->ptr, *xp = (double *)x->ptr;
+  switch (S->kind) {
+    case MTJG_CG: solve_cg(S, A, bp, xp); break;
+    case MTJG_BICGSTAB: solve_bicgstab(S, A, bp, xp); break;
+    case MTJG_GMRES: solve_gmres(S, A, bp, xp); break;
+    case MTJG_IR: solve_ir(S, A, bp, xp); break;
+    case MTJG_MIXED_CG: solve_mixed(S, A, bp, xp); break;
+  }
+  KState k;
+  state_read(S, S->st, k);
+  int code = status_to_code(k.status);
+  if (result) {
+    result->iterations = k.iter;
+    result->residual = k.residual;
+    result->reason = code;
+  }
+  if (code != MTJG_OK) {
+    const char *why = code == MTJG_NOT_CONVERGED_ITERATIONS   ? "Iterations"
+                      : code == MTJG_NOT_CONVERGED_DIVERGENCE ? "Divergence"
+                      : code == MTJG_NOT_CONVERGED_BREAKDOWN  ? "Breakdown"
+                                                              : "MatrixSingular";
+    set_last_error(std::string("Solver did not converge: ") + why + ". Residual=" + std::to_string(k.residual) +
+                   ", iterations=" + std::to_string(k.iter));
+  }
+  return code;
+  MTJG_API_END
+}
+
+extern "C" int mtjg_solver_inner_iterations(mtjg_solver S) { return S ? S->inner_total : -1; }

@@ -1,0 +1,571 @@
+// CompRowMatrix on the device and its CSR products (S/CompRowMatrix.java).
+//
+// SpMV kernel ("row-block stream"): the rows are cut on the host into
+// row-blocks holding at most SPMV_CAP stored entries (and at most
+// SPMV_MAX_ROWS rows).  A 256-thread workgroup takes a row-block and
+//   1. streams the block's column indices and values with unit-stride,
+//      fully coalesced loads (8 entries per thread, all issued before any
+//      use), gathers x at those columns, and stages the products
+//      val[j] * x[col[j]] in LDS;
+//   2. reduces each row from LDS.  When the block holds at least 128 rows a
+//      thread owns a row and adds its products sequentially in storage
+//      order, exactly as MTJ's loop (S/CompRowMatrix.java:304-312) does, so
+//      the result is bitwise MTJ's; blocks with fewer, longer rows give each
+//      row a power-of-two group of lanes that add strided partials and
+//      combine them with an xor butterfly.
+// A row longer than SPMV_CAP gets a row-block of its own, reduced by the
+// whole workgroup straight from global memory.  The kernel is HBM-bound
+// (12 bytes per stored entry, 20 per row), so MFMA is not used.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+#include "mtjg_internal.hpp"
+#include "reduce.hpp"
+
+namespace mtjg {
+
+constexpr int SPMV_CAP = 2048;           // products staged per row-block
+constexpr int SPMV_PER = SPMV_CAP / BS;  // entries per thread per row-block
+constexpr int SPMV_MAX_ROWS = 1024;      // rows per row-block
+
+// Row-block schedule: greedy cut so every block holds <= SPMV_CAP entries,
+// except single rows longer than that.
+static std::vector<int32_t> build_row_blocks(const std::vector<int32_t> &rp, int32_t nr) {
+  std::vector<int32_t> rb;
+  rb.reserve((size_t)nr / 64 + 2);
+  rb.push_back(0);
+  int32_t r = 0;
+  while (r < nr) {
+    int32_t len = rp[r + 1] - rp[r];
+    if (len > SPMV_CAP) {
+      ++r;
+      rb.push_back(r);
+      continue;
+    }
+    int32_t acc = 0, rows = 0;
+    while (r < nr && rows < SPMV_MAX_ROWS) {
+      int32_t l = rp[r + 1] - rp[r];
+      if (acc + l > SPMV_CAP) break;
+      acc += l;
+      ++r;
+      ++rows;
+    }
+    rb.push_back(r);
+  }
+  return rb;
+}
+
+__device__ __forceinline__ int pow2_floor(int v) { return 1 << (31 - __clz(v)); }
+
+// Writes row `row`'s result and accumulates the epilogue dots.
+template <typename XT, int MODE>
+__device__ __forceinline__ void spmv_store(int row, XT s, XT *__restrict__ y, const XT *y0,
+                                           double alpha, double inv_alpha, const XT *w,
+                                           double &acc0, double &acc1) {
+  if (MODE == SPMV_MULT || MODE == SPMV_DOT) {
+    XT v = (s != XT(0)) ? s : XT(0);
+    y[row] = v;
+    if (MODE == SPMV_DOT) {
+      acc0 += (double)w[row] * (double)v;
+      acc1 += (double)v * (double)v;
+    }
+  } else if (MODE == SPMV_MULTADD) {
+    y[row] = y0[row] + (XT)alpha * s;
+  } else {  // SPMV_TMULTADD: the initial value was folded into s
+    y[row] = s * (XT)alpha;
+  }
+}
+
+template <typename XT, int MODE>
+__device__ __forceinline__ XT spmv_init(int row, const XT *y0, double inv_alpha) {
+  if (MODE == SPMV_TMULTADD) return y0[row] * (XT)inv_alpha;
+  return XT(0);
+}
+
+// Scalar epilogues of fused solver kernels: run by the last workgroup.
+__device__ __forceinline__ void spmv_finish(KState *st, int fin, double s0, double s1) {
+  if (fin == FIN_CG_ALPHA) {
+    st->alpha = st->rho / s0;  // S/CG.java:78
+    st->rho_1 = st->rho;       // S/CG.java:83
+  } else if (fin == FIN_BICG_ALPHA) {
+    st->alpha = st->rho_1 / s0;  // S/BiCGstab.java:99
+  } else if (fin == FIN_BICG_OMEGA) {
+    st->s[0] = s0;
+    st->s[1] = s1;
+  }
+}
+
+template <typename VT, typename XT, int MODE>
+__global__ __launch_bounds__(BS) void k_spmv(int nrb, const int32_t *__restrict__ rb,
+                                             const int32_t *__restrict__ rp,
+                                             const int32_t *__restrict__ ci,
+                                             const VT *__restrict__ val, const XT *__restrict__ x,
+                                             XT *y, const XT *y0, double alpha, double inv_alpha,
+                                             const XT *w, KState *st, int fin, double *partials,
+                                             uint32_t *ticket) {
+  if (MODE == SPMV_DOT && st->status != ST_RUNNING) return;
+  __shared__ XT prod[SPMV_CAP];
+  double acc0 = 0.0, acc1 = 0.0;
+  for (int b = blockIdx.x; b < nrb; b += gridDim.x) {
+    const int r0 = rb[b], r1 = rb[b + 1];
+    const int j0 = rp[r0];
+    const int nnzb = rp[r1] - j0;
+    if (nnzb > SPMV_CAP) {
+      // one long row: strided partials from global memory, fixed tree
+      double v[1] = {0.0};
+      XT s = 0;
+      for (int k = threadIdx.x; k < nnzb; k += BS) s += (XT)val[j0 + k] * x[ci[j0 + k]];
+      v[0] = (double)s;
+      block_sum<1>(v);
+      if (threadIdx.x == 0) {
+        XT tot = (XT)v[0];
+        if (MODE == SPMV_TMULTADD) tot = spmv_init<XT, MODE>(r0, y0, inv_alpha) + tot;
+        spmv_store<XT, MODE>(r0, tot, y, y0, alpha, inv_alpha, w, acc0, acc1);
+      }
+      continue;
+    }
+    // 1. stage products in LDS (coalesced col/val streaming, then gathers)
+    int c[SPMV_PER];
+    VT a[SPMV_PER];
+    XT xv[SPMV_PER];
+#pragma unroll
+    for (int u = 0; u < SPMV_PER; ++u) {
+      const int k = threadIdx.x + u * BS;
+      if (k < nnzb) {
+        c[u] = __builtin_nontemporal_load(&ci[j0 + k]);
+        a[u] = __builtin_nontemporal_load(&val[j0 + k]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SPMV_PER; ++u) {
+      const int k = threadIdx.x + u * BS;
+      if (k < nnzb) xv[u] = x[c[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < SPMV_PER; ++u) {
+      const int k = threadIdx.x + u * BS;
+      if (k < nnzb) prod[k] = (XT)a[u] * xv[u];
+    }
+    __syncthreads();
+    // 2. per-row reduction from LDS
+    const int nrows = r1 - r0;
+    int tpr = 1;
+    if (nrows * 2 <= BS) tpr = min(64, pow2_floor(BS / max(nrows, 1)));
+    const int groups = BS / tpr;
+    const int lane = threadIdx.x & (tpr - 1);
+    for (int g = threadIdx.x / tpr; g < nrows; g += groups) {
+      const int row = r0 + g;
+      const int beg = rp[row] - j0, end = rp[row + 1] - j0;
+      XT s = (lane == 0) ? spmv_init<XT, MODE>(row, y0, inv_alpha) : XT(0);
+      for (int k = beg + lane; k < end; k += tpr) s += prod[k];
+      for (int off = tpr >> 1; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+      if (lane == 0) spmv_store<XT, MODE>(row, s, y, y0, alpha, inv_alpha, w, acc0, acc1);
+    }
+    __syncthreads();
+  }
+  if (MODE == SPMV_DOT) {
+    double v[2] = {acc0, acc1};
+    if (grid_sum<2>(v, partials, ticket) && threadIdx.x == 0) spmv_finish(st, fin, v[0], v[1]);
+  }
+}
+
+template <typename VT, typename XT>
+static void spmv_launch(mtjg_csr A, const VT *val, const XT *x, XT *y, const SpmvArgs &a,
+                        hipStream_t s) {
+  if (A->nr == 0) {
+    if (a.mode == SPMV_DOT) fail(MTJG_E_INVALID_ARGUMENT, "empty matrix in fused product");
+    return;
+  }
+  int grid = A->nrb;
+  if (a.mode == SPMV_DOT) grid = std::min(grid, RED_MAX_BLOCKS);
+  else grid = std::min(grid, 1 << 20);
+  const XT *y0 = (const XT *)a.y0;
+  const XT *w = (const XT *)a.w;
+  switch (a.mode) {
+    case SPMV_MULT:
+      k_spmv<VT, XT, SPMV_MULT><<<grid, BS, 0, s>>>(A->nrb, A->rb, A->rp, A->ci, val, x, y, y0, a.alpha,
+                                                   a.inv_alpha, w, a.st, a.fin, a.partials, a.ticket);
+      break;
+    case SPMV_MULTADD:
+      k_spmv<VT, XT, SPMV_MULTADD><<<grid, BS, 0, s>>>(A->nrb, A->rb, A->rp, A->ci, val, x, y, y0,
+                                                      a.alpha, a.inv_alpha, w, a.st, a.fin,
+                                                      a.partials, a.ticket);
+      break;
+    case SPMV_TMULTADD:
+      k_spmv<VT, XT, SPMV_TMULTADD><<<grid, BS, 0, s>>>(A->nrb, A->rb, A->rp, A->ci, val, x, y, y0,
+                                                       a.alpha, a.inv_alpha, w, a.st, a.fin,
+                                                       a.partials, a.ticket);
+      break;
+    case SPMV_DOT:
+      k_spmv<VT, XT, SPMV_DOT><<<grid, BS, 0, s>>>(A->nrb, A->rb, A->rp, A->ci, val, x, y, y0, a.alpha,
+                                                  a.inv_alpha, w, a.st, a.fin, a.partials, a.ticket);
+      break;
+    default:
+      fail(MTJG_E_INVALID_ARGUMENT, "bad spmv mode");
+  }
+  HIPCHK(hipGetLastError());
+}
+
+void spmv_f64(mtjg_csr A, const double *x, double *y, const SpmvArgs &a, hipStream_t s) {
+  spmv_launch<double, double>(A, A->val, x, y, a, s);
+}
+
+void spmv_f32(mtjg_csr A, const float *x, float *y, const SpmvArgs &a, hipStream_t s) {
+  csr_ensure_fp32(A);
+  spmv_launch<float, float>(A, A->val32, x, y, a, s);
+}
+
+__global__ void k_to_f32(int64_t n, const double *a, float *b) {
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+    b[i] = (float)a[i];
+}
+
+void csr_ensure_fp32(mtjg_csr A) {
+  if (A->val32 || A->nnz == 0) return;
+  HIPCHK(hipMalloc(&A->val32, sizeof(float) * (size_t)A->nnz));
+  k_to_f32<<<grid_for(A->nnz), BS, 0, A->ctx->stream>>>(A->nnz, A->val, A->val32);
+  HIPCHK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// construction
+
+static void csr_free(mtjg_csr_s *A) {
+  if (!A) return;
+  if (A->trans) csr_free(A->trans);
+  (void)hipFree(A->rp);
+  (void)hipFree(A->ci);
+  (void)hipFree(A->val);
+  (void)hipFree(A->val32);
+  (void)hipFree(A->rb);
+  delete A;
+}
+
+// Uploads a validated host CSR (sorted unique columns) and builds its
+// schedule.
+static mtjg_csr_s *csr_upload(mtjg_ctx ctx, int32_t nr, int32_t nc, std::vector<int32_t> &&rp,
+                              const int32_t *ci, const double *val) {
+  auto *A = new mtjg_csr_s();
+  A->ctx = ctx;
+  A->nr = nr;
+  A->nc = nc;
+  A->nnz = rp[nr];
+  A->h_rp = std::move(rp);
+  try {
+    size_t nnz1 = (size_t)std::max<int64_t>(A->nnz, 1);
+    HIPCHK(hipMalloc(&A->rp, sizeof(int32_t) * (size_t)(nr + 1)));
+    HIPCHK(hipMalloc(&A->ci, sizeof(int32_t) * nnz1));
+    HIPCHK(hipMalloc(&A->val, sizeof(double) * nnz1));
+    HIPCHK(hipMemcpyAsync(A->rp, A->h_rp.data(), sizeof(int32_t) * (size_t)(nr + 1),
+                          hipMemcpyHostToDevice, ctx->stream));
+    if (A->nnz > 0) {
+      HIPCHK(hipMemcpyAsync(A->ci, ci, sizeof(int32_t) * (size_t)A->nnz, hipMemcpyHostToDevice,
+                            ctx->stream));
+      HIPCHK(hipMemcpyAsync(A->val, val, sizeof(double) * (size_t)A->nnz, hipMemcpyHostToDevice,
+                            ctx->stream));
+    }
+    std::vector<int32_t> rb = build_row_blocks(A->h_rp, nr);
+    A->nrb = (int32_t)rb.size() - 1;
+    HIPCHK(hipMalloc(&A->rb, sizeof(int32_t) * rb.size()));
+    HIPCHK(hipMemcpyAsync(A->rb, rb.data(), sizeof(int32_t) * rb.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  } catch (...) {
+    csr_free(A);
+    throw;
+  }
+  return A;
+}
+
+// CSR of A' with each column's entries in ascending row order, so that the
+// sequential sum of a transposed row adds the scatter contributions of
+// S/CompRowMatrix.java:361-363 in MTJ's order.
+mtjg_csr_s *csr_transpose(mtjg_csr A) {
+  if (A->trans) return A->trans;
+  mtjg_ctx ctx = A->ctx;
+  std::vector<int32_t> ci((size_t)A->nnz);
+  std::vector<double> val((size_t)A->nnz);
+  if (A->nnz > 0) {
+    HIPCHK(hipMemcpyAsync(ci.data(), A->ci, sizeof(int32_t) * (size_t)A->nnz, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipMemcpyAsync(val.data(), A->val, sizeof(double) * (size_t)A->nnz,
+                          hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  std::vector<int32_t> trp((size_t)A->nc + 1, 0);
+  for (int64_t j = 0; j < A->nnz; ++j) trp[(size_t)ci[j] + 1]++;
+  for (int32_t c = 0; c < A->nc; ++c) trp[c + 1] += trp[c];
+  std::vector<int32_t> pos(trp.begin(), trp.end() - 1);
+  std::vector<int32_t> tci((size_t)A->nnz);
+  std::vector<double> tval((size_t)A->nnz);
+  for (int32_t i = 0; i < A->nr; ++i)
+    for (int32_t j = A->h_rp[i]; j < A->h_rp[i + 1]; ++j) {
+      int32_t p = pos[ci[j]]++;
+      tci[p] = i;
+      tval[p] = val[j];
+    }
+  A->trans = csr_upload(ctx, A->nc, A->nr, std::move(trp), tci.data(), tval.data());
+  return A->trans;
+}
+
+}  // namespace mtjg
+
+using namespace mtjg;
+
+// ===========================================================================
+// C ABI
+
+extern "C" int mtjg_csr_create(mtjg_ctx ctx, int32_t n_rows, int32_t n_cols, int64_t nnz,
+                               const int32_t *row_pointer, const int32_t *column_index,
+                               const double *data, mtjg_csr *out) {
+  MTJG_API_BEGIN
+  if (!ctx || !out || !row_pointer) fail(MTJG_E_NULL, "null argument");
+  if (n_rows < 0 || n_cols < 0 || nnz < 0) fail(MTJG_E_INVALID_ARGUMENT, "negative size");
+  if (nnz > INT32_MAX) fail(MTJG_E_INVALID_ARGUMENT, "nnz exceeds int32 indexing");
+  if (nnz > 0 && (!column_index || !data)) fail(MTJG_E_NULL, "null argument");
+  std::vector<int32_t> rp(row_pointer, row_pointer + (size_t)n_rows + 1);
+  if (rp[0] != 0 || rp[n_rows] != nnz)
+    fail(MTJG_E_INVALID_ARGUMENT, "row pointers must run from 0 to nnz");
+  bool sorted = true;
+  for (int32_t i = 0; i < n_rows; ++i) {
+    if (rp[i + 1] < rp[i]) fail(MTJG_E_INVALID_ARGUMENT, "row pointers must be non-decreasing");
+    for (int32_t j = rp[i]; j < rp[i + 1]; ++j) {
+      int32_t c = column_index[j];
+      if (c < 0 || c >= n_cols)
+        fail(MTJG_E_INVALID_ARGUMENT, "nz[" + std::to_string(i) + "][" + std::to_string(j - rp[i]) +
+                                          "]=" + std::to_string(c) +
+                                          ", which is not a valid column index");
+      if (j > rp[i] && column_index[j - 1] >= c) sorted = false;
+    }
+  }
+  const int32_t *ci = column_index;
+  const double *val = data;
+  std::vector<int32_t> sci;
+  std::vector<double> sval;
+  if (!sorted) {
+    // CompRowMatrix keeps columns sorted within each row
+    // (S/CompRowMatrix.java:206); the values follow their columns.
+    sci.resize((size_t)nnz);
+    sval.resize((size_t)nnz);
+    std::vector<int32_t> perm;
+    for (int32_t i = 0; i < n_rows; ++i) {
+      int32_t b = rp[i], e = rp[i + 1];
+      perm.resize((size_t)(e - b));
+      std::iota(perm.begin(), perm.end(), b);
+      std::stable_sort(perm.begin(), perm.end(),
+                       [&](int32_t p, int32_t q) { return column_index[p] < column_index[q]; });
+      for (int32_t k = 0; k < e - b; ++k) {
+        sci[(size_t)(b + k)] = column_index[perm[k]];
+        sval[(size_t)(b + k)] = data[perm[k]];
+        if (k > 0 && sci[(size_t)(b + k)] == sci[(size_t)(b + k - 1)])
+          fail(MTJG_E_INVALID_ARGUMENT,
+               "duplicate column " + std::to_string(sci[(size_t)(b + k)]) + " in row " + std::to_string(i));
+      }
+    }
+    ci = sci.data();
+    val = sval.data();
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  *out = csr_upload(ctx, n_rows, n_cols, std::move(rp), ci, val);
+  MTJG_API_END
+}
+
+extern "C" int mtjg_csr_destroy(mtjg_csr A) {
+  MTJG_API_BEGIN
+  if (!A) return MTJG_OK;
+  HIPCHK(hipStreamSynchronize(A->ctx->stream));
+  csr_free(A);
+  MTJG_API_END
+}
+
+extern "C" int mtjg_csr_shape(mtjg_csr A, int32_t *n_rows, int32_t *n_cols, int64_t *nnz) {
+  MTJG_API_BEGIN
+  if (!A) fail(MTJG_E_NULL, "null matrix");
+  if (n_rows) *n_rows = A->nr;
+  if (n_cols) *n_cols = A->nc;
+  if (nnz) *nnz = A->nnz;
+  MTJG_API_END
+}
+
+extern "C" int mtjg_csr_download(mtjg_csr A, int32_t *row_pointer, int32_t *column_index,
+                                 double *data) {
+  MTJG_API_BEGIN
+  if (!A) fail(MTJG_E_NULL, "null matrix");
+  hipStream_t s = A->ctx->stream;
+  if (row_pointer) memcpy(row_pointer, A->h_rp.data(), sizeof(int32_t) * (size_t)(A->nr + 1));
+  if (column_index && A->nnz)
+    HIPCHK(hipMemcpyAsync(column_index, A->ci, sizeof(int32_t) * (size_t)A->nnz, hipMemcpyDeviceToHost, s));
+  if (data && A->nnz)
+    HIPCHK(hipMemcpyAsync(data, A->val, sizeof(double) * (size_t)A->nnz, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  MTJG_API_END
+}
+
+static void csr_values_changed(mtjg_csr A) {
+  // derived copies of the values are rebuilt on next use
+  if (A->val32) {
+    HIPCHK(hipStreamSynchronize(A->ctx->stream));
+    (void)hipFree(A->val32);
+    A->val32 = nullptr;
+  }
+  if (A->trans) {
+    HIPCHK(hipStreamSynchronize(A->ctx->stream));
+    csr_free(A->trans);
+    A->trans = nullptr;
+  }
+}
+
+extern "C" int mtjg_csr_set_values(mtjg_csr A, const double *data) {
+  MTJG_API_BEGIN
+  if (!A || (!data && A->nnz)) fail(MTJG_E_NULL, "null argument");
+  csr_values_changed(A);
+  if (A->nnz)
+    HIPCHK(hipMemcpyAsync(A->val, data, sizeof(double) * (size_t)A->nnz, hipMemcpyHostToDevice,
+                          A->ctx->stream));
+  HIPCHK(hipStreamSynchronize(A->ctx->stream));
+  MTJG_API_END
+}
+
+// Position of (row, column) in the pattern or -1 (S/CompRowMatrix.java:
+// 406-431 through S/Arrays.java:119-121); host binary search over a
+// downloaded row.
+static int64_t csr_find(mtjg_csr A, int32_t row, int32_t column) {
+  if (row < 0 || row >= A->nr || column < 0 || column >= A->nc)
+    fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "index out of range (" + std::to_string(row) + ", " +
+                                         std::to_string(column) + ")");
+  int32_t b = A->h_rp[row], e = A->h_rp[row + 1];
+  if (e == b) return -1;
+  std::vector<int32_t> cols((size_t)(e - b));
+  HIPCHK(hipMemcpyAsync(cols.data(), A->ci + b, sizeof(int32_t) * cols.size(), hipMemcpyDeviceToHost,
+                        A->ctx->stream));
+  HIPCHK(hipStreamSynchronize(A->ctx->stream));
+  auto it = std::lower_bound(cols.begin(), cols.end(), column);
+  if (it == cols.end() || *it != column) return -1;
+  return b + (it - cols.begin());
+}
+
+extern "C" int mtjg_csr_get(mtjg_csr A, int32_t row, int32_t column, double *out) {
+  MTJG_API_BEGIN
+  if (!A || !out) fail(MTJG_E_NULL, "null argument");
+  int64_t p = csr_find(A, row, column);
+  if (p < 0) {
+    *out = 0;
+  } else {
+    HIPCHK(hipMemcpyAsync(out, A->val + p, sizeof(double), hipMemcpyDeviceToHost, A->ctx->stream));
+    HIPCHK(hipStreamSynchronize(A->ctx->stream));
+  }
+  MTJG_API_END
+}
+
+extern "C" int mtjg_csr_set(mtjg_csr A, int32_t row, int32_t column, double value) {
+  MTJG_API_BEGIN
+  if (!A) fail(MTJG_E_NULL, "null matrix");
+  int64_t p = csr_find(A, row, column);
+  if (p < 0)
+    fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "Entry (" + std::to_string(row + 1) + ", " +
+                                         std::to_string(column + 1) +
+                                         ") is not in the matrix structure");
+  csr_values_changed(A);
+  HIPCHK(hipMemcpyAsync(A->val + p, &value, sizeof(double), hipMemcpyHostToDevice, A->ctx->stream));
+  HIPCHK(hipStreamSynchronize(A->ctx->stream));
+  MTJG_API_END
+}
+
+// checkMultAdd / checkTransMultAdd (M/AbstractMatrix.java:169-176, 204-211).
+static void check_mult(mtjg_csr A, mtjg_vec x, mtjg_vec y, bool trans) {
+  if (!A || !x || !y) fail(MTJG_E_NULL, "null argument");
+  int64_t xin = trans ? A->nr : A->nc, yout = trans ? A->nc : A->nr;
+  if (x->n != xin)
+    fail(MTJG_E_INDEX_OUT_OF_BOUNDS, std::string(trans ? "A.numRows" : "A.numColumns") +
+                                         " != x.size (" + std::to_string(xin) + " != " +
+                                         std::to_string(x->n) + ")");
+  if (y->n != yout)
+    fail(MTJG_E_INDEX_OUT_OF_BOUNDS, std::string(trans ? "A.numColumns" : "A.numRows") +
+                                         " != y.size (" + std::to_string(yout) + " != " +
+                                         std::to_string(y->n) + ")");
+  if (x->dtype != MTJG_F64 || y->dtype != MTJG_F64)
+    fail(MTJG_E_UNSUPPORTED, "CompRowMatrix products take FP64 vectors");
+  if (x->ptr == y->ptr && (x->n > 0))
+    fail(MTJG_E_INVALID_ARGUMENT, "x and y must not alias");
+}
+
+extern "C" int mtjg_csr_mult(mtjg_csr A, mtjg_vec x, mtjg_vec y) {
+  MTJG_API_BEGIN
+  check_mult(A, x, y, false);
+  SpmvArgs a;
+  a.mode = SPMV_MULT;
+  spmv_f64(A, (const double *)x->ptr, (double *)y->ptr, a, A->ctx->stream);
+  MTJG_API_END
+}
+
+extern "C" int mtjg_csr_mult_add(mtjg_csr A, double alpha, mtjg_vec x, mtjg_vec y) {
+  MTJG_API_BEGIN
+  check_mult(A, x, y, false);
+  SpmvArgs a;
+  a.mode = SPMV_MULTADD;
+  a.alpha = alpha;
+  a.y0 = y->ptr;
+  spmv_f64(A, (const double *)x->ptr, (double *)y->ptr, a, A->ctx->stream);
+  MTJG_API_END
+}
+
+extern "C" int mtjg_csr_mult_scaled(mtjg_csr A, double alpha, mtjg_vec x, mtjg_vec y) {
+  MTJG_API_BEGIN
+  check_mult(A, x, y, false);
+  if (y->n) HIPCHK(hipMemsetAsync(y->ptr, 0, sizeof(double) * (size_t)y->n, A->ctx->stream));
+  SpmvArgs a;
+  a.mode = SPMV_MULTADD;
+  a.alpha = alpha;
+  a.y0 = y->ptr;
+  spmv_f64(A, (const double *)x->ptr, (double *)y->ptr, a, A->ctx->stream);
+  MTJG_API_END
+}
+
+extern "C" int mtjg_csr_trans_mult(mtjg_csr A, mtjg_vec x, mtjg_vec y) {
+  MTJG_API_BEGIN
+  check_mult(A, x, y, true);
+  mtjg_csr_s *T = csr_transpose(A);
+  SpmvArgs a;
+  a.mode = SPMV_MULT;
+  spmv_f64(T, (const double *)x->ptr, (double *)y->ptr, a, A->ctx->stream);
+  MTJG_API_END
+}
+
+extern "C" int mtjg_csr_trans_mult_add(mtjg_csr A, double alpha, mtjg_vec x, mtjg_vec y) {
+  MTJG_API_BEGIN
+  check_mult(A, x, y, true);
+  mtjg_csr_s *T = csr_transpose(A);
+  SpmvArgs a;
+  a.mode = SPMV_TMULTADD;
+  a.alpha = alpha;
+  a.inv_alpha = 1. / alpha;
+  a.y0 = y->ptr;
+  spmv_f64(T, (const double *)x->ptr, (double *)y->ptr, a, A->ctx->stream);
+  MTJG_API_END
+}
+
+extern "C" int mtjg_csr_trans_mult_scaled(mtjg_csr A, double alpha, mtjg_vec x, mtjg_vec y) {
+  MTJG_API_BEGIN
+  check_mult(A, x, y, true);
+  if (y->n) HIPCHK(hipMemsetAsync(y->ptr, 0, sizeof(double) * (size_t)y->n, A->ctx->stream));
+  return mtjg_csr_trans_mult_add(A, alpha, x, y);
+  MTJG_API_END
+}
+
+extern "C" int mtjg_csr_row_histogram(mtjg_csr A, int64_t *bins, int nbins) {
+  MTJG_API_BEGIN
+  if (!A || !bins) fail(MTJG_E_NULL, "null argument");
+  for (int b = 0; b < nbins; ++b) bins[b] = 0;
+  for (int32_t i = 0; i < A->nr; ++i) {
+    int32_t len = A->h_rp[i + 1] - A->h_rp[i];
+    int b = 0;
+    while (len > 0) {
+      ++b;
+      len >>= 1;
+    }
+    if (b >= nbins) b = nbins - 1;
+    if (nbins > 0) bins[b]++;
+  }
+  MTJG_API_END
+}

@@ -1,0 +1,159 @@
+/*
+ * mtj_oracle.h -- CPU restatement of MTJ's unstructured-sparse hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation: only the test suite, the smoke check and the CPU-baseline
+ * leg of the benchmark load it.  Nothing in the product library links or
+ * calls it.
+ *
+ * Every function restates one MTJ routine in the same operation order, with
+ * IEEE binary64 arithmetic and no FMA contraction (built with
+ * -O2 -ffp-contract=off -fno-fast-math), so that results equal what a JVM
+ * running MTJ computes on the same inputs.  Reference citations use
+ *   S/ = src/main/java/no/uib/cipr/matrix/sparse/
+ *   M/ = src/main/java/no/uib/cipr/matrix/
+ *
+ * Parity pinning (what checks each function independently of its own
+ * formula) is listed in the test files tests/test_oracle_*.py; the one
+ * function whose arithmetic is fixed by a third-party library (the LAPACK
+ * dtrtrs back-substitution reached from GMRES) follows reference BLAS dtrsm
+ * and is pinned against scipy.linalg.solve_triangular.
+ */
+#ifndef MTJ_ORACLE_H
+#define MTJ_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Outcome codes, mirroring IterativeSolverNotConvergedException.Reason
+ * (M/NotConvergedException.java:33-49) and the RuntimeExceptions of the
+ * preconditioners. */
+enum {
+    OC_OK = 0,
+    OC_ITERATIONS = 1,      /* Reason.Iterations */
+    OC_DIVERGENCE = 2,      /* Reason.Divergence */
+    OC_BREAKDOWN = 3,       /* Reason.Breakdown  */
+    OC_ZERO_DIAGONAL = 10,  /* S/DiagonalPreconditioner.java:71-72 */
+    OC_MISSING_DIAGONAL = 11, /* S/ILU.java:134-136 */
+    OC_ZERO_PIVOT = 12,     /* S/ILU.java:105-107 */
+    OC_SINGULAR = 13,       /* MatrixSingularException from dtrtrs */
+    OC_BAD_ARG = 20
+};
+
+/* ---- CompRowMatrix (S/CompRowMatrix.java) -------------------------------- */
+void oc_csr_mult(int nr, const int32_t *rp, const int32_t *ci, const double *a,
+                 const double *x, double *y);
+void oc_csr_mult_add(int nr, const int32_t *rp, const int32_t *ci,
+                     const double *a, double alpha, const double *x, double *y);
+void oc_csr_trans_mult(int nr, int nc, const int32_t *rp, const int32_t *ci,
+                       const double *a, const double *x, double *y);
+void oc_csr_trans_mult_add(int nr, int nc, const int32_t *rp, const int32_t *ci,
+                           const double *a, double alpha, const double *x,
+                           double *y);
+double oc_csr_get(const int32_t *rp, const int32_t *ci, const double *a, int row,
+                  int col);
+
+/* ---- DenseVector (M/DenseVector.java) ------------------------------------ */
+void oc_dv_zero(int n, double *x);
+void oc_dv_scale(int n, double alpha, double *x);
+void oc_dv_set(int n, const double *y, double *x);
+void oc_dv_set_scaled(int n, double alpha, const double *y, double *x);
+void oc_dv_add(int n, const double *y, double *x);
+void oc_dv_add_scaled(int n, double alpha, const double *y, double *x);
+double oc_dv_dot(int n, const double *x, const double *y);
+double oc_dv_norm1(int n, const double *x);
+double oc_dv_norm2(int n, const double *x);
+double oc_dv_norm2_robust(int n, const double *x);
+double oc_dv_norm_inf(int n, const double *x);
+
+/* ---- DefaultIterationMonitor (S/DefaultIterationMonitor.java) ------------ */
+typedef struct {
+    int iter;
+    int max_iter;
+    double rtol, atol, dtol;
+    double init_r;
+    double residual;
+} oc_monitor;
+
+void oc_monitor_init(oc_monitor *m, int max_iter, double rtol, double atol,
+                     double dtol);
+/* 1 = converged, 0 = continue, OC_DIVERGENCE / OC_ITERATIONS as negative
+ * values: -OC_DIVERGENCE, -OC_ITERATIONS. */
+int oc_monitor_converged(oc_monitor *m, double r);
+
+/* ---- Preconditioners ------------------------------------------------------ */
+enum { OC_PREC_IDENTITY = 0, OC_PREC_DIAGONAL = 1, OC_PREC_ILU = 2 };
+
+typedef struct {
+    int kind;
+    int n;
+    double *invdiag;     /* DiagonalPreconditioner */
+    int32_t *rp, *ci;    /* ILU: copied pattern */
+    double *lu;          /* ILU: factor values */
+    int32_t *diagind;    /* ILU: diagonal positions */
+    double *y;           /* ILU: temporary */
+} oc_prec;
+
+void oc_prec_identity(oc_prec *p, int n);
+/* Returns OC_OK or OC_ZERO_DIAGONAL (row in *bad_row, 0-based). */
+int oc_prec_diagonal(oc_prec *p, int n, const int32_t *rp, const int32_t *ci,
+                     const double *a, int *bad_row);
+/* Returns OC_OK, OC_MISSING_DIAGONAL or OC_ZERO_PIVOT (row in *bad_row). */
+int oc_prec_ilu(oc_prec *p, int n, const int32_t *rp, const int32_t *ci,
+                const double *a, int *bad_row);
+void oc_prec_apply(const oc_prec *p, const double *b, double *x);
+void oc_prec_trans_apply(const oc_prec *p, const double *b, double *x);
+void oc_prec_free(oc_prec *p);
+/* Accessors for tests: the factored values and diagonal index (ILU). */
+const double *oc_prec_ilu_values(const oc_prec *p);
+const int32_t *oc_prec_ilu_diagind(const oc_prec *p);
+
+/* Triangular views over an ILU factor (S/UnitLowerCompRowMatrix.java,
+ * S/UpperCompRowMatrix.java). */
+void oc_unit_lower_solve(int n, const int32_t *rp, const int32_t *ci,
+                         const double *a, const int32_t *diagind,
+                         const double *b, double *x);
+void oc_unit_lower_trans_solve(int n, const int32_t *rp, const int32_t *ci,
+                               const double *a, const int32_t *diagind,
+                               const double *b, double *x);
+void oc_upper_solve(int n, const int32_t *rp, const int32_t *ci,
+                    const double *a, const int32_t *diagind, const double *b,
+                    double *x);
+void oc_upper_trans_solve(int n, const int32_t *rp, const int32_t *ci,
+                          const double *a, const int32_t *diagind,
+                          const double *b, double *x);
+
+/* ---- GMRES helpers --------------------------------------------------------- */
+/* GivensRotation (M/GivensRotation.java:42-58). */
+void oc_givens(double x, double y, double *c, double *s);
+/* Upper-triangular solve of the leading k x k block of column-major H (leading
+ * dimension ld) in place on s; reference-BLAS dtrsm column order as reached
+ * through dtrtrs (M/AbstractTriangDenseMatrix.java:228-257).  Returns OC_OK or
+ * OC_SINGULAR. */
+int oc_triu_solve(int k, const double *h, int ld, double *s);
+
+/* ---- Solvers ---------------------------------------------------------------- */
+/* All solvers take x as the initial guess and return the solution in x.
+ * Return OC_OK on convergence, else OC_ITERATIONS / OC_DIVERGENCE /
+ * OC_BREAKDOWN / OC_SINGULAR; m->iter and m->residual hold the monitor state
+ * at exit, as IterativeSolverNotConvergedException reports them. */
+int oc_cg(int n, const int32_t *rp, const int32_t *ci, const double *a,
+          const oc_prec *M, const double *b, double *x, oc_monitor *m);
+int oc_bicgstab(int n, const int32_t *rp, const int32_t *ci, const double *a,
+                const oc_prec *M, const double *b, double *x, oc_monitor *m);
+int oc_gmres(int n, const int32_t *rp, const int32_t *ci, const double *a,
+             const oc_prec *M, int restart, const double *b, double *x,
+             oc_monitor *m);
+int oc_ir(int n, const int32_t *rp, const int32_t *ci, const double *a,
+          const oc_prec *M, const double *b, double *x, oc_monitor *m);
+
+/* ---- timing helper for the CPU baseline ------------------------------------ */
+double oc_now(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

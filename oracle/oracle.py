@@ -1,0 +1,221 @@
+"""ctypes view of the CPU restatement of MTJ's loops (build/libmtjoracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
+the checker, and by bench.py's CPU-baseline leg.  The product never loads it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "build", "libmtjoracle.so")
+
+OK, ITERATIONS, DIVERGENCE, BREAKDOWN = 0, 1, 2, 3
+ZERO_DIAGONAL, MISSING_DIAGONAL, ZERO_PIVOT, SINGULAR, BAD_ARG = 10, 11, 12, 13, 20
+PREC_IDENTITY, PREC_DIAGONAL, PREC_ILU = 0, 1, 2
+
+
+class Monitor(C.Structure):
+    _fields_ = [("iter", C.c_int), ("max_iter", C.c_int), ("rtol", C.c_double),
+                ("atol", C.c_double), ("dtol", C.c_double), ("init_r", C.c_double),
+                ("residual", C.c_double)]
+
+
+class Prec(C.Structure):
+    _fields_ = [("kind", C.c_int), ("n", C.c_int), ("invdiag", C.c_void_p), ("rp", C.c_void_p),
+                ("ci", C.c_void_p), ("lu", C.c_void_p), ("diagind", C.c_void_p),
+                ("y", C.c_void_p)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        lib = C.CDLL(LIB)
+        P = C.c_void_p
+        D = C.c_double
+        lib.oc_csr_get.restype = D
+        for name in ("oc_dv_dot",):
+            getattr(lib, name).restype = D
+        for name in ("oc_dv_norm1", "oc_dv_norm2", "oc_dv_norm2_robust", "oc_dv_norm_inf", "oc_now"):
+            getattr(lib, name).restype = D
+        lib.oc_csr_mult.argtypes = [C.c_int, P, P, P, P, P]
+        lib.oc_csr_mult_add.argtypes = [C.c_int, P, P, P, D, P, P]
+        lib.oc_csr_trans_mult.argtypes = [C.c_int, C.c_int, P, P, P, P, P]
+        lib.oc_csr_trans_mult_add.argtypes = [C.c_int, C.c_int, P, P, P, D, P, P]
+        lib.oc_csr_get.argtypes = [P, P, P, C.c_int, C.c_int]
+        lib.oc_dv_scale.argtypes = [C.c_int, D, P]
+        lib.oc_dv_set_scaled.argtypes = [C.c_int, D, P, P]
+        lib.oc_dv_add_scaled.argtypes = [C.c_int, D, P, P]
+        lib.oc_dv_add.argtypes = [C.c_int, P, P]
+        lib.oc_dv_dot.argtypes = [C.c_int, P, P]
+        for name in ("oc_dv_norm1", "oc_dv_norm2", "oc_dv_norm2_robust", "oc_dv_norm_inf"):
+            getattr(lib, name).argtypes = [C.c_int, P]
+        lib.oc_monitor_init.argtypes = [C.POINTER(Monitor), C.c_int, D, D, D]
+        lib.oc_monitor_converged.argtypes = [C.POINTER(Monitor), D]
+        lib.oc_prec_identity.argtypes = [C.POINTER(Prec), C.c_int]
+        lib.oc_prec_diagonal.argtypes = [C.POINTER(Prec), C.c_int, P, P, P, C.POINTER(C.c_int)]
+        lib.oc_prec_ilu.argtypes = [C.POINTER(Prec), C.c_int, P, P, P, C.POINTER(C.c_int)]
+        lib.oc_prec_apply.argtypes = [C.POINTER(Prec), P, P]
+        lib.oc_prec_trans_apply.argtypes = [C.POINTER(Prec), P, P]
+        lib.oc_prec_free.argtypes = [C.POINTER(Prec)]
+        lib.oc_givens.argtypes = [D, D, C.POINTER(D), C.POINTER(D)]
+        lib.oc_triu_solve.argtypes = [C.c_int, P, C.c_int, P]
+        for name in ("oc_unit_lower_solve", "oc_unit_lower_trans_solve", "oc_upper_solve",
+                     "oc_upper_trans_solve"):
+            getattr(lib, name).argtypes = [C.c_int, P, P, P, P, P, P]
+        sol = [C.c_int, P, P, P, C.POINTER(Prec), P, P, C.POINTER(Monitor)]
+        lib.oc_cg.argtypes = sol
+        lib.oc_bicgstab.argtypes = sol
+        lib.oc_ir.argtypes = sol
+        lib.oc_gmres.argtypes = [C.c_int, P, P, P, C.POINTER(Prec), C.c_int, P, P,
+                                 C.POINTER(Monitor)]
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data if a is not None and a.size else None
+
+
+class Csr:
+    """Host CSR triple kept alive for the oracle calls."""
+
+    def __init__(self, n_rows, n_cols, rp, ci, va):
+        self.nr, self.nc = int(n_rows), int(n_cols)
+        self.rp = np.ascontiguousarray(rp, np.int32)
+        self.ci = np.ascontiguousarray(ci, np.int32)
+        self.va = np.ascontiguousarray(va, np.float64)
+
+    def mult(self, x):
+        y = np.empty(self.nr)
+        load().oc_csr_mult(self.nr, _p(self.rp), _p(self.ci), _p(self.va), _p(np.ascontiguousarray(x, float)), _p(y))
+        return y
+
+    def mult_add(self, alpha, x, y):
+        y = np.array(y, dtype=float)
+        load().oc_csr_mult_add(self.nr, _p(self.rp), _p(self.ci), _p(self.va), alpha,
+                               _p(np.ascontiguousarray(x, float)), _p(y))
+        return y
+
+    def trans_mult(self, x):
+        y = np.empty(self.nc)
+        load().oc_csr_trans_mult(self.nr, self.nc, _p(self.rp), _p(self.ci), _p(self.va),
+                                 _p(np.ascontiguousarray(x, float)), _p(y))
+        return y
+
+    def trans_mult_add(self, alpha, x, y):
+        y = np.array(y, dtype=float)
+        load().oc_csr_trans_mult_add(self.nr, self.nc, _p(self.rp), _p(self.ci), _p(self.va), alpha,
+                                     _p(np.ascontiguousarray(x, float)), _p(y))
+        return y
+
+    def get(self, i, j):
+        return load().oc_csr_get(_p(self.rp), _p(self.ci), _p(self.va), i, j)
+
+
+class Precond:
+    """Identity / DiagonalPreconditioner / ILU restatement."""
+
+    def __init__(self, kind, A=None, n=None):
+        self.s = Prec()
+        self.bad = C.c_int(-1)
+        lib = load()
+        if kind == PREC_IDENTITY:
+            lib.oc_prec_identity(C.byref(self.s), n if n is not None else A.nr)
+            self.code = OK
+        elif kind == PREC_DIAGONAL:
+            self.code = lib.oc_prec_diagonal(C.byref(self.s), A.nr, _p(A.rp), _p(A.ci), _p(A.va),
+                                             C.byref(self.bad))
+        else:
+            self.code = lib.oc_prec_ilu(C.byref(self.s), A.nr, _p(A.rp), _p(A.ci), _p(A.va),
+                                        C.byref(self.bad))
+        self.A = A
+
+    def apply(self, b):
+        x = np.empty(self.s.n)
+        load().oc_prec_apply(C.byref(self.s), _p(np.ascontiguousarray(b, float)), _p(x))
+        return x
+
+    def trans_apply(self, b):
+        x = np.empty(self.s.n)
+        load().oc_prec_trans_apply(C.byref(self.s), _p(np.ascontiguousarray(b, float)), _p(x))
+        return x
+
+    def ilu_values(self):
+        nnz = int(self.A.rp[-1])
+        buf = (C.c_double * nnz).from_address(self.s.lu)
+        return np.frombuffer(buf, dtype=np.float64).copy()
+
+    def ilu_diagind(self):
+        buf = (C.c_int32 * self.s.n).from_address(self.s.diagind)
+        return np.frombuffer(buf, dtype=np.int32).copy()
+
+    def __del__(self):
+        try:
+            load().oc_prec_free(C.byref(self.s))
+        except Exception:
+            pass
+
+
+def monitor(max_iter=100000, rtol=1e-5, atol=1e-50, dtol=1e5):
+    m = Monitor()
+    load().oc_monitor_init(C.byref(m), max_iter, rtol, atol, dtol)
+    return m
+
+
+def solve(method, A, b, x0, prec=None, mon=None, restart=30):
+    """Runs one restated solver; returns (code, x, iterations, residual)."""
+    lib = load()
+    mon = mon or monitor()
+    prec = prec or Precond(PREC_IDENTITY, n=A.nr)
+    x = np.array(x0, dtype=float)
+    b = np.ascontiguousarray(b, float)
+    args = (A.nr, _p(A.rp), _p(A.ci), _p(A.va), C.byref(prec.s))
+    if method == "cg":
+        code = lib.oc_cg(*args, _p(b), _p(x), C.byref(mon))
+    elif method == "bicgstab":
+        code = lib.oc_bicgstab(*args, _p(b), _p(x), C.byref(mon))
+    elif method == "ir":
+        code = lib.oc_ir(*args, _p(b), _p(x), C.byref(mon))
+    elif method == "gmres":
+        code = lib.oc_gmres(*args, restart, _p(b), _p(x), C.byref(mon))
+    else:
+        raise ValueError(method)
+    return code, x, mon.iter, mon.residual
+
+
+def dv_dot(x, y):
+    return load().oc_dv_dot(len(x), _p(np.ascontiguousarray(x, float)), _p(np.ascontiguousarray(y, float)))
+
+
+def dv_norm(x, kind):
+    lib = load()
+    x = np.ascontiguousarray(x, float)
+    fn = {"one": lib.oc_dv_norm1, "two": lib.oc_dv_norm2, "robust": lib.oc_dv_norm2_robust,
+          "inf": lib.oc_dv_norm_inf}[kind]
+    return fn(len(x), _p(x))
+
+
+def givens(x, y):
+    c, s = C.c_double(), C.c_double()
+    load().oc_givens(x, y, C.byref(c), C.byref(s))
+    return c.value, s.value
+
+
+def triu_solve(H, k, s):
+    """In-place back-substitution of the leading k x k block of column-major H."""
+    H = np.asfortranarray(H, dtype=float)
+    s = np.array(s, dtype=float)
+    code = load().oc_triu_solve(k, H.ctypes.data, H.shape[0], s.ctypes.data)
+    return code, s

@@ -1,0 +1,118 @@
+"""GPU parity of DenseVector operations (M/DenseVector.java), the
+DiagonalPreconditioner and ILU(0) (S/DiagonalPreconditioner.java, S/ILU.java)
+against the oracle.  Elementwise operations and the level-scheduled ILU
+factor and triangular solves are bitwise; reductions are within 1e-14
+relative (summation order only)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import mtjgpu as M
+import oracle
+from mtjgpu import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def test_vector_ops(ctx):
+    n = 100003
+    a, b = P.uniform(n, 1), P.uniform(n, 2)
+    x, y = M.DenseVector(a, ctx), M.DenseVector(b, ctx)
+    np.testing.assert_array_equal(x.copy().scale(0.3).getData(), a * 0.3)
+    np.testing.assert_array_equal(x.copy().add(-1.7, y).getData(), a + (-1.7) * b)
+    np.testing.assert_array_equal(x.copy().add(y).getData(), a + b)
+    np.testing.assert_array_equal(x.copy().set(2.0, y).getData(), 2.0 * b)
+    # alpha == 0 shortcuts: set zeroes (NaN not copied), add is a no-op
+    yn = b.copy()
+    yn[7] = np.nan
+    w = M.DenseVector(yn, ctx)
+    np.testing.assert_array_equal(x.copy().set(0.0, w).getData(), np.zeros(n))
+    np.testing.assert_array_equal(x.copy().add(0.0, w).getData(), a)
+    assert x.dot(y) == pytest.approx(oracle.dv_dot(a, b), rel=1e-13)
+    assert x.norm(M.Norm.Two) == pytest.approx(oracle.dv_norm(a, "two"), rel=1e-14)
+    assert x.norm(M.Norm.One) == pytest.approx(oracle.dv_norm(a, "one"), rel=1e-14)
+    assert x.norm(M.Norm.Infinity) == oracle.dv_norm(a, "inf")
+    assert x.norm(M.Norm.TwoRobust) == pytest.approx(oracle.dv_norm(a, "robust"), rel=1e-13)
+    assert np.isnan(w.norm(M.Norm.Infinity))
+    with pytest.raises(M.IndexOutOfBoundsException):
+        x.add(M.DenseVector(5, ctx))
+
+
+def test_jacobi_bitwise_and_errors(ctx):
+    sysm = P.varcoef_poisson(3, 20, seed=1)
+    n, _, rp, ci, va = sysm
+    A = M.CompRowMatrix(*sysm, ctx=ctx)
+    Mg = M.DiagonalPreconditioner(n, ctx)
+    Mg.setMatrix(A)
+    Mo = oracle.Precond(oracle.PREC_DIAGONAL, oracle.Csr(*sysm))
+    bh = P.uniform(n, 3)
+    got = Mg.apply(M.DenseVector(bh, ctx), M.DenseVector(n, ctx)).getData()
+    np.testing.assert_array_equal(got, Mo.apply(bh))
+    bad = sp.csr_matrix(np.array([[1.0, 2.0], [3.0, 0.0]]))
+    bad.eliminate_zeros()
+    B = M.CompRowMatrix(bad, ctx=ctx)
+    Mb = M.DiagonalPreconditioner(2, ctx)
+    with pytest.raises(M.ZeroDiagonalException) as e:
+        Mb.setMatrix(B)
+    assert e.value.row == 1
+
+
+@pytest.mark.parametrize("name", ["convdiff_16", "p2d_64", "random_spd"])
+def test_ilu_apply_bitwise(ctx, name):
+    if name == "random_spd":
+        rng = np.random.default_rng(5)
+        n = 400
+        d = sp.random(n, n, density=0.02, random_state=5, format="csr")
+        d = d + d.T + sp.diags(np.full(n, 10.0))
+        d = d.tocsr()
+        d.sort_indices()
+        sysm = (n, n, d.indptr, d.indices, d.data)
+    else:
+        sysm = P.convdiff3d(16, seed=2) if name == "convdiff_16" else P.poisson2d(64)
+    n = sysm[0]
+    A = M.CompRowMatrix(*sysm, ctx=ctx)
+    Mg = M.ILU(A)
+    Mg.setMatrix(A)
+    Mo = oracle.Precond(oracle.PREC_ILU, oracle.Csr(*sysm))
+    assert Mo.code == oracle.OK
+    bh = P.uniform(n, 4)
+    b = M.DenseVector(bh, ctx)
+    np.testing.assert_array_equal(Mg.apply(b, M.DenseVector(n, ctx)).getData(), Mo.apply(bh))
+    np.testing.assert_array_equal(Mg.transApply(b, M.DenseVector(n, ctx)).getData(), Mo.trans_apply(bh))
+    # setMatrix again with new values on the same pattern refactors
+    sysm2 = (sysm[0], sysm[1], sysm[2], sysm[3], np.asarray(sysm[4]) * 2.0)
+    A2 = M.CompRowMatrix(*sysm2, ctx=ctx)
+    Mg.setMatrix(A2)
+    Mo2 = oracle.Precond(oracle.PREC_ILU, oracle.Csr(*sysm2))
+    np.testing.assert_array_equal(Mg.apply(b, M.DenseVector(n, ctx)).getData(), Mo2.apply(bh))
+
+
+def test_ilu_errors(ctx):
+    m = sp.csr_matrix(np.array([[1.0, 1.0], [1.0, 0.0]]))
+    m.eliminate_zeros()
+    A = M.CompRowMatrix(m, ctx=ctx)
+    with pytest.raises(M.MissingDiagonalException) as e:
+        M.ILU(A).setMatrix(A)
+    assert e.value.row == 1
+    A = M.CompRowMatrix(2, 2, [0, 2, 4], [0, 1, 0, 1], [0.0, 1.0, 1.0, 1.0], ctx=ctx)
+    with pytest.raises(M.ZeroPivotException) as e:
+        M.ILU(A).setMatrix(A)
+    assert e.value.row == 1
+
+
+@pytest.mark.parametrize("name", ["ilu_tridiagonal", "ilu_pentadiagonal"])
+def test_ilu_reference_known_answers(ctx, name):
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "mtj_fixtures.json")))[name]
+    for n in (1, 2, 3, 6, 9):
+        d = np.diag(np.full(n, fx["diagonal"]))
+        for k, v in enumerate(fx["offdiagonal"], start=1):
+            if k < n:
+                d += np.diag(np.full(n - k, v), k) + np.diag(np.full(n - k, v), -k)
+        A = M.CompRowMatrix(sp.csr_matrix(d), ctx=ctx)
+        ilu = M.ILU(A)
+        ilu.setMatrix(A)
+        b = d @ np.full(n, fx["x"])
+        x = ilu.apply(M.DenseVector(b, ctx), M.DenseVector(n, ctx)).getData()
+        assert np.linalg.norm(b - d @ x) <= fx["residual_tol"]
