@@ -172,7 +172,9 @@ void csr_ensure_fp32(mtjg_csr A);
 
 // ---- preconditioners (precond.hip) ----
 enum PrecKind : int { PREC_IDENTITY = 0, PREC_DIAGONAL = 1, PREC_ILU = 2 };
-void prec_apply_dev(mtjg_prec M, const double *b, double *x, hipStream_t s, KState *st);
+// Applies M (identity when M is null) to n entries; kernels skip work once st's
+// status leaves running (st may be null).
+void prec_apply_dev(mtjg_prec M, int64_t n, const double *b, double *x, hipStream_t s, KState *st);
 void prec_trans_apply_dev(mtjg_prec M, const double *b, double *x, hipStream_t s);
 int prec_kind(mtjg_prec M);
 const double *prec_invdiag(mtjg_prec M);

@@ -386,9 +386,8 @@ static void ilu_build_transpose(mtjg_prec M) {
   M->trans_ready = true;
 }
 
-void prec_apply_dev(mtjg_prec M, const double *b, double *x, hipStream_t s, KState *st) {
-  const int32_t n = M->n;
-  if (M->kind == PREC_IDENTITY) {
+void prec_apply_dev(mtjg_prec M, int64_t n, const double *b, double *x, hipStream_t s, KState *st) {
+  if (!M || M->kind == PREC_IDENTITY) {
     // IdentityPreconditioner: x.set(b) (S/AbstractIterativeSolver.java:84-86)
     if (b != x && n > 0) k_copy_if_running<<<grid_for(n), BS, 0, s>>>(n, b, x, st);
   } else if (M->kind == PREC_DIAGONAL) {
@@ -404,7 +403,7 @@ void prec_apply_dev(mtjg_prec M, const double *b, double *x, hipStream_t s, KSta
 
 void prec_trans_apply_dev(mtjg_prec M, const double *b, double *x, hipStream_t s) {
   if (M->kind != PREC_ILU) {
-    prec_apply_dev(M, b, x, s, nullptr);
+    prec_apply_dev(M, M->n, b, x, s, nullptr);
     return;
   }
   if (!M->factored) fail(MTJG_E_INVALID_ARGUMENT, "ILU.setMatrix must precede transApply");
@@ -490,7 +489,7 @@ static void check_apply(mtjg_prec M, mtjg_vec b, mtjg_vec x) {
 extern "C" int mtjg_prec_apply(mtjg_prec M, mtjg_vec b, mtjg_vec x) {
   MTJG_API_BEGIN
   check_apply(M, b, x);
-  prec_apply_dev(M, (const double *)b->ptr, (double *)x->ptr, M->ctx->stream, nullptr);
+  prec_apply_dev(M, M->n, (const double *)b->ptr, (double *)x->ptr, M->ctx->stream, nullptr);
   MTJG_API_END
 }
 

@@ -582,7 +582,7 @@ void solve_cg(mtjg_solver S, mtjg_csr A, double *b, double *x) {
     } else {
       if (update) k_cg_resid<double, PK_EXTERNAL, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->partials);
       else k_cg_resid<double, PK_EXTERNAL, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->partials);
-      prec_apply_dev(S->M, r, z, s, st);
+      prec_apply_dev(S->M, n, r, z, s, st);
       k_cg_rho<double><<<g, BS, 0, s>>>(n, r, z, st, S->partials);
     }
   };
@@ -646,10 +646,10 @@ void solve_bicgstab(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   ta.ticket = &st->ticket[KSLOT + 3];
   run_chunks(S, st, pk == PK_EXTERNAL ? 1 : chunk_for(A), [&]() {
     k_bicg_p<<<g, BS, 0, s>>>(n, p, r, v, st);
-    if (pk != PK_IDENTITY) prec_apply_dev(S->M, p, phat, s, st);
+    if (pk != PK_IDENTITY) prec_apply_dev(S->M, n, p, phat, s, st);
     spmv_f64(A, phat, v, va, s);
     k_bicg_s<<<g, BS, 0, s>>>(n, x, phat, r, v, sv, st, S->partials);
-    if (pk != PK_IDENTITY) prec_apply_dev(S->M, sv, shat, s, st);
+    if (pk != PK_IDENTITY) prec_apply_dev(S->M, n, sv, shat, s, st);
     spmv_f64(A, shat, t, ta, s);
     k_bicg_omega<<<1, 1, 0, s>>>(st);
     k_bicg_top<true><<<g, BS, 0, s>>>(n, x, shat, r, sv, t, rtilde, st, S->partials);
@@ -699,7 +699,7 @@ void solve_gmres(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   auto residual = [&]() {
     // u = b - A x; r = M^-1 u (S/GMRES.java:131-132, :170-171)
     spmv_f64(A, x, u, ra, s);
-    prec_apply_dev(S->M, u, r, s, nullptr);
+    prec_apply_dev(S->M, n, u, r, s, nullptr);
   };
   residual();
   k_gmres_start<<<g, BS, 0, s>>>(n, r, st, gd, 1, S->partials);
@@ -711,7 +711,7 @@ void solve_gmres(mtjg_solver S, mtjg_csr A, double *b, double *x) {
     k_gmres_v0<<<g, BS, 0, s>>>(n, r, Vk(0), st);
     for (int i = 0; i < S->restart && !k.cycle_done; ++i) {
       spmv_f64(A, Vk(i), u, ma, s);
-      prec_apply_dev(S->M, u, w, s, nullptr);
+      prec_apply_dev(S->M, n, u, w, s, nullptr);
       for (int kk = 0; kk <= i; ++kk)
         k_gmres_mgs<<<g, BS, 0, s>>>(n, w, kk > 0 ? Vk(kk - 1) : nullptr, Vk(kk), kk, i, st, gd, S->partials);
       k_gmres_arnoldi_end<<<g, BS, 0, s>>>(n, w, Vk(i), i, st, gd, S->partials);
@@ -747,7 +747,7 @@ void solve_ir(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   k_norm_check<<<g, BS, 0, s>>>(n, r, st, 0, S->partials);
   HIPCHK(hipGetLastError());
   run_chunks(S, st, 1, [&]() {
-    prec_apply_dev(S->M, r, z, s, st);
+    prec_apply_dev(S->M, n, r, z, s, st);
     k_add_if_running<<<g, BS, 0, s>>>(n, x, z, st);
     spmv_f64(A, x, r, ra, s);
     k_norm_check<<<g, BS, 0, s>>>(n, r, st, 1, S->partials);

@@ -13,6 +13,7 @@ M/ = src/main/java/no/uib/cipr/matrix/.
 """
 import ctypes as C
 import enum
+import sys
 
 import numpy as np
 
@@ -125,6 +126,18 @@ def _raise(code, row=None):
     raise MtjgError(code, msg)
 
 
+def _finalize(obj):
+    """Releases a handle when its Python object dies.  At interpreter exit the
+    process is ending and the driver reclaims device memory, and handles may
+    be finalized in any order, so nothing is released then."""
+    if sys.is_finalizing():
+        return
+    try:
+        obj.close()
+    except Exception:
+        pass
+
+
 def _ck(code):
     if code != L.MTJG_OK:
         _raise(code)
@@ -156,10 +169,7 @@ class Context:
             self.handle = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 _default = None
@@ -272,10 +282,7 @@ class DenseVector:
             self.handle = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 # ---------------------------------------------------------------------------
@@ -394,10 +401,7 @@ class CompRowMatrix:
             self.handle = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 # ---------------------------------------------------------------------------
@@ -424,10 +428,7 @@ class _Preconditioner:
             self.handle = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class DiagonalPreconditioner(_Preconditioner):
@@ -544,10 +545,7 @@ class _Solver:
             self.handle = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class CG(_Solver):
