@@ -60,6 +60,19 @@ def workload(name):
     raise SystemExit("unknown workload " + name)
 
 
+def measured_traffic(workload):
+    """HBM bytes per SpMV launch from the committed rocprofv3 PMC summary of
+    this workload (FETCH_SIZE and WRITE_SIZE, each scaled by the same
+    counter's reading for a 1 GiB copy), or None if there is none."""
+    import re
+    path = os.path.join(ROOT, "profiles", "spmv_%s_hbm_traffic.md" % workload)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        m = re.search(r"traffic_bytes_per_launch:\s*([0-9]+)", f.read())
+    return (int(m.group(1)), os.path.relpath(path, ROOT)) if m else None
+
+
 def cpu_baseline(sysm, seconds):
     """Oracle restatement of MTJ's loop (one core) on the same matrix."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -84,6 +97,103 @@ def cpu_baseline(sysm, seconds):
                       % (calls, el, per)}
 
 
+def copy_calibration(steps):
+    """Device-to-device copies of 1 GiB: the known-size reference whose HBM
+    counters (FETCH_SIZE, WRITE_SIZE) scale the SpMV's counter readings."""
+    import mtjgpu as M
+    ctx = M.Context(0)
+    n = 1 << 27  # 2^27 doubles = 1 GiB
+    a = M.DenseVector(n, ctx)
+    b = M.DenseVector(n, ctx)
+    timer = HipTimer(ctx.stream)
+    hip = timer.hip
+    hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+    timer.start()
+    for _ in range(steps):
+        assert hip.hipMemcpyAsync(b.device_ptr(), a.device_ptr(), n * 8, 3, timer.stream) == 0
+    ms = timer.stop_ms() / steps
+    print(json.dumps({"mode": "copy", "bytes_read": n * 8, "bytes_written": n * 8,
+                      "ms_per_copy": round(ms, 5), "GBps_rw": round(2 * n * 8 / (ms * 1e-3) / 1e9, 1)}))
+    return 0
+
+
+SOLVE_WORKLOADS = {
+    # name: (description, generator, solver, preconditioner, rtol, oracle-feasible)
+    "cg_poisson2d_256": ("2D 5-point Poisson 256^2, Jacobi-CG to rtol 1e-8", "p2d", "cg", "diag", 1e-8, True),
+    "cg_poisson3d_256": ("3D 7-point Poisson 256^3, Jacobi-CG to rtol 1e-8", "p3d", "cg", "diag", 1e-8, False),
+    "bicgstab_ilu_convdiff_128": ("3D upwind convection-diffusion 128^3 (Pe 10), BiCGstab + ILU(0) to rtol 1e-8",
+                                  "cd", "bicgstab", "ilu", 1e-8, False),
+    "gmres_ilu_convdiff_128": ("3D upwind convection-diffusion 128^3 (Pe 10), GMRES(30) + ILU(0) to rtol 1e-8",
+                               "cd", "gmres", "ilu", 1e-8, False),
+    "mixed_varcoef_192": ("3D variable-kappa Poisson 192^3, FP32 inner Jacobi-CG (1e-4) under FP64 refinement to 1e-10",
+                          "vk", "mixed", "diag", 1e-10, False),
+    "cg_varcoef_192": ("3D variable-kappa Poisson 192^3, FP64 Jacobi-CG to 1e-10 (oracle of the mixed solve)",
+                       "vk", "cg", "diag", 1e-10, False),
+}
+
+
+def solve_mode(name, repeats):
+    """Times one solve of a named configuration: x_true ~ U[-1,1) (seed 1),
+    b = A x_true, x0 = 0.  Reports iterations, device time, true relative
+    residual and error, and (where it runs in seconds) the oracle's solve."""
+    import mtjgpu as M
+    from mtjgpu import problems as P
+    desc, gen, method, prec, rtol, with_oracle = SOLVE_WORKLOADS[name]
+    sysm = {"p2d": lambda: P.poisson2d(256), "p3d": lambda: P.poisson3d(256),
+            "cd": lambda: P.convdiff3d(128, peclet=10.0, seed=1),
+            "vk": lambda: P.varcoef_poisson(3, 192, seed=1)}[gen]()
+    n, m, rp, ci, va = sysm
+    import scipy.sparse as sp
+    Ah = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    x_true = P.uniform(n, 1)
+    bh = Ah @ x_true
+    ctx = M.Context(0)
+    A = M.CompRowMatrix(n, m, rp, ci, va, ctx=ctx)
+    b = M.DenseVector(bh, ctx)
+    x = M.DenseVector(n, ctx)
+    cls = {"cg": M.CG, "bicgstab": M.BiCGstab, "gmres": M.GMRES, "mixed": M.MixedCG}[method]
+    S = cls(x)
+    if prec == "diag":
+        Mp = M.DiagonalPreconditioner(n, ctx)
+    else:
+        Mp = M.ILU(A)
+    t0 = time.perf_counter()
+    Mp.setMatrix(A)
+    setup = time.perf_counter() - t0
+    if method != "mixed":
+        S.setPreconditioner(Mp)
+    S.setIterationMonitor(M.DefaultIterationMonitor(100000 if method != "mixed" else 100, rtol))
+    times = []
+    for _ in range(repeats):
+        x.zero()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        S.solve(A, b, x)
+        ctx.synchronize()
+        times.append(time.perf_counter() - t0)
+    xs = x.getData()
+    out = {"mode": "solve", "workload": name, "config": desc, "n": n, "nnz": len(va),
+           "method": method, "preconditioner": prec, "rtol": rtol,
+           "iterations": S.getIterationMonitor().iterations(),
+           "gpu_s": round(min(times), 5), "gpu_s_all": [round(t, 5) for t in times],
+           "setup_s": round(setup, 4),
+           "true_rel_residual": float(np.linalg.norm(bh - Ah @ xs) / np.linalg.norm(bh)),
+           "rel_error_vs_x_true": float(np.linalg.norm(xs - x_true) / np.linalg.norm(x_true))}
+    if method == "mixed":
+        out["inner_iterations"] = S.inner_iterations()
+    if with_oracle:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        O = oracle.Csr(n, m, rp, ci, va)
+        op = oracle.Precond(oracle.PREC_DIAGONAL if prec == "diag" else oracle.PREC_ILU, O)
+        t0 = time.perf_counter()
+        code, xo, it_o, _ = oracle.solve(method, O, bh, np.zeros(n), op, oracle.monitor(100000, rtol))
+        out["oracle"] = {"kind": "port", "cores": 1, "iterations": it_o, "cpu_s": round(time.perf_counter() - t0, 4),
+                         "rel_diff_x": float(np.linalg.norm(xs - xo) / np.linalg.norm(xo))}
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,11 +202,18 @@ def main():
     ap.add_argument("--workload", default="poisson3d_256")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", choices=["spmv", "copy", "solve"], default="spmv")
+    ap.add_argument("--solve", default="cg_poisson2d_256", choices=sorted(SOLVE_WORKLOADS))
+    ap.add_argument("--repeats", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         return 2
+    if args.mode == "copy":
+        return copy_calibration(args.steps)
+    if args.mode == "solve":
+        return solve_mode(args.solve, args.repeats)
 
     import mtjgpu as M
     from mtjgpu import problems as P
@@ -142,6 +259,10 @@ def main():
                      "kernel": "k_spmv<double,double,SPMV_MULT>",
                      "kernel_ms": round(per_ms, 5)},
     }
+    traffic = measured_traffic(args.workload)
+    if traffic:
+        out["roofline"]["traffic"] = traffic[0]
+        out["roofline"]["traffic_source"] = traffic[1]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sysm, args.cpu_seconds)
     print(json.dumps(out))

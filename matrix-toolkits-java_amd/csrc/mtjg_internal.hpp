@@ -114,9 +114,12 @@ struct mtjg_csr_s {
   int32_t *ci = nullptr;
   double *val = nullptr;
   float *val32 = nullptr;  // FP32 copy of val (mixed precision), lazily built
-  // SpMV row-block schedule: row-block b covers rows [rb[b], rb[b+1]).
+  // SpMV row-block schedule: row-block b covers rows [rb[b].x, rb[b+1].x)
+  // and stored entries [rb[b].y, rb[b+1].y); `slabs` orders the blocks over
+  // the XCDs.
   int32_t nrb = 0;
-  int32_t *rb = nullptr;
+  int32_t slabs = 1;
+  int2 *rb = nullptr;
   // Host copy of the row pointers (schedules, level analysis, validation).
   std::vector<int32_t> h_rp;
   // Transpose (CSR of A'), built on first transposed product.

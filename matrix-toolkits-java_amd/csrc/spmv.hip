@@ -27,20 +27,21 @@ namespace mtjg {
 
 constexpr int SPMV_CAP = 2048;           // products staged per row-block
 constexpr int SPMV_PER = SPMV_CAP / BS;  // entries per thread per row-block
-constexpr int SPMV_MAX_ROWS = 1024;      // rows per row-block
+constexpr int SPMV_MAX_ROWS = 512;       // rows per row-block (2 per thread)
 
 // Row-block schedule: greedy cut so every block holds <= SPMV_CAP entries,
-// except single rows longer than that.
-static std::vector<int32_t> build_row_blocks(const std::vector<int32_t> &rp, int32_t nr) {
-  std::vector<int32_t> rb;
+// except single rows longer than that.  Entry b is (first row, first stored
+// entry) of row-block b; entry nrb closes the last block.
+static std::vector<int2> build_row_blocks(const std::vector<int32_t> &rp, int32_t nr) {
+  std::vector<int2> rb;
   rb.reserve((size_t)nr / 64 + 2);
-  rb.push_back(0);
+  rb.push_back(make_int2(0, 0));
   int32_t r = 0;
   while (r < nr) {
     int32_t len = rp[r + 1] - rp[r];
     if (len > SPMV_CAP) {
       ++r;
-      rb.push_back(r);
+      rb.push_back(make_int2(r, rp[r]));
       continue;
     }
     int32_t acc = 0, rows = 0;
@@ -51,9 +52,21 @@ static std::vector<int32_t> build_row_blocks(const std::vector<int32_t> &rp, int
       ++r;
       ++rows;
     }
-    rb.push_back(r);
+    rb.push_back(make_int2(r, rp[r]));
   }
   return rb;
+}
+
+// Number of XCD slabs used to order row-blocks (see ticket_to_block).  Plain
+// row order measured fastest on the 256^3 Poisson product (5.25 TB/s against
+// 4.99 TB/s with 16 slabs); MTJG_SPMV_SLABS overrides it for experiments.
+static int choose_slabs(int32_t nrb) {
+  (void)nrb;
+  if (const char *e = getenv("MTJG_SPMV_SLABS")) {
+    int v = atoi(e);
+    if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16 || v == 32) return v;
+  }
+  return 1;
 }
 
 __device__ __forceinline__ int pow2_floor(int v) { return 1 << (31 - __clz(v)); }
@@ -96,21 +109,33 @@ __device__ __forceinline__ void spmv_finish(KState *st, int fin, double s0, doub
   }
 }
 
+// Maps a launch-order ticket to a row-block.  Workgroups are dispatched
+// round-robin over the XCDs, so with `slabs` = 8 the tickets of XCD k
+// (t % 8 == k) sweep the k-th contiguous eighth of the row-blocks in order:
+// the x entries a stencil row gathers from neighbouring planes were fetched
+// into that XCD's own L2 by the rows just before it.  slabs = 1 keeps plain
+// row order.  Returns -1 for tickets past the end of a slab.
+__device__ __forceinline__ int ticket_to_block(int t, int nrb, int slabs, int per) {
+  const int k = t % slabs, l = t / slabs;
+  const int b = k * per + l;
+  return (l < per && b < nrb) ? b : -1;
+}
+
+// One row-block of y = A x (and the mode's epilogue) by one workgroup.
 template <typename VT, typename XT, int MODE>
-__global__ __launch_bounds__(BS) void k_spmv(int nrb, const int32_t *__restrict__ rb,
-                                             const int32_t *__restrict__ rp,
-                                             const int32_t *__restrict__ ci,
-                                             const VT *__restrict__ val, const XT *__restrict__ x,
-                                             XT *y, const XT *y0, double alpha, double inv_alpha,
-                                             const XT *w, KState *st, int fin, double *partials,
-                                             uint32_t *ticket) {
-  if (MODE == SPMV_DOT && st->status != ST_RUNNING) return;
-  __shared__ XT prod[SPMV_CAP];
-  double acc0 = 0.0, acc1 = 0.0;
-  for (int b = blockIdx.x; b < nrb; b += gridDim.x) {
-    const int r0 = rb[b], r1 = rb[b + 1];
-    const int j0 = rp[r0];
-    const int nnzb = rp[r1] - j0;
+__device__ __forceinline__ void spmv_rowblock(int b, XT *prod, const int2 *__restrict__ info,
+                                              const int32_t *__restrict__ rp,
+                                              const int32_t *__restrict__ ci,
+                                              const VT *__restrict__ val, const XT *__restrict__ x,
+                                              XT *y, const XT *y0, double alpha, double inv_alpha,
+                                              const XT *w, double &acc0, double &acc1) {
+  constexpr int RPER = SPMV_MAX_ROWS / BS;
+  {
+    const int2 s0 = info[b], s1 = info[b + 1];
+    const int r0 = s0.x, r1 = s1.x;
+    const int j0 = s0.y;
+    const int nnzb = s1.y - j0;
+    const int nrows = r1 - r0;
     if (nnzb > SPMV_CAP) {
       // one long row: strided partials from global memory, fixed tree
       double v[1] = {0.0};
@@ -123,9 +148,10 @@ __global__ __launch_bounds__(BS) void k_spmv(int nrb, const int32_t *__restrict_
         if (MODE == SPMV_TMULTADD) tot = spmv_init<XT, MODE>(r0, y0, inv_alpha) + tot;
         spmv_store<XT, MODE>(r0, tot, y, y0, alpha, inv_alpha, w, acc0, acc1);
       }
-      continue;
+      return;
     }
-    // 1. stage products in LDS (coalesced col/val streaming, then gathers)
+    // 1. stage products in LDS: coalesced col/val streaming, the block's row
+    //    pointers alongside, then the x gathers
     int c[SPMV_PER];
     VT a[SPMV_PER];
     XT xv[SPMV_PER];
@@ -133,8 +159,20 @@ __global__ __launch_bounds__(BS) void k_spmv(int nrb, const int32_t *__restrict_
     for (int u = 0; u < SPMV_PER; ++u) {
       const int k = threadIdx.x + u * BS;
       if (k < nnzb) {
-        c[u] = __builtin_nontemporal_load(&ci[j0 + k]);
-        a[u] = __builtin_nontemporal_load(&val[j0 + k]);
+        c[u] = ci[j0 + k];
+        a[u] = val[j0 + k];
+      }
+    }
+    const bool one_lane = nrows * 2 > BS;  // uniform over the workgroup
+    int rs[RPER], re[RPER];
+    if (one_lane) {
+#pragma unroll
+      for (int u = 0; u < RPER; ++u) {
+        const int g = threadIdx.x + u * BS;
+        if (g < nrows) {
+          rs[u] = rp[r0 + g] - j0;
+          re[u] = rp[r0 + g + 1] - j0;
+        }
       }
     }
 #pragma unroll
@@ -149,25 +187,73 @@ __global__ __launch_bounds__(BS) void k_spmv(int nrb, const int32_t *__restrict_
     }
     __syncthreads();
     // 2. per-row reduction from LDS
-    const int nrows = r1 - r0;
-    int tpr = 1;
-    if (nrows * 2 <= BS) tpr = min(64, pow2_floor(BS / max(nrows, 1)));
-    const int groups = BS / tpr;
-    const int lane = threadIdx.x & (tpr - 1);
-    for (int g = threadIdx.x / tpr; g < nrows; g += groups) {
-      const int row = r0 + g;
-      const int beg = rp[row] - j0, end = rp[row + 1] - j0;
-      XT s = (lane == 0) ? spmv_init<XT, MODE>(row, y0, inv_alpha) : XT(0);
-      for (int k = beg + lane; k < end; k += tpr) s += prod[k];
-      for (int off = tpr >> 1; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-      if (lane == 0) spmv_store<XT, MODE>(row, s, y, y0, alpha, inv_alpha, w, acc0, acc1);
+    if (one_lane) {
+      // a thread owns a row: sequential sum in storage order (MTJ's order)
+#pragma unroll
+      for (int u = 0; u < RPER; ++u) {
+        const int g = threadIdx.x + u * BS;
+        if (g < nrows) {
+          const int row = r0 + g;
+          XT s = spmv_init<XT, MODE>(row, y0, inv_alpha);
+          for (int k = rs[u]; k < re[u]; ++k) s += prod[k];
+          spmv_store<XT, MODE>(row, s, y, y0, alpha, inv_alpha, w, acc0, acc1);
+        }
+      }
+    } else {
+      const int tpr = min(64, pow2_floor(BS / max(nrows, 1)));
+      const int groups = BS / tpr;
+      const int lane = threadIdx.x & (tpr - 1);
+      for (int g = threadIdx.x / tpr; g < nrows; g += groups) {
+        const int row = r0 + g;
+        const int beg = rp[row] - j0, end = rp[row + 1] - j0;
+        XT s = (lane == 0) ? spmv_init<XT, MODE>(row, y0, inv_alpha) : XT(0);
+        for (int k = beg + lane; k < end; k += tpr) s += prod[k];
+        for (int off = tpr >> 1; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (lane == 0) spmv_store<XT, MODE>(row, s, y, y0, alpha, inv_alpha, w, acc0, acc1);
+      }
     }
+  }
+}
+
+// Plain products: one workgroup per row-block ticket.
+template <typename VT, typename XT, int MODE>
+__global__ __launch_bounds__(BS) void k_spmv(int nrb, int slabs, const int2 *__restrict__ info,
+                                             const int32_t *__restrict__ rp,
+                                             const int32_t *__restrict__ ci,
+                                             const VT *__restrict__ val, const XT *__restrict__ x,
+                                             XT *y, const XT *y0, double alpha, double inv_alpha) {
+  __shared__ XT prod[SPMV_CAP];
+  const int per = (nrb + slabs - 1) / slabs;
+  const int b = ticket_to_block(blockIdx.x, nrb, slabs, per);
+  if (b < 0) return;
+  double acc0 = 0.0, acc1 = 0.0;
+  spmv_rowblock<VT, XT, MODE>(b, prod, info, rp, ci, val, x, y, y0, alpha, inv_alpha, nullptr, acc0,
+                              acc1);
+}
+
+// Fused solver product q = A p with the grid sums w.q and q.q: a bounded grid
+// walks the row-block tickets, and the last workgroup runs the solver
+// epilogue.  Returns at once when the solve has left the running state.
+template <typename VT, typename XT>
+__global__ __launch_bounds__(BS) void k_spmv_dot(int nrb, int slabs, const int2 *__restrict__ info,
+                                                 const int32_t *__restrict__ rp,
+                                                 const int32_t *__restrict__ ci,
+                                                 const VT *__restrict__ val, const XT *__restrict__ x,
+                                                 XT *y, const XT *w, KState *st, int fin,
+                                                 double *partials, uint32_t *ticket) {
+  if (st->status != ST_RUNNING) return;
+  __shared__ XT prod[SPMV_CAP];
+  double acc0 = 0.0, acc1 = 0.0;
+  const int per = (nrb + slabs - 1) / slabs;
+  const int ntickets = per * slabs;
+  for (int t = blockIdx.x; t < ntickets; t += gridDim.x) {
+    const int b = ticket_to_block(t, nrb, slabs, per);
+    if (b < 0) continue;
+    spmv_rowblock<VT, XT, SPMV_DOT>(b, prod, info, rp, ci, val, x, y, nullptr, 1.0, 1.0, w, acc0, acc1);
     __syncthreads();
   }
-  if (MODE == SPMV_DOT) {
-    double v[2] = {acc0, acc1};
-    if (grid_sum<2>(v, partials, ticket) && threadIdx.x == 0) spmv_finish(st, fin, v[0], v[1]);
-  }
+  double v[2] = {acc0, acc1};
+  if (grid_sum<2>(v, partials, ticket) && threadIdx.x == 0) spmv_finish(st, fin, v[0], v[1]);
 }
 
 template <typename VT, typename XT>
@@ -177,29 +263,31 @@ static void spmv_launch(mtjg_csr A, const VT *val, const XT *x, XT *y, const Spm
     if (a.mode == SPMV_DOT) fail(MTJG_E_INVALID_ARGUMENT, "empty matrix in fused product");
     return;
   }
-  int grid = A->nrb;
+  // one workgroup per row-block ticket; fused reductions use at most
+  // RED_MAX_BLOCKS workgroups (a multiple of the slab count) in grid-stride
+  const int per = (A->nrb + A->slabs - 1) / A->slabs;
+  int grid = per * A->slabs;
   if (a.mode == SPMV_DOT) grid = std::min(grid, RED_MAX_BLOCKS);
-  else grid = std::min(grid, 1 << 20);
+  else grid = std::min(grid, 1 << 24);
   const XT *y0 = (const XT *)a.y0;
   const XT *w = (const XT *)a.w;
+  const int sl = A->slabs;
   switch (a.mode) {
     case SPMV_MULT:
-      k_spmv<VT, XT, SPMV_MULT><<<grid, BS, 0, s>>>(A->nrb, A->rb, A->rp, A->ci, val, x, y, y0, a.alpha,
-                                                   a.inv_alpha, w, a.st, a.fin, a.partials, a.ticket);
+      k_spmv<VT, XT, SPMV_MULT><<<grid, BS, 0, s>>>(A->nrb, sl, A->rb, A->rp, A->ci, val, x, y, y0,
+                                                   a.alpha, a.inv_alpha);
       break;
     case SPMV_MULTADD:
-      k_spmv<VT, XT, SPMV_MULTADD><<<grid, BS, 0, s>>>(A->nrb, A->rb, A->rp, A->ci, val, x, y, y0,
-                                                      a.alpha, a.inv_alpha, w, a.st, a.fin,
-                                                      a.partials, a.ticket);
+      k_spmv<VT, XT, SPMV_MULTADD><<<grid, BS, 0, s>>>(A->nrb, sl, A->rb, A->rp, A->ci, val, x, y, y0,
+                                                      a.alpha, a.inv_alpha);
       break;
     case SPMV_TMULTADD:
-      k_spmv<VT, XT, SPMV_TMULTADD><<<grid, BS, 0, s>>>(A->nrb, A->rb, A->rp, A->ci, val, x, y, y0,
-                                                       a.alpha, a.inv_alpha, w, a.st, a.fin,
-                                                       a.partials, a.ticket);
+      k_spmv<VT, XT, SPMV_TMULTADD><<<grid, BS, 0, s>>>(A->nrb, sl, A->rb, A->rp, A->ci, val, x, y,
+                                                       y0, a.alpha, a.inv_alpha);
       break;
     case SPMV_DOT:
-      k_spmv<VT, XT, SPMV_DOT><<<grid, BS, 0, s>>>(A->nrb, A->rb, A->rp, A->ci, val, x, y, y0, a.alpha,
-                                                  a.inv_alpha, w, a.st, a.fin, a.partials, a.ticket);
+      k_spmv_dot<VT, XT><<<grid, BS, 0, s>>>(A->nrb, sl, A->rb, A->rp, A->ci, val, x, y, w, a.st,
+                                             a.fin, a.partials, a.ticket);
       break;
     default:
       fail(MTJG_E_INVALID_ARGUMENT, "bad spmv mode");
@@ -265,10 +353,11 @@ static mtjg_csr_s *csr_upload(mtjg_ctx ctx, int32_t nr, int32_t nc, std::vector<
       HIPCHK(hipMemcpyAsync(A->val, val, sizeof(double) * (size_t)A->nnz, hipMemcpyHostToDevice,
                             ctx->stream));
     }
-    std::vector<int32_t> rb = build_row_blocks(A->h_rp, nr);
+    std::vector<int2> rb = build_row_blocks(A->h_rp, nr);
     A->nrb = (int32_t)rb.size() - 1;
-    HIPCHK(hipMalloc(&A->rb, sizeof(int32_t) * rb.size()));
-    HIPCHK(hipMemcpyAsync(A->rb, rb.data(), sizeof(int32_t) * rb.size(), hipMemcpyHostToDevice,
+    A->slabs = choose_slabs(A->nrb);
+    HIPCHK(hipMalloc(&A->rb, sizeof(int2) * rb.size()));
+    HIPCHK(hipMemcpyAsync(A->rb, rb.data(), sizeof(int2) * rb.size(), hipMemcpyHostToDevice,
                           ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
   } catch (...) {
