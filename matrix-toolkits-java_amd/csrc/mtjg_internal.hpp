@@ -81,6 +81,17 @@ struct RedWS {
   double *out;       // [8]
 };
 
+// Workspace of the two-level grid reduction (grid_sum_h in reduce.hpp) for
+// grids of up to `cap` workgroups in groups of RED_GROUP.
+constexpr int RED_GROUP = 256;
+struct GridRed {
+  double *partials = nullptr;   // [RED_MAX_K * cap]
+  double *gpart = nullptr;      // [RED_MAX_K * gcap]
+  uint32_t *gticket = nullptr;  // [gcap], zero between launches
+  uint32_t *fticket = nullptr;  // [1], zero between launches
+  int cap = 0, gcap = 0;
+};
+
 // Grid size for an elementwise or reducing kernel over n items.
 inline int grid_for(int64_t n, int per_thread = 4, int cap = 2048) {
   int64_t g = (n + (int64_t)BS * per_thread - 1) / ((int64_t)BS * per_thread);
@@ -164,8 +175,7 @@ struct SpmvArgs {
   const void *w = nullptr;   // SPMV_DOT: first dot operand
   KState *st = nullptr;      // SPMV_DOT: solver state (status checked on entry)
   int fin = FIN_NONE;
-  double *partials = nullptr;
-  uint32_t *ticket = nullptr;
+  GridRed red{};             // SPMV_DOT: two-level reduction workspace
 };
 
 void spmv_f64(mtjg_csr A, const double *x, double *y, const SpmvArgs &a, hipStream_t s);
@@ -179,6 +189,9 @@ enum PrecKind : int { PREC_IDENTITY = 0, PREC_DIAGONAL = 1, PREC_ILU = 2 };
 // status leaves running (st may be null).
 void prec_apply_dev(mtjg_prec M, int64_t n, const double *b, double *x, hipStream_t s, KState *st);
 void prec_trans_apply_dev(mtjg_prec M, const double *b, double *x, hipStream_t s);
+// Raises MTJG_E_DEVICE if a single-launch triangular solve hit its poll budget
+// (synchronises the stream).
+void prec_check(mtjg_prec M);
 int prec_kind(mtjg_prec M);
 const double *prec_invdiag(mtjg_prec M);
 int32_t prec_n(mtjg_prec M);

@@ -47,6 +47,10 @@ struct mtjg_prec_s {
   int32_t *trp = nullptr, *tci = nullptr, *tdiag = nullptr, *tperm = nullptr;
   double *tval = nullptr;
   Levels lev_ut, lev_lt;
+  // single-launch solves: ticket counters [4] and the poll-budget error flag
+  uint32_t *sf_next = nullptr;
+  int32_t *sf_err = nullptr;
+  bool level_launch = true;   // false: single-launch solves (MTJG_ILU_SYNCFREE=1)
 };
 
 namespace mtjg {
@@ -174,6 +178,96 @@ __global__ __launch_bounds__(BS) void k_trsv_level(int32_t cnt, const int32_t *r
   }
 }
 
+// ---- single-launch triangular solve ----------------------------------------
+//
+// Rows are taken in level order in chunks of 64 (one per wave) from an atomic
+// ticket counter, so every row a chunk depends on sits in a chunk with a
+// smaller ticket, which a running (or finished) wave has already claimed:
+// some unfinished row always has all of its inputs, and the solve cannot
+// deadlock.  The output vector starts filled with TRSV_PENDING, a signalling
+// NaN that IEEE arithmetic never produces; a finished row stores its value
+// with a 64-bit agent-scope atomic store, and readers load dependencies with
+// agent-scope atomic loads until the value is no longer pending, so the value
+// itself is the completion flag.  Each lane sums its row's entries strictly
+// in storage order (pausing while an input is pending), which keeps MTJ's
+// operation order.  A bounded poll budget turns any violation of these
+// assumptions into an error flag instead of a hang.
+constexpr uint64_t TRSV_PENDING = 0x7FF4DEAD5EA1ED00ULL;
+constexpr long long TRSV_POLL_BUDGET = 1LL << 20;
+
+__device__ __forceinline__ uint64_t load_bits(const double *p) {
+  return __hip_atomic_load((const uint64_t *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(BS) void k_fill_pending(int64_t n, double *x) {
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+    *(uint64_t *)&x[i] = TRSV_PENDING;
+}
+
+template <int FORM>
+__global__ __launch_bounds__(BS) void k_trsv_syncfree(int32_t n, const int32_t *__restrict__ rows,
+                                                      const int32_t *__restrict__ rp,
+                                                      const int32_t *__restrict__ ci,
+                                                      const int32_t *__restrict__ diag,
+                                                      const double *__restrict__ a,
+                                                      const double *__restrict__ b, double *x,
+                                                      uint32_t *next, int32_t *err) {
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    int base = 0;
+    if (lane == 0) base = (int)atomicAdd(next, 64u);
+    base = __shfl(base, 0, 64);
+    if (base >= n) return;
+    const int t = base + lane;
+    bool done = t >= n;
+    int i = 0, j = 0, end = 0;
+    double s = 0.0;
+    if (!done) {
+      i = rows[t];
+      if (FORM == TRSV_ROW_UNIT) { j = rp[i]; end = diag[i]; s = 0.0; }
+      else if (FORM == TRSV_ROW_DIV) { j = diag[i] + 1; end = rp[i + 1]; s = 0.0; }
+      else if (FORM == TRSV_COL_DIV) { j = rp[i]; end = diag[i]; s = b[i]; }
+      else { j = rp[i + 1] - 1; end = diag[i]; s = b[i]; }
+    }
+    long long budget = TRSV_POLL_BUDGET;
+    int backoff = 1;
+    while (__builtin_amdgcn_ballot_w64(!done) != 0ULL) {
+      if (!done) {
+        // consume entries in order while their inputs are available
+        for (;;) {
+          const bool more = (FORM == TRSV_COL_UNIT_REV) ? (j > end) : (j < end);
+          if (!more) break;
+          const uint64_t bits = load_bits(&x[ci[j]]);
+          if (bits == TRSV_PENDING) break;
+          const double xv = __longlong_as_double((long long)bits);
+          if (FORM == TRSV_ROW_UNIT || FORM == TRSV_ROW_DIV) s += a[j] * xv;
+          else s -= a[j] * xv;
+          if (FORM == TRSV_COL_UNIT_REV) --j; else ++j;
+        }
+        const bool more = (FORM == TRSV_COL_UNIT_REV) ? (j > end) : (j < end);
+        if (!more) {
+          double v;
+          if (FORM == TRSV_ROW_UNIT) v = b[i] - s;
+          else if (FORM == TRSV_ROW_DIV) v = (b[i] - s) / a[diag[i]];
+          else if (FORM == TRSV_COL_DIV) v = s / a[diag[i]];
+          else v = s;
+          __hip_atomic_store((uint64_t *)&x[i], (uint64_t)__double_as_longlong(v), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+          done = true;
+        } else if (--budget < 0) {
+          atomicExch(err, 1);
+          done = true;
+        }
+      }
+      // wave-uniform back-off while any lane still waits
+      if (__builtin_amdgcn_ballot_w64(!done) != 0ULL) {
+        for (int k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(2);
+        backoff = backoff < 32 ? 2 * backoff : 32;
+      }
+    }
+  }
+}
+
 __global__ void k_gather(int64_t n, const int32_t *perm, const double *src, double *dst) {
   for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
     dst[i] = src[perm[i]];
@@ -217,6 +311,22 @@ static void run_levels(const Levels &L, hipStream_t s, const int32_t *rp, const 
     int32_t cnt = L.ptr[l + 1] - L.ptr[l];
     k_trsv_level<FORM><<<(cnt + BS - 1) / BS, BS, 0, s>>>(cnt, L.rows + L.ptr[l], rp, ci, diag, a, b, x);
   }
+  HIPCHK(hipGetLastError());
+}
+
+// One triangular solve as a single launch (see k_trsv_syncfree); `slot`
+// selects the ticket counter so consecutive solves never share one.
+template <int FORM>
+static void run_syncfree(mtjg_prec M, const Levels &L, int slot, hipStream_t s, const int32_t *rp,
+                         const int32_t *ci, const int32_t *diag, const double *a, const double *b,
+                         double *x) {
+  const int32_t n = M->n;
+  if (n == 0) return;
+  k_fill_pending<<<grid_for(n), BS, 0, s>>>(n, x);
+  HIPCHK(hipMemsetAsync(M->sf_next + slot, 0, sizeof(uint32_t), s));
+  // one workgroup per CU bounds the number of polling lanes
+  const int grid = std::min((n + BS - 1) / BS, 256);
+  k_trsv_syncfree<FORM><<<grid, BS, 0, s>>>(n, L.rows, rp, ci, diag, a, b, x, M->sf_next + slot, M->sf_err);
   HIPCHK(hipGetLastError());
 }
 
@@ -395,8 +505,13 @@ void prec_apply_dev(mtjg_prec M, int64_t n, const double *b, double *x, hipStrea
   } else {
     if (!M->factored) fail(MTJG_E_INVALID_ARGUMENT, "ILU.setMatrix must precede apply");
     // L.solve(b, y); U.solve(y, x) (S/ILU.java:64-70)
-    run_levels<TRSV_ROW_UNIT>(M->lev_l, s, M->rp, M->ci, M->diag, M->lu, b, M->y);
-    run_levels<TRSV_ROW_DIV>(M->lev_u, s, M->rp, M->ci, M->diag, M->lu, M->y, x);
+    if (M->level_launch) {
+      run_levels<TRSV_ROW_UNIT>(M->lev_l, s, M->rp, M->ci, M->diag, M->lu, b, M->y);
+      run_levels<TRSV_ROW_DIV>(M->lev_u, s, M->rp, M->ci, M->diag, M->lu, M->y, x);
+    } else {
+      run_syncfree<TRSV_ROW_UNIT>(M, M->lev_l, 0, s, M->rp, M->ci, M->diag, M->lu, b, M->y);
+      run_syncfree<TRSV_ROW_DIV>(M, M->lev_u, 1, s, M->rp, M->ci, M->diag, M->lu, M->y, x);
+    }
   }
   HIPCHK(hipGetLastError());
 }
@@ -409,8 +524,26 @@ void prec_trans_apply_dev(mtjg_prec M, const double *b, double *x, hipStream_t s
   if (!M->factored) fail(MTJG_E_INVALID_ARGUMENT, "ILU.setMatrix must precede transApply");
   if (!M->trans_ready) ilu_build_transpose(M);
   // U.transSolve(b, y); L.transSolve(y, x) (S/ILU.java:72-78)
-  run_levels<TRSV_COL_DIV>(M->lev_ut, s, M->trp, M->tci, M->tdiag, M->tval, b, M->y);
-  run_levels<TRSV_COL_UNIT_REV>(M->lev_lt, s, M->trp, M->tci, M->tdiag, M->tval, M->y, x);
+  if (M->level_launch) {
+    run_levels<TRSV_COL_DIV>(M->lev_ut, s, M->trp, M->tci, M->tdiag, M->tval, b, M->y);
+    run_levels<TRSV_COL_UNIT_REV>(M->lev_lt, s, M->trp, M->tci, M->tdiag, M->tval, M->y, x);
+  } else {
+    run_syncfree<TRSV_COL_DIV>(M, M->lev_ut, 2, s, M->trp, M->tci, M->tdiag, M->tval, b, M->y);
+    run_syncfree<TRSV_COL_UNIT_REV>(M, M->lev_lt, 3, s, M->trp, M->tci, M->tdiag, M->tval, M->y, x);
+  }
+}
+
+// Raises if a single-launch triangular solve ran out of its poll budget
+// since the last check (synchronises the stream).
+void prec_check(mtjg_prec M) {
+  if (!M || M->kind != PREC_ILU || !M->sf_err) return;
+  int32_t e = 0;
+  HIPCHK(hipMemcpyAsync(&e, M->sf_err, sizeof(int32_t), hipMemcpyDeviceToHost, M->ctx->stream));
+  HIPCHK(hipStreamSynchronize(M->ctx->stream));
+  if (e) {
+    HIPCHK(hipMemsetAsync(M->sf_err, 0, sizeof(int32_t), M->ctx->stream));
+    fail(MTJG_E_DEVICE, "ILU triangular solve exceeded its dependency poll budget");
+  }
 }
 
 int prec_kind(mtjg_prec M) { return M ? M->kind : PREC_IDENTITY; }
@@ -450,6 +583,12 @@ extern "C" int mtjg_prec_ilu_create(mtjg_ctx ctx, mtjg_csr LU, mtjg_prec *out) {
   HIPCHK(hipMalloc(&M->y, sizeof(double) * (size_t)std::max(M->n, 1)));
   HIPCHK(hipMemsetAsync(M->y, 0, sizeof(double) * (size_t)std::max(M->n, 1), ctx->stream));
   HIPCHK(hipMalloc(&M->bad, sizeof(int32_t)));
+  HIPCHK(hipMalloc(&M->sf_next, sizeof(uint32_t) * 4));
+  HIPCHK(hipMalloc(&M->sf_err, sizeof(int32_t)));
+  HIPCHK(hipMemsetAsync(M->sf_next, 0, sizeof(uint32_t) * 4, ctx->stream));
+  HIPCHK(hipMemsetAsync(M->sf_err, 0, sizeof(int32_t), ctx->stream));
+  M->level_launch = true;
+  if (const char *e = getenv("MTJG_ILU_SYNCFREE")) M->level_launch = atoi(e) == 0;
   *out = M;
   MTJG_API_END
 }
@@ -490,6 +629,7 @@ extern "C" int mtjg_prec_apply(mtjg_prec M, mtjg_vec b, mtjg_vec x) {
   MTJG_API_BEGIN
   check_apply(M, b, x);
   prec_apply_dev(M, M->n, (const double *)b->ptr, (double *)x->ptr, M->ctx->stream, nullptr);
+  prec_check(M);
   MTJG_API_END
 }
 
@@ -497,6 +637,7 @@ extern "C" int mtjg_prec_trans_apply(mtjg_prec M, mtjg_vec b, mtjg_vec x) {
   MTJG_API_BEGIN
   check_apply(M, b, x);
   prec_trans_apply_dev(M, (const double *)b->ptr, (double *)x->ptr, M->ctx->stream);
+  prec_check(M);
   MTJG_API_END
 }
 
@@ -507,6 +648,8 @@ extern "C" int mtjg_prec_destroy(mtjg_prec M) {
   (void)hipFree(M->invdiag);
   (void)hipFree(M->bad);
   (void)hipFree(M->y);
+  (void)hipFree(M->sf_next);
+  (void)hipFree(M->sf_err);
   ilu_free_storage(M);
   delete M;
   MTJG_API_END

@@ -47,6 +47,20 @@ __device__ __forceinline__ double load_agent(const double *p) {
   return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Publication protocol of the grid reductions.  Partials are written with
+// agent-scope atomic stores (global_store ... sc1: written through to the
+// device-wide coherence point) and read with agent-scope atomic loads (sc1:
+// served from that point, never from a stale line of another XCD's L2).  The
+// only ordering needed is that a workgroup's partial stores have completed
+// before its ticket increment, which s_waitcnt vmcnt(0) provides; a
+// release/acquire fence pair would add an L2 write-back and invalidate per
+// workgroup.  Readers issue their loads after the __syncthreads that follows
+// the ticket, so they cannot run ahead of it.
+__device__ __forceinline__ void stores_complete() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+__device__ __forceinline__ uint32_t take_ticket(uint32_t *t) {
+  return __hip_atomic_fetch_add(t, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Grid-wide sum.  Every workgroup publishes its K partial sums to
 // partials[k * RED_MAX_BLOCKS + blockIdx.x] and takes a ticket; the
 // workgroup that takes the last ticket adds all partials in index order and
@@ -60,13 +74,12 @@ __device__ bool grid_sum(double (&v)[K], double *partials, uint32_t *ticket) {
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; ++k) store_agent(&partials[k * RED_MAX_BLOCKS + blockIdx.x], v[k]);
-    __atomic_thread_fence(__ATOMIC_RELEASE);
-    uint32_t t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    stores_complete();
+    uint32_t t = take_ticket(ticket);
     last = (t == gridDim.x - 1);
   }
   __syncthreads();
   if (!last) return false;
-  __atomic_thread_fence(__ATOMIC_ACQUIRE);
   double acc[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) acc[k] = 0.0;
@@ -79,6 +92,179 @@ __device__ bool grid_sum(double (&v)[K], double *partials, uint32_t *ticket) {
   for (int k = 0; k < K; ++k) v[k] = acc[k];
   if (threadIdx.x == 0)
     __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
+// Two-level grid-wide sum for grids of any size.  Workgroup b publishes its K
+// partials to partials[k * cap + b] and takes a ticket on its group of
+// RED_GROUP consecutive workgroups; the last workgroup of a group adds the
+// group's partials in index order into gpart[k * gcap + g] and takes a
+// ticket on the final counter; the last group adds the group sums in index
+// order.  Every sum has a fixed order, so results are bitwise reproducible;
+// both counters are reset by their last user.  The workspace is a GridRed.
+template <int K>
+__device__ bool grid_sum_h(double (&v)[K], const GridRed &r) {
+  __shared__ int last;
+  block_sum<K>(v);
+  const int b = blockIdx.x, g = b / RED_GROUP;
+  const int ngroups = (gridDim.x + RED_GROUP - 1) / RED_GROUP;
+  const int gsize = min(RED_GROUP, (int)gridDim.x - g * RED_GROUP);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) store_agent(&r.partials[k * r.cap + b], v[k]);
+    stores_complete();
+    uint32_t t = take_ticket(&r.gticket[g]);
+    last = (t == (uint32_t)gsize - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    acc[k] = (threadIdx.x < gsize) ? load_agent(&r.partials[k * r.cap + g * RED_GROUP + threadIdx.x]) : 0.0;
+  block_sum<K>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) store_agent(&r.gpart[k * r.gcap + g], acc[k]);
+    __hip_atomic_store(&r.gticket[g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    stores_complete();
+    uint32_t t = take_ticket(r.fticket);
+    last = (t == (uint32_t)ngroups - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = 0.0;
+  for (int i = threadIdx.x; i < ngroups; i += BS) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] += load_agent(&r.gpart[k * r.gcap + i]);
+  }
+  block_sum<K>(acc);
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = acc[k];
+  if (threadIdx.x == 0) __hip_atomic_store(r.fticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
+// ---- compensated sums ------------------------------------------------------
+//
+// The solvers' dot products and norms are accumulated as unevaluated sums
+// hi + lo (Ogita, Rump and Oishi's Dot2: error-free TwoProduct via fma and
+// TwoSum; the library is built with -ffp-contract=off, so these
+// transformations are not rewritten).  The result is as accurate as a dot
+// product evaluated in twice the working precision and then rounded, i.e.
+// essentially the correctly rounded value, so iteration counts follow
+// exact-arithmetic behaviour instead of one particular summation order.
+// Combination order stays fixed, so results remain bitwise reproducible.
+struct dd {
+  double hi, lo;
+};
+
+__device__ __forceinline__ dd dd_add(dd a, dd b) {
+  const double s = a.hi + b.hi;
+  const double bb = s - a.hi;
+  const double e = (a.hi - (s - bb)) + (b.hi - bb);
+  return {s, e + (a.lo + b.lo)};
+}
+
+// acc += a * b with the product's rounding error kept.
+__device__ __forceinline__ void dd_fma(dd &acc, double a, double b) {
+  const double p = a * b;
+  const double pe = fma(a, b, -p);
+  const double s = acc.hi + p;
+  const double bb = s - acc.hi;
+  const double e = (acc.hi - (s - bb)) + (p - bb);
+  acc.hi = s;
+  acc.lo += e + pe;
+}
+
+template <int K>
+__device__ __forceinline__ void block_sum_dd(dd (&v)[K]) {
+  __shared__ double sh[2 * K][BS / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      dd o{__shfl_xor(v[k].hi, off, 64), __shfl_xor(v[k].lo, off, 64)};
+      v[k] = dd_add(v[k], o);
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      sh[2 * k][wid] = v[k].hi;
+      sh[2 * k + 1][wid] = v[k].lo;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    dd a = dd_add(dd{sh[2 * k][0], sh[2 * k + 1][0]}, dd{sh[2 * k][1], sh[2 * k + 1][1]});
+    dd b = dd_add(dd{sh[2 * k][2], sh[2 * k + 1][2]}, dd{sh[2 * k][3], sh[2 * k + 1][3]});
+    v[k] = dd_add(a, b);
+  }
+  __syncthreads();
+}
+
+// Two-level grid sum of compensated partials (protocol of grid_sum_h; each
+// sum occupies two partial slots, so K <= RED_MAX_K / 2).  Returns true in
+// the last workgroup with out[k] = hi + lo of the grid total.
+template <int K>
+__device__ bool grid_sum_dd(dd (&v)[K], const GridRed &r, double (&out)[K]) {
+  static_assert(2 * K <= RED_MAX_K, "too many sums");
+  __shared__ int last;
+  block_sum_dd<K>(v);
+  const int b = blockIdx.x, g = b / RED_GROUP;
+  const int ngroups = (gridDim.x + RED_GROUP - 1) / RED_GROUP;
+  const int gsize = min(RED_GROUP, (int)gridDim.x - g * RED_GROUP);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      store_agent(&r.partials[(2 * k) * r.cap + b], v[k].hi);
+      store_agent(&r.partials[(2 * k + 1) * r.cap + b], v[k].lo);
+    }
+    stores_complete();
+    uint32_t t = take_ticket(&r.gticket[g]);
+    last = (t == (uint32_t)gsize - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+  dd acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    acc[k] = dd{0.0, 0.0};
+    if ((int)threadIdx.x < gsize) {
+      acc[k].hi = load_agent(&r.partials[(2 * k) * r.cap + g * RED_GROUP + threadIdx.x]);
+      acc[k].lo = load_agent(&r.partials[(2 * k + 1) * r.cap + g * RED_GROUP + threadIdx.x]);
+    }
+  }
+  block_sum_dd<K>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      store_agent(&r.gpart[(2 * k) * r.gcap + g], acc[k].hi);
+      store_agent(&r.gpart[(2 * k + 1) * r.gcap + g], acc[k].lo);
+    }
+    __hip_atomic_store(&r.gticket[g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    stores_complete();
+    uint32_t t = take_ticket(r.fticket);
+    last = (t == (uint32_t)ngroups - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = dd{0.0, 0.0};
+  for (int i = threadIdx.x; i < ngroups; i += BS) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      acc[k] = dd_add(acc[k], dd{load_agent(&r.gpart[(2 * k) * r.gcap + i]),
+                                 load_agent(&r.gpart[(2 * k + 1) * r.gcap + i])});
+  }
+  block_sum_dd<K>(acc);
+#pragma unroll
+  for (int k = 0; k < K; ++k) out[k] = acc[k].hi + acc[k].lo;
+  if (threadIdx.x == 0) __hip_atomic_store(r.fticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return true;
 }
 

@@ -37,7 +37,7 @@ struct mtjg_solver_s {
   KState *st_in = nullptr;   // inner (FP32) solve of MTJG_MIXED_CG
   KState *h_st = nullptr;    // pinned staging [2]
   int32_t *h_slot = nullptr; // pinned status slots [2]
-  double *partials = nullptr;
+  GridRed red{};             // two-level reduction workspace (grown on demand)
   hipEvent_t ev[2] = {nullptr, nullptr};
   std::vector<double *> v64;  // scratch FP64 vectors
   std::vector<float *> v32;   // scratch FP32 vectors (mixed)
@@ -52,6 +52,23 @@ namespace {
 
 constexpr int KSLOT = 0;  // ticket slots inside KState
 
+// Vector kernels split the vector into tiles of BS * VPT elements; a thread
+// owns elements tile + threadIdx.x + u * BS (unit stride across the wave, all
+// loads of a thread independent).  The grid holds at most VGRID_MAX
+// workgroups (eight per CU, all co-resident) that walk the tiles in
+// grid-stride order, so a reducing kernel takes at most VGRID_MAX tickets.
+constexpr int VPT = 4;
+constexpr int VGRID_MAX = 2048;
+inline int vgrid(int64_t n) {
+  return (int)std::min<int64_t>(VGRID_MAX, std::max<int64_t>(1, (n + BS * VPT - 1) / (BS * VPT)));
+}
+#define TILES(base)                                                                          \
+  for (int64_t base = (int64_t)blockIdx.x * (BS * VPT) + threadIdx.x; base - threadIdx.x < n; \
+       base += (int64_t)gridDim.x * (BS * VPT))
+#define VLOOP(i)                                                                                  \
+  TILES(i##_b)                                                                                    \
+  _Pragma("unroll") for (int64_t i##_u = 0, i = i##_b; i##_u < VPT; ++i##_u, i += BS) if (i < n)
+
 // ---------------------------------------------------------------------------
 // CG kernels
 
@@ -65,36 +82,49 @@ enum PrecK : int { PK_IDENTITY = 0, PK_JACOBI = 1, PK_EXTERNAL = 2 };
 // running, rho = r.z (S/CG.java:68) and beta = rho / rho_1 (S/CG.java:73).
 template <typename T, int PK, bool UPDATE>
 __global__ __launch_bounds__(BS) void k_cg_resid(int64_t n, T *x, const T *p, T *r, const T *q,
-                                                 const T *invdiag, T *z, KState *st,
-                                                 double *partials) {
+                                                 const T *invdiag, T *z, KState *st, GridRed red) {
   if (st->status != ST_RUNNING) return;
   const T alpha = (T)st->alpha;
-  double v[2] = {0.0, 0.0};
-  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
-    T ri = r[i];
-    if (UPDATE && alpha != T(0)) {
-      x[i] = x[i] + alpha * p[i];
-      ri = ri + (-alpha) * q[i];
-      r[i] = ri;
+  dd v[2] = {{0.0, 0.0}, {0.0, 0.0}};
+  TILES(base) {
+    T ri[VPT];
+#pragma unroll
+    for (int u = 0; u < VPT; ++u) {
+      const int64_t i = base + u * BS;
+      if (i < n) {
+        ri[u] = r[i];
+        if (UPDATE && alpha != T(0)) {
+          x[i] = x[i] + alpha * p[i];
+          ri[u] = ri[u] + (-alpha) * q[i];
+          r[i] = ri[u];
+        }
+      }
     }
-    if (PK == PK_JACOBI) {
-      T zi = ri * invdiag[i];
-      z[i] = zi;
-      v[0] += (double)ri * (double)zi;
-    } else if (PK == PK_IDENTITY) {
-      v[0] += (double)ri * (double)ri;
+#pragma unroll
+    for (int u = 0; u < VPT; ++u) {
+      const int64_t i = base + u * BS;
+      if (i < n) {
+        if (PK == PK_JACOBI) {
+          T zi = ri[u] * invdiag[i];
+          z[i] = zi;
+          dd_fma(v[0], (double)ri[u], (double)zi);
+        } else if (PK == PK_IDENTITY) {
+          dd_fma(v[0], (double)ri[u], (double)ri[u]);
+        }
+        dd_fma(v[1], (double)ri[u], (double)ri[u]);
+      }
     }
-    v[1] += (double)ri * (double)ri;
   }
-  if (grid_sum<2>(v, partials, &st->ticket[KSLOT]) && threadIdx.x == 0) {
+  double tot[2];
+  if (grid_sum_dd<2>(v, red, tot) && threadIdx.x == 0) {
     if (UPDATE) st->iter++;
-    int s = monitor_check(st, sqrt(v[1]));
+    int s = monitor_check(st, sqrt(tot[1]));
     if (s != ST_RUNNING) {
       st->status = s;
       return;
     }
     if (PK != PK_EXTERNAL) {
-      st->rho = v[0];
+      st->rho = tot[0];
       if (st->iter > 0) st->beta = st->rho / st->rho_1;
     }
   }
@@ -102,14 +132,13 @@ __global__ __launch_bounds__(BS) void k_cg_resid(int64_t n, T *x, const T *p, T 
 
 // rho = r.z for a preconditioner applied by separate launches (ILU).
 template <typename T>
-__global__ __launch_bounds__(BS) void k_cg_rho(int64_t n, const T *r, const T *z, KState *st,
-                                               double *partials) {
+__global__ __launch_bounds__(BS) void k_cg_rho(int64_t n, const T *r, const T *z, KState *st, GridRed red) {
   if (st->status != ST_RUNNING) return;
-  double v[1] = {0.0};
-  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
-    v[0] += (double)r[i] * (double)z[i];
-  if (grid_sum<1>(v, partials, &st->ticket[KSLOT + 1]) && threadIdx.x == 0) {
-    st->rho = v[0];
+  dd v[1] = {{0.0, 0.0}};
+  VLOOP(i) dd_fma(v[0], (double)r[i], (double)z[i]);
+  double tot[1];
+  if (grid_sum_dd<1>(v, red, tot) && threadIdx.x == 0) {
+    st->rho = tot[0];
     if (st->iter > 0) st->beta = st->rho / st->rho_1;
   }
 }
@@ -121,7 +150,7 @@ __global__ __launch_bounds__(BS) void k_cg_p(int64_t n, T *p, const T *z, const 
   if (st->status != ST_RUNNING) return;
   const bool first = st->iter == 0;
   const T beta = (T)st->beta;
-  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+  VLOOP(i) {
     if (first) {
       p[i] = z[i];
     } else {
@@ -142,12 +171,11 @@ __global__ __launch_bounds__(BS) void k_cg_p(int64_t n, T *p, const T *z, const 
 template <bool UPDATE>
 __global__ __launch_bounds__(BS) void k_bicg_top(int64_t n, double *x, const double *shat, double *r,
                                                  const double *s, const double *t,
-                                                 const double *rtilde, KState *st,
-                                                 double *partials) {
+                                                 const double *rtilde, KState *st, GridRed red) {
   if (st->status != ST_RUNNING) return;
   const double omega = st->omega;
-  double v[2] = {0.0, 0.0};
-  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+  dd v[2] = {{0.0, 0.0}, {0.0, 0.0}};
+  VLOOP(i) {
     double ri;
     if (UPDATE) {
       if (omega != 0) x[i] = x[i] + omega * shat[i];
@@ -157,20 +185,21 @@ __global__ __launch_bounds__(BS) void k_bicg_top(int64_t n, double *x, const dou
     } else {
       ri = r[i];
     }
-    v[0] += ri * ri;
-    v[1] += rtilde[i] * ri;
+    dd_fma(v[0], ri, ri);
+    dd_fma(v[1], rtilde[i], ri);
   }
-  if (grid_sum<2>(v, partials, &st->ticket[KSLOT]) && threadIdx.x == 0) {
+  double tot[2];
+  if (grid_sum_dd<2>(v, red, tot) && threadIdx.x == 0) {
     if (UPDATE) {
       st->rho_2 = st->rho_1;
       st->iter++;
     }
-    int stt = monitor_check(st, sqrt(v[0]));
+    int stt = monitor_check(st, sqrt(tot[0]));
     if (stt != ST_RUNNING) {
       st->status = stt;
       return;
     }
-    st->rho_1 = v[1];
+    st->rho_1 = tot[1];
     if (st->rho_1 == 0 || st->omega == 0) {
       st->status = ST_BREAKDOWN;
       return;
@@ -185,7 +214,7 @@ __global__ __launch_bounds__(BS) void k_bicg_p(int64_t n, double *p, const doubl
   if (st->status != ST_RUNNING) return;
   const bool first = st->iter == 0;
   const double beta = st->beta, omega = st->omega;
-  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+  VLOOP(i) {
     if (first) {
       p[i] = r[i];
     } else {
@@ -201,22 +230,22 @@ __global__ __launch_bounds__(BS) void k_bicg_p(int64_t n, double *p, const doubl
 // s = r + (-alpha) v; x += alpha phat; then the half-step check on ||s||
 // (S/BiCGstab.java:100-104).
 __global__ __launch_bounds__(BS) void k_bicg_s(int64_t n, double *x, const double *phat, const double *r,
-                                               const double *v, double *s, KState *st,
-                                               double *partials) {
+                                               const double *v, double *s, KState *st, GridRed red) {
   if (st->status != ST_RUNNING) return;
   const double alpha = st->alpha;
-  double acc[1] = {0.0};
-  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+  dd acc[1] = {{0.0, 0.0}};
+  VLOOP(i) {
     double si = r[i];
     if (alpha != 0) {
       si = si + (-alpha) * v[i];
       x[i] = x[i] + alpha * phat[i];
     }
     s[i] = si;
-    acc[0] += si * si;
+    dd_fma(acc[0], si, si);
   }
-  if (grid_sum<1>(acc, partials, &st->ticket[KSLOT + 1]) && threadIdx.x == 0) {
-    int stt = monitor_check(st, sqrt(acc[0]));
+  double tot[1];
+  if (grid_sum_dd<1>(acc, red, tot) && threadIdx.x == 0) {
+    int stt = monitor_check(st, sqrt(tot[0]));
     if (stt != ST_RUNNING) st->status = stt;
   }
 }
@@ -239,13 +268,14 @@ struct GmresDev {
 // iter.next() of the outer loop; the outer check; then s = normr e1 and
 // the inner check at i = 0 (S/GMRES.java:137-144).
 __global__ __launch_bounds__(BS) void k_gmres_start(int64_t n, const double *r, KState *st,
-                                                    GmresDev g, int first, double *partials) {
+                                                    GmresDev g, int first, GridRed red) {
   if (st->status != ST_RUNNING) return;
-  double v[1] = {0.0};
-  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
-    v[0] += r[i] * r[i];
-  if (grid_sum<1>(v, partials, &st->ticket[KSLOT]) && threadIdx.x == 0) {
-    double normr = sqrt(v[0]);
+  dd v[1] = {{0.0, 0.0}};
+  VLOOP(i)
+    dd_fma(v[0], r[i], r[i]);
+  double tot[1];
+  if (grid_sum_dd<1>(v, red, tot) && threadIdx.x == 0) {
+    double normr = sqrt(tot[0]);
     if (!first) st->iter++;
     int stt = monitor_check(st, normr);
     if (stt != ST_RUNNING) {
@@ -267,7 +297,7 @@ __global__ __launch_bounds__(BS) void k_gmres_start(int64_t n, const double *r, 
 __global__ __launch_bounds__(BS) void k_gmres_v0(int64_t n, const double *r, double *v0, const KState *st) {
   if (st->status != ST_RUNNING) return;
   const double a = 1 / st->normr;
-  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+  VLOOP(i)
     v0[i] = (a == 0) ? 0.0 : a * r[i];
 }
 
@@ -275,20 +305,21 @@ __global__ __launch_bounds__(BS) void k_gmres_v0(int64_t n, const double *r, dou
 // (S/GMRES.java:151) for k > 0, then H(k,i) = w.v_k (:150).
 __global__ __launch_bounds__(BS) void k_gmres_mgs(int64_t n, double *w, const double *vprev,
                                                   const double *vk, int k, int i, KState *st,
-                                                  GmresDev g, double *partials) {
+                                                  GmresDev g, GridRed red) {
   if (st->status != ST_RUNNING) return;
   const int ld = g.m + 1;
   const double h = (k > 0) ? -g.H[(k - 1) + (size_t)i * ld] : 0.0;
-  double v[1] = {0.0};
-  for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
+  dd v[1] = {{0.0, 0.0}};
+  VLOOP(e) {
     double we = w[e];
     if (h != 0) {
       we = we + h * vprev[e];
       w[e] = we;
     }
-    v[0] += we * vk[e];
+    dd_fma(v[0], we, vk[e]);
   }
-  if (grid_sum<1>(v, partials, &st->ticket[KSLOT + 1]) && threadIdx.x == 0) g.H[k + (size_t)i * ld] = v[0];
+  double tot[1];
+  if (grid_sum_dd<1>(v, red, tot) && threadIdx.x == 0) g.H[k + (size_t)i * ld] = tot[0];
 }
 
 // GivensRotation constructor and apply (M/GivensRotation.java:42-92).
@@ -318,23 +349,24 @@ __device__ void givens_apply(double c, double s, double &a1, double &a2) {
 // column i, forms rotation i, rotates H and s (:157-162), advances i and the
 // iteration counter, and runs the next inner check (:144).
 __global__ __launch_bounds__(BS) void k_gmres_arnoldi_end(int64_t n, double *w, const double *vi, int i,
-                                                          KState *st, GmresDev g, double *partials) {
+                                                          KState *st, GmresDev g, GridRed red) {
   if (st->status != ST_RUNNING) return;
   const int ld = g.m + 1;
   const double h = -g.H[i + (size_t)i * ld];
-  double v[1] = {0.0};
-  for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
+  dd v[1] = {{0.0, 0.0}};
+  VLOOP(e) {
     double we = w[e];
     if (h != 0) {
       we = we + h * vi[e];
       w[e] = we;
     }
-    v[0] += we * we;
+    dd_fma(v[0], we, we);
   }
-  if (grid_sum<1>(v, partials, &st->ticket[KSLOT]) && threadIdx.x == 0) {
+  double tot[1];
+  if (grid_sum_dd<1>(v, red, tot) && threadIdx.x == 0) {
     double *Hc = g.H + (size_t)i * ld;
     double *s = g.gs, *rc = g.gs + ld, *rs = g.gs + 2 * ld;
-    Hc[i + 1] = sqrt(v[0]);
+    Hc[i + 1] = sqrt(tot[0]);
     st->s[2] = Hc[i + 1];  // ||w||, used to scale v_{i+1} (:154)
     for (int k = 0; k < i; ++k) givens_apply(rc[k], rs[k], Hc[k], Hc[k + 1]);
     givens(Hc[i], Hc[i + 1], rc[i], rs[i]);
@@ -356,7 +388,7 @@ __global__ __launch_bounds__(BS) void k_gmres_arnoldi_end(int64_t n, double *w, 
 __global__ __launch_bounds__(BS) void k_gmres_scale(int64_t n, const double *w, double *vnext, const KState *st) {
   if (st->status != ST_RUNNING) return;
   const double a = 1. / st->s[2];
-  for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS)
+  VLOOP(e)
     vnext[e] = (a == 0) ? 0.0 : a * w[e];
 }
 
@@ -387,7 +419,7 @@ __global__ __launch_bounds__(BS) void k_gmres_xupdate(int64_t n, double *x, cons
   if (st->status != ST_RUNNING) return;
   const int k = st->inner_i;
   const double *s = g.gs;
-  for (int64_t e = blockIdx.x * (int64_t)BS + threadIdx.x; e < n; e += (int64_t)gridDim.x * BS) {
+  VLOOP(e) {
     double xe = x[e];
     for (int j = 0; j < k; ++j) {
       const double sj = s[j];
@@ -402,22 +434,23 @@ __global__ __launch_bounds__(BS) void k_gmres_xupdate(int64_t n, double *x, cons
 
 // ||r|| and the monitor check; with advance, iter.next() first.
 __global__ __launch_bounds__(BS) void k_norm_check(int64_t n, const double *r, KState *st, int advance,
-                                                   double *partials) {
+                                                   GridRed red) {
   if (st->status != ST_RUNNING) return;
-  double v[1] = {0.0};
-  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
-    v[0] += r[i] * r[i];
-  if (grid_sum<1>(v, partials, &st->ticket[KSLOT]) && threadIdx.x == 0) {
+  dd v[1] = {{0.0, 0.0}};
+  VLOOP(i)
+    dd_fma(v[0], r[i], r[i]);
+  double tot[1];
+  if (grid_sum_dd<1>(v, red, tot) && threadIdx.x == 0) {
     if (advance) st->iter++;
-    int stt = monitor_check(st, sqrt(v[0]));
+    int stt = monitor_check(st, sqrt(tot[0]));
     if (stt != ST_RUNNING) st->status = stt;
-    st->normr = sqrt(v[0]);
+    st->normr = sqrt(tot[0]);
   }
 }
 
 __global__ __launch_bounds__(BS) void k_add_if_running(int64_t n, double *x, const double *z, const KState *st) {
   if (st->status != ST_RUNNING) return;
-  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+  VLOOP(i)
     x[i] = x[i] + z[i];
 }
 
@@ -425,20 +458,20 @@ __global__ __launch_bounds__(BS) void k_add_if_running(int64_t n, double *x, con
 __global__ __launch_bounds__(BS) void k_demote_scaled(int64_t n, const double *r, float *r32, const KState *st) {
   if (st->status != ST_RUNNING) return;
   const double sc = 1.0 / st->normr;
-  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+  VLOOP(i)
     r32[i] = (float)(r[i] * sc);
 }
 
 __global__ __launch_bounds__(BS) void k_promote_add(int64_t n, double *x, const float *d32, const KState *st) {
   if (st->status != ST_RUNNING) return;
   const double sc = st->normr;
-  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+  VLOOP(i)
     x[i] = x[i] + sc * (double)d32[i];
 }
 
 __global__ __launch_bounds__(BS) void k_invdiag32(int64_t n, const int32_t *rp, const int32_t *ci,
                                                   const double *val, float *inv) {
-  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+  VLOOP(i) {
     double d = 0;
     for (int32_t j = rp[i]; j < rp[i + 1]; ++j)
       if (ci[j] == (int32_t)i) d = val[j];
@@ -447,7 +480,7 @@ __global__ __launch_bounds__(BS) void k_invdiag32(int64_t n, const int32_t *rp, 
 }
 
 __global__ void k_copy_f32_zero(int64_t n, const float *a, float *b, float *z) {
-  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+  VLOOP(i) {
     b[i] = a[i];
     z[i] = 0.0f;
   }
@@ -540,6 +573,32 @@ float *scratch32(mtjg_solver S, size_t k) {
 
 // Iterations queued per status read: aims at a few hundred microseconds of
 // device work per chunk, fewer for large systems.
+void free_red(GridRed &r) {
+  (void)hipFree(r.partials);
+  (void)hipFree(r.gpart);
+  (void)hipFree(r.gticket);
+  r = GridRed{};
+}
+
+// Sizes the reduction workspace for the largest grid a solve of A launches:
+// the vector kernels' vgrid(n) and the fused product's one workgroup per
+// row-block.  Counters start at zero and are left at zero by every launch.
+void ensure_red(mtjg_solver S, mtjg_csr A) {
+  int need = std::max(vgrid(S->n), A->nrb + 64);
+  if (need <= S->red.cap) return;
+  HIPCHK(hipStreamSynchronize(S->ctx->stream));
+  free_red(S->red);
+  GridRed r;
+  r.cap = need;
+  r.gcap = (need + RED_GROUP - 1) / RED_GROUP;
+  HIPCHK(hipMalloc(&r.partials, sizeof(double) * RED_MAX_K * (size_t)r.cap));
+  HIPCHK(hipMalloc(&r.gpart, sizeof(double) * RED_MAX_K * (size_t)r.gcap));
+  HIPCHK(hipMalloc(&r.gticket, sizeof(uint32_t) * (size_t)(r.gcap + 1)));
+  HIPCHK(hipMemsetAsync(r.gticket, 0, sizeof(uint32_t) * (size_t)(r.gcap + 1), S->ctx->stream));
+  r.fticket = r.gticket + r.gcap;
+  S->red = r;
+}
+
 int chunk_for(mtjg_csr A) {
   double bytes = 12.0 * (double)A->nnz + 108.0 * (double)A->nr;
   int c = (int)(2.0e8 / std::max(bytes, 1.0));
@@ -570,20 +629,21 @@ void solve_cg(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   ra.alpha = -1;
   ra.y0 = b;
   spmv_f64(A, x, r, ra, s);
-  const int g = grid_for(n);
+  const int g = vgrid(n);
+  ensure_red(S, A);
   KState *st = S->st;
   auto resid = [&](bool update) {
     if (pk == PK_IDENTITY) {
-      if (update) k_cg_resid<double, PK_IDENTITY, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->partials);
-      else k_cg_resid<double, PK_IDENTITY, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->partials);
+      if (update) k_cg_resid<double, PK_IDENTITY, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->red);
+      else k_cg_resid<double, PK_IDENTITY, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->red);
     } else if (pk == PK_JACOBI) {
-      if (update) k_cg_resid<double, PK_JACOBI, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->partials);
-      else k_cg_resid<double, PK_JACOBI, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->partials);
+      if (update) k_cg_resid<double, PK_JACOBI, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->red);
+      else k_cg_resid<double, PK_JACOBI, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->red);
     } else {
-      if (update) k_cg_resid<double, PK_EXTERNAL, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->partials);
-      else k_cg_resid<double, PK_EXTERNAL, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->partials);
+      if (update) k_cg_resid<double, PK_EXTERNAL, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->red);
+      else k_cg_resid<double, PK_EXTERNAL, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->red);
       prec_apply_dev(S->M, n, r, z, s, st);
-      k_cg_rho<double><<<g, BS, 0, s>>>(n, r, z, st, S->partials);
+      k_cg_rho<double><<<g, BS, 0, s>>>(n, r, z, st, S->red);
     }
   };
   resid(false);
@@ -593,8 +653,7 @@ void solve_cg(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   qa.w = p;
   qa.st = st;
   qa.fin = FIN_CG_ALPHA;
-  qa.partials = S->partials;
-  qa.ticket = &st->ticket[KSLOT + 2];
+  qa.red = S->red;
   run_chunks(S, st, pk == PK_EXTERNAL ? 1 : chunk_for(A), [&]() {
     k_cg_p<double><<<g, BS, 0, s>>>(n, p, z, st);
     spmv_f64(A, p, q, qa, s);
@@ -630,29 +689,28 @@ void solve_bicgstab(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   ra.y0 = b;
   spmv_f64(A, x, r, ra, s);
   if (n) HIPCHK(hipMemcpyAsync(rtilde, r, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
-  const int g = grid_for(n);
-  k_bicg_top<false><<<g, BS, 0, s>>>(n, x, shat, r, sv, t, rtilde, st, S->partials);
+  const int g = vgrid(n);
+  ensure_red(S, A);
+  k_bicg_top<false><<<g, BS, 0, s>>>(n, x, shat, r, sv, t, rtilde, st, S->red);
   HIPCHK(hipGetLastError());
   SpmvArgs va;
   va.mode = SPMV_DOT;
   va.w = rtilde;
   va.st = st;
   va.fin = FIN_BICG_ALPHA;
-  va.partials = S->partials;
-  va.ticket = &st->ticket[KSLOT + 2];
+  va.red = S->red;
   SpmvArgs ta = va;
   ta.w = sv;
   ta.fin = FIN_BICG_OMEGA;
-  ta.ticket = &st->ticket[KSLOT + 3];
   run_chunks(S, st, pk == PK_EXTERNAL ? 1 : chunk_for(A), [&]() {
     k_bicg_p<<<g, BS, 0, s>>>(n, p, r, v, st);
     if (pk != PK_IDENTITY) prec_apply_dev(S->M, n, p, phat, s, st);
     spmv_f64(A, phat, v, va, s);
-    k_bicg_s<<<g, BS, 0, s>>>(n, x, phat, r, v, sv, st, S->partials);
+    k_bicg_s<<<g, BS, 0, s>>>(n, x, phat, r, v, sv, st, S->red);
     if (pk != PK_IDENTITY) prec_apply_dev(S->M, n, sv, shat, s, st);
     spmv_f64(A, shat, t, ta, s);
     k_bicg_omega<<<1, 1, 0, s>>>(st);
-    k_bicg_top<true><<<g, BS, 0, s>>>(n, x, shat, r, sv, t, rtilde, st, S->partials);
+    k_bicg_top<true><<<g, BS, 0, s>>>(n, x, shat, r, sv, t, rtilde, st, S->red);
     HIPCHK(hipGetLastError());
   });
 }
@@ -688,7 +746,8 @@ void solve_gmres(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   KState *st = S->st;
   state_init(S, st, S->mon.max_iter, S->mon.rtol, S->mon.atol, S->mon.dtol);
   GmresDev gd{S->H, S->gs, S->restart};
-  const int g = grid_for(n);
+  const int g = vgrid(n);
+  ensure_red(S, A);
   auto Vk = [&](int k) { return S->V + (size_t)k * (size_t)n; };
   SpmvArgs ra;
   ra.mode = SPMV_MULTADD;
@@ -702,7 +761,7 @@ void solve_gmres(mtjg_solver S, mtjg_csr A, double *b, double *x) {
     prec_apply_dev(S->M, n, u, r, s, nullptr);
   };
   residual();
-  k_gmres_start<<<g, BS, 0, s>>>(n, r, st, gd, 1, S->partials);
+  k_gmres_start<<<g, BS, 0, s>>>(n, r, st, gd, 1, S->red);
   HIPCHK(hipGetLastError());
   KState k;
   for (;;) {
@@ -713,8 +772,8 @@ void solve_gmres(mtjg_solver S, mtjg_csr A, double *b, double *x) {
       spmv_f64(A, Vk(i), u, ma, s);
       prec_apply_dev(S->M, n, u, w, s, nullptr);
       for (int kk = 0; kk <= i; ++kk)
-        k_gmres_mgs<<<g, BS, 0, s>>>(n, w, kk > 0 ? Vk(kk - 1) : nullptr, Vk(kk), kk, i, st, gd, S->partials);
-      k_gmres_arnoldi_end<<<g, BS, 0, s>>>(n, w, Vk(i), i, st, gd, S->partials);
+        k_gmres_mgs<<<g, BS, 0, s>>>(n, w, kk > 0 ? Vk(kk - 1) : nullptr, Vk(kk), kk, i, st, gd, S->red);
+      k_gmres_arnoldi_end<<<g, BS, 0, s>>>(n, w, Vk(i), i, st, gd, S->red);
       k_gmres_scale<<<g, BS, 0, s>>>(n, w, Vk(i + 1), st);
       HIPCHK(hipGetLastError());
       state_read(S, st, k);
@@ -724,7 +783,7 @@ void solve_gmres(mtjg_solver S, mtjg_csr A, double *b, double *x) {
     k_gmres_backsolve<<<1, 1, 0, s>>>(st, gd);
     k_gmres_xupdate<<<g, BS, 0, s>>>(n, x, S->V, st, gd);
     residual();
-    k_gmres_start<<<g, BS, 0, s>>>(n, r, st, gd, 0, S->partials);
+    k_gmres_start<<<g, BS, 0, s>>>(n, r, st, gd, 0, S->red);
     HIPCHK(hipGetLastError());
   }
 }
@@ -742,15 +801,16 @@ void solve_ir(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   ra.mode = SPMV_MULTADD;
   ra.alpha = -1;
   ra.y0 = b;
-  const int g = grid_for(n);
+  const int g = vgrid(n);
+  ensure_red(S, A);
   spmv_f64(A, x, r, ra, s);
-  k_norm_check<<<g, BS, 0, s>>>(n, r, st, 0, S->partials);
+  k_norm_check<<<g, BS, 0, s>>>(n, r, st, 0, S->red);
   HIPCHK(hipGetLastError());
   run_chunks(S, st, 1, [&]() {
     prec_apply_dev(S->M, n, r, z, s, st);
     k_add_if_running<<<g, BS, 0, s>>>(n, x, z, st);
     spmv_f64(A, x, r, ra, s);
-    k_norm_check<<<g, BS, 0, s>>>(n, r, st, 1, S->partials);
+    k_norm_check<<<g, BS, 0, s>>>(n, r, st, 1, S->red);
     HIPCHK(hipGetLastError());
   });
 }
@@ -768,7 +828,8 @@ void solve_mixed(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   float *r32 = scratch32(S, 0), *d32 = scratch32(S, 1), *p32 = scratch32(S, 2), *q32 = scratch32(S, 3),
         *z32 = scratch32(S, 4), *rin = scratch32(S, 5), *inv32 = scratch32(S, 6);
   csr_ensure_fp32(A);
-  const int g = grid_for(n);
+  const int g = vgrid(n);
+  ensure_red(S, A);
   k_invdiag32<<<g, BS, 0, s>>>(n, A->rp, A->ci, A->val, inv32);
   KState *st = S->st, *si = S->st_in;
   state_init(S, st, S->mon.max_iter, S->mon.rtol, S->mon.atol, S->mon.dtol);
@@ -777,15 +838,14 @@ void solve_mixed(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   ra.alpha = -1;
   ra.y0 = b;
   spmv_f64(A, x, r, ra, s);
-  k_norm_check<<<g, BS, 0, s>>>(n, r, st, 0, S->partials);
+  k_norm_check<<<g, BS, 0, s>>>(n, r, st, 0, S->red);
   HIPCHK(hipGetLastError());
   SpmvArgs qa;
   qa.mode = SPMV_DOT;
   qa.w = p32;
   qa.st = si;
   qa.fin = FIN_CG_ALPHA;
-  qa.partials = S->partials;
-  qa.ticket = &si->ticket[KSLOT + 2];
+  qa.red = S->red;
   S->inner_total = 0;
   const int chunk = chunk_for(A);
   for (;;) {
@@ -797,12 +857,12 @@ void solve_mixed(mtjg_solver S, mtjg_csr A, double *b, double *x) {
     k_demote_scaled<<<g, BS, 0, s>>>(n, r, r32, st);
     k_copy_f32_zero<<<g, BS, 0, s>>>(n, r32, rin, d32);
     state_init(S, si, S->inner_max, S->inner_rtol, 0.0, 1e300);
-    k_cg_resid<float, PK_JACOBI, false><<<g, BS, 0, s>>>(n, d32, p32, rin, q32, inv32, z32, si, S->partials);
+    k_cg_resid<float, PK_JACOBI, false><<<g, BS, 0, s>>>(n, d32, p32, rin, q32, inv32, z32, si, S->red);
     HIPCHK(hipGetLastError());
     run_chunks(S, si, chunk, [&]() {
       k_cg_p<float><<<g, BS, 0, s>>>(n, p32, z32, si);
       spmv_f32(A, p32, q32, qa, s);
-      k_cg_resid<float, PK_JACOBI, true><<<g, BS, 0, s>>>(n, d32, p32, rin, q32, inv32, z32, si, S->partials);
+      k_cg_resid<float, PK_JACOBI, true><<<g, BS, 0, s>>>(n, d32, p32, rin, q32, inv32, z32, si, S->red);
       HIPCHK(hipGetLastError());
     });
     KState ki;
@@ -811,7 +871,7 @@ void solve_mixed(mtjg_solver S, mtjg_csr A, double *b, double *x) {
     // x += ||r|| d; r = b - A x; outer check
     k_promote_add<<<g, BS, 0, s>>>(n, x, d32, st);
     spmv_f64(A, x, r, ra, s);
-    k_norm_check<<<g, BS, 0, s>>>(n, r, st, 1, S->partials);
+    k_norm_check<<<g, BS, 0, s>>>(n, r, st, 1, S->red);
     HIPCHK(hipGetLastError());
   }
 }
@@ -835,7 +895,6 @@ extern "C" int mtjg_solver_create(mtjg_ctx ctx, int kind, mtjg_vec template_vec,
   HIPCHK(hipMemsetAsync(S->st_in, 0, sizeof(KState), ctx->stream));
   HIPCHK(hipHostMalloc(&S->h_st, sizeof(KState) * 2, hipHostMallocDefault));
   HIPCHK(hipHostMalloc(&S->h_slot, sizeof(int32_t) * 2, hipHostMallocDefault));
-  HIPCHK(hipMalloc(&S->partials, sizeof(double) * RED_MAX_K * RED_MAX_BLOCKS));
   HIPCHK(hipEventCreateWithFlags(&S->ev[0], hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&S->ev[1], hipEventDisableTiming));
   // scratch vectors come from template.copy() (S/CG.java:51-56)
@@ -855,7 +914,7 @@ extern "C" int mtjg_solver_destroy(mtjg_solver S) {
   gmres_free(S);
   (void)hipFree(S->st);
   (void)hipFree(S->st_in);
-  (void)hipFree(S->partials);
+  free_red(S->red);
   (void)hipHostFree(S->h_st);
   (void)hipHostFree(S->h_slot);
   (void)hipEventDestroy(S->ev[0]);
@@ -926,6 +985,7 @@ extern "C" int mtjg_solver_solve(mtjg_solver S, mtjg_csr A, mtjg_vec b, mtjg_vec
     case MTJG_IR: solve_ir(S, A, bp, xp); break;
     case MTJG_MIXED_CG: solve_mixed(S, A, bp, xp); break;
   }
+  prec_check(S->M);
   KState k;
   state_read(S, S->st, k);
   int code = status_to_code(k.status);

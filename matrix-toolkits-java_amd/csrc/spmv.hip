@@ -28,6 +28,7 @@ namespace mtjg {
 constexpr int SPMV_CAP = 2048;           // products staged per row-block
 constexpr int SPMV_PER = SPMV_CAP / BS;  // entries per thread per row-block
 constexpr int SPMV_MAX_ROWS = 512;       // rows per row-block (2 per thread)
+constexpr int SPMV_DOT_GRID = 2048;      // workgroups of the fused product
 
 // Row-block schedule: greedy cut so every block holds <= SPMV_CAP entries,
 // except single rows longer than that.  Entry b is (first row, first stored
@@ -75,13 +76,13 @@ __device__ __forceinline__ int pow2_floor(int v) { return 1 << (31 - __clz(v)); 
 template <typename XT, int MODE>
 __device__ __forceinline__ void spmv_store(int row, XT s, XT *__restrict__ y, const XT *y0,
                                            double alpha, double inv_alpha, const XT *w,
-                                           double &acc0, double &acc1) {
+                                           dd &acc0, dd &acc1) {
   if (MODE == SPMV_MULT || MODE == SPMV_DOT) {
     XT v = (s != XT(0)) ? s : XT(0);
     y[row] = v;
     if (MODE == SPMV_DOT) {
-      acc0 += (double)w[row] * (double)v;
-      acc1 += (double)v * (double)v;
+      dd_fma(acc0, (double)w[row], (double)v);
+      dd_fma(acc1, (double)v, (double)v);
     }
   } else if (MODE == SPMV_MULTADD) {
     y[row] = y0[row] + (XT)alpha * s;
@@ -128,7 +129,7 @@ __device__ __forceinline__ void spmv_rowblock(int b, XT *prod, const int2 *__res
                                               const int32_t *__restrict__ ci,
                                               const VT *__restrict__ val, const XT *__restrict__ x,
                                               XT *y, const XT *y0, double alpha, double inv_alpha,
-                                              const XT *w, double &acc0, double &acc1) {
+                                              const XT *w, dd &acc0, dd &acc1) {
   constexpr int RPER = SPMV_MAX_ROWS / BS;
   {
     const int2 s0 = info[b], s1 = info[b + 1];
@@ -226,34 +227,36 @@ __global__ __launch_bounds__(BS) void k_spmv(int nrb, int slabs, const int2 *__r
   const int per = (nrb + slabs - 1) / slabs;
   const int b = ticket_to_block(blockIdx.x, nrb, slabs, per);
   if (b < 0) return;
-  double acc0 = 0.0, acc1 = 0.0;
+  dd acc0{0.0, 0.0}, acc1{0.0, 0.0};
   spmv_rowblock<VT, XT, MODE>(b, prod, info, rp, ci, val, x, y, y0, alpha, inv_alpha, nullptr, acc0,
                               acc1);
 }
 
-// Fused solver product q = A p with the grid sums w.q and q.q: a bounded grid
-// walks the row-block tickets, and the last workgroup runs the solver
-// epilogue.  Returns at once when the solve has left the running state.
+// Fused solver product q = A p with the grid sums w.q and q.q: at most
+// SPMV_DOT_GRID co-resident workgroups walk the row-block tickets, so only
+// that many take a reduction ticket; a two-level deterministic reduction
+// ends in the solver epilogue in the last workgroup.  Every workgroup returns
+// at once when the solve has left the running state.
 template <typename VT, typename XT>
 __global__ __launch_bounds__(BS) void k_spmv_dot(int nrb, int slabs, const int2 *__restrict__ info,
                                                  const int32_t *__restrict__ rp,
                                                  const int32_t *__restrict__ ci,
                                                  const VT *__restrict__ val, const XT *__restrict__ x,
-                                                 XT *y, const XT *w, KState *st, int fin,
-                                                 double *partials, uint32_t *ticket) {
+                                                 XT *y, const XT *w, KState *st, int fin, GridRed red) {
   if (st->status != ST_RUNNING) return;
   __shared__ XT prod[SPMV_CAP];
-  double acc0 = 0.0, acc1 = 0.0;
+  dd acc0{0.0, 0.0}, acc1{0.0, 0.0};
   const int per = (nrb + slabs - 1) / slabs;
   const int ntickets = per * slabs;
   for (int t = blockIdx.x; t < ntickets; t += gridDim.x) {
     const int b = ticket_to_block(t, nrb, slabs, per);
-    if (b < 0) continue;
-    spmv_rowblock<VT, XT, SPMV_DOT>(b, prod, info, rp, ci, val, x, y, nullptr, 1.0, 1.0, w, acc0, acc1);
-    __syncthreads();
+    if (b >= 0)
+      spmv_rowblock<VT, XT, SPMV_DOT>(b, prod, info, rp, ci, val, x, y, nullptr, 1.0, 1.0, w, acc0, acc1);
+    __syncthreads();  // prod is reused by the next row-block
   }
-  double v[2] = {acc0, acc1};
-  if (grid_sum<2>(v, partials, ticket) && threadIdx.x == 0) spmv_finish(st, fin, v[0], v[1]);
+  dd v[2] = {acc0, acc1};
+  double tot[2];
+  if (grid_sum_dd<2>(v, red, tot) && threadIdx.x == 0) spmv_finish(st, fin, tot[0], tot[1]);
 }
 
 template <typename VT, typename XT>
@@ -263,12 +266,9 @@ static void spmv_launch(mtjg_csr A, const VT *val, const XT *x, XT *y, const Spm
     if (a.mode == SPMV_DOT) fail(MTJG_E_INVALID_ARGUMENT, "empty matrix in fused product");
     return;
   }
-  // one workgroup per row-block ticket; fused reductions use at most
-  // RED_MAX_BLOCKS workgroups (a multiple of the slab count) in grid-stride
+  // one workgroup per row-block ticket
   const int per = (A->nrb + A->slabs - 1) / A->slabs;
-  int grid = per * A->slabs;
-  if (a.mode == SPMV_DOT) grid = std::min(grid, RED_MAX_BLOCKS);
-  else grid = std::min(grid, 1 << 24);
+  const int grid = (a.mode == SPMV_DOT) ? std::min(per * A->slabs, SPMV_DOT_GRID) : per * A->slabs;
   const XT *y0 = (const XT *)a.y0;
   const XT *w = (const XT *)a.w;
   const int sl = A->slabs;
@@ -286,8 +286,9 @@ static void spmv_launch(mtjg_csr A, const VT *val, const XT *x, XT *y, const Spm
                                                        y0, a.alpha, a.inv_alpha);
       break;
     case SPMV_DOT:
+      if (grid > a.red.cap) fail(MTJG_E_INVALID_ARGUMENT, "reduction workspace smaller than the grid");
       k_spmv_dot<VT, XT><<<grid, BS, 0, s>>>(A->nrb, sl, A->rb, A->rp, A->ci, val, x, y, w, a.st,
-                                             a.fin, a.partials, a.ticket);
+                                             a.fin, a.red);
       break;
     default:
       fail(MTJG_E_INVALID_ARGUMENT, "bad spmv mode");

@@ -103,13 +103,12 @@ __global__ __launch_bounds__(BS) void k_maxabs(int64_t n, const T *x, double *pa
   }
   if (threadIdx.x == 0) {
     store_agent(&partials[blockIdx.x], sh[0]);
-    __atomic_thread_fence(__ATOMIC_RELEASE);
-    uint32_t t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    stores_complete();
+    uint32_t t = take_ticket(ticket);
     last = (t == gridDim.x - 1);
   }
   __syncthreads();
   if (!last) return;
-  __atomic_thread_fence(__ATOMIC_ACQUIRE);
   if (threadIdx.x == 0) {
     double r = 0.0;
     for (int i = 0; i < (int)gridDim.x; ++i) r = nan_max(load_agent(&partials[i]), r);
@@ -170,13 +169,12 @@ __global__ __launch_bounds__(BS) void k_norm2_robust(int64_t n, const T *x, doub
   if (threadIdx.x == 0) {
     store_agent(&partials[blockIdx.x], shs[0]);
     store_agent(&partials[RED_MAX_BLOCKS + blockIdx.x], shq[0]);
-    __atomic_thread_fence(__ATOMIC_RELEASE);
-    uint32_t t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    stores_complete();
+    uint32_t t = take_ticket(ticket);
     last = (t == gridDim.x - 1);
   }
   __syncthreads();
   if (!last) return;
-  __atomic_thread_fence(__ATOMIC_ACQUIRE);
   if (threadIdx.x == 0) {
     double sa = 0, qa = 0;
     for (int i = 0; i < (int)gridDim.x; ++i)

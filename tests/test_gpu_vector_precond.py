@@ -87,6 +87,24 @@ def test_ilu_apply_bitwise(ctx, name):
     np.testing.assert_array_equal(Mg.apply(b, M.DenseVector(n, ctx)).getData(), Mo2.apply(bh))
 
 
+@pytest.mark.parametrize("name", ["convdiff_16", "p2d_64"])
+def test_ilu_single_launch_solves_bitwise(ctx, monkeypatch, name):
+    """The opt-in single-launch triangular solves (dependency values polled
+    in place) give the same bits as the per-level launches and the oracle."""
+    monkeypatch.setenv("MTJG_ILU_SYNCFREE", "1")
+    sysm = P.convdiff3d(16, seed=2) if name == "convdiff_16" else P.poisson2d(64)
+    n = sysm[0]
+    A = M.CompRowMatrix(*sysm, ctx=ctx)
+    Mg = M.ILU(A)
+    Mg.setMatrix(A)
+    Mo = oracle.Precond(oracle.PREC_ILU, oracle.Csr(*sysm))
+    bh = P.uniform(n, 4)
+    b = M.DenseVector(bh, ctx)
+    for _ in range(3):
+        np.testing.assert_array_equal(Mg.apply(b, M.DenseVector(n, ctx)).getData(), Mo.apply(bh))
+        np.testing.assert_array_equal(Mg.transApply(b, M.DenseVector(n, ctx)).getData(), Mo.trans_apply(bh))
+
+
 def test_ilu_errors(ctx):
     m = sp.csr_matrix(np.array([[1.0, 1.0], [1.0, 0.0]]))
     m.eliminate_zeros()
