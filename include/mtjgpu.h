@@ -234,6 +234,45 @@ int mtjg_mixed_set_inner(mtjg_solver S, double inner_rtol,
 int mtjg_solver_solve(mtjg_solver S, mtjg_csr A, mtjg_vec b, mtjg_vec x,
                       mtjg_result *result);
 
+/* ---- Batched serving of many systems (no MTJ counterpart) ------------------ */
+/* A batch solves many independent SPD systems with mixed-precision CG (FP64
+ * residual r = b - A x with the FP64 values, FP32 inner Jacobi-CG on r/||r||,
+ * FP64 update of x), all active systems advancing together in the same
+ * launches.  Requests queue on the host and are admitted between steps while
+ * a slot, work-table room (ceil(n / rows_per_workgroup) workgroups per
+ * system) and arena memory are free; finished systems retire after each
+ * step.  A system's result bits depend only on the system and
+ * rows_per_workgroup, not on the rest of the batch.  The outer loop follows
+ * DefaultIterationMonitor (rtol relative to ||b||, atol 1e-50, dtol 1e5,
+ * max_outer refinement steps). */
+typedef struct mtjg_batch_s *mtjg_batch;
+
+/* arena_bytes of device memory hold every admitted system (CSR in FP64 and
+ * FP32, 3 FP64 and 6 FP32 vectors); each step runs iters_per_step batch
+ * iterations from one captured hipGraph. */
+int mtjg_batch_create(mtjg_ctx ctx, int32_t max_slots, int32_t max_workgroups,
+                      int64_t arena_bytes, int32_t rows_per_workgroup,
+                      int32_t iters_per_step, mtjg_batch *out);
+int mtjg_batch_destroy(mtjg_batch B);
+/* Queues A x = b (CSR copied on the host; x0 = 0); *id names the request. */
+int mtjg_batch_submit(mtjg_batch B, int32_t n, int64_t nnz,
+                      const int32_t *row_pointer, const int32_t *column_index,
+                      const double *data, const double *b, double rtol,
+                      int32_t max_outer, double inner_rtol, int32_t max_inner,
+                      int64_t *id);
+/* Admits what fits, runs one step, retires finished systems. */
+int mtjg_batch_step(mtjg_batch B, int32_t *n_active, int32_t *n_completed);
+/* Collects a finished request (x may be NULL); returns its status
+ * (MTJG_OK or MTJG_NOT_CONVERGED_*); iterations = refinement steps. */
+int mtjg_batch_result(mtjg_batch B, int64_t id, double *x, mtjg_result *result,
+                      int32_t *inner_iterations);
+/* Ids of requests retired since the last call (at most max_ids; the rest
+ * stay reported for the next call). */
+int mtjg_batch_completed(mtjg_batch B, int64_t *ids, int32_t max_ids,
+                         int32_t *count);
+int mtjg_batch_info(mtjg_batch B, int32_t *queued, int32_t *active,
+                    int64_t *arena_used, int64_t *arena_size);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,8 +50,6 @@ struct mtjg_solver_s {
 
 namespace {
 
-constexpr int KSLOT = 0;  // ticket slots inside KState
-
 // Vector kernels split the vector into tiles of BS * VPT elements; a thread
 // owns elements tile + threadIdx.x + u * BS (unit stride across the wave, all
 // loads of a thread independent).  The grid holds at most VGRID_MAX

@@ -103,6 +103,13 @@ PROTOTYPES = [
     ("mtjg_gmres_set_restart", C.c_int, [P, I32]),
     ("mtjg_mixed_set_inner", C.c_int, [P, D, I32]),
     ("mtjg_solver_solve", C.c_int, [P, P, P, P, C.POINTER(Result)]),
+    ("mtjg_batch_create", C.c_int, [P, I32, I32, I64, I32, I32, PP]),
+    ("mtjg_batch_destroy", C.c_int, [P]),
+    ("mtjg_batch_submit", C.c_int, [P, I32, I64, P, P, P, P, D, I32, D, I32, PI64]),
+    ("mtjg_batch_step", C.c_int, [P, PI32, PI32]),
+    ("mtjg_batch_result", C.c_int, [P, I64, P, C.POINTER(Result), PI32]),
+    ("mtjg_batch_completed", C.c_int, [P, PI64, I32, PI32]),
+    ("mtjg_batch_info", C.c_int, [P, PI32, PI32, PI64, PI64]),
 ]
 
 _lib = None

@@ -1,0 +1,115 @@
+"""GPU tests of the batched, continuously admitted mixed-precision CG engine
+(mtjg_batch_*, the serving path).
+
+Bars: every system reaches the requested FP64 relative residual
+||b - A x|| / ||b|| <= 1e-8 (the outer monitor is relative to ||b|| with
+x0 = 0); a system's solution is bitwise the same alone and in a mix of other
+systems; requests that do not fit the memory budget wait and are solved
+later; a broken system (NaN right-hand side) ends in divergence without
+affecting the others.  The FP64 Jacobi-CG oracle solution is the accuracy
+reference (same tolerance)."""
+import numpy as np
+import pytest
+
+import mtjgpu as M
+import oracle
+from mtjgpu import problems as P
+from mtjgpu import serving as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _systems(count, seed=3, n_max=40000):
+    reqs = S.request_stream(count, n_min=1000, n_max=n_max, seed=seed)
+    return reqs, [S.make_system(r) for r in reqs]
+
+
+def _run(ctx, systems, **kw):
+    B = S.BatchServer(ctx, **kw)
+    try:
+        ids = {B.submit(*sysm[:4], sysm[4], rtol=1e-8): k for k, sysm in enumerate(systems)}
+        out = {}
+        for _ in range(100000):
+            B.step()
+            for rid in B.completed():
+                k = ids[rid]
+                out[k] = B.result(rid, systems[k][0])
+            if len(out) == len(systems):
+                break
+        return out
+    finally:
+        B.close()
+
+
+def test_batch_reaches_fp64_residual(ctx):
+    reqs, systems = _systems(12)
+    out = _run(ctx, systems, max_slots=64, max_workgroups=4096, arena_bytes=1 << 30)
+    assert len(out) == len(systems)
+    for k, (x, code, outer, inner, res) in out.items():
+        assert code == 0, (k, code)
+        assert S.true_residual(systems[k], x) <= 1e-8
+        assert inner > 0 and outer >= 1
+
+
+def test_same_alone_and_in_mix(ctx):
+    reqs, systems = _systems(10, seed=11)
+    mix = _run(ctx, systems, max_slots=64, max_workgroups=4096, arena_bytes=1 << 30)
+    for k in (0, 4, 9):
+        alone = _run(ctx, [systems[k]], max_slots=4, max_workgroups=4096, arena_bytes=1 << 30)
+        np.testing.assert_array_equal(alone[0][0], mix[k][0])
+        assert alone[0][2:4] == mix[k][2:4]
+
+
+def test_matches_fp64_oracle_solution(ctx):
+    reqs, systems = _systems(3, seed=5, n_max=6000)
+    out = _run(ctx, systems, max_slots=8, max_workgroups=512, arena_bytes=1 << 28)
+    for k, sysm in enumerate(systems):
+        n, rp, ci, va, b = sysm
+        O = oracle.Csr(n, n, rp, ci, va)
+        code, xo, it, _ = oracle.solve("cg", O, b, np.zeros(n), oracle.Precond(oracle.PREC_DIAGONAL, O),
+                                       oracle.monitor(100000, 1e-10))
+        assert code == 0
+        xg = out[k][0]
+        assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-5
+
+
+def test_admission_waits_for_memory(ctx):
+    reqs, systems = _systems(8, seed=21, n_max=20000)
+    need = max(S_bytes(s) for s in systems)
+    # room for about two systems at a time: the rest queue and are admitted
+    # as earlier ones retire
+    out = _run(ctx, systems, max_slots=64, max_workgroups=4096, arena_bytes=int(2.2 * need))
+    assert len(out) == len(systems)
+    for k, (x, code, *_r) in out.items():
+        assert code == 0
+        assert S.true_residual(systems[k], x) <= 1e-8
+
+
+def S_bytes(sysm):
+    n, rp, ci, va, b = sysm
+    nnz = len(ci)
+    al = lambda v: (v + 255) // 256 * 256
+    return al(4 * (n + 1)) + al(4 * nnz) + al(8 * nnz) + al(4 * nnz) + al(4 * n) + 3 * al(8 * n) + 5 * al(4 * n)
+
+
+def test_nan_system_is_isolated(ctx):
+    reqs, systems = _systems(4, seed=31, n_max=8000)
+    n, rp, ci, va, b = systems[2]
+    bad = b.copy()
+    bad[n // 2] = np.nan
+    systems[2] = (n, rp, ci, va, bad)
+    out = _run(ctx, systems, max_slots=16, max_workgroups=1024, arena_bytes=1 << 28)
+    assert out[2][1] == M.Reason.Divergence.value
+    for k in (0, 1, 3):
+        assert out[k][1] == 0 and S.true_residual(systems[k], out[k][0]) <= 1e-8
+
+
+def test_serving_loops_small_stream(ctx):
+    reqs, systems = _systems(16, seed=41, n_max=20000)
+    st_b, res_b = S.serve_batched(ctx, reqs, systems, max_slots=64, max_workgroups=4096, arena_bytes=1 << 30)
+    st_p, res_p = S.serve_plain(ctx, reqs, systems)
+    assert st_b["solves"] == st_p["solves"] == 16
+    for k in range(16):
+        assert res_b[k]["code"] == 0 and res_p[k]["code"] == 0
+        assert S.true_residual(systems[k], res_b[k]["x"]) <= 1e-8
+        assert S.true_residual(systems[k], res_p[k]["x"]) <= 1e-8
