@@ -222,6 +222,59 @@ def serve_mode(count, lam, seed):
     return 0
 
 
+def mmload_mode(grid, repeats, cpu_seconds):
+    """Matrix Market input: the 3D 7-point Poisson matrix grid^3 written in
+    MTJ's MatrixVectorWriter layout to a temporary file, then read into a
+    device CompRowMatrix (parse + CSR assembly + upload + SpMV schedule).
+    The CPU baseline is the oracle's single-threaded restatement of
+    CompRowMatrix(MatrixVectorReader) on a leading slice of the same file
+    (its size line rewritten to the slice's entry count)."""
+    import tempfile
+    import mtjgpu as M
+    from mtjgpu import matrix_market as MM
+    from mtjgpu import problems as P
+    n, _, rp, ci, va = P.poisson3d(grid)
+    ctx = M.Context(0)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "poisson3d_%d.mtx" % grid)
+        P.write_matrix_market(path, n, n, rp, ci, va)
+        size = os.path.getsize(path)
+        best = None
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            A = MM.load_matrix(path, ctx)
+            ctx.synchronize()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+            A.close()
+        t0 = time.perf_counter()
+        MM.read_matrix(path)
+        host_dt = time.perf_counter() - t0
+        out = {"mode": "mmload", "workload": "poisson3d_%d" % grid, "file_bytes": size, "nnz": int(ci.size),
+               "best_s_to_device": round(best, 4), "MB_per_s_to_device": round(size / best / 1e6, 1),
+               "host_parse_s": round(host_dt, 4), "threads": min(os.cpu_count() or 1, 16)}
+        if cpu_seconds > 0:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle
+            with open(path, "rb") as f:
+                head = f.readline()
+                f.readline()
+                # about 19 MB/s for the restatement: size the slice to the budget
+                want = int(cpu_seconds * 18e6)
+                body = f.read(want)
+            body = body[:body.rfind(b"\n") + 1]
+            cnt = body.count(b"\n")
+            text = head + b"%10d %10d %19d\n" % (n, n, cnt) + body
+            t0 = time.perf_counter()
+            code = oracle.mm_matrix(text)[0]
+            dt = time.perf_counter() - t0
+            assert code == oracle.OK
+            out["cpu_baseline"] = {"value": round(len(text) / dt / 1e6, 2), "unit": "MB/s", "cores": 1,
+                                   "kind": "port", "sample": "first %d entries of the same file" % cnt}
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,7 +283,8 @@ def main():
     ap.add_argument("--workload", default="poisson3d_256")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", choices=["spmv", "copy", "solve", "serve"], default="spmv")
+    ap.add_argument("--mode", choices=["spmv", "copy", "solve", "serve", "mmload"], default="spmv")
+    ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--lam", type=float, default=2000.0)
     ap.add_argument("--seed", type=int, default=7)
@@ -247,6 +301,8 @@ def main():
         return solve_mode(args.solve, args.repeats)
     if args.mode == "serve":
         return serve_mode(args.requests, args.lam, args.seed)
+    if args.mode == "mmload":
+        return mmload_mode(args.grid, args.repeats, 0.0 if args.no_cpu else args.cpu_seconds)
 
     import mtjgpu as M
     from mtjgpu import problems as P

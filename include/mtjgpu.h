@@ -63,7 +63,8 @@ enum {
     MTJG_E_UNSUPPORTED = -7,
     MTJG_E_OUT_OF_MEMORY = -8,
     MTJG_E_DEVICE = -9,
-    MTJG_E_NULL = -10
+    MTJG_E_NULL = -10,
+    MTJG_E_IO = -11             /* IOException / EOFException (unreadable file, bad header, early end) */
 };
 
 /* Vector element types. */
@@ -272,6 +273,46 @@ int mtjg_batch_completed(mtjg_batch B, int64_t *ids, int32_t max_ids,
                          int32_t *count);
 int mtjg_batch_info(mtjg_batch B, int32_t *queued, int32_t *active,
                     int64_t *arena_used, int64_t *arena_size);
+
+/* ------------------------------------------------------------------------
+ * Matrix Market input (host side).  A parsed file is held in an mtjg_mm
+ * handle: a CSR matrix exactly as CompRowMatrix(MatrixVectorReader) builds it
+ * (S/CompRowMatrix.java:86-166: sorted unique columns per row, last duplicate
+ * wins, symmetric / skew-symmetric files mirrored after the direct entries),
+ * or a dense vector as DenseVector(MatrixVectorReader) builds it
+ * (M/DenseVector.java:52-96).  Header, tokenizer and number rules follow
+ * MatrixVectorReader (M/io/MatrixVectorReader.java:68-87, :121-211, :606-657).
+ * Errors: MTJG_E_IO (IOException/EOFException), MTJG_E_UNSUPPORTED (pattern,
+ * dense or complex matrix; pattern or complex vector), MTJG_E_INVALID_ARGUMENT
+ * (number format, MatrixInfo validation, column index outside the matrix),
+ * MTJG_E_INDEX_OUT_OF_BOUNDS (row index outside the matrix).
+ * ---------------------------------------------------------------------- */
+
+typedef struct mtjg_mm_s *mtjg_mm;
+
+/* MatrixInfo.MatrixSymmetry of a parsed matrix. */
+enum { MTJG_MM_GENERAL = 0, MTJG_MM_SYMMETRIC = 1, MTJG_MM_SKEW_SYMMETRIC = 2 };
+
+/* new CompRowMatrix(new MatrixVectorReader(new FileReader(path))). */
+int mtjg_mm_read_matrix(const char *path, mtjg_mm *out);
+/* The same from text already in memory. */
+int mtjg_mm_parse_matrix(const char *text, int64_t length, mtjg_mm *out);
+/* new DenseVector(new MatrixVectorReader(new FileReader(path))). */
+int mtjg_mm_read_vector(const char *path, mtjg_mm *out);
+int mtjg_mm_parse_vector(const char *text, int64_t length, mtjg_mm *out);
+/* numRows(), numColumns(), stored entries (vector: entries read), symmetry. */
+int mtjg_mm_shape(mtjg_mm m, int32_t *rows, int32_t *cols, int64_t *nnz,
+                  int32_t *symmetry);
+/* getRowPointers(), getColumnIndices(), getData() into caller arrays of
+ * rows+1, nnz and nnz elements (any may be null). */
+int mtjg_mm_copy_csr(mtjg_mm m, int32_t *row_pointer, int32_t *column_index,
+                     double *data);
+/* getData() of a parsed vector into x[rows]. */
+int mtjg_mm_copy_vector(mtjg_mm m, double *x);
+/* Device CompRowMatrix / DenseVector from a parsed file. */
+int mtjg_mm_to_csr(mtjg_ctx ctx, mtjg_mm m, mtjg_csr *out);
+int mtjg_mm_to_vec(mtjg_ctx ctx, mtjg_mm m, mtjg_vec *out);
+int mtjg_mm_destroy(mtjg_mm m);
 
 #ifdef __cplusplus
 }

@@ -16,6 +16,7 @@
  * (S/CompRowMatrix.java:178-208).
  */
 #include <math.h>
+#include <stdio.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -236,4 +237,27 @@ void mtjp_powerlaw_fill(int32_t n, const int32_t *len, uint64_t seed,
         val[base + pos] = 1.0 + sum;
         rp[i + 1] = base + L + 1;
     }
+}
+
+/* Writes a CSR matrix as a Matrix Market coordinate file in the layout of
+ * MTJ's MatrixVectorWriter (M/io/MatrixVectorWriter.java:89-116, :249-263):
+ * MatrixInfo header, "%10d %10d %19d" size line, "%10d %10d % .12e" entries,
+ * 1-based indices.  With lower_only, only entries on or below the diagonal
+ * are written and the header says "symmetric".  Returns 0 on success. */
+int mtjp_write_mm(const char *path, int32_t nr, int32_t nc, const int32_t *rp,
+                  const int32_t *ci, const double *val, int lower_only) {
+    FILE *f = fopen(path, "w");
+    if (!f) return -1;
+    static char buf[1 << 20];
+    setvbuf(f, buf, _IOFBF, sizeof buf);
+    int64_t count = 0;
+    for (int32_t i = 0; i < nr; ++i)
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k)
+            if (!lower_only || ci[k] <= i) count++;
+    fprintf(f, "%%%%MatrixMarket matrix coordinate real %s\n", lower_only ? "symmetric" : "general");
+    fprintf(f, "%10d %10d %19lld\n", nr, nc, (long long)count);
+    for (int32_t i = 0; i < nr; ++i)
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k)
+            if (!lower_only || ci[k] <= i) fprintf(f, "%10d %10d % .12e\n", i + 1, ci[k] + 1, val[k]);
+    return fclose(f) == 0 ? 0 : -1;
 }

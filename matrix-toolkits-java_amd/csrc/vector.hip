@@ -259,11 +259,12 @@ static const char *kErr[] = {"",
                              "unsupported operation",
                              "out of memory",
                              "device error",
-                             "null handle"};
+                             "null handle",
+                             "input/output or format error"};
 
 extern "C" const char *mtjg_strerror(int code) {
   if (code >= 0 && code <= 3) return kStatus[code];
-  if (code < 0 && code >= -10) return kErr[-code];
+  if (code < 0 && code >= -11) return kErr[-code];
   return "unknown status";
 }
 

@@ -52,6 +52,10 @@ class UnsupportedOperationException(MtjgError):
     pass
 
 
+class IOException(MtjgError, OSError):
+    """java.io.IOException / EOFException (unreadable or malformed input)."""
+
+
 class MatrixSingularException(MtjgError):
     pass
 
@@ -117,6 +121,8 @@ def _raise(code, row=None):
         raise MatrixSingularException(code, msg)
     if code == L.E_UNSUPPORTED:
         raise UnsupportedOperationException(code, msg)
+    if code == L.E_IO:
+        raise IOException(code, msg)
     if code == L.E_ZERO_DIAGONAL:
         raise ZeroDiagonalException(code, msg, row)
     if code == L.E_MISSING_DIAGONAL:

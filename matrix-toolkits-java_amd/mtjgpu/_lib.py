@@ -26,6 +26,7 @@ E_UNSUPPORTED = -7
 E_OUT_OF_MEMORY = -8
 E_DEVICE = -9
 E_NULL = -10
+E_IO = -11
 
 F64, F32 = 0, 1
 NORM_ONE, NORM_TWO, NORM_TWO_ROBUST, NORM_INFINITY = 0, 1, 2, 3
@@ -110,6 +111,16 @@ PROTOTYPES = [
     ("mtjg_batch_result", C.c_int, [P, I64, P, C.POINTER(Result), PI32]),
     ("mtjg_batch_completed", C.c_int, [P, PI64, I32, PI32]),
     ("mtjg_batch_info", C.c_int, [P, PI32, PI32, PI64, PI64]),
+    ("mtjg_mm_read_matrix", C.c_int, [C.c_char_p, PP]),
+    ("mtjg_mm_parse_matrix", C.c_int, [C.c_char_p, I64, PP]),
+    ("mtjg_mm_read_vector", C.c_int, [C.c_char_p, PP]),
+    ("mtjg_mm_parse_vector", C.c_int, [C.c_char_p, I64, PP]),
+    ("mtjg_mm_shape", C.c_int, [P, PI32, PI32, PI64, PI32]),
+    ("mtjg_mm_copy_csr", C.c_int, [P, P, P, P]),
+    ("mtjg_mm_copy_vector", C.c_int, [P, P]),
+    ("mtjg_mm_to_csr", C.c_int, [P, P, PP]),
+    ("mtjg_mm_to_vec", C.c_int, [P, P, PP]),
+    ("mtjg_mm_destroy", C.c_int, [P]),
 ]
 
 _lib = None

@@ -35,6 +35,7 @@ def _load():
         lib.mtjp_powerlaw_lengths.restype = C.c_int64
         lib.mtjp_powerlaw_lengths.argtypes = [C.c_int32, C.c_double, C.c_int32, C.c_uint64, P]
         lib.mtjp_powerlaw_fill.argtypes = [C.c_int32, P, C.c_uint64, P, P, P]
+        lib.mtjp_write_mm.argtypes = [C.c_char_p, C.c_int32, C.c_int32, P, P, P, C.c_int]
         _lib = lib
     return _lib
 
@@ -115,3 +116,14 @@ def spmv_bytes(n_rows, nnz, value_bytes=8, index_bytes=4):
     """Algorithmic HBM bytes of one y = A x: values and column indices once,
     row pointers once, x read once, y written once."""
     return (value_bytes + index_bytes) * nnz + index_bytes * (n_rows + 1) + 2 * value_bytes * n_rows
+
+
+def write_matrix_market(path, n_rows, n_cols, rp, ci, va, lower_only=False):
+    """Writes CSR arrays as a Matrix Market file in MTJ's MatrixVectorWriter
+    layout (lower triangle with a "symmetric" header if lower_only)."""
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    va = np.ascontiguousarray(va, np.float64)
+    if _load().mtjp_write_mm(os.fsencode(path), int(n_rows), int(n_cols), rp.ctypes.data,
+                             ci.ctypes.data, va.ctypes.data, int(bool(lower_only))) != 0:
+        raise OSError("cannot write %s" % path)

@@ -40,7 +40,10 @@ enum {
     OC_MISSING_DIAGONAL = 11, /* S/ILU.java:134-136 */
     OC_ZERO_PIVOT = 12,     /* S/ILU.java:105-107 */
     OC_SINGULAR = 13,       /* MatrixSingularException from dtrtrs */
-    OC_BAD_ARG = 20
+    OC_BAD_ARG = 20,        /* IllegalArgumentException (incl. NumberFormatException) */
+    OC_IO = 21,             /* IOException / EOFException of MatrixVectorReader */
+    OC_UNSUPPORTED = 22,    /* UnsupportedOperationException */
+    OC_INDEX = 23           /* IndexOutOfBoundsException */
 };
 
 /* ---- CompRowMatrix (S/CompRowMatrix.java) -------------------------------- */
@@ -152,6 +155,20 @@ int oc_ir(int n, const int32_t *rp, const int32_t *ci, const double *a,
 
 /* ---- timing helper for the CPU baseline ------------------------------------ */
 double oc_now(void);
+
+/* Matrix Market input (mtj_oracle_mm.c): CompRowMatrix(MatrixVectorReader)
+ * (S/CompRowMatrix.java:86-166) and DenseVector(MatrixVectorReader)
+ * (M/DenseVector.java:52-96) on text in memory.  For a vector, val holds the
+ * dense entries and rp/ci are null. */
+typedef struct {
+    int32_t rows, cols, symmetry;
+    long nnz;
+    int32_t *rp, *ci;
+    double *val;
+} oc_mm;
+int oc_mm_matrix(const char *text, long len, oc_mm *out);
+int oc_mm_vector(const char *text, long len, oc_mm *out);
+void oc_mm_free(oc_mm *m);
 
 #ifdef __cplusplus
 }

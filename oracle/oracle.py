@@ -219,3 +219,41 @@ def triu_solve(H, k, s):
     s = np.array(s, dtype=float)
     code = load().oc_triu_solve(k, H.ctypes.data, H.shape[0], s.ctypes.data)
     return code, s
+
+
+class _MM(C.Structure):
+    _fields_ = [("rows", C.c_int32), ("cols", C.c_int32), ("symmetry", C.c_int32), ("nnz", C.c_long),
+                ("rp", C.POINTER(C.c_int32)), ("ci", C.POINTER(C.c_int32)), ("val", C.POINTER(C.c_double))]
+
+
+IO, UNSUPPORTED, INDEX = 21, 22, 23
+
+
+def mm_matrix(text):
+    """CompRowMatrix(MatrixVectorReader) on `text` (bytes).  Returns
+    (code, rows, cols, rp, ci, val, symmetry); arrays are None on error."""
+    lib = load()
+    m = _MM()
+    code = lib.oc_mm_matrix(text, C.c_long(len(text)), C.byref(m))
+    if code != OK:
+        return code, None, None, None, None, None, None
+    try:
+        rp = np.ctypeslib.as_array(m.rp, (m.rows + 1,)).copy()
+        ci = np.ctypeslib.as_array(m.ci, (max(m.nnz, 1),))[:m.nnz].copy()
+        va = np.ctypeslib.as_array(m.val, (max(m.nnz, 1),))[:m.nnz].copy()
+        return code, m.rows, m.cols, rp, ci, va, m.symmetry
+    finally:
+        lib.oc_mm_free(C.byref(m))
+
+
+def mm_vector(text):
+    """DenseVector(MatrixVectorReader) on `text` (bytes): (code, x)."""
+    lib = load()
+    m = _MM()
+    code = lib.oc_mm_vector(text, C.c_long(len(text)), C.byref(m))
+    if code != OK:
+        return code, None
+    try:
+        return code, np.ctypeslib.as_array(m.val, (max(m.rows, 1),))[:m.rows].copy()
+    finally:
+        lib.oc_mm_free(C.byref(m))
