@@ -4,9 +4,11 @@
 // inner Jacobi-CG on r / ||r|| to a relative tolerance, FP64 update
 // x += ||r|| d), but every active system advances in the same launches.
 //
-// Work layout: system s is cut into nb_s row tiles of rows_per_wg rows; a
-// device work table maps each workgroup of a launch to (slot, tile).  One
-// batch iteration is three launches over the whole table:
+// Work layout: system s is cut into nb_s row tiles of rows_per_wg rows, and
+// each tile into SpMV row-blocks (spmv_rowblock.hpp) that never cross a tile
+// boundary; a device work table maps each workgroup of a launch to
+// (slot, tile, the slot's partial-sum offset).  One batch iteration is three
+// launches over the whole table:
 //   B1  phase INNER: q = A32 p and p.q per slot -> alpha
 //       phase OUTER: r = b - A64 x (MTJ's multAdd order) and ||r||, then the
 //                    DefaultIterationMonitor check of the outer loop
@@ -17,12 +19,18 @@
 // Per-slot sums are fixed-order compensated reductions whose last workgroup
 // (per-slot ticket) runs the slot's scalar epilogue, so a slot's result bits
 // depend only on its own system and tile size, never on what else is in the
-// batch.  Between steps the host retires finished slots and admits queued
-// requests while slots, work-table entries and arena memory are free; each
-// step replays a hipGraph of `iters` batch iterations captured once per grid
-// size.
+// batch.
+//
+// Host pipeline: each mtjg_batch_step launches a hipGraph of `iters` batch
+// iterations (captured once per grid size) followed by a snapshot of the
+// slot table into pinned memory, and only then retires the systems the
+// previous step's snapshot shows finished and admits queued requests, so the
+// host work overlaps the running graph.  A finished slot is inert (its
+// kernels return at once), so the running graph may still list it; new
+// systems are uploaded on a second stream into free arena ranges, and their
+// slot records and the new work table are copied on the main stream after
+// the running graph, before the next one.
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <deque>
@@ -31,6 +39,7 @@
 
 #include "mtjg_internal.hpp"
 #include "reduce.hpp"
+#include "spmv_rowblock.hpp"
 
 namespace mtjg {
 namespace {
@@ -46,20 +55,22 @@ struct BSlot {
   const float *v32;
   const float *inv;
   const double *b;
+  const int2 *info;        // row-blocks (first row, first entry), tile-aligned
+  const int32_t *tile_blk; // first row-block of each tile, nb + 1 entries
   double *x, *r;
   float *rr, *d, *p, *q, *z;
-  int32_t n, nb, rows_per_wg, part_off;
+  int64_t rid;             // request id (host bookkeeping)
+  int32_t n, nb, rows_per_wg;
   // control
   int32_t active, status, phase, action3;
   int32_t outer, inner, inner_total, max_outer, max_inner;
   double rtol, atol, dtol, init_r, rnorm;
   double rho, alpha, beta, inner_init, inner_rtol;
   uint32_t ticket;
-  int32_t pad;
 };
 
 struct BWork {
-  int32_t slot, tile;
+  int32_t slot, tile, part_off, pad;
 };
 
 struct BDev {
@@ -75,14 +86,15 @@ __device__ __forceinline__ bool slot_live(const BSlot &S) { return S.active && S
 // Per-slot two-sum reduction; returns true in the slot's last workgroup with
 // the totals in out[].
 template <int K>
-__device__ bool slot_sum(dd (&v)[K], BSlot &S, int tile, const BDev &d, double (&out)[K]) {
+__device__ bool slot_sum(dd (&v)[K], BSlot &S, const BWork &w, const BDev &d, double (&out)[K]) {
   __shared__ int last;
   block_sum_dd<K>(v);
+  const int base = w.part_off;
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      store_agent(&d.partials[(2 * k) * d.cap + S.part_off + tile], v[k].hi);
-      store_agent(&d.partials[(2 * k + 1) * d.cap + S.part_off + tile], v[k].lo);
+      store_agent(&d.partials[(2 * k) * d.cap + base + w.tile], v[k].hi);
+      store_agent(&d.partials[(2 * k + 1) * d.cap + base + w.tile], v[k].lo);
     }
     stores_complete();
     uint32_t t = take_ticket(&S.ticket);
@@ -96,8 +108,8 @@ __device__ bool slot_sum(dd (&v)[K], BSlot &S, int tile, const BDev &d, double (
   for (int j = threadIdx.x; j < S.nb; j += BS) {
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      acc[k] = dd_add(acc[k], dd{load_agent(&d.partials[(2 * k) * d.cap + S.part_off + j]),
-                                 load_agent(&d.partials[(2 * k + 1) * d.cap + S.part_off + j])});
+      acc[k] = dd_add(acc[k], dd{load_agent(&d.partials[(2 * k) * d.cap + base + j]),
+                                 load_agent(&d.partials[(2 * k + 1) * d.cap + base + j])});
   }
   block_sum_dd<K>(acc);
 #pragma unroll
@@ -113,37 +125,40 @@ __device__ bool slot_sum(dd (&v)[K], BSlot &S, int tile, const BDev &d, double (
        i < i##_end; i += BS)
 
 __global__ __launch_bounds__(BS) void k_batch_b1(BDev d) {
+  __shared__ double prod[SPMV_CAP];
   if ((int)blockIdx.x >= *d.nwork) return;
   const BWork w = d.work[blockIdx.x];
   BSlot &S = d.slots[w.slot];
   if (!slot_live(S)) return;
   const int phase = S.phase;
   if (phase == PH_START) return;
-  dd v[1] = {{0.0, 0.0}};
+  const int blk0 = S.tile_blk[w.tile], blk1 = S.tile_blk[w.tile + 1];
+  dd acc0{0.0, 0.0}, acc1{0.0, 0.0};
   if (phase == PH_INNER) {
-    TILE_ROWS(i, S, w.tile) {
-      float s = 0.0f;
-      for (int32_t k = S.rp[i]; k < S.rp[i + 1]; ++k) s += S.v32[k] * S.p[S.ci[k]];
-      S.q[i] = s;
-      dd_fma(v[0], (double)S.p[i], (double)s);
+    // q = A32 p with p.q (row-block SpMV, FP32 values and vectors)
+    for (int blk = blk0; blk < blk1; ++blk) {
+      if (blk > blk0) __syncthreads();
+      spmv_rowblock<float, float, SPMV_DOT>(blk, reinterpret_cast<float *>(prod), S.info, S.rp, S.ci, S.v32,
+                                            S.p, S.q, nullptr, 1.0, 1.0, S.p, acc0, acc1);
     }
   } else {
-    // r = b - A x in MTJ's multAdd order: dot sequential, then b + (-1) dot
-    TILE_ROWS(i, S, w.tile) {
-      double s = 0.0;
-      for (int32_t k = S.rp[i]; k < S.rp[i + 1]; ++k) s += S.v64[k] * S.x[S.ci[k]];
-      const double ri = S.b[i] + (-1.0) * s;
-      S.r[i] = ri;
-      dd_fma(v[0], ri, ri);
+    // r = b + (-1) * (A64 x): MTJ's multAdd order, with r.r
+    for (int blk = blk0; blk < blk1; ++blk) {
+      if (blk > blk0) __syncthreads();
+      spmv_rowblock<double, double, SPMV_RESID>(blk, prod, S.info, S.rp, S.ci, S.v64, S.x, S.r, S.b, -1.0,
+                                                1.0, nullptr, acc0, acc1);
     }
+    acc0 = acc1;
   }
+  dd v[1] = {acc0};
   double tot[1];
-  if (slot_sum<1>(v, S, w.tile, d, tot) && threadIdx.x == 0) {
+  if (slot_sum<1>(v, S, w, d, tot) && threadIdx.x == 0) {
     if (phase == PH_INNER) {
       S.alpha = S.rho / tot[0];
     } else {
       const double rn = sqrt(tot[0]);
       S.rnorm = rn;
+      S.action3 = A3_NONE;
       if (S.outer == 0) S.init_r = rn;
       if (rn < java_max(S.rtol * S.init_r, S.atol)) S.status = ST_CONVERGED;
       else if (rn > S.dtol * S.init_r) S.status = ST_DIVERGENCE;
@@ -186,7 +201,7 @@ __global__ __launch_bounds__(BS) void k_batch_b2(BDev d) {
     }
   }
   double tot[2];
-  if (slot_sum<2>(v, S, w.tile, d, tot) && threadIdx.x == 0) {
+  if (slot_sum<2>(v, S, w, d, tot) && threadIdx.x == 0) {
     const double rn = sqrt(tot[1]);
     if (phase == PH_START) {
       S.rho = tot[0];
@@ -232,13 +247,6 @@ __global__ __launch_bounds__(BS) void k_batch_b3(BDev d) {
   if (act == A3_PROMOTE && w.tile == 0 && threadIdx.x == 0) S.outer++;
 }
 
-// Clears action3 after B3 so a slot that starts its outer check does not
-// repeat the last action (single thread per slot).
-__global__ void k_batch_clear(BDev d, int nslots) {
-  int s = blockIdx.x * BS + threadIdx.x;
-  if (s < nslots) d.slots[s].action3 = A3_NONE;
-}
-
 __global__ void k_invdiag_f32(int32_t n, const int32_t *rp, const int32_t *ci, const double *v, float *inv,
                               float *v32, int64_t nnz) {
   int64_t t = blockIdx.x * (int64_t)BS + threadIdx.x;
@@ -249,6 +257,10 @@ __global__ void k_invdiag_f32(int32_t n, const int32_t *rp, const int32_t *ci, c
       if (ci[k] == (int32_t)i) dg = v[k];
     inv[i] = (float)(1.0 / dg);
   }
+}
+
+__global__ void k_zero_f64(int64_t n, double *x) {
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) x[i] = 0.0;
 }
 
 // ---- host side ---------------------------------------------------------
@@ -302,6 +314,8 @@ struct Request {
   int32_t n;
   std::vector<int32_t> rp, ci;
   std::vector<double> val, b;
+  std::vector<int2> info;       // tile-aligned row-blocks
+  std::vector<int32_t> tile_blk;
   double rtol;
   int32_t max_outer, max_inner;
   double inner_rtol;
@@ -313,10 +327,51 @@ struct Done {
   std::vector<double> x;
 };
 
-size_t req_bytes(int32_t n, int64_t nnz) {
-  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  return al(4 * (size_t)(n + 1)) + al(4 * (size_t)nnz) + al(8 * (size_t)nnz) + al(4 * (size_t)nnz) +
-         al(4 * (size_t)n) + 3 * al(8 * (size_t)n) + 5 * al(4 * (size_t)n);
+struct HostSlot {
+  int64_t id = -1;  // request id, -1 when free
+  size_t off = 0, bytes = 0;
+  int32_t n = 0, nb = 0;
+  double *x = nullptr;
+};
+
+inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t req_bytes(const Request &R) {
+  const size_t n = (size_t)R.n, nnz = R.ci.size();
+  return al256(4 * (n + 1)) + al256(4 * nnz) + al256(8 * nnz) + al256(4 * nnz) + al256(4 * n) + 3 * al256(8 * n) +
+         5 * al256(4 * n) + al256(sizeof(int2) * R.info.size()) + al256(4 * R.tile_blk.size());
+}
+
+// Row-blocks of each tile of `rows` rows, cut as for the CSR products
+// (at most SPMV_CAP entries and SPMV_MAX_ROWS rows, a longer row alone).
+void build_tile_blocks(Request &R, int rows) {
+  const int32_t n = R.n;
+  const int nb = (n + rows - 1) / rows;
+  R.info.clear();
+  R.tile_blk.assign((size_t)nb + 1, 0);
+  R.info.push_back(make_int2(0, 0));
+  for (int t = 0; t < nb; ++t) {
+    R.tile_blk[t] = (int32_t)R.info.size() - 1;
+    const int32_t end = std::min(n, (t + 1) * rows);
+    int32_t r = t * rows;
+    while (r < end) {
+      const int32_t len = R.rp[r + 1] - R.rp[r];
+      if (len > SPMV_CAP) {
+        ++r;
+      } else {
+        int32_t acc = 0, cnt = 0;
+        while (r < end && cnt < SPMV_MAX_ROWS) {
+          const int32_t l = R.rp[r + 1] - R.rp[r];
+          if (acc + l > SPMV_CAP) break;
+          acc += l;
+          ++r;
+          ++cnt;
+        }
+      }
+      R.info.push_back(make_int2(r, R.rp[r]));
+    }
+  }
+  R.tile_blk[nb] = (int32_t)R.info.size() - 1;
 }
 
 }  // namespace
@@ -326,18 +381,27 @@ using namespace mtjg;
 
 struct mtjg_batch_s {
   mtjg_ctx ctx = nullptr;
+  hipStream_t copy = nullptr;  // uploads of admitted systems, solution downloads
   int max_slots = 0, max_work = 0, iters = 32;
   int rows_per_wg = 1024;
   Arena arena;
   BSlot *d_slots = nullptr;     // device slot table
-  BSlot *h_slots = nullptr;     // pinned mirror
   BWork *d_work = nullptr;
   int32_t *d_nwork = nullptr;
   double *d_partials = nullptr;
-  std::vector<BWork> work;
-  std::vector<int64_t> slot_req;   // request id per slot (-1 free)
-  std::vector<size_t> slot_off;    // arena offset per slot
-  std::vector<size_t> slot_bytes;
+  // pinned staging, double-buffered: work table (+ count), new slot records,
+  // slot-table snapshots
+  BWork *h_work[2] = {nullptr, nullptr};
+  BSlot *h_new[2] = {nullptr, nullptr};
+  BSlot *h_snap[2] = {nullptr, nullptr};
+  hipEvent_t ev_stage[2] = {nullptr, nullptr};  // staging buffer i consumed
+  hipEvent_t ev_snap[2] = {nullptr, nullptr};   // snapshot i written
+  hipEvent_t ev_copy = nullptr;                 // uploads on the copy stream done
+  int stage = 0;
+  int snap_pending = -1, snap_hi = 0;          // buffer of the unprocessed snapshot
+  int table_work = 0;                           // entries in the device work table
+  int64_t table_rows = 0;
+  std::vector<HostSlot> slots;
   std::deque<Request> queue;
   std::map<int64_t, Done> done;
   std::vector<int64_t> finished;  // ids retired since the last mtjg_batch_completed
@@ -348,65 +412,106 @@ struct mtjg_batch_s {
 
 namespace {
 
-BDev bdev(mtjg_batch B) {
-  return BDev{B->d_slots, B->d_work, B->d_nwork, B->d_partials, B->max_work};
-}
+BDev bdev(mtjg_batch B) { return BDev{B->d_slots, B->d_work, B->d_nwork, B->d_partials, B->max_work}; }
 
-// Rebuilds the work table from the active slots (tile-major per slot).
-void rebuild_work(mtjg_batch B) {
-  B->work.clear();
-  int off = 0;
-  for (int s = 0; s < B->max_slots; ++s) {
-    if (B->slot_req[s] < 0) continue;
-    BSlot &S = B->h_slots[s];
-    S.part_off = off;
-    for (int t = 0; t < S.nb; ++t) B->work.push_back(BWork{s, t});
-    off += S.nb;
-  }
+int active_slots(mtjg_batch B) {
+  int a = 0;
+  for (const auto &h : B->slots) a += h.id >= 0;
+  return a;
 }
 
 int used_work(mtjg_batch B) {
   int u = 0;
-  for (int s = 0; s < B->max_slots; ++s)
-    if (B->slot_req[s] >= 0) u += B->h_slots[s].nb;
+  for (const auto &h : B->slots)
+    if (h.id >= 0) u += h.nb;
   return u;
 }
 
-void upload_tables(mtjg_batch B) {
+hipGraphExec_t graph_for(mtjg_batch B, int grid) {
+  auto it = B->graphs.find(grid);
+  if (it != B->graphs.end()) return it->second;
   hipStream_t st = B->ctx->stream;
-  rebuild_work(B);
-  int32_t nw = (int32_t)B->work.size();
-  if (nw) HIPCHK(hipMemcpyAsync(B->d_work, B->work.data(), sizeof(BWork) * nw, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(B->d_nwork, &nw, sizeof(int32_t), hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(B->d_slots, B->h_slots, sizeof(BSlot) * B->max_slots, hipMemcpyHostToDevice, st));
-  HIPCHK(hipStreamSynchronize(st));
+  hipGraph_t g;
+  BDev d = bdev(B);
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < B->iters; ++k) {
+    k_batch_b1<<<grid, BS, 0, st>>>(d);
+    k_batch_b2<<<grid, BS, 0, st>>>(d);
+    k_batch_b3<<<grid, BS, 0, st>>>(d);
+  }
+  HIPCHK(hipStreamEndCapture(st, &g));
+  hipGraphExec_t ex;
+  HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+  HIPCHK(hipGraphDestroy(g));
+  HIPCHK(hipGraphUpload(ex, st));
+  B->graphs[grid] = ex;
+  return ex;
 }
 
-// Admits queued requests while a slot, work-table room and arena memory
-// are free (first come, first served; a request that does not fit blocks
-// the queue, so large requests are not starved).
-int admit(mtjg_batch B) {
-  hipStream_t st = B->ctx->stream;
+// Retires the systems the pending snapshot shows finished: solution to the
+// host (copy stream, after the snapshot), slot and arena range freed.
+int retire(mtjg_batch B) {
+  if (B->snap_pending < 0) return 0;
+  const int i = B->snap_pending;
+  B->snap_pending = -1;
+  HIPCHK(hipEventSynchronize(B->ev_snap[i]));
+  const BSlot *snap = B->h_snap[i];
+  int n = 0;
+  bool waited = false;
+  for (int s = 0; s < B->snap_hi; ++s) {
+    HostSlot &H = B->slots[s];
+    const BSlot &S = snap[s];
+    if (H.id < 0 || S.rid != H.id || S.status == ST_RUNNING) continue;
+    if (!waited) {
+      HIPCHK(hipStreamWaitEvent(B->copy, B->ev_snap[i], 0));
+      waited = true;
+    }
+    Done D;
+    D.status = S.status;
+    D.outer = S.outer;
+    D.inner = S.inner_total;
+    D.residual = S.rnorm;
+    D.x.resize((size_t)H.n);
+    HIPCHK(hipMemcpyAsync(D.x.data(), H.x, 8 * (size_t)H.n, hipMemcpyDeviceToHost, B->copy));
+    B->done[H.id] = std::move(D);
+    B->finished.push_back(H.id);
+    B->arena.release(H.off, H.bytes);
+    H = HostSlot{};
+    ++n;
+  }
+  if (waited) HIPCHK(hipStreamSynchronize(B->copy));
+  return n;
+}
+
+// Admits queued requests while a slot, work-table room and arena memory are
+// free (first come, first served; a request that does not fit blocks the
+// queue, so large requests are not starved).  Uploads go on the copy
+// stream; the new slot records are staged in h_new[stage].
+int admit(mtjg_batch B, std::vector<int> &new_slots) {
+  hipStream_t cs = B->copy;
   int admitted = 0;
+  int used = used_work(B);
+  // admitted requests keep their host arrays until the uploads are done
+  std::vector<Request> sent;
   while (!B->queue.empty()) {
     Request &R = B->queue.front();
     int slot = -1;
     for (int s = 0; s < B->max_slots; ++s)
-      if (B->slot_req[s] < 0) {
+      if (B->slots[s].id < 0) {
         slot = s;
         break;
       }
     if (slot < 0) break;
-    const int nb = (R.n + B->rows_per_wg - 1) / B->rows_per_wg;
-    if (used_work(B) + nb > B->max_work) break;
+    const int nb = (int)R.tile_blk.size() - 1;
+    if (used + nb > B->max_work) break;
     const int64_t nnz = (int64_t)R.ci.size();
-    const size_t bytes = req_bytes(R.n, nnz);
+    const size_t bytes = req_bytes(R);
     const size_t off = B->arena.alloc(bytes);
     if (off == SIZE_MAX) break;
     char *p = B->arena.base + off;
     auto take = [&](size_t b) {
       char *q = p;
-      p += (b + 255) & ~(size_t)255;
+      p += al256(b);
       return q;
     };
     int32_t *rp = (int32_t *)take(4 * (size_t)(R.n + 1));
@@ -422,12 +527,16 @@ int admit(mtjg_batch B) {
     float *pv = (float *)take(4 * (size_t)R.n);
     float *qv = (float *)take(4 * (size_t)R.n);
     float *zv = (float *)take(4 * (size_t)R.n);
-    HIPCHK(hipMemcpyAsync(rp, R.rp.data(), 4 * (size_t)(R.n + 1), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(ci, R.ci.data(), 4 * (size_t)nnz, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(v64, R.val.data(), 8 * (size_t)nnz, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(b, R.b.data(), 8 * (size_t)R.n, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemsetAsync(x, 0, 8 * (size_t)R.n, st));
-    k_invdiag_f32<<<grid_for(std::max<int64_t>(nnz, R.n)), BS, 0, st>>>(R.n, rp, ci, v64, inv, v32, nnz);
+    int2 *info = (int2 *)take(sizeof(int2) * R.info.size());
+    int32_t *tb = (int32_t *)take(4 * R.tile_blk.size());
+    HIPCHK(hipMemcpyAsync(rp, R.rp.data(), 4 * (size_t)(R.n + 1), hipMemcpyHostToDevice, cs));
+    HIPCHK(hipMemcpyAsync(ci, R.ci.data(), 4 * (size_t)nnz, hipMemcpyHostToDevice, cs));
+    HIPCHK(hipMemcpyAsync(v64, R.val.data(), 8 * (size_t)nnz, hipMemcpyHostToDevice, cs));
+    HIPCHK(hipMemcpyAsync(b, R.b.data(), 8 * (size_t)R.n, hipMemcpyHostToDevice, cs));
+    HIPCHK(hipMemcpyAsync(info, R.info.data(), sizeof(int2) * R.info.size(), hipMemcpyHostToDevice, cs));
+    HIPCHK(hipMemcpyAsync(tb, R.tile_blk.data(), 4 * R.tile_blk.size(), hipMemcpyHostToDevice, cs));
+    k_zero_f64<<<grid_for(R.n), BS, 0, cs>>>(R.n, x);
+    k_invdiag_f32<<<grid_for(std::max<int64_t>(nnz, R.n)), BS, 0, cs>>>(R.n, rp, ci, v64, inv, v32, nnz);
     HIPCHK(hipGetLastError());
     BSlot S{};
     S.rp = rp;
@@ -436,6 +545,8 @@ int admit(mtjg_batch B) {
     S.v32 = v32;
     S.inv = inv;
     S.b = b;
+    S.info = info;
+    S.tile_blk = tb;
     S.x = x;
     S.r = r;
     S.rr = rr;
@@ -443,6 +554,7 @@ int admit(mtjg_batch B) {
     S.p = pv;
     S.q = qv;
     S.z = zv;
+    S.rid = R.id;
     S.n = R.n;
     S.nb = nb;
     S.rows_per_wg = B->rows_per_wg;
@@ -456,63 +568,57 @@ int admit(mtjg_batch B) {
     S.atol = 1e-50;
     S.dtol = 1e5;
     S.inner_rtol = R.inner_rtol;
-    B->h_slots[slot] = S;
-    B->slot_req[slot] = R.id;
-    B->slot_off[slot] = off;
-    B->slot_bytes[slot] = bytes;
+    B->h_new[B->stage][slot] = S;
+    new_slots.push_back(slot);
+    HostSlot &H = B->slots[slot];
+    H.id = R.id;
+    H.off = off;
+    H.bytes = bytes;
+    H.n = R.n;
+    H.nb = nb;
+    H.x = x;
+    used += nb;
+    sent.push_back(std::move(R));
     B->queue.pop_front();
     ++admitted;
   }
+  if (admitted) HIPCHK(hipStreamSynchronize(cs));
   return admitted;
 }
 
-hipGraphExec_t graph_for(mtjg_batch B, int grid) {
-  auto it = B->graphs.find(grid);
-  if (it != B->graphs.end()) return it->second;
+// Stages and enqueues (main stream, after everything already queued there)
+// the new slot records and the rebuilt work table.
+void publish_tables(mtjg_batch B, const std::vector<int> &new_slots, bool uploads) {
   hipStream_t st = B->ctx->stream;
-  hipGraph_t g;
-  BDev d = bdev(B);
-  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  for (int k = 0; k < B->iters; ++k) {
-    k_batch_b1<<<grid, BS, 0, st>>>(d);
-    k_batch_b2<<<grid, BS, 0, st>>>(d);
-    k_batch_b3<<<grid, BS, 0, st>>>(d);
-    k_batch_clear<<<(B->max_slots + BS - 1) / BS, BS, 0, st>>>(d, B->max_slots);
+  const int i = B->stage;
+  if (uploads) {
+    HIPCHK(hipEventRecord(B->ev_copy, B->copy));
+    HIPCHK(hipStreamWaitEvent(st, B->ev_copy, 0));
   }
-  HIPCHK(hipStreamEndCapture(st, &g));
-  hipGraphExec_t ex;
-  HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
-  HIPCHK(hipGraphDestroy(g));
-  B->graphs[grid] = ex;
-  return ex;
-}
-
-// Copies finished slots' results out and frees their slots.
-int retire(mtjg_batch B) {
-  hipStream_t st = B->ctx->stream;
-  HIPCHK(hipMemcpyAsync(B->h_slots, B->d_slots, sizeof(BSlot) * B->max_slots, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  int n = 0;
+  for (int s : new_slots)
+    HIPCHK(hipMemcpyAsync(B->d_slots + s, &B->h_new[i][s], sizeof(BSlot), hipMemcpyHostToDevice, st));
+  BWork *w = B->h_work[i];
+  int nw = 0;
+  int64_t rows = 0;
+  int off = 0;
   for (int s = 0; s < B->max_slots; ++s) {
-    if (B->slot_req[s] < 0) continue;
-    BSlot &S = B->h_slots[s];
-    if (S.status == ST_RUNNING) continue;
-    Done D;
-    D.status = S.status;
-    D.outer = S.outer;
-    D.inner = S.inner_total;
-    D.residual = S.rnorm;
-    D.x.resize((size_t)S.n);
-    HIPCHK(hipMemcpyAsync(D.x.data(), S.x, 8 * (size_t)S.n, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    B->done[B->slot_req[s]] = std::move(D);
-    B->finished.push_back(B->slot_req[s]);
-    B->arena.release(B->slot_off[s], B->slot_bytes[s]);
-    B->slot_req[s] = -1;
-    S = BSlot{};
-    ++n;
+    const HostSlot &H = B->slots[s];
+    if (H.id < 0) continue;
+    for (int t = 0; t < H.nb; ++t) w[nw++] = BWork{s, t, off, 0};
+    off += H.nb;
+    rows += H.n;
   }
-  return n;
+  // the count rides behind the entries in the same pinned buffer
+  int32_t *cnt = reinterpret_cast<int32_t *>(w + B->max_work);
+  *cnt = nw;
+  if (nw) HIPCHK(hipMemcpyAsync(B->d_work, w, sizeof(BWork) * nw, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(B->d_nwork, cnt, sizeof(int32_t), hipMemcpyHostToDevice, st));
+  HIPCHK(hipEventRecord(B->ev_stage[i], st));
+  B->table_work = nw;
+  B->table_rows = rows;
+  B->stage ^= 1;
+  // the next stage's buffers must be free before the host writes them
+  HIPCHK(hipEventSynchronize(B->ev_stage[B->stage]));
 }
 
 }  // namespace
@@ -536,16 +642,30 @@ extern "C" int mtjg_batch_create(mtjg_ctx ctx, int32_t max_slots, int32_t max_wo
   B->iters = iters_per_step;
   B->rows_per_wg = rows_per_workgroup;
   B->arena.init((size_t)arena_bytes);
+  HIPCHK(hipStreamCreateWithFlags(&B->copy, hipStreamNonBlocking));
   HIPCHK(hipMalloc(&B->d_slots, sizeof(BSlot) * max_slots));
-  HIPCHK(hipHostMalloc(&B->h_slots, sizeof(BSlot) * max_slots, hipHostMallocDefault));
-  memset(B->h_slots, 0, sizeof(BSlot) * max_slots);
+  HIPCHK(hipMemsetAsync(B->d_slots, 0, sizeof(BSlot) * max_slots, ctx->stream));
   HIPCHK(hipMalloc(&B->d_work, sizeof(BWork) * max_workgroups));
   HIPCHK(hipMalloc(&B->d_nwork, sizeof(int32_t)));
+  HIPCHK(hipMemsetAsync(B->d_nwork, 0, sizeof(int32_t), ctx->stream));
   HIPCHK(hipMalloc(&B->d_partials, sizeof(double) * 4 * (size_t)max_workgroups));
-  B->slot_req.assign(max_slots, -1);
-  B->slot_off.assign(max_slots, 0);
-  B->slot_bytes.assign(max_slots, 0);
-  upload_tables(B);
+  for (int i = 0; i < 2; ++i) {
+    HIPCHK(hipHostMalloc(&B->h_work[i], sizeof(BWork) * (max_workgroups + 1), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc(&B->h_new[i], sizeof(BSlot) * max_slots, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc(&B->h_snap[i], sizeof(BSlot) * max_slots, hipHostMallocDefault));
+    HIPCHK(hipEventCreateWithFlags(&B->ev_stage[i], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&B->ev_snap[i], hipEventDisableTiming));
+  }
+  HIPCHK(hipEventCreateWithFlags(&B->ev_copy, hipEventDisableTiming));
+  B->slots.assign(max_slots, HostSlot{});
+  // capture, instantiate and launch once (with an empty work table, so every
+  // workgroup returns at once) every grid size a step can use, so that no
+  // step pays for a graph's first launch
+  for (int grid = 64;; grid *= 2) {
+    HIPCHK(hipGraphLaunch(graph_for(B, std::min(grid, max_workgroups)), ctx->stream));
+    if (grid >= max_workgroups) break;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   *out = B;
   MTJG_API_END
 }
@@ -554,13 +674,22 @@ extern "C" int mtjg_batch_destroy(mtjg_batch B) {
   MTJG_API_BEGIN
   if (!B) return MTJG_OK;
   HIPCHK(hipStreamSynchronize(B->ctx->stream));
+  HIPCHK(hipStreamSynchronize(B->copy));
   for (auto &kv : B->graphs) (void)hipGraphExecDestroy(kv.second);
   (void)hipFree(B->arena.base);
   (void)hipFree(B->d_slots);
-  (void)hipHostFree(B->h_slots);
   (void)hipFree(B->d_work);
   (void)hipFree(B->d_nwork);
   (void)hipFree(B->d_partials);
+  for (int i = 0; i < 2; ++i) {
+    (void)hipHostFree(B->h_work[i]);
+    (void)hipHostFree(B->h_new[i]);
+    (void)hipHostFree(B->h_snap[i]);
+    (void)hipEventDestroy(B->ev_stage[i]);
+    (void)hipEventDestroy(B->ev_snap[i]);
+  }
+  (void)hipEventDestroy(B->ev_copy);
+  (void)hipStreamDestroy(B->copy);
   delete B;
   MTJG_API_END
 }
@@ -586,14 +715,15 @@ extern "C" int mtjg_batch_submit(mtjg_batch B, int32_t n, int64_t nnz, const int
     }
     if (!diag) fail(MTJG_E_MISSING_DIAGONAL, "missing diagonal in row " + std::to_string(i));
   }
-  const int nb = (n + B->rows_per_wg - 1) / B->rows_per_wg;
-  if (nb > B->max_work || req_bytes(n, nnz) > B->arena.size)
-    fail(MTJG_E_OUT_OF_MEMORY, "system larger than the batch can ever hold");
   Request R;
-  R.id = B->next_id++;
   R.n = n;
   R.rp.assign(row_pointer, row_pointer + n + 1);
+  build_tile_blocks(R, B->rows_per_wg);
+  const int nb = (int)R.tile_blk.size() - 1;
   R.ci.assign(column_index, column_index + nnz);
+  if (nb > B->max_work || req_bytes(R) > B->arena.size)
+    fail(MTJG_E_OUT_OF_MEMORY, "system larger than the batch can ever hold");
+  R.id = B->next_id++;
   R.val.assign(data, data + nnz);
   R.b.assign(b, b + n);
   R.rtol = rtol;
@@ -608,27 +738,41 @@ extern "C" int mtjg_batch_submit(mtjg_batch B, int32_t n, int64_t nnz, const int
 extern "C" int mtjg_batch_step(mtjg_batch B, int32_t *n_active, int32_t *n_completed) {
   MTJG_API_BEGIN
   if (!B) fail(MTJG_E_NULL, "null batch");
-  int admitted = admit(B);
-  int active = 0;
-  for (int s = 0; s < B->max_slots; ++s) active += B->slot_req[s] >= 0;
-  if (admitted) upload_tables(B);
-  int completed = 0;
-  if (active) {
-    const int need = (int)B->work.size();
+  hipStream_t st = B->ctx->stream;
+  // 1. run the current table (published by the previous call) and snapshot
+  //    the slot table behind it
+  if (B->table_work > 0) {
     int grid = 64;
-    while (grid < need) grid *= 2;
+    while (grid < B->table_work) grid *= 2;
     grid = std::min(grid, B->max_work);
-    HIPCHK(hipGraphLaunch(graph_for(B, grid), B->ctx->stream));
-    completed = retire(B);
-    if (completed) upload_tables(B);
+    HIPCHK(hipGraphLaunch(graph_for(B, grid), st));
+    int hi = 0;
+    for (int s = 0; s < B->max_slots; ++s)
+      if (B->slots[s].id >= 0) hi = s + 1;
+    const int i = B->steps & 1;
+    HIPCHK(hipMemcpyAsync(B->h_snap[i], B->d_slots, sizeof(BSlot) * hi, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(B->ev_snap[i], st));
+    // 2. while it runs: retire what the previous snapshot shows finished
+    const int completed = retire(B);
+    B->snap_pending = i;
+    B->snap_hi = hi;
+    if (n_completed) *n_completed = completed;
     ++B->steps;
+  } else {
+    // nothing queued on the device: consume a leftover snapshot
+    const int completed = retire(B);
+    if (n_completed) *n_completed = completed;
   }
-  if (n_active) {
-    int a = 0;
-    for (int s = 0; s < B->max_slots; ++s) a += B->slot_req[s] >= 0;
-    *n_active = a;
+  // 3. admit, then publish the next table behind the running graph
+  std::vector<int> new_slots;
+  const int admitted = admit(B, new_slots);
+  bool changed = admitted > 0;
+  if (!changed) {
+    // a retirement also changes the table
+    changed = used_work(B) != B->table_work;
   }
-  if (n_completed) *n_completed = completed;
+  if (changed) publish_tables(B, new_slots, admitted > 0);
+  if (n_active) *n_active = active_slots(B);
   MTJG_API_END
 }
 
@@ -667,10 +811,8 @@ extern "C" int mtjg_batch_info(mtjg_batch B, int32_t *queued, int32_t *active, i
                                int64_t *arena_size) {
   MTJG_API_BEGIN
   if (!B) fail(MTJG_E_NULL, "null batch");
-  int a = 0;
-  for (int s = 0; s < B->max_slots; ++s) a += B->slot_req[s] >= 0;
   if (queued) *queued = (int32_t)B->queue.size();
-  if (active) *active = a;
+  if (active) *active = active_slots(B);
   if (arena_used) *arena_used = (int64_t)B->arena.used;
   if (arena_size) *arena_size = (int64_t)B->arena.size;
   MTJG_API_END

@@ -157,6 +157,7 @@ enum SpmvMode : int {
   SPMV_MULTADD = 1,   // y = y0 + alpha * (A x)
   SPMV_TMULTADD = 2,  // y = ((y0 * inv_alpha) + sum) * alpha (transposed storage)
   SPMV_DOT = 3,       // y = A x, plus grid sums w.y and y.y, then fin()
+  SPMV_RESID = 4,     // y = y0 + alpha * (A x), plus y.y (batched residual)
 };
 
 // Solver epilogue selector for SPMV_DOT.

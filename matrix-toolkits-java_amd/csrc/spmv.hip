@@ -1,62 +1,20 @@
 // CompRowMatrix on the device and its CSR products (S/CompRowMatrix.java).
 //
-// SpMV kernel ("row-block stream"): the rows are cut on the host into
-// row-blocks holding at most SPMV_CAP stored entries (and at most
-// SPMV_MAX_ROWS rows).  A 256-thread workgroup takes a row-block and
-//   1. streams the block's column indices and values with unit-stride,
-//      fully coalesced loads (8 entries per thread, all issued before any
-//      use), gathers x at those columns, and stages the products
-//      val[j] * x[col[j]] in LDS;
-//   2. reduces each row from LDS.  When the block holds at least 128 rows a
-//      thread owns a row and adds its products sequentially in storage
-//      order, exactly as MTJ's loop (S/CompRowMatrix.java:304-312) does, so
-//      the result is bitwise MTJ's; blocks with fewer, longer rows give each
-//      row a power-of-two group of lanes that add strided partials and
-//      combine them with an xor butterfly.
-// A row longer than SPMV_CAP gets a row-block of its own, reduced by the
-// whole workgroup straight from global memory.  The kernel is HBM-bound
-// (12 bytes per stored entry, 20 per row), so MFMA is not used.
+// SpMV kernels ("row-block stream"): one 256-thread workgroup per host-cut
+// row-block (the routine in spmv_rowblock.hpp), so stencil rows are summed
+// in MTJ's storage order and come out bitwise MTJ's.  The kernels are
+// HBM-bound (12 bytes per stored entry, 20 per row), so MFMA is not used.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
 
 #include "mtjg_internal.hpp"
 #include "reduce.hpp"
+#include "spmv_rowblock.hpp"
 
 namespace mtjg {
 
-constexpr int SPMV_CAP = 2048;           // products staged per row-block
-constexpr int SPMV_PER = SPMV_CAP / BS;  // entries per thread per row-block
-constexpr int SPMV_MAX_ROWS = 512;       // rows per row-block (2 per thread)
-constexpr int SPMV_DOT_GRID = 2048;      // workgroups of the fused product
-
-// Row-block schedule: greedy cut so every block holds <= SPMV_CAP entries,
-// except single rows longer than that.  Entry b is (first row, first stored
-// entry) of row-block b; entry nrb closes the last block.
-static std::vector<int2> build_row_blocks(const std::vector<int32_t> &rp, int32_t nr) {
-  std::vector<int2> rb;
-  rb.reserve((size_t)nr / 64 + 2);
-  rb.push_back(make_int2(0, 0));
-  int32_t r = 0;
-  while (r < nr) {
-    int32_t len = rp[r + 1] - rp[r];
-    if (len > SPMV_CAP) {
-      ++r;
-      rb.push_back(make_int2(r, rp[r]));
-      continue;
-    }
-    int32_t acc = 0, rows = 0;
-    while (r < nr && rows < SPMV_MAX_ROWS) {
-      int32_t l = rp[r + 1] - rp[r];
-      if (acc + l > SPMV_CAP) break;
-      acc += l;
-      ++r;
-      ++rows;
-    }
-    rb.push_back(make_int2(r, rp[r]));
-  }
-  return rb;
-}
+constexpr int SPMV_DOT_GRID = 2048;  // workgroups of the fused product
 
 // Number of XCD slabs used to order row-blocks (see ticket_to_block).  Plain
 // row order measured fastest on the 256^3 Poisson product (5.25 TB/s against
@@ -68,33 +26,6 @@ static int choose_slabs(int32_t nrb) {
     if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16 || v == 32) return v;
   }
   return 1;
-}
-
-__device__ __forceinline__ int pow2_floor(int v) { return 1 << (31 - __clz(v)); }
-
-// Writes row `row`'s result and accumulates the epilogue dots.
-template <typename XT, int MODE>
-__device__ __forceinline__ void spmv_store(int row, XT s, XT *__restrict__ y, const XT *y0,
-                                           double alpha, double inv_alpha, const XT *w,
-                                           dd &acc0, dd &acc1) {
-  if (MODE == SPMV_MULT || MODE == SPMV_DOT) {
-    XT v = (s != XT(0)) ? s : XT(0);
-    y[row] = v;
-    if (MODE == SPMV_DOT) {
-      dd_fma(acc0, (double)w[row], (double)v);
-      dd_fma(acc1, (double)v, (double)v);
-    }
-  } else if (MODE == SPMV_MULTADD) {
-    y[row] = y0[row] + (XT)alpha * s;
-  } else {  // SPMV_TMULTADD: the initial value was folded into s
-    y[row] = s * (XT)alpha;
-  }
-}
-
-template <typename XT, int MODE>
-__device__ __forceinline__ XT spmv_init(int row, const XT *y0, double inv_alpha) {
-  if (MODE == SPMV_TMULTADD) return y0[row] * (XT)inv_alpha;
-  return XT(0);
 }
 
 // Scalar epilogues of fused solver kernels: run by the last workgroup.
@@ -120,100 +51,6 @@ __device__ __forceinline__ int ticket_to_block(int t, int nrb, int slabs, int pe
   const int k = t % slabs, l = t / slabs;
   const int b = k * per + l;
   return (l < per && b < nrb) ? b : -1;
-}
-
-// One row-block of y = A x (and the mode's epilogue) by one workgroup.
-template <typename VT, typename XT, int MODE>
-__device__ __forceinline__ void spmv_rowblock(int b, XT *prod, const int2 *__restrict__ info,
-                                              const int32_t *__restrict__ rp,
-                                              const int32_t *__restrict__ ci,
-                                              const VT *__restrict__ val, const XT *__restrict__ x,
-                                              XT *y, const XT *y0, double alpha, double inv_alpha,
-                                              const XT *w, dd &acc0, dd &acc1) {
-  constexpr int RPER = SPMV_MAX_ROWS / BS;
-  {
-    const int2 s0 = info[b], s1 = info[b + 1];
-    const int r0 = s0.x, r1 = s1.x;
-    const int j0 = s0.y;
-    const int nnzb = s1.y - j0;
-    const int nrows = r1 - r0;
-    if (nnzb > SPMV_CAP) {
-      // one long row: strided partials from global memory, fixed tree
-      double v[1] = {0.0};
-      XT s = 0;
-      for (int k = threadIdx.x; k < nnzb; k += BS) s += (XT)val[j0 + k] * x[ci[j0 + k]];
-      v[0] = (double)s;
-      block_sum<1>(v);
-      if (threadIdx.x == 0) {
-        XT tot = (XT)v[0];
-        if (MODE == SPMV_TMULTADD) tot = spmv_init<XT, MODE>(r0, y0, inv_alpha) + tot;
-        spmv_store<XT, MODE>(r0, tot, y, y0, alpha, inv_alpha, w, acc0, acc1);
-      }
-      return;
-    }
-    // 1. stage products in LDS: coalesced col/val streaming, the block's row
-    //    pointers alongside, then the x gathers
-    int c[SPMV_PER];
-    VT a[SPMV_PER];
-    XT xv[SPMV_PER];
-#pragma unroll
-    for (int u = 0; u < SPMV_PER; ++u) {
-      const int k = threadIdx.x + u * BS;
-      if (k < nnzb) {
-        c[u] = ci[j0 + k];
-        a[u] = val[j0 + k];
-      }
-    }
-    const bool one_lane = nrows * 2 > BS;  // uniform over the workgroup
-    int rs[RPER], re[RPER];
-    if (one_lane) {
-#pragma unroll
-      for (int u = 0; u < RPER; ++u) {
-        const int g = threadIdx.x + u * BS;
-        if (g < nrows) {
-          rs[u] = rp[r0 + g] - j0;
-          re[u] = rp[r0 + g + 1] - j0;
-        }
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < SPMV_PER; ++u) {
-      const int k = threadIdx.x + u * BS;
-      if (k < nnzb) xv[u] = x[c[u]];
-    }
-#pragma unroll
-    for (int u = 0; u < SPMV_PER; ++u) {
-      const int k = threadIdx.x + u * BS;
-      if (k < nnzb) prod[k] = (XT)a[u] * xv[u];
-    }
-    __syncthreads();
-    // 2. per-row reduction from LDS
-    if (one_lane) {
-      // a thread owns a row: sequential sum in storage order (MTJ's order)
-#pragma unroll
-      for (int u = 0; u < RPER; ++u) {
-        const int g = threadIdx.x + u * BS;
-        if (g < nrows) {
-          const int row = r0 + g;
-          XT s = spmv_init<XT, MODE>(row, y0, inv_alpha);
-          for (int k = rs[u]; k < re[u]; ++k) s += prod[k];
-          spmv_store<XT, MODE>(row, s, y, y0, alpha, inv_alpha, w, acc0, acc1);
-        }
-      }
-    } else {
-      const int tpr = min(64, pow2_floor(BS / max(nrows, 1)));
-      const int groups = BS / tpr;
-      const int lane = threadIdx.x & (tpr - 1);
-      for (int g = threadIdx.x / tpr; g < nrows; g += groups) {
-        const int row = r0 + g;
-        const int beg = rp[row] - j0, end = rp[row + 1] - j0;
-        XT s = (lane == 0) ? spmv_init<XT, MODE>(row, y0, inv_alpha) : XT(0);
-        for (int k = beg + lane; k < end; k += tpr) s += prod[k];
-        for (int off = tpr >> 1; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        if (lane == 0) spmv_store<XT, MODE>(row, s, y, y0, alpha, inv_alpha, w, acc0, acc1);
-      }
-    }
-  }
 }
 
 // Plain products: one workgroup per row-block ticket.

@@ -1,0 +1,184 @@
+// The row-block SpMV building block shared by the CSR products (spmv.hip)
+// and the batched serving engine (batch.hip): the host-side row-block cutter
+// and the device routine one workgroup runs per row-block.  The rows are cut
+// into row-blocks holding at most SPMV_CAP stored entries (and at most
+// SPMV_MAX_ROWS rows); a workgroup
+//   1. streams the block's column indices and values with unit-stride,
+//      fully coalesced loads (SPMV_PER entries per thread, all issued before
+//      any use), gathers x at those columns, and stages the products
+//      val[j] * x[col[j]] in LDS;
+//   2. reduces each row from LDS.  When the block holds at least 128 rows a
+//      thread owns a row and adds its products sequentially in storage
+//      order, exactly as MTJ's loop (S/CompRowMatrix.java:304-312) does;
+//      blocks with fewer, longer rows give each row a power-of-two group of
+//      lanes that add strided partials and combine them with an xor
+//      butterfly.
+// A row longer than SPMV_CAP gets a row-block of its own, reduced by the
+// whole workgroup straight from global memory.
+#pragma once
+
+#include <vector>
+
+#include "mtjg_internal.hpp"
+#include "reduce.hpp"
+
+namespace mtjg {
+
+constexpr int SPMV_CAP = 2048;           // products staged per row-block
+constexpr int SPMV_PER = SPMV_CAP / BS;  // entries per thread per row-block
+constexpr int SPMV_MAX_ROWS = 512;       // rows per row-block (2 per thread)
+
+// Row-block schedule: greedy cut so every block holds <= SPMV_CAP entries,
+// except single rows longer than that.  Entry b is (first row, first stored
+// entry) of row-block b; entry nrb closes the last block.
+inline std::vector<int2> build_row_blocks(const std::vector<int32_t> &rp, int32_t nr) {
+  std::vector<int2> rb;
+  rb.reserve((size_t)nr / 64 + 2);
+  rb.push_back(make_int2(0, 0));
+  int32_t r = 0;
+  while (r < nr) {
+    int32_t len = rp[r + 1] - rp[r];
+    if (len > SPMV_CAP) {
+      ++r;
+      rb.push_back(make_int2(r, rp[r]));
+      continue;
+    }
+    int32_t acc = 0, rows = 0;
+    while (r < nr && rows < SPMV_MAX_ROWS) {
+      int32_t l = rp[r + 1] - rp[r];
+      if (acc + l > SPMV_CAP) break;
+      acc += l;
+      ++r;
+      ++rows;
+    }
+    rb.push_back(make_int2(r, rp[r]));
+  }
+  return rb;
+}
+
+__device__ __forceinline__ int pow2_floor(int v) { return 1 << (31 - __clz(v)); }
+
+// Writes row `row`'s result and accumulates the epilogue dots.
+template <typename XT, int MODE>
+__device__ __forceinline__ void spmv_store(int row, XT s, XT *__restrict__ y, const XT *y0,
+                                           double alpha, double inv_alpha, const XT *w,
+                                           dd &acc0, dd &acc1) {
+  if (MODE == SPMV_MULT || MODE == SPMV_DOT) {
+    XT v = (s != XT(0)) ? s : XT(0);
+    y[row] = v;
+    if (MODE == SPMV_DOT) {
+      dd_fma(acc0, (double)w[row], (double)v);
+      dd_fma(acc1, (double)v, (double)v);
+    }
+  } else if (MODE == SPMV_MULTADD) {
+    y[row] = y0[row] + (XT)alpha * s;
+  } else if (MODE == SPMV_RESID) {
+    const XT r = y0[row] + (XT)alpha * s;
+    y[row] = r;
+    dd_fma(acc1, (double)r, (double)r);
+  } else {  // SPMV_TMULTADD: the initial value was folded into s
+    y[row] = s * (XT)alpha;
+  }
+}
+
+template <typename XT, int MODE>
+__device__ __forceinline__ XT spmv_init(int row, const XT *y0, double inv_alpha) {
+  if (MODE == SPMV_TMULTADD) return y0[row] * (XT)inv_alpha;
+  return XT(0);
+}
+
+// One row-block of y = A x (and the mode's epilogue) by one workgroup.
+template <typename VT, typename XT, int MODE>
+__device__ __forceinline__ void spmv_rowblock(int b, XT *prod, const int2 *__restrict__ info,
+                                              const int32_t *__restrict__ rp,
+                                              const int32_t *__restrict__ ci,
+                                              const VT *__restrict__ val, const XT *__restrict__ x,
+                                              XT *y, const XT *y0, double alpha, double inv_alpha,
+                                              const XT *w, dd &acc0, dd &acc1) {
+  constexpr int RPER = SPMV_MAX_ROWS / BS;
+  {
+    const int2 s0 = info[b], s1 = info[b + 1];
+    const int r0 = s0.x, r1 = s1.x;
+    const int j0 = s0.y;
+    const int nnzb = s1.y - j0;
+    const int nrows = r1 - r0;
+    if (nnzb > SPMV_CAP) {
+      // one long row: strided partials from global memory, fixed tree
+      double v[1] = {0.0};
+      XT s = 0;
+      for (int k = threadIdx.x; k < nnzb; k += BS) s += (XT)val[j0 + k] * x[ci[j0 + k]];
+      v[0] = (double)s;
+      block_sum<1>(v);
+      if (threadIdx.x == 0) {
+        XT tot = (XT)v[0];
+        if (MODE == SPMV_TMULTADD) tot = spmv_init<XT, MODE>(r0, y0, inv_alpha) + tot;
+        spmv_store<XT, MODE>(r0, tot, y, y0, alpha, inv_alpha, w, acc0, acc1);
+      }
+      return;
+    }
+    // 1. stage products in LDS: coalesced col/val streaming, the block's row
+    //    pointers alongside, then the x gathers
+    int c[SPMV_PER];
+    VT a[SPMV_PER];
+    XT xv[SPMV_PER];
+#pragma unroll
+    for (int u = 0; u < SPMV_PER; ++u) {
+      const int k = threadIdx.x + u * BS;
+      if (k < nnzb) {
+        c[u] = ci[j0 + k];
+        a[u] = val[j0 + k];
+      }
+    }
+    const bool one_lane = nrows * 2 > BS;  // uniform over the workgroup
+    int rs[RPER], re[RPER];
+    if (one_lane) {
+#pragma unroll
+      for (int u = 0; u < RPER; ++u) {
+        const int g = threadIdx.x + u * BS;
+        if (g < nrows) {
+          rs[u] = rp[r0 + g] - j0;
+          re[u] = rp[r0 + g + 1] - j0;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SPMV_PER; ++u) {
+      const int k = threadIdx.x + u * BS;
+      if (k < nnzb) xv[u] = x[c[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < SPMV_PER; ++u) {
+      const int k = threadIdx.x + u * BS;
+      if (k < nnzb) prod[k] = (XT)a[u] * xv[u];
+    }
+    __syncthreads();
+    // 2. per-row reduction from LDS
+    if (one_lane) {
+      // a thread owns a row: sequential sum in storage order (MTJ's order)
+#pragma unroll
+      for (int u = 0; u < RPER; ++u) {
+        const int g = threadIdx.x + u * BS;
+        if (g < nrows) {
+          const int row = r0 + g;
+          XT s = spmv_init<XT, MODE>(row, y0, inv_alpha);
+          for (int k = rs[u]; k < re[u]; ++k) s += prod[k];
+          spmv_store<XT, MODE>(row, s, y, y0, alpha, inv_alpha, w, acc0, acc1);
+        }
+      }
+    } else {
+      const int tpr = min(64, pow2_floor(BS / max(nrows, 1)));
+      const int groups = BS / tpr;
+      const int lane = threadIdx.x & (tpr - 1);
+      for (int g = threadIdx.x / tpr; g < nrows; g += groups) {
+        const int row = r0 + g;
+        const int beg = rp[row] - j0, end = rp[row + 1] - j0;
+        XT s = (lane == 0) ? spmv_init<XT, MODE>(row, y0, inv_alpha) : XT(0);
+        for (int k = beg + lane; k < end; k += tpr) s += prod[k];
+        for (int off = tpr >> 1; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (lane == 0) spmv_store<XT, MODE>(row, s, y, y0, alpha, inv_alpha, w, acc0, acc1);
+      }
+    }
+  }
+}
+
+}  // namespace mtjg

@@ -99,7 +99,7 @@ def _stencil_text(nx, ny, nz, style, seed=0):
         for t, (i, j, v) in enumerate(zip(rows, ci, va)):
             sep = " " if t % 3 else "\n% interleaved comment line\n"
             toks.append("%d %d %r%s" % (i + 1, j + 1, float(v), sep))
-        return ("%%MatrixMarket matrix coordinate real general\r\n%d %d %d\r\n" % (n, n, va.size)
+        return ("%%%%MatrixMarket matrix coordinate real general\r\n%d %d %d\r\n" % (n, n, va.size)
                 + "".join(toks).replace("\n", "\r\n")).encode()
     if style == "symmetric_dups":
         low = rows >= ci
@@ -154,7 +154,7 @@ def test_vectors_match_oracle():
              b"%%MatrixMarket vector coordinate real\n2 1\n3 1\n",
              b"%%MatrixMarket vector array real\n3\n1 2\n",
              b"%%MatrixMarket matrix array real\n3\n1 2 3\n",
-             ("%%MatrixMarket vector array real\n200000\n" + "".join("%r\n" % v for v in
+             ("%%MatrixMarket vector array real\n200000\n" + "".join("%r\n" % float(v) for v in
               np.random.default_rng(2).normal(size=200000))).encode()]
     for text in cases:
         code, x = oracle.mm_vector(text)
@@ -183,7 +183,7 @@ def test_device_matrix_from_file_multiplies_like_oracle(ctx, tmp_path):
     y = M.DenseVector(np.zeros(r), ctx)
     A.mult(M.DenseVector(x, ctx), y)
     assert y.getData().tobytes() == O.mult(x).tobytes()
-    vtext = ("%%MatrixMarket vector array real\n%d\n" % c + "".join("%r\n" % v for v in x)).encode()
+    vtext = ("%%MatrixMarket vector array real\n" + "%d\n" % c + "".join("%r\n" % float(v) for v in x)).encode()
     vf = tmp_path / "x.mtx"
     vf.write_bytes(vtext)
     xv = MM.load_vector(str(vf), ctx)
