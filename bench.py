@@ -286,6 +286,8 @@ def main():
     ap.add_argument("--mode", choices=["spmv", "copy", "solve", "serve", "mmload"], default="spmv")
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--requests", type=int, default=512)
+    ap.add_argument("--serve-requests", type=int, default=256,
+                    help="requests of the served-vs-plain comparison attached to the SpMV line (0: none)")
     ap.add_argument("--lam", type=float, default=2000.0)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--solve", default="cg_poisson2d_256", choices=sorted(SOLVE_WORKLOADS))
@@ -354,8 +356,31 @@ def main():
         out["roofline"]["traffic_source"] = traffic[1]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sysm, args.cpu_seconds)
+    if args.serve_requests > 0:
+        for o in (A, x, y):
+            o.close()
+        out["serving"] = serve_summary(ctx, args.serve_requests, args.lam, args.seed)
     print(json.dumps(out))
     return 0
+
+
+def serve_summary(ctx, count, lam, seed):
+    """The metric's second half on a bounded stream: `count` requests served
+    by the batch engine and by plain per-request FP64 Jacobi-CG."""
+    from mtjgpu import serving as S
+    reqs = S.request_stream(count, lam=lam, seed=seed)
+    systems = [S.make_system(r) for r in reqs]
+    st_b, res_b = S.serve_batched(ctx, reqs, systems)
+    st_p, res_p = S.serve_plain(ctx, reqs, systems)
+    worst = max(S.true_residual(systems[k], res_b[k]["x"]) for k in range(count))
+    return {"requests": count, "lambda_per_s": lam, "rtol": 1e-8,
+            "batched_solves_per_s": round(st_b["solves_per_s"], 2), "batched_p50_ms": round(st_b["p50_ms"], 1),
+            "batched_p99_ms": round(st_b["p99_ms"], 1),
+            "plain_solves_per_s": round(st_p["solves_per_s"], 2), "plain_p50_ms": round(st_p["p50_ms"], 1),
+            "plain_p99_ms": round(st_p["p99_ms"], 1),
+            "speedup": round(st_b["solves_per_s"] / st_p["solves_per_s"], 3),
+            "batched_worst_true_rel_residual": worst,
+            "all_converged": all(res_b[k]["code"] == 0 and res_p[k]["code"] == 0 for k in range(count))}
 
 
 if __name__ == "__main__":
