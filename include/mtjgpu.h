@@ -80,9 +80,13 @@ enum {
     MTJG_BICGSTAB = 1,  /* S/BiCGstab.java */
     MTJG_GMRES = 2,     /* S/GMRES.java */
     MTJG_IR = 3,        /* S/IR.java */
-    MTJG_MIXED_CG = 4   /* FP32 inner Jacobi-CG under FP64 refinement; no
+    MTJG_MIXED_CG = 4,  /* FP32 inner Jacobi-CG under FP64 refinement; no
                            MTJ counterpart (S/IR.java:57-70 is its outer
                            loop's shape) */
+    MTJG_BICG = 5,      /* S/BiCG.java (uses transMult and transApply) */
+    MTJG_CGS = 6,       /* S/CGS.java */
+    MTJG_QMR = 7        /* S/QMR.java with M1 = M2 = the preconditioner
+                           (uses transMultAdd and transApply) */
 };
 
 const char *mtjg_strerror(int code);

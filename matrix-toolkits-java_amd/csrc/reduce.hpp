@@ -269,10 +269,10 @@ __device__ bool grid_sum_dd(dd (&v)[K], const GridRed &r, double (&out)[K]) {
 }
 
 // java.lang.Math.max(double, double).
-__device__ __forceinline__ double java_max(double a, double b) {
+__host__ __device__ __forceinline__ double java_max(double a, double b) {
   if (a != a) return a;
   if (b != b) return b;
-  if (a == 0.0 && b == 0.0) return signbit(a) ? b : a;
+  if (a == 0.0 && b == 0.0) return __builtin_signbit(a) ? b : a;
   return (a >= b) ? a : b;
 }
 

@@ -1,5 +1,6 @@
-// Iterative solvers (S/CG.java, S/BiCGstab.java, S/GMRES.java, S/IR.java)
-// driving fused HIP kernels with every operand device resident.
+// Iterative solvers (S/CG.java, S/BiCGstab.java, S/GMRES.java, S/IR.java,
+// S/BiCG.java, S/CGS.java, S/QMR.java) with every operand device resident;
+// CG, BiCGstab, GMRES and IR drive fused HIP kernels.
 //
 // Each solver keeps its scalars (rho, alpha, omega, beta, the Hessenberg
 // matrix, the iteration-monitor state) in a device struct.  Reductions end in
@@ -874,6 +875,253 @@ void solve_mixed(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Transpose-using solvers: BiCG (S/BiCG.java:63-102), CGS (S/CGS.java:65-105)
+// and QMR (S/QMR.java:117-213).  These drive the library's own vector,
+// product and preconditioner entry points from the host, one device call per
+// MTJ statement in MTJ's order, and read each dot and norm back (as MTJ's
+// loops branch on them); the DefaultIterationMonitor predicate runs on the
+// host.
+
+// A device vector handle over solver scratch memory (not owned).
+struct DV {
+  mtjg_vec_s v;
+  DV(mtjg_solver S, double *p) {
+    v.ctx = S->ctx;
+    v.n = S->n;
+    v.dtype = MTJG_F64;
+    v.ptr = p;
+    v.owned = false;
+  }
+  mtjg_vec h() { return &v; }
+};
+
+void ck(int rc) {
+  if (rc != MTJG_OK) fail(rc, mtjg_last_error());
+}
+
+double dot(DV &x, DV &y) {
+  double r = 0;
+  ck(mtjg_vec_dot(x.h(), y.h(), &r));
+  return r;
+}
+
+double norm2(DV &x) {
+  double r = 0;
+  ck(mtjg_vec_norm(x.h(), MTJG_NORM_TWO, &r));
+  return r;
+}
+
+// M.apply / M.transApply; the identity preconditioner copies (x.set(b)).
+void papply(mtjg_solver S, DV &b, DV &x) {
+  if (S->M) ck(mtjg_prec_apply(S->M, b.h(), x.h()));
+  else ck(mtjg_vec_set(x.h(), b.h()));
+}
+
+void ptrans(mtjg_solver S, DV &b, DV &x) {
+  if (S->M) ck(mtjg_prec_trans_apply(S->M, b.h(), x.h()));
+  else ck(mtjg_vec_set(x.h(), b.h()));
+}
+
+// DefaultIterationMonitor on the host (S/DefaultIterationMonitor.java:
+// 132-156, S/AbstractIterationMonitor.java:81-93).
+struct HostMonitor {
+  mtjg_solver S;
+  KState k{};
+  explicit HostMonitor(mtjg_solver s) : S(s) {
+    k.max_iter = s->mon.max_iter;
+    k.rtol = s->mon.rtol;
+    k.atol = s->mon.atol;
+    k.dtol = s->mon.dtol;
+    k.status = ST_RUNNING;
+  }
+  // iter.converged(r): true when the loop must stop
+  bool converged(double r) {
+    k.residual = r;
+    if (k.iter == 0) k.init_r = r;
+    if (r < java_max(k.rtol * k.init_r, k.atol)) k.status = ST_CONVERGED;
+    else if (r > k.dtol * k.init_r) k.status = ST_DIVERGENCE;
+    else if (k.iter >= k.max_iter) k.status = ST_ITERATIONS;
+    else if (r != r) k.status = ST_DIVERGENCE;
+    if (S->rep) S->rep(r, k.iter, S->rep_user);
+    return k.status != ST_RUNNING;
+  }
+  void breakdown() { k.status = ST_BREAKDOWN; }
+  // publishes the outcome where mtjg_solver_solve reads it
+  void finish() {
+    HIPCHK(hipMemcpyAsync(S->st, &k, sizeof(KState), hipMemcpyHostToDevice, S->ctx->stream));
+    HIPCHK(hipStreamSynchronize(S->ctx->stream));
+  }
+};
+
+void solve_bicg(mtjg_solver S, mtjg_csr A, mtjg_vec b, mtjg_vec x) {
+  DV z(S, scratch(S, 0)), p(S, scratch(S, 1)), q(S, scratch(S, 2)), r(S, scratch(S, 3)),
+      zt(S, scratch(S, 4)), pt(S, scratch(S, 5)), qt(S, scratch(S, 6)), rt(S, scratch(S, 7));
+  HostMonitor mon(S);
+  double rho_1 = 1, rho_2 = 1, alpha = 1, beta = 1;
+  ck(mtjg_vec_set(r.h(), b));
+  ck(mtjg_csr_mult_add(A, -1, x, r.h()));
+  ck(mtjg_vec_set(rt.h(), r.h()));
+  for (mon.k.iter = 0; !mon.converged(norm2(r)); mon.k.iter++) {
+    papply(S, r, z);
+    ptrans(S, rt, zt);
+    rho_1 = dot(z, rt);
+    if (rho_1 == 0.) {
+      mon.breakdown();
+      break;
+    }
+    if (mon.k.iter == 0) {
+      ck(mtjg_vec_set(p.h(), z.h()));
+      ck(mtjg_vec_set(pt.h(), zt.h()));
+    } else {
+      beta = rho_1 / rho_2;
+      ck(mtjg_vec_scale(p.h(), beta));
+      ck(mtjg_vec_add(p.h(), z.h()));
+      ck(mtjg_vec_scale(pt.h(), beta));
+      ck(mtjg_vec_add(pt.h(), zt.h()));
+    }
+    ck(mtjg_csr_mult(A, p.h(), q.h()));
+    ck(mtjg_csr_trans_mult(A, pt.h(), qt.h()));
+    alpha = rho_1 / dot(pt, q);
+    ck(mtjg_vec_add_scaled(x, alpha, p.h()));
+    ck(mtjg_vec_add_scaled(r.h(), -alpha, q.h()));
+    ck(mtjg_vec_add_scaled(rt.h(), -alpha, qt.h()));
+    rho_2 = rho_1;
+  }
+  (void)beta;
+  mon.finish();
+}
+
+void solve_cgs(mtjg_solver S, mtjg_csr A, mtjg_vec b, mtjg_vec x) {
+  DV p(S, scratch(S, 0)), q(S, scratch(S, 1)), u(S, scratch(S, 2)), phat(S, scratch(S, 3)),
+      qhat(S, scratch(S, 4)), vhat(S, scratch(S, 5)), uhat(S, scratch(S, 6)), sum(S, scratch(S, 7)),
+      r(S, scratch(S, 8)), rt(S, scratch(S, 9));
+  HostMonitor mon(S);
+  double rho_1 = 0, rho_2 = 0, alpha = 0, beta = 0;
+  ck(mtjg_vec_set(r.h(), b));
+  ck(mtjg_csr_mult_add(A, -1, x, r.h()));
+  ck(mtjg_vec_set(rt.h(), r.h()));
+  for (mon.k.iter = 0; !mon.converged(norm2(r)); mon.k.iter++) {
+    rho_1 = dot(rt, r);
+    if (rho_1 == 0) {
+      mon.breakdown();
+      break;
+    }
+    if (mon.k.iter == 0) {
+      ck(mtjg_vec_set(u.h(), r.h()));
+      ck(mtjg_vec_set(p.h(), u.h()));
+    } else {
+      beta = rho_1 / rho_2;
+      ck(mtjg_vec_set(u.h(), r.h()));
+      ck(mtjg_vec_add_scaled(u.h(), beta, q.h()));
+      ck(mtjg_vec_set(sum.h(), q.h()));
+      ck(mtjg_vec_add_scaled(sum.h(), beta, p.h()));
+      ck(mtjg_vec_set(p.h(), u.h()));
+      ck(mtjg_vec_add_scaled(p.h(), beta, sum.h()));
+    }
+    papply(S, p, phat);
+    ck(mtjg_csr_mult(A, phat.h(), vhat.h()));
+    alpha = rho_1 / dot(rt, vhat);
+    ck(mtjg_vec_set_scaled(q.h(), -alpha, vhat.h()));
+    ck(mtjg_vec_add(q.h(), u.h()));
+    ck(mtjg_vec_set(sum.h(), u.h()));
+    ck(mtjg_vec_add(sum.h(), q.h()));
+    papply(S, sum, uhat);
+    ck(mtjg_vec_add_scaled(x, alpha, uhat.h()));
+    ck(mtjg_csr_mult(A, uhat.h(), qhat.h()));
+    ck(mtjg_vec_add_scaled(r.h(), -alpha, qhat.h()));
+    rho_2 = rho_1;
+  }
+  mon.finish();
+}
+
+void solve_qmr(mtjg_solver S, mtjg_csr A, mtjg_vec b, mtjg_vec x) {
+  DV r(S, scratch(S, 0)), y(S, scratch(S, 1)), z(S, scratch(S, 2)), v(S, scratch(S, 3)), w(S, scratch(S, 4)),
+      vt(S, scratch(S, 5)), wt(S, scratch(S, 6)), yt(S, scratch(S, 7)), zt(S, scratch(S, 8)),
+      p(S, scratch(S, 9)), q(S, scratch(S, 10)), pt(S, scratch(S, 11)), d(S, scratch(S, 12)),
+      s(S, scratch(S, 13));
+  HostMonitor mon(S);
+  double rho = 0, rho_1 = 0, xi = 0, gamma = 1., gamma_1 = 0, theta = 0, theta_1 = 0, eta = -1., delta = 0, ep = 0,
+         beta = 0;
+  ck(mtjg_vec_set(r.h(), b));
+  ck(mtjg_csr_mult_add(A, -1, x, r.h()));
+  ck(mtjg_vec_set(vt.h(), r.h()));
+  papply(S, vt, y);
+  rho = norm2(y);
+  ck(mtjg_vec_set(wt.h(), r.h()));
+  ptrans(S, wt, z);
+  xi = norm2(z);
+  for (mon.k.iter = 0; !mon.converged(norm2(r)); mon.k.iter++) {
+    if (rho == 0 || xi == 0) {
+      mon.breakdown();
+      break;
+    }
+    ck(mtjg_vec_set_scaled(v.h(), 1 / rho, vt.h()));
+    ck(mtjg_vec_scale(y.h(), 1 / rho));
+    ck(mtjg_vec_set_scaled(w.h(), 1 / xi, wt.h()));
+    ck(mtjg_vec_scale(z.h(), 1 / xi));
+    delta = dot(z, y);
+    if (delta == 0) {
+      mon.breakdown();
+      break;
+    }
+    papply(S, y, yt);
+    ptrans(S, z, zt);
+    if (mon.k.iter == 0) {
+      ck(mtjg_vec_set(p.h(), yt.h()));
+      ck(mtjg_vec_set(q.h(), zt.h()));
+    } else {
+      ck(mtjg_vec_scale(p.h(), -xi * delta / ep));
+      ck(mtjg_vec_add(p.h(), yt.h()));
+      ck(mtjg_vec_scale(q.h(), -rho * delta / ep));
+      ck(mtjg_vec_add(q.h(), zt.h()));
+    }
+    ck(mtjg_csr_mult(A, p.h(), pt.h()));
+    ep = dot(q, pt);
+    if (ep == 0) {
+      mon.breakdown();
+      break;
+    }
+    beta = ep / delta;
+    if (beta == 0) {
+      mon.breakdown();
+      break;
+    }
+    ck(mtjg_vec_set_scaled(vt.h(), -beta, v.h()));
+    ck(mtjg_vec_add(vt.h(), pt.h()));
+    papply(S, vt, y);
+    rho_1 = rho;
+    rho = norm2(y);
+    // A.transMultAdd(q, w_tld.set(-beta, w)): transMultAdd with alpha 1
+    ck(mtjg_vec_set_scaled(wt.h(), -beta, w.h()));
+    ck(mtjg_csr_trans_mult_add(A, 1.0, q.h(), wt.h()));
+    ptrans(S, wt, z);
+    xi = norm2(z);
+    gamma_1 = gamma;
+    theta_1 = theta;
+    theta = rho / (gamma_1 * beta);
+    gamma = 1 / std::sqrt(1 + theta * theta);
+    if (gamma == 0) {
+      mon.breakdown();
+      break;
+    }
+    eta = -eta * rho_1 * gamma * gamma / (beta * gamma_1 * gamma_1);
+    if (mon.k.iter == 0) {
+      ck(mtjg_vec_set_scaled(d.h(), eta, p.h()));
+      ck(mtjg_vec_set_scaled(s.h(), eta, pt.h()));
+    } else {
+      const double val = theta_1 * theta_1 * gamma * gamma;
+      ck(mtjg_vec_scale(d.h(), val));
+      ck(mtjg_vec_add_scaled(d.h(), eta, p.h()));
+      ck(mtjg_vec_scale(s.h(), val));
+      ck(mtjg_vec_add_scaled(s.h(), eta, pt.h()));
+    }
+    ck(mtjg_vec_add(x, d.h()));
+    ck(mtjg_vec_add_scaled(r.h(), -1, s.h()));
+  }
+  mon.finish();
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -882,7 +1130,7 @@ void solve_mixed(mtjg_solver S, mtjg_csr A, double *b, double *x) {
 extern "C" int mtjg_solver_create(mtjg_ctx ctx, int kind, mtjg_vec template_vec, mtjg_solver *out) {
   MTJG_API_BEGIN
   if (!ctx || !template_vec || !out) fail(MTJG_E_NULL, "null argument");
-  if (kind < MTJG_CG || kind > MTJG_MIXED_CG) fail(MTJG_E_INVALID_ARGUMENT, "unknown solver kind");
+  if (kind < MTJG_CG || kind > MTJG_QMR) fail(MTJG_E_INVALID_ARGUMENT, "unknown solver kind");
   auto *S = new mtjg_solver_s();
   S->ctx = ctx;
   S->kind = kind;
@@ -896,7 +1144,14 @@ extern "C" int mtjg_solver_create(mtjg_ctx ctx, int kind, mtjg_vec template_vec,
   HIPCHK(hipEventCreateWithFlags(&S->ev[0], hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&S->ev[1], hipEventDisableTiming));
   // scratch vectors come from template.copy() (S/CG.java:51-56)
-  int nvec = kind == MTJG_CG ? 4 : kind == MTJG_BICGSTAB ? 8 : kind == MTJG_GMRES ? 3 : kind == MTJG_IR ? 2 : 1;
+  int nvec = kind == MTJG_CG         ? 4
+             : kind == MTJG_BICGSTAB ? 8
+             : kind == MTJG_GMRES    ? 3
+             : kind == MTJG_IR       ? 2
+             : kind == MTJG_BICG     ? 8
+             : kind == MTJG_CGS      ? 10
+             : kind == MTJG_QMR      ? 14
+                                     : 1;
   for (int k = 0; k < nvec; ++k) scratch(S, (size_t)k);
   HIPCHK(hipStreamSynchronize(ctx->stream));
   *out = S;
@@ -982,6 +1237,9 @@ extern "C" int mtjg_solver_solve(mtjg_solver S, mtjg_csr A, mtjg_vec b, mtjg_vec
     case MTJG_GMRES: solve_gmres(S, A, bp, xp); break;
     case MTJG_IR: solve_ir(S, A, bp, xp); break;
     case MTJG_MIXED_CG: solve_mixed(S, A, bp, xp); break;
+    case MTJG_BICG: solve_bicg(S, A, b, x); break;
+    case MTJG_CGS: solve_cgs(S, A, b, x); break;
+    case MTJG_QMR: solve_qmr(S, A, b, x); break;
   }
   prec_check(S->M);
   KState k;

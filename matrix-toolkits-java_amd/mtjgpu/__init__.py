@@ -569,6 +569,21 @@ class IR(_Solver):
     KIND = L.IR
 
 
+class BiCG(_Solver):
+    """S/BiCG.java (uses transMult and the preconditioner's transApply)."""
+    KIND = L.BICG
+
+
+class CGS(_Solver):
+    """S/CGS.java."""
+    KIND = L.CGS
+
+
+class QMR(_Solver):
+    """S/QMR.java with M1 = M2 = the preconditioner (setPreconditioner form)."""
+    KIND = L.QMR
+
+
 class GMRES(_Solver):
     """S/GMRES.java (restart 30 by default)."""
     KIND = L.GMRES

@@ -30,7 +30,7 @@ E_IO = -11
 
 F64, F32 = 0, 1
 NORM_ONE, NORM_TWO, NORM_TWO_ROBUST, NORM_INFINITY = 0, 1, 2, 3
-CG, BICGSTAB, GMRES, IR, MIXED_CG = 0, 1, 2, 3, 4
+CG, BICGSTAB, GMRES, IR, MIXED_CG, BICG, CGS, QMR = 0, 1, 2, 3, 4, 5, 6, 7
 
 
 class Monitor(C.Structure):

@@ -587,6 +587,172 @@ int oc_ir(int n, const int32_t *rp, const int32_t *ci, const double *a,
     return ret;
 }
 
+/* S/BiCG.java:63-102: biconjugate gradients; the shadow system uses
+ * transApply and transMult. */
+int oc_bicg(int n, const int32_t *rp, const int32_t *ci, const double *a,
+            const oc_prec *M, const double *b, double *x, oc_monitor *m) {
+    double *z = vec_alloc(n), *p = vec_alloc(n), *q = vec_alloc(n), *r = vec_alloc(n),
+           *zt = vec_alloc(n), *pt = vec_alloc(n), *qt = vec_alloc(n), *rt = vec_alloc(n);
+    double rho_1 = 1, rho_2 = 1, alpha = 1, beta = 1;
+    int ret = OC_OK;
+    oc_dv_set(n, b, r);
+    oc_csr_mult_add(n, rp, ci, a, -1, x, r);
+    oc_dv_set(n, r, rt);
+    for (m->iter = 0;; m->iter++) {
+        int c = check(m, oc_dv_norm2(n, r));
+        if (c) { ret = c - 1; break; }
+        oc_prec_apply(M, r, z);
+        oc_prec_trans_apply(M, rt, zt);
+        rho_1 = oc_dv_dot(n, z, rt);
+        if (rho_1 == 0.) { ret = OC_BREAKDOWN; break; }
+        if (m->iter == 0) {
+            oc_dv_set(n, z, p);
+            oc_dv_set(n, zt, pt);
+        } else {
+            beta = rho_1 / rho_2;
+            oc_dv_scale(n, beta, p);
+            oc_dv_add(n, z, p);
+            oc_dv_scale(n, beta, pt);
+            oc_dv_add(n, zt, pt);
+        }
+        oc_csr_mult(n, rp, ci, a, p, q);
+        oc_csr_trans_mult(n, n, rp, ci, a, pt, qt);
+        alpha = rho_1 / oc_dv_dot(n, pt, q);
+        oc_dv_add_scaled(n, alpha, p, x);
+        oc_dv_add_scaled(n, -alpha, q, r);
+        oc_dv_add_scaled(n, -alpha, qt, rt);
+        rho_2 = rho_1;
+    }
+    (void)beta;
+    free(z); free(p); free(q); free(r); free(zt); free(pt); free(qt); free(rt);
+    return ret;
+}
+
+/* S/CGS.java:65-105: conjugate gradients squared. */
+int oc_cgs(int n, const int32_t *rp, const int32_t *ci, const double *a,
+           const oc_prec *M, const double *b, double *x, oc_monitor *m) {
+    double *p = vec_alloc(n), *q = vec_alloc(n), *u = vec_alloc(n), *phat = vec_alloc(n),
+           *vhat = vec_alloc(n), *uhat = vec_alloc(n), *qhat = vec_alloc(n), *sum = vec_alloc(n),
+           *r = vec_alloc(n), *rt = vec_alloc(n);
+    double rho_1 = 0, rho_2 = 0, alpha = 0, beta = 0;
+    int ret = OC_OK;
+    oc_dv_set(n, b, r);
+    oc_csr_mult_add(n, rp, ci, a, -1, x, r);
+    oc_dv_set(n, r, rt);
+    for (m->iter = 0;; m->iter++) {
+        int c = check(m, oc_dv_norm2(n, r));
+        if (c) { ret = c - 1; break; }
+        rho_1 = oc_dv_dot(n, rt, r);
+        if (rho_1 == 0) { ret = OC_BREAKDOWN; break; }
+        if (m->iter == 0) {
+            oc_dv_set(n, r, u);
+            oc_dv_set(n, u, p);
+        } else {
+            beta = rho_1 / rho_2;
+            oc_dv_set(n, r, u);
+            oc_dv_add_scaled(n, beta, q, u);
+            oc_dv_set(n, q, sum);
+            oc_dv_add_scaled(n, beta, p, sum);
+            oc_dv_set(n, u, p);
+            oc_dv_add_scaled(n, beta, sum, p);
+        }
+        oc_prec_apply(M, p, phat);
+        oc_csr_mult(n, rp, ci, a, phat, vhat);
+        alpha = rho_1 / oc_dv_dot(n, rt, vhat);
+        oc_dv_set_scaled(n, -alpha, vhat, q);
+        oc_dv_add(n, u, q);
+        oc_dv_set(n, u, sum);
+        oc_dv_add(n, q, sum);
+        oc_prec_apply(M, sum, uhat);
+        oc_dv_add_scaled(n, alpha, uhat, x);
+        oc_csr_mult(n, rp, ci, a, uhat, qhat);
+        oc_dv_add_scaled(n, -alpha, qhat, r);
+        rho_2 = rho_1;
+    }
+    free(p); free(q); free(u); free(phat); free(vhat); free(uhat); free(qhat); free(sum);
+    free(r); free(rt);
+    return ret;
+}
+
+/* S/QMR.java:117-213: quasi-minimal residual with M1 = M2 = M (the
+ * setPreconditioner form, S/QMR.java:214-220). */
+int oc_qmr(int n, const int32_t *rp, const int32_t *ci, const double *a,
+           const oc_prec *M, const double *b, double *x, oc_monitor *m) {
+    double *r = vec_alloc(n), *y = vec_alloc(n), *z = vec_alloc(n), *v = vec_alloc(n),
+           *w = vec_alloc(n), *vt = vec_alloc(n), *wt = vec_alloc(n), *yt = vec_alloc(n),
+           *zt = vec_alloc(n), *p = vec_alloc(n), *q = vec_alloc(n), *pt = vec_alloc(n),
+           *d = vec_alloc(n), *s = vec_alloc(n);
+    double rho = 0, rho_1 = 0, xi = 0, gamma = 1., gamma_1 = 0, theta = 0, theta_1 = 0,
+           eta = -1., delta = 0, ep = 0, beta = 0;
+    int ret = OC_OK;
+    oc_dv_set(n, b, r);
+    oc_csr_mult_add(n, rp, ci, a, -1, x, r);
+    oc_dv_set(n, r, vt);
+    oc_prec_apply(M, vt, y);
+    rho = oc_dv_norm2(n, y);
+    oc_dv_set(n, r, wt);
+    oc_prec_trans_apply(M, wt, z);
+    xi = oc_dv_norm2(n, z);
+    for (m->iter = 0;; m->iter++) {
+        int c = check(m, oc_dv_norm2(n, r));
+        if (c) { ret = c - 1; break; }
+        if (rho == 0) { ret = OC_BREAKDOWN; break; }
+        if (xi == 0) { ret = OC_BREAKDOWN; break; }
+        oc_dv_set_scaled(n, 1 / rho, vt, v);
+        oc_dv_scale(n, 1 / rho, y);
+        oc_dv_set_scaled(n, 1 / xi, wt, w);
+        oc_dv_scale(n, 1 / xi, z);
+        delta = oc_dv_dot(n, z, y);
+        if (delta == 0) { ret = OC_BREAKDOWN; break; }
+        oc_prec_apply(M, y, yt);
+        oc_prec_trans_apply(M, z, zt);
+        if (m->iter == 0) {
+            oc_dv_set(n, yt, p);
+            oc_dv_set(n, zt, q);
+        } else {
+            oc_dv_scale(n, -xi * delta / ep, p);
+            oc_dv_add(n, yt, p);
+            oc_dv_scale(n, -rho * delta / ep, q);
+            oc_dv_add(n, zt, q);
+        }
+        oc_csr_mult(n, rp, ci, a, p, pt);
+        ep = oc_dv_dot(n, q, pt);
+        if (ep == 0) { ret = OC_BREAKDOWN; break; }
+        beta = ep / delta;
+        if (beta == 0) { ret = OC_BREAKDOWN; break; }
+        oc_dv_set_scaled(n, -beta, v, vt);
+        oc_dv_add(n, pt, vt);
+        oc_prec_apply(M, vt, y);
+        rho_1 = rho;
+        rho = oc_dv_norm2(n, y);
+        oc_dv_set_scaled(n, -beta, w, wt);
+        oc_csr_trans_mult_add(n, n, rp, ci, a, 1, q, wt);
+        oc_prec_trans_apply(M, wt, z);
+        xi = oc_dv_norm2(n, z);
+        gamma_1 = gamma;
+        theta_1 = theta;
+        theta = rho / (gamma_1 * beta);
+        gamma = 1 / sqrt(1 + theta * theta);
+        if (gamma == 0) { ret = OC_BREAKDOWN; break; }
+        eta = -eta * rho_1 * gamma * gamma / (beta * gamma_1 * gamma_1);
+        if (m->iter == 0) {
+            oc_dv_set_scaled(n, eta, p, d);
+            oc_dv_set_scaled(n, eta, pt, s);
+        } else {
+            double val = theta_1 * theta_1 * gamma * gamma;
+            oc_dv_scale(n, val, d);
+            oc_dv_add_scaled(n, eta, p, d);
+            oc_dv_scale(n, val, s);
+            oc_dv_add_scaled(n, eta, pt, s);
+        }
+        oc_dv_add(n, d, x);
+        oc_dv_add_scaled(n, -1, s, r);
+    }
+    free(r); free(y); free(z); free(v); free(w); free(vt); free(wt); free(yt); free(zt);
+    free(p); free(q); free(pt); free(d); free(s);
+    return ret;
+}
+
 double oc_now(void) {
     struct timespec ts;
     clock_gettime(CLOCK_MONOTONIC, &ts);

@@ -154,6 +154,13 @@ int oc_ir(int n, const int32_t *rp, const int32_t *ci, const double *a,
           const oc_prec *M, const double *b, double *x, oc_monitor *m);
 
 /* ---- timing helper for the CPU baseline ------------------------------------ */
+/* S/BiCG.java:63-102, S/CGS.java:65-105, S/QMR.java:117-213 (M1 = M2 = M). */
+int oc_bicg(int n, const int32_t *rp, const int32_t *ci, const double *a,
+            const oc_prec *M, const double *b, double *x, oc_monitor *m);
+int oc_cgs(int n, const int32_t *rp, const int32_t *ci, const double *a,
+           const oc_prec *M, const double *b, double *x, oc_monitor *m);
+int oc_qmr(int n, const int32_t *rp, const int32_t *ci, const double *a,
+           const oc_prec *M, const double *b, double *x, oc_monitor *m);
 double oc_now(void);
 
 /* Matrix Market input (mtj_oracle_mm.c): CompRowMatrix(MatrixVectorReader)

@@ -78,6 +78,9 @@ def load():
         lib.oc_cg.argtypes = sol
         lib.oc_bicgstab.argtypes = sol
         lib.oc_ir.argtypes = sol
+        lib.oc_bicg.argtypes = sol
+        lib.oc_cgs.argtypes = sol
+        lib.oc_qmr.argtypes = sol
         lib.oc_gmres.argtypes = [C.c_int, P, P, P, C.POINTER(Prec), C.c_int, P, P,
                                  C.POINTER(Monitor)]
         _lib = lib
@@ -190,6 +193,8 @@ def solve(method, A, b, x0, prec=None, mon=None, restart=30):
         code = lib.oc_ir(*args, _p(b), _p(x), C.byref(mon))
     elif method == "gmres":
         code = lib.oc_gmres(*args, restart, _p(b), _p(x), C.byref(mon))
+    elif method in ("bicg", "cgs", "qmr"):
+        code = getattr(lib, "oc_" + method)(*args, _p(b), _p(x), C.byref(mon))
     else:
         raise ValueError(method)
     return code, x, mon.iter, mon.residual
