@@ -179,6 +179,17 @@ int mtjg_prec_diagonal_create(mtjg_ctx ctx, int32_t n, mtjg_prec *out);
 /* new ILU(CompRowMatrix LU): LU supplies the pattern of the factor
  * (S/ILU.java:54-62). */
 int mtjg_prec_ilu_create(mtjg_ctx ctx, mtjg_csr LU, mtjg_prec *out);
+/* new SSOR(F, reverse, omegaF, omegaR) (S/SSOR.java:81-93): symmetric
+ * successive over-relaxation; F supplies the size (square, else
+ * MTJG_E_INVALID_ARGUMENT), and each relaxation must lie in [0, 2]
+ * (S/SSOR.java:116-124).  setMatrix(A) takes A's pattern and values and
+ * fails with MTJG_E_MISSING_DIAGONAL on a row without a diagonal entry.
+ * apply(b, x) starts its sweeps from x's incoming values, as MTJ's does
+ * (S/SSOR.java:143-194); transApply is apply (S/SSOR.java:196-199). */
+int mtjg_prec_ssor_create(mtjg_ctx ctx, mtjg_csr F, int32_t reverse,
+                          double omega_f, double omega_r, mtjg_prec *out);
+/* setOmega(omegaF, omegaR) (S/SSOR.java:116-124). */
+int mtjg_prec_ssor_set_omega(mtjg_prec M, double omega_f, double omega_r);
 /* setMatrix(A) (S/Preconditioner.java:63; S/DiagonalPreconditioner.java:
  * 64-76; S/ILU.java:80-84).  On MTJG_E_ZERO_DIAGONAL, MTJG_E_MISSING_DIAGONAL
  * or MTJG_E_ZERO_PIVOT, *bad_row (may be NULL) is the 0-based row. */

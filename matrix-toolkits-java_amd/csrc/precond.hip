@@ -1,6 +1,7 @@
-// Preconditioners on the device: DiagonalPreconditioner (Jacobi) and ILU(0)
-// (S/DiagonalPreconditioner.java, S/ILU.java, S/UnitLowerCompRowMatrix.java,
-// S/UpperCompRowMatrix.java).
+// Preconditioners on the device: DiagonalPreconditioner (Jacobi), ILU(0) and
+// SSOR (S/DiagonalPreconditioner.java, S/ILU.java, S/UnitLowerCompRowMatrix.java,
+// S/UpperCompRowMatrix.java, S/SSOR.java).  SSOR's two sweeps use the same
+// level sets as ILU's triangular solves.
 //
 // ILU(0) is level scheduled.  Rows are grouped on the host into dependency
 // levels (a row's level is one more than the highest level among the rows its
@@ -51,6 +52,10 @@ struct mtjg_prec_s {
   uint32_t *sf_next = nullptr;
   int32_t *sf_err = nullptr;
   bool level_launch = true;   // false: single-launch solves (MTJG_ILU_SYNCFREE=1)
+  // SSOR: relaxations, reverse sweep flag, half iterate (values in lu)
+  double omega_f = 1, omega_r = 1;
+  bool reverse = true;
+  double *xx = nullptr;
 };
 
 namespace mtjg {
@@ -176,6 +181,28 @@ __global__ __launch_bounds__(BS) void k_trsv_level(int32_t cnt, const int32_t *r
     for (int32_t j = rp[i + 1] - 1; j > diag[i]; --j) s -= a[j] * x[ci[j]];
     x[i] = s;
   }
+}
+
+// One level of an SSOR sweep (S/SSOR.java:143-194), one thread per row in
+// MTJ's order: sigma adds the strictly-lower entries against xx and the
+// strictly-upper ones against xd, then sigma = (b_i - sigma) / a_ii.  The
+// forward sweep writes xx_i = xd_i + omega (sigma - xd_i) (levels of the
+// lower triangle; xd still holds x's incoming values), the backward sweep
+// xd_i = xx_i + omega (sigma - xx_i) (levels of the upper triangle, from the
+// last row up).
+template <bool FORWARD>
+__global__ __launch_bounds__(BS) void k_ssor_level(int32_t cnt, const int32_t *rows, const int32_t *rp,
+                                                   const int32_t *ci, const int32_t *diag, const double *a,
+                                                   const double *b, double *xd, double *xx, double omega) {
+  int32_t t = blockIdx.x * BS + threadIdx.x;
+  if (t >= cnt) return;
+  const int32_t i = rows[t];
+  double sigma = 0;
+  for (int32_t j = rp[i]; j < diag[i]; ++j) sigma += a[j] * xx[ci[j]];
+  for (int32_t j = diag[i] + 1; j < rp[i + 1]; ++j) sigma += a[j] * xd[ci[j]];
+  sigma = (b[i] - sigma) / a[diag[i]];
+  if (FORWARD) xx[i] = xd[i] + omega * (sigma - xd[i]);
+  else xd[i] = xx[i] + omega * (sigma - xx[i]);
 }
 
 // ---- single-launch triangular solve ----------------------------------------
@@ -330,6 +357,30 @@ static void run_syncfree(mtjg_prec M, const Levels &L, int slot, hipStream_t s, 
   HIPCHK(hipGetLastError());
 }
 
+// SSOR.apply (S/SSOR.java:143-194): forward sweep into xx, then the reverse
+// sweep into x, or x = xx when the reverse sweep is off.  MTJ's initial copy
+// xx = x is not needed: the forward sweep writes every xx_i before any row
+// reads it.
+static void run_ssor(mtjg_prec M, hipStream_t s, const double *b, double *x) {
+  const Levels &L = M->lev_l, &U = M->lev_u;
+  for (int32_t l = 0; l < L.nlev; ++l) {
+    int32_t cnt = L.ptr[l + 1] - L.ptr[l];
+    k_ssor_level<true><<<(cnt + BS - 1) / BS, BS, 0, s>>>(cnt, L.rows + L.ptr[l], M->rp, M->ci, M->diag, M->lu,
+                                                          b, x, M->xx, M->omega_f);
+  }
+  if (!M->reverse) {
+    if (M->n) HIPCHK(hipMemcpyAsync(x, M->xx, sizeof(double) * (size_t)M->n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipGetLastError());
+    return;
+  }
+  for (int32_t l = 0; l < U.nlev; ++l) {
+    int32_t cnt = U.ptr[l + 1] - U.ptr[l];
+    k_ssor_level<false><<<(cnt + BS - 1) / BS, BS, 0, s>>>(cnt, U.rows + U.ptr[l], M->rp, M->ci, M->diag, M->lu,
+                                                           b, x, M->xx, M->omega_r);
+  }
+  HIPCHK(hipGetLastError());
+}
+
 static void free_levels(Levels &L) {
   if (L.rows) (void)hipFree(L.rows);
   L.rows = nullptr;
@@ -369,9 +420,11 @@ static void reset_bad(mtjg_prec M) {
   HIPCHK(hipMemcpyAsync(M->bad, &big, sizeof(int32_t), hipMemcpyHostToDevice, M->ctx->stream));
 }
 
-// ILU.setMatrix: LU.set(A) takes A's pattern and values
-// (S/CompRowMatrix.java:449-471), then factor() (S/ILU.java:86-125).
-static void ilu_set_matrix(mtjg_prec M, mtjg_csr A, int32_t *bad_row) {
+// LU.set(A) / F.set(A) (S/CompRowMatrix.java:449-471): takes A's pattern
+// (rebuilding the diagonal positions and the lower / upper level sets when it
+// differs from the stored one) and values.  A missing diagonal fails with
+// `missing` + the 1-based row, MTJ's message.
+static void take_pattern(mtjg_prec M, mtjg_csr A, int32_t *bad_row, const char *missing) {
   mtjg_ctx ctx = M->ctx;
   hipStream_t s = ctx->stream;
   const int32_t n = A->nr;
@@ -407,7 +460,7 @@ static void ilu_set_matrix(mtjg_prec M, mtjg_csr A, int32_t *bad_row) {
       if (bad_row) *bad_row = b;
       ilu_free_storage(M);
       M->h_rp.clear();
-      fail(MTJG_E_MISSING_DIAGONAL, "Missing diagonal entry on row " + std::to_string(b + 1));
+      fail(MTJG_E_MISSING_DIAGONAL, missing + std::to_string(b + 1));
     }
     std::vector<int32_t> hd((size_t)n);
     if (n) HIPCHK(hipMemcpyAsync(hd.data(), M->diag, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, s));
@@ -428,6 +481,12 @@ static void ilu_set_matrix(mtjg_prec M, mtjg_csr A, int32_t *bad_row) {
   }
   if (A->nnz)
     HIPCHK(hipMemcpyAsync(M->lu, A->val, sizeof(double) * (size_t)A->nnz, hipMemcpyDeviceToDevice, s));
+}
+
+// ILU.setMatrix: LU.set(A), then factor() (S/ILU.java:86-125).
+static void ilu_set_matrix(mtjg_prec M, mtjg_csr A, int32_t *bad_row) {
+  hipStream_t s = M->ctx->stream;
+  take_pattern(M, A, bad_row, "Missing diagonal entry on row ");
   reset_bad(M);
   // rows of the first L level have no strictly-lower entries: nothing to do
   for (int32_t l = 1; l < M->lev_l.nlev; ++l) {
@@ -502,6 +561,9 @@ void prec_apply_dev(mtjg_prec M, int64_t n, const double *b, double *x, hipStrea
     if (b != x && n > 0) k_copy_if_running<<<grid_for(n), BS, 0, s>>>(n, b, x, st);
   } else if (M->kind == PREC_DIAGONAL) {
     if (n > 0) k_jacobi_apply<<<grid_for(n), BS, 0, s>>>(n, b, M->invdiag, x, st);
+  } else if (M->kind == PREC_SSOR) {
+    if (!M->factored) fail(MTJG_E_INVALID_ARGUMENT, "SSOR.setMatrix must precede apply");
+    run_ssor(M, s, b, x);
   } else {
     if (!M->factored) fail(MTJG_E_INVALID_ARGUMENT, "ILU.setMatrix must precede apply");
     // L.solve(b, y); U.solve(y, x) (S/ILU.java:64-70)
@@ -593,6 +655,38 @@ extern "C" int mtjg_prec_ilu_create(mtjg_ctx ctx, mtjg_csr LU, mtjg_prec *out) {
   MTJG_API_END
 }
 
+extern "C" int mtjg_prec_ssor_create(mtjg_ctx ctx, mtjg_csr F, int32_t reverse, double omega_f, double omega_r,
+                                     mtjg_prec *out) {
+  MTJG_API_BEGIN
+  if (!ctx || !F || !out) fail(MTJG_E_NULL, "null argument");
+  if (F->nr != F->nc) fail(MTJG_E_INVALID_ARGUMENT, "SSOR only applies to square matrices");
+  if (omega_f < 0 || omega_f > 2) fail(MTJG_E_INVALID_ARGUMENT, "omegaF must be between 0 and 2");
+  if (omega_r < 0 || omega_r > 2) fail(MTJG_E_INVALID_ARGUMENT, "omegaR must be between 0 and 2");
+  auto *M = new mtjg_prec_s();
+  M->ctx = ctx;
+  M->kind = PREC_SSOR;
+  M->n = F->nr;
+  M->reverse = reverse != 0;
+  M->omega_f = omega_f;
+  M->omega_r = omega_r;
+  HIPCHK(hipMalloc(&M->xx, sizeof(double) * (size_t)std::max(M->n, 1)));
+  HIPCHK(hipMemsetAsync(M->xx, 0, sizeof(double) * (size_t)std::max(M->n, 1), ctx->stream));
+  HIPCHK(hipMalloc(&M->bad, sizeof(int32_t)));
+  *out = M;
+  MTJG_API_END
+}
+
+extern "C" int mtjg_prec_ssor_set_omega(mtjg_prec M, double omega_f, double omega_r) {
+  MTJG_API_BEGIN
+  if (!M) fail(MTJG_E_NULL, "null argument");
+  if (M->kind != PREC_SSOR) fail(MTJG_E_INVALID_ARGUMENT, "not an SSOR preconditioner");
+  if (omega_f < 0 || omega_f > 2) fail(MTJG_E_INVALID_ARGUMENT, "omegaF must be between 0 and 2");
+  if (omega_r < 0 || omega_r > 2) fail(MTJG_E_INVALID_ARGUMENT, "omegaR must be between 0 and 2");
+  M->omega_f = omega_f;
+  M->omega_r = omega_r;
+  MTJG_API_END
+}
+
 extern "C" int mtjg_prec_set_matrix(mtjg_prec M, mtjg_csr A, int32_t *bad_row) {
   MTJG_API_BEGIN
   if (!M || !A) fail(MTJG_E_NULL, "null argument");
@@ -612,6 +706,11 @@ extern "C" int mtjg_prec_set_matrix(mtjg_prec M, mtjg_csr A, int32_t *bad_row) {
   } else if (M->kind == PREC_ILU) {
     if (A->nr != M->n || A->nc != M->n) fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "matrix size differs from ILU size");
     ilu_set_matrix(M, A, bad_row);
+  } else if (M->kind == PREC_SSOR) {
+    if (A->nr != M->n || A->nc != M->n) fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "matrix size differs from SSOR size");
+    M->factored = false;
+    take_pattern(M, A, bad_row, "Missing diagonal on row ");
+    M->factored = true;
   }
   MTJG_API_END
 }
@@ -621,7 +720,7 @@ static void check_apply(mtjg_prec M, mtjg_vec b, mtjg_vec x) {
   if (b->dtype != MTJG_F64 || x->dtype != MTJG_F64)
     fail(MTJG_E_INVALID_ARGUMENT, "Vector must be DenseVectors");
   if (b->n != M->n || x->n != M->n) fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "vector size differs from preconditioner size");
-  if (M->kind == PREC_ILU && b->ptr == x->ptr && M->n > 0)
+  if ((M->kind == PREC_ILU || M->kind == PREC_SSOR) && b->ptr == x->ptr && M->n > 0)
     fail(MTJG_E_INVALID_ARGUMENT, "b and x must not alias");
 }
 
@@ -650,6 +749,7 @@ extern "C" int mtjg_prec_destroy(mtjg_prec M) {
   (void)hipFree(M->y);
   (void)hipFree(M->sf_next);
   (void)hipFree(M->sf_err);
+  (void)hipFree(M->xx);
   ilu_free_storage(M);
   delete M;
   MTJG_API_END

@@ -459,6 +459,22 @@ class ILU(_Preconditioner):
         self.handle = h
 
 
+class SSOR(_Preconditioner):
+    """S/SSOR.java: SSOR(F, reverse=True, omegaF=1, omegaR=1).  apply starts
+    its sweeps from x's incoming values."""
+
+    def __init__(self, F, reverse=True, omegaF=1.0, omegaR=1.0, ctx=None):
+        self.ctx = ctx or F.ctx
+        self._lib = self.ctx._lib
+        h = C.c_void_p()
+        _ck(self._lib.mtjg_prec_ssor_create(self.ctx.handle, F.handle, int(bool(reverse)), float(omegaF),
+                                            float(omegaR), C.byref(h)))
+        self.handle = h
+
+    def setOmega(self, omegaF, omegaR):
+        _ck(self._lib.mtjg_prec_ssor_set_omega(self.handle, float(omegaF), float(omegaR)))
+
+
 # ---------------------------------------------------------------------------
 # monitor and solvers
 

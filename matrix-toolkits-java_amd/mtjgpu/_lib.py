@@ -92,6 +92,8 @@ PROTOTYPES = [
     ("mtjg_csr_row_histogram", C.c_int, [P, PI64, C.c_int]),
     ("mtjg_prec_diagonal_create", C.c_int, [P, I32, PP]),
     ("mtjg_prec_ilu_create", C.c_int, [P, P, PP]),
+    ("mtjg_prec_ssor_create", C.c_int, [P, P, I32, D, D, PP]),
+    ("mtjg_prec_ssor_set_omega", C.c_int, [P, D, D]),
     ("mtjg_prec_set_matrix", C.c_int, [P, P, PI32]),
     ("mtjg_prec_apply", C.c_int, [P, P, P]),
     ("mtjg_prec_trans_apply", C.c_int, [P, P, P]),

@@ -319,9 +319,70 @@ void oc_upper_trans_solve(int n, const int32_t *rp, const int32_t *ci,
  * (S/AbstractIterativeSolver.java:84-86); DiagonalPreconditioner
  * x_i = b_i * invdiag_i (S/DiagonalPreconditioner.java:54-55); ILU L.solve
  * then U.solve through y (S/ILU.java:64-70). */
+/* S/SSOR.java:81-141: the relaxations are checked by setOmega (:116-124),
+ * setMatrix copies A (F.set(A)) and locates the diagonals (:126-141). */
+int oc_prec_ssor(oc_prec *p, int n, const int32_t *rp, const int32_t *ci,
+                 const double *a, int reverse, double omega_f, double omega_r,
+                 int *bad_row) {
+    memset(p, 0, sizeof(*p));
+    p->kind = OC_PREC_SSOR;
+    p->n = n;
+    if (omega_f < 0 || omega_f > 2 || omega_r < 0 || omega_r > 2) return OC_BAD_ARG;
+    p->omega_f = omega_f;
+    p->omega_r = omega_r;
+    p->reverse = reverse;
+    int32_t nnz = rp[n];
+    p->rp = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
+    p->ci = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1));
+    p->lu = (double *)malloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
+    p->diagind = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    p->xx = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    memcpy(p->rp, rp, sizeof(int32_t) * (size_t)(n + 1));
+    memcpy(p->ci, ci, sizeof(int32_t) * (size_t)nnz);
+    memcpy(p->lu, a, sizeof(double) * (size_t)nnz);
+    for (int k = 0; k < n; ++k) {
+        p->diagind[k] = bsearch_range(p->ci, k, p->rp[k], p->rp[k + 1]);
+        if (p->diagind[k] < 0) {
+            if (bad_row) *bad_row = k;
+            return OC_MISSING_DIAGONAL;
+        }
+    }
+    return OC_OK;
+}
+
+/* S/SSOR.java:143-194: forward sweep from x's incoming values into xx, then
+ * (reverse) the backward sweep from xx into x. */
+static void ssor_apply(const oc_prec *p, const double *bd, double *xd) {
+    int n = p->n;
+    const int32_t *rowptr = p->rp, *colind = p->ci, *diagind = p->diagind;
+    const double *data = p->lu;
+    double *xx = p->xx;
+    memcpy(xx, xd, sizeof(double) * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        double sigma = 0;
+        for (int j = rowptr[i]; j < diagind[i]; ++j) sigma += data[j] * xx[colind[j]];
+        for (int j = diagind[i] + 1; j < rowptr[i + 1]; ++j) sigma += data[j] * xd[colind[j]];
+        sigma = (bd[i] - sigma) / data[diagind[i]];
+        xx[i] = xd[i] + p->omega_f * (sigma - xd[i]);
+    }
+    if (!p->reverse) {
+        memcpy(xd, xx, sizeof(double) * (size_t)n);
+        return;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double sigma = 0;
+        for (int j = rowptr[i]; j < diagind[i]; ++j) sigma += data[j] * xx[colind[j]];
+        for (int j = diagind[i] + 1; j < rowptr[i + 1]; ++j) sigma += data[j] * xd[colind[j]];
+        sigma = (bd[i] - sigma) / data[diagind[i]];
+        xd[i] = xx[i] + p->omega_r * (sigma - xx[i]);
+    }
+}
+
 void oc_prec_apply(const oc_prec *p, const double *b, double *x) {
     int n = p->n;
-    if (p->kind == OC_PREC_IDENTITY) {
+    if (p->kind == OC_PREC_SSOR) {
+        ssor_apply(p, b, x);
+    } else if (p->kind == OC_PREC_IDENTITY) {
         oc_dv_set(n, b, x);
     } else if (p->kind == OC_PREC_DIAGONAL) {
         for (int i = 0; i < n; ++i) x[i] = b[i] * p->invdiag[i];
@@ -331,7 +392,8 @@ void oc_prec_apply(const oc_prec *p, const double *b, double *x) {
     }
 }
 
-/* transApply: identity and Jacobi are symmetric; ILU is U' then L'
+/* transApply: identity and Jacobi are symmetric, SSOR assumes a symmetric
+ * matrix and applies itself (S/SSOR.java:196-199); ILU is U' then L'
  * (S/ILU.java:72-78). */
 void oc_prec_trans_apply(const oc_prec *p, const double *b, double *x) {
     int n = p->n;
@@ -350,6 +412,7 @@ void oc_prec_free(oc_prec *p) {
     free(p->lu);
     free(p->diagind);
     free(p->y);
+    free(p->xx);
     memset(p, 0, sizeof(*p));
 }
 

@@ -88,7 +88,7 @@ void oc_monitor_init(oc_monitor *m, int max_iter, double rtol, double atol,
 int oc_monitor_converged(oc_monitor *m, double r);
 
 /* ---- Preconditioners ------------------------------------------------------ */
-enum { OC_PREC_IDENTITY = 0, OC_PREC_DIAGONAL = 1, OC_PREC_ILU = 2 };
+enum { OC_PREC_IDENTITY = 0, OC_PREC_DIAGONAL = 1, OC_PREC_ILU = 2, OC_PREC_SSOR = 3 };
 
 typedef struct {
     int kind;
@@ -98,6 +98,9 @@ typedef struct {
     double *lu;          /* ILU: factor values */
     int32_t *diagind;    /* ILU: diagonal positions */
     double *y;           /* ILU: temporary */
+    double omega_f, omega_r; /* SSOR relaxation of the forward / reverse sweep */
+    int reverse;         /* SSOR: run the reverse sweep */
+    double *xx;          /* SSOR: half iterate */
 } oc_prec;
 
 void oc_prec_identity(oc_prec *p, int n);
@@ -107,6 +110,12 @@ int oc_prec_diagonal(oc_prec *p, int n, const int32_t *rp, const int32_t *ci,
 /* Returns OC_OK, OC_MISSING_DIAGONAL or OC_ZERO_PIVOT (row in *bad_row). */
 int oc_prec_ilu(oc_prec *p, int n, const int32_t *rp, const int32_t *ci,
                 const double *a, int *bad_row);
+/* SSOR(F, reverse, omegaF, omegaR) + setMatrix(A) (S/SSOR.java:81-141):
+ * OC_BAD_ARG for a relaxation outside [0, 2], OC_MISSING_DIAGONAL (row in
+ * *bad_row).  Its apply reads x's incoming values as the first iterate. */
+int oc_prec_ssor(oc_prec *p, int n, const int32_t *rp, const int32_t *ci,
+                 const double *a, int reverse, double omega_f, double omega_r,
+                 int *bad_row);
 void oc_prec_apply(const oc_prec *p, const double *b, double *x);
 void oc_prec_trans_apply(const oc_prec *p, const double *b, double *x);
 void oc_prec_free(oc_prec *p);

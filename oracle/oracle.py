@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "build", "libmtjoracle.so")
 
 OK, ITERATIONS, DIVERGENCE, BREAKDOWN = 0, 1, 2, 3
 ZERO_DIAGONAL, MISSING_DIAGONAL, ZERO_PIVOT, SINGULAR, BAD_ARG = 10, 11, 12, 13, 20
-PREC_IDENTITY, PREC_DIAGONAL, PREC_ILU = 0, 1, 2
+PREC_IDENTITY, PREC_DIAGONAL, PREC_ILU, PREC_SSOR = 0, 1, 2, 3
 
 
 class Monitor(C.Structure):
@@ -26,7 +26,8 @@ class Monitor(C.Structure):
 class Prec(C.Structure):
     _fields_ = [("kind", C.c_int), ("n", C.c_int), ("invdiag", C.c_void_p), ("rp", C.c_void_p),
                 ("ci", C.c_void_p), ("lu", C.c_void_p), ("diagind", C.c_void_p),
-                ("y", C.c_void_p)]
+                ("y", C.c_void_p), ("omega_f", C.c_double), ("omega_r", C.c_double),
+                ("reverse", C.c_int), ("xx", C.c_void_p)]
 
 
 _lib = None
@@ -66,6 +67,8 @@ def load():
         lib.oc_prec_identity.argtypes = [C.POINTER(Prec), C.c_int]
         lib.oc_prec_diagonal.argtypes = [C.POINTER(Prec), C.c_int, P, P, P, C.POINTER(C.c_int)]
         lib.oc_prec_ilu.argtypes = [C.POINTER(Prec), C.c_int, P, P, P, C.POINTER(C.c_int)]
+        lib.oc_prec_ssor.argtypes = [C.POINTER(Prec), C.c_int, P, P, P, C.c_int, C.c_double, C.c_double,
+                                     C.POINTER(C.c_int)]
         lib.oc_prec_apply.argtypes = [C.POINTER(Prec), P, P]
         lib.oc_prec_trans_apply.argtypes = [C.POINTER(Prec), P, P]
         lib.oc_prec_free.argtypes = [C.POINTER(Prec)]
@@ -128,9 +131,9 @@ class Csr:
 
 
 class Precond:
-    """Identity / DiagonalPreconditioner / ILU restatement."""
+    """Identity / DiagonalPreconditioner / ILU / SSOR restatement."""
 
-    def __init__(self, kind, A=None, n=None):
+    def __init__(self, kind, A=None, n=None, reverse=True, omega_f=1.0, omega_r=1.0):
         self.s = Prec()
         self.bad = C.c_int(-1)
         lib = load()
@@ -140,18 +143,22 @@ class Precond:
         elif kind == PREC_DIAGONAL:
             self.code = lib.oc_prec_diagonal(C.byref(self.s), A.nr, _p(A.rp), _p(A.ci), _p(A.va),
                                              C.byref(self.bad))
+        elif kind == PREC_SSOR:
+            self.code = lib.oc_prec_ssor(C.byref(self.s), A.nr, _p(A.rp), _p(A.ci), _p(A.va), int(reverse),
+                                         float(omega_f), float(omega_r), C.byref(self.bad))
         else:
             self.code = lib.oc_prec_ilu(C.byref(self.s), A.nr, _p(A.rp), _p(A.ci), _p(A.va),
                                         C.byref(self.bad))
         self.A = A
 
-    def apply(self, b):
-        x = np.empty(self.s.n)
+    def apply(self, b, x0=None):
+        """x = M^-1 b; x0 is x's incoming content (read by SSOR), zero if None."""
+        x = np.zeros(self.s.n) if x0 is None else np.array(x0, dtype=float)
         load().oc_prec_apply(C.byref(self.s), _p(np.ascontiguousarray(b, float)), _p(x))
         return x
 
-    def trans_apply(self, b):
-        x = np.empty(self.s.n)
+    def trans_apply(self, b, x0=None):
+        x = np.zeros(self.s.n) if x0 is None else np.array(x0, dtype=float)
         load().oc_prec_trans_apply(C.byref(self.s), _p(np.ascontiguousarray(b, float)), _p(x))
         return x
 
