@@ -315,6 +315,8 @@ def main():
     bytes_spmv = P.spmv_bytes(n, nnz)
     ctx = M.Context(0)
     A = M.CompRowMatrix(n, m, rp, ci, va, ctx=ctx)
+    fmt, stored = A.format()
+    kernel = "k_sell<double,double,SPMV_MULT>" if fmt == "sell" else "k_spmv<double,double,SPMV_MULT>"
     x = M.DenseVector(P.uniform(m, 1), ctx)
     y = M.DenseVector(n, ctx)
     for _ in range(args.warmup):
@@ -347,7 +349,7 @@ def main():
                    "pct_hbm_peak": round(100 * gbps / HBM_PEAK_GBPS, 2)},
         "roofline": {"bound": "hbm", "achieved": round(gbps, 2), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 4), "traffic": None,
-                     "kernel": "k_spmv<double,double,SPMV_MULT>",
+                     "kernel": kernel, "format": fmt, "stored_entries": stored,
                      "kernel_ms": round(per_ms, 5)},
     }
     traffic = measured_traffic(args.workload)

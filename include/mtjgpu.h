@@ -146,6 +146,14 @@ int mtjg_csr_shape(mtjg_csr A, int32_t *n_rows, int32_t *n_cols, int64_t *nnz);
  * copied to caller memory; any pointer may be NULL. */
 int mtjg_csr_download(mtjg_csr A, int32_t *row_pointer, int32_t *column_index,
                       double *data);
+/* Device storage chosen for the products (no MTJ counterpart): the row-block
+ * CSR kernels (MTJG_FORMAT_CSR) or, for patterns whose rows in each group of
+ * 64 have similar lengths, an added SELL-64 copy (MTJG_FORMAT_SELL; padding
+ * at most a quarter of the stored entries).  *stored counts the entries the
+ * products stream, padding included.  MTJG_SPMV_FORMAT=csr|sell in the
+ * environment at creation forces the choice. */
+enum { MTJG_FORMAT_CSR = 0, MTJG_FORMAT_SELL = 1 };
+int mtjg_csr_format(mtjg_csr A, int32_t *format, int64_t *stored);
 /* set(Matrix) with an identical pattern (S/CompRowMatrix.java:449-471). */
 int mtjg_csr_set_values(mtjg_csr A, const double *data);
 /* get(row, column): 0 outside the pattern (S/CompRowMatrix.java:406-417). */

@@ -131,6 +131,19 @@ struct mtjg_csr_s {
   int32_t nrb = 0;
   int32_t slabs = 1;
   int2 *rb = nullptr;
+  // SELL-64 copy used by the products when its padding is small: rows in
+  // slices of 64 (one wavefront, a lane per row), each slice stored column-
+  // major with the width of its longest row, padding columns -1 and values
+  // 0.  Slice s holds entries [sptr[s], sptr[s+1]).  `sell_stale` marks
+  // values changed since the copy was filled.
+  bool sell = false;
+  bool sell_stale = false;
+  int32_t nslice = 0;
+  int64_t h_sptr_total = 0;  // entries of the copy, padding included
+  int32_t *sptr = nullptr;
+  int32_t *scol = nullptr;
+  double *sval = nullptr;
+  float *sval32 = nullptr;
   // Host copy of the row pointers (schedules, level analysis, validation).
   std::vector<int32_t> h_rp;
   // Transpose (CSR of A'), built on first transposed product.

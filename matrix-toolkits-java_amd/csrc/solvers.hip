@@ -582,8 +582,11 @@ void free_red(GridRed &r) {
 // Sizes the reduction workspace for the largest grid a solve of A launches:
 // the vector kernels' vgrid(n) and the fused product's one workgroup per
 // row-block.  Counters start at zero and are left at zero by every launch.
+// Every reducing kernel of a solve runs at most VGRID_MAX (vector kernels)
+// or 2048 (fused products, row-block or SELL) workgroups.
 void ensure_red(mtjg_solver S, mtjg_csr A) {
-  int need = std::max(vgrid(S->n), A->nrb + 64);
+  (void)A;
+  const int need = std::max(VGRID_MAX, 2048);
   if (need <= S->red.cap) return;
   HIPCHK(hipStreamSynchronize(S->ctx->stream));
   free_red(S->red);

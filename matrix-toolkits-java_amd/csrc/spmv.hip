@@ -1,9 +1,15 @@
 // CompRowMatrix on the device and its CSR products (S/CompRowMatrix.java).
 //
-// SpMV kernels ("row-block stream"): one 256-thread workgroup per host-cut
-// row-block (the routine in spmv_rowblock.hpp), so stencil rows are summed
-// in MTJ's storage order and come out bitwise MTJ's.  The kernels are
-// HBM-bound (12 bytes per stored entry, 20 per row), so MFMA is not used.
+// Two device formats, both summing rows in MTJ's storage order:
+//  * SELL-64 (chosen when rows in each group of 64 have similar lengths, the
+//    padding being at most a quarter of the entries): a wavefront per slice
+//    of 64 rows, a lane per row, entries column-major within the slice so
+//    every load is coalesced and no LDS or barrier is needed; every row is
+//    summed sequentially, so every product is bitwise MTJ's;
+//  * row-block CSR (any pattern): one 256-thread workgroup per host-cut
+//    row-block (spmv_rowblock.hpp), bitwise MTJ's for blocks of >= 128 rows.
+// The kernels are HBM-bound (12 bytes per stored entry, 20 per row), so MFMA
+// is not used.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -96,6 +102,96 @@ __global__ __launch_bounds__(BS) void k_spmv_dot(int nrb, int slabs, const int2 
   if (grid_sum_dd<2>(v, red, tot) && threadIdx.x == 0) spmv_finish(st, fin, tot[0], tot[1]);
 }
 
+// Products from the SELL-64 copy: four slices (wavefronts) per workgroup.
+template <typename VT, typename XT, int MODE>
+__global__ __launch_bounds__(BS) void k_sell(int nslice, int n, const int32_t *__restrict__ sptr,
+                                             const int32_t *__restrict__ scol, const VT *__restrict__ sval,
+                                             const XT *__restrict__ x, XT *y, const XT *y0, double alpha,
+                                             double inv_alpha) {
+  const int slice = blockIdx.x * (BS / SELL_C) + (int)threadIdx.x / SELL_C;
+  if (slice >= nslice) return;
+  dd acc0{0.0, 0.0}, acc1{0.0, 0.0};
+  sell_row<VT, XT, MODE>(slice, threadIdx.x % SELL_C, n, sptr, scol, sval, x, y, y0, alpha, inv_alpha, nullptr,
+                         acc0, acc1);
+}
+
+// Fused solver product from the SELL-64 copy (see k_spmv_dot): at most
+// SPMV_DOT_GRID workgroups walk the slices in grid-stride order.
+template <typename VT, typename XT>
+__global__ __launch_bounds__(BS) void k_sell_dot(int nslice, int n, const int32_t *__restrict__ sptr,
+                                                 const int32_t *__restrict__ scol, const VT *__restrict__ sval,
+                                                 const XT *__restrict__ x, XT *y, const XT *w, KState *st,
+                                                 int fin, GridRed red) {
+  if (st->status != ST_RUNNING) return;
+  dd acc0{0.0, 0.0}, acc1{0.0, 0.0};
+  constexpr int WPB = BS / SELL_C;
+  for (int slice = blockIdx.x * WPB + (int)threadIdx.x / SELL_C; slice < nslice; slice += gridDim.x * WPB)
+    sell_row<VT, XT, SPMV_DOT>(slice, threadIdx.x % SELL_C, n, sptr, scol, sval, x, y, nullptr, 1.0, 1.0, w,
+                               acc0, acc1);
+  dd v[2] = {acc0, acc1};
+  double tot[2];
+  if (grid_sum_dd<2>(v, red, tot) && threadIdx.x == 0) spmv_finish(st, fin, tot[0], tot[1]);
+}
+
+// Refills the SELL-64 values from the CSR values (one thread per row).
+__global__ void k_sell_fill(int32_t nr, const int32_t *rp, const int32_t *sptr, const double *val, double *sval) {
+  const int32_t row = blockIdx.x * BS + threadIdx.x;
+  if (row >= nr) return;
+  const int32_t off = sptr[row / SELL_C] + row % SELL_C;
+  for (int32_t k = 0, j = rp[row]; j < rp[row + 1]; ++k, ++j) sval[off + k * SELL_C] = val[j];
+}
+
+__global__ void k_to_f32(int64_t n, const double *a, float *b) {
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+    b[i] = (float)a[i];
+}
+
+// Brings the SELL-64 values (and their FP32 copy when `fp32`) up to date.
+static void sell_ready(mtjg_csr A, bool fp32) {
+  hipStream_t s = A->ctx->stream;
+  if (A->sell_stale) {
+    k_sell_fill<<<(A->nr + BS - 1) / BS, BS, 0, s>>>(A->nr, A->rp, A->sptr, A->val, A->sval);
+    HIPCHK(hipGetLastError());
+    A->sell_stale = false;
+  }
+  if (fp32 && !A->sval32) {
+    const int64_t total = A->h_sptr_total;
+    HIPCHK(hipMalloc(&A->sval32, sizeof(float) * (size_t)std::max<int64_t>(total, 1)));
+    if (total) k_to_f32<<<grid_for(total), BS, 0, s>>>(total, A->sval, A->sval32);
+    HIPCHK(hipGetLastError());
+  }
+}
+
+template <typename VT, typename XT>
+static void sell_launch(mtjg_csr A, const VT *sval, const XT *x, XT *y, const SpmvArgs &a, hipStream_t s) {
+  const int grid = (A->nslice + BS / SELL_C - 1) / (BS / SELL_C);
+  const XT *y0 = (const XT *)a.y0;
+  switch (a.mode) {
+    case SPMV_MULT:
+      k_sell<VT, XT, SPMV_MULT><<<grid, BS, 0, s>>>(A->nslice, A->nr, A->sptr, A->scol, sval, x, y, y0, a.alpha,
+                                                   a.inv_alpha);
+      break;
+    case SPMV_MULTADD:
+      k_sell<VT, XT, SPMV_MULTADD><<<grid, BS, 0, s>>>(A->nslice, A->nr, A->sptr, A->scol, sval, x, y, y0,
+                                                      a.alpha, a.inv_alpha);
+      break;
+    case SPMV_TMULTADD:
+      k_sell<VT, XT, SPMV_TMULTADD><<<grid, BS, 0, s>>>(A->nslice, A->nr, A->sptr, A->scol, sval, x, y, y0,
+                                                       a.alpha, a.inv_alpha);
+      break;
+    case SPMV_DOT: {
+      const int g = std::min(grid, SPMV_DOT_GRID);
+      if (g > a.red.cap) fail(MTJG_E_INVALID_ARGUMENT, "reduction workspace smaller than the grid");
+      k_sell_dot<VT, XT><<<g, BS, 0, s>>>(A->nslice, A->nr, A->sptr, A->scol, sval, x, y, (const XT *)a.w, a.st,
+                                          a.fin, a.red);
+      break;
+    }
+    default:
+      fail(MTJG_E_INVALID_ARGUMENT, "bad spmv mode");
+  }
+  HIPCHK(hipGetLastError());
+}
+
 template <typename VT, typename XT>
 static void spmv_launch(mtjg_csr A, const VT *val, const XT *x, XT *y, const SpmvArgs &a,
                         hipStream_t s) {
@@ -134,17 +230,22 @@ static void spmv_launch(mtjg_csr A, const VT *val, const XT *x, XT *y, const Spm
 }
 
 void spmv_f64(mtjg_csr A, const double *x, double *y, const SpmvArgs &a, hipStream_t s) {
+  if (A->sell && A->nr > 0) {
+    sell_ready(A, false);
+    sell_launch<double, double>(A, A->sval, x, y, a, s);
+    return;
+  }
   spmv_launch<double, double>(A, A->val, x, y, a, s);
 }
 
 void spmv_f32(mtjg_csr A, const float *x, float *y, const SpmvArgs &a, hipStream_t s) {
+  if (A->sell && A->nr > 0) {
+    sell_ready(A, true);
+    sell_launch<float, float>(A, A->sval32, x, y, a, s);
+    return;
+  }
   csr_ensure_fp32(A);
   spmv_launch<float, float>(A, A->val32, x, y, a, s);
-}
-
-__global__ void k_to_f32(int64_t n, const double *a, float *b) {
-  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
-    b[i] = (float)a[i];
 }
 
 void csr_ensure_fp32(mtjg_csr A) {
@@ -165,7 +266,58 @@ static void csr_free(mtjg_csr_s *A) {
   (void)hipFree(A->val);
   (void)hipFree(A->val32);
   (void)hipFree(A->rb);
+  (void)hipFree(A->sptr);
+  (void)hipFree(A->scol);
+  (void)hipFree(A->sval);
+  (void)hipFree(A->sval32);
   delete A;
+}
+
+// Builds the SELL-64 copy when its padding is at most a quarter of the
+// stored entries (stencils and other regular patterns; MTJG_SPMV_FORMAT=csr
+// or =sell forces the choice).  Irregular row lengths keep the row-block
+// products only.
+static void sell_build(mtjg_csr_s *A, const int32_t *ci, const double *val) {
+  const int32_t nr = A->nr;
+  if (nr == 0) return;
+  int mode = 0;  // 0 auto, 1 never, 2 always
+  if (const char *e = getenv("MTJG_SPMV_FORMAT")) {
+    if (!strcmp(e, "csr")) mode = 1;
+    else if (!strcmp(e, "sell")) mode = 2;
+  }
+  if (mode == 1) return;
+  const int32_t nslice = (nr + SELL_C - 1) / SELL_C;
+  std::vector<int32_t> sptr((size_t)nslice + 1, 0);
+  int64_t total = 0;
+  for (int32_t sl = 0; sl < nslice; ++sl) {
+    int32_t w = 0;
+    const int32_t r1 = std::min(nr, (sl + 1) * SELL_C);
+    for (int32_t r = sl * SELL_C; r < r1; ++r) w = std::max(w, A->h_rp[r + 1] - A->h_rp[r]);
+    total += (int64_t)w * SELL_C;
+    if (total > INT32_MAX) return;
+    sptr[(size_t)sl + 1] = (int32_t)total;
+  }
+  if (mode == 0 && (double)total > 1.25 * (double)A->nnz + SELL_C) return;
+  std::vector<int32_t> scol((size_t)std::max<int64_t>(total, 1), -1);
+  std::vector<double> sval((size_t)std::max<int64_t>(total, 1), 0.0);
+  for (int32_t r = 0; r < nr; ++r) {
+    const int64_t off = sptr[(size_t)(r / SELL_C)] + r % SELL_C;
+    for (int32_t k = 0, j = A->h_rp[r]; j < A->h_rp[r + 1]; ++k, ++j) {
+      scol[(size_t)(off + (int64_t)k * SELL_C)] = ci[j];
+      sval[(size_t)(off + (int64_t)k * SELL_C)] = val[j];
+    }
+  }
+  hipStream_t s = A->ctx->stream;
+  HIPCHK(hipMalloc(&A->sptr, sizeof(int32_t) * sptr.size()));
+  HIPCHK(hipMalloc(&A->scol, sizeof(int32_t) * scol.size()));
+  HIPCHK(hipMalloc(&A->sval, sizeof(double) * sval.size()));
+  HIPCHK(hipMemcpyAsync(A->sptr, sptr.data(), sizeof(int32_t) * sptr.size(), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(A->scol, scol.data(), sizeof(int32_t) * scol.size(), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(A->sval, sval.data(), sizeof(double) * sval.size(), hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+  A->nslice = nslice;
+  A->h_sptr_total = total;
+  A->sell = true;
 }
 
 // Uploads a validated host CSR (sorted unique columns) and builds its
@@ -198,6 +350,7 @@ static mtjg_csr_s *csr_upload(mtjg_ctx ctx, int32_t nr, int32_t nc, std::vector<
     HIPCHK(hipMemcpyAsync(A->rb, rb.data(), sizeof(int2) * rb.size(), hipMemcpyHostToDevice,
                           ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    sell_build(A, ci, val);
   } catch (...) {
     csr_free(A);
     throw;
@@ -306,6 +459,14 @@ extern "C" int mtjg_csr_destroy(mtjg_csr A) {
   MTJG_API_END
 }
 
+extern "C" int mtjg_csr_format(mtjg_csr A, int32_t *format, int64_t *stored) {
+  MTJG_API_BEGIN
+  if (!A) fail(MTJG_E_NULL, "null matrix");
+  if (format) *format = A->sell ? MTJG_FORMAT_SELL : MTJG_FORMAT_CSR;
+  if (stored) *stored = A->sell ? A->h_sptr_total : A->nnz;
+  MTJG_API_END
+}
+
 extern "C" int mtjg_csr_shape(mtjg_csr A, int32_t *n_rows, int32_t *n_cols, int64_t *nnz) {
   MTJG_API_BEGIN
   if (!A) fail(MTJG_E_NULL, "null matrix");
@@ -331,10 +492,13 @@ extern "C" int mtjg_csr_download(mtjg_csr A, int32_t *row_pointer, int32_t *colu
 
 static void csr_values_changed(mtjg_csr A) {
   // derived copies of the values are rebuilt on next use
-  if (A->val32) {
+  if (A->sell) A->sell_stale = true;
+  if (A->val32 || A->sval32) {
     HIPCHK(hipStreamSynchronize(A->ctx->stream));
     (void)hipFree(A->val32);
+    (void)hipFree(A->sval32);
     A->val32 = nullptr;
+    A->sval32 = nullptr;
   }
   if (A->trans) {
     HIPCHK(hipStreamSynchronize(A->ctx->stream));

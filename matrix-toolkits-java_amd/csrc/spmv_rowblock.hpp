@@ -181,4 +181,45 @@ __device__ __forceinline__ void spmv_rowblock(int b, XT *prod, const int2 *__res
   }
 }
 
+// ---- SELL-64 -----------------------------------------------------------------
+
+constexpr int SELL_C = 64;       // rows per slice: one wavefront, a lane per row
+constexpr int SELL_UNROLL = 8;   // entries of a row in flight per lane
+
+// Row slice*64 + lane of y = A x (and the mode's epilogue) from the SELL-64
+// copy: the lane adds its row's products in storage order, exactly MTJ's
+// sequential loop (S/CompRowMatrix.java:304-312), loading eight entries and
+// their x values before using any; padding entries (column -1) are skipped.
+template <typename VT, typename XT, int MODE>
+__device__ __forceinline__ void sell_row(int slice, int lane, int n, const int32_t *__restrict__ sptr,
+                                         const int32_t *__restrict__ scol, const VT *__restrict__ sval,
+                                         const XT *__restrict__ x, XT *y, const XT *y0, double alpha,
+                                         double inv_alpha, const XT *w, dd &acc0, dd &acc1) {
+  const int row = slice * SELL_C + lane;
+  const int off = sptr[slice];
+  const int width = (sptr[slice + 1] - off) / SELL_C;
+  XT s = (row < n) ? spmv_init<XT, MODE>(row, y0, inv_alpha) : XT(0);
+  for (int k0 = 0; k0 < width; k0 += SELL_UNROLL) {
+    int c[SELL_UNROLL];
+    VT v[SELL_UNROLL];
+    XT xv[SELL_UNROLL];
+#pragma unroll
+    for (int u = 0; u < SELL_UNROLL; ++u) {
+      c[u] = -1;
+      if (k0 + u < width) {
+        const int e = off + (k0 + u) * SELL_C + lane;
+        c[u] = scol[e];
+        v[u] = sval[e];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SELL_UNROLL; ++u)
+      if (c[u] >= 0) xv[u] = x[c[u]];
+#pragma unroll
+    for (int u = 0; u < SELL_UNROLL; ++u)
+      if (c[u] >= 0) s += (XT)v[u] * xv[u];
+  }
+  if (row < n) spmv_store<XT, MODE>(row, s, y, y0, alpha, inv_alpha, w, acc0, acc1);
+}
+
 }  // namespace mtjg

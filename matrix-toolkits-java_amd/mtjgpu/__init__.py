@@ -319,6 +319,12 @@ class CompRowMatrix:
         _ck(self._lib.mtjg_csr_shape(self.handle, C.byref(r), C.byref(c), C.byref(z)))
         return r.value, c.value, z.value
 
+    def format(self):
+        """("csr" | "sell", entries streamed by a product incl. padding)."""
+        f, z = C.c_int32(), C.c_int64()
+        _ck(self._lib.mtjg_csr_format(self.handle, C.byref(f), C.byref(z)))
+        return ("sell" if f.value == 1 else "csr"), z.value
+
     def numRows(self):
         return self.shape()[0]
 

@@ -1,8 +1,11 @@
 """GPU parity of the CompRowMatrix products (S/CompRowMatrix.java:294-388)
-through the C ABI against the oracle.
+through the C ABI against the oracle, for each device format: the automatic
+choice, the row-block CSR kernels only (MTJG_SPMV_FORMAT=csr) and the SELL-64
+copy for every matrix (MTJG_SPMV_FORMAT=sell).
 
-Bar: bitwise equality wherever the kernel sums a row sequentially (rows in
-row-blocks of >= 128 rows: every stencil matrix); otherwise per row
+Bar: bitwise equality wherever the kernel sums a row sequentially (every row
+of the SELL-64 kernels; rows in row-blocks of >= 128 rows for the CSR
+kernels, so every stencil matrix); otherwise per row
 |y_gpu - y_ref| <= 1e-14 * sum_j |a_ij x_j| (only the summation order
 differs)."""
 import json
@@ -21,12 +24,23 @@ GOLD = os.path.join(os.path.dirname(__file__), "golden")
 TOL = 1e-14
 
 
+@pytest.fixture(params=["auto", "csr", "sell"], autouse=True)
+def spmv_format(request, monkeypatch):
+    # read when a matrix is created
+    monkeypatch.setenv("MTJG_SPMV_FORMAT", request.param)
+    return request.param
+
+
+def _sell_forced():
+    return os.environ.get("MTJG_SPMV_FORMAT") == "sell"
+
+
 def _abs_rowsum(rp, ci, va, x):
     return np.add.reduceat(np.abs(va * x[ci]), rp[:-1]) * (np.diff(rp) > 0) if len(va) else np.zeros(len(rp) - 1)
 
 
 def _check(y, ref, rp, ci, va, x, exact):
-    if exact:
+    if exact or _sell_forced():
         np.testing.assert_array_equal(y, ref)
     else:
         bound = TOL * _abs_rowsum(rp, ci, va, x) + 1e-300
@@ -89,6 +103,8 @@ def test_powerlaw_rows(ctx, n, seed):
     y.upload(yh)
     got = A.multAdd(1.5, x, y).getData()
     ref = O.mult_add(1.5, xh, yh)
+    if _sell_forced():
+        np.testing.assert_array_equal(got, ref)
     assert np.all(np.abs(got - ref) <= 1.5 * TOL * _abs_rowsum(rp, ci, va, xh) + 1e-15 * np.abs(ref) + 1e-300)
     # transpose: compare to the transposed oracle bound
     m = sp.csr_matrix((va, ci, rp), shape=(n, n))
@@ -177,3 +193,24 @@ def test_run_to_run_deterministic(ctx):
     a = A.mult(x, M.DenseVector(30000, ctx)).getData()
     for _ in range(3):
         np.testing.assert_array_equal(A.mult(x, M.DenseVector(30000, ctx)).getData(), a)
+
+
+def test_value_updates_reach_every_copy(ctx):
+    # setValues / set change the CSR values; the SELL-64 copy (and the FP32
+    # copies) must follow before the next product
+    sysm = P.varcoef_poisson(3, 14, 12, 10, seed=7)
+    n, m, rp, ci, va = sysm
+    A, _ = _pair(ctx, sysm)
+    xh = P.uniform(m, 2)
+    x = M.DenseVector(xh, ctx)
+    y = M.DenseVector(n, ctx)
+    A.mult(x, y)
+    va2 = va * 1.5 + 0.25
+    A.setValues(va2)
+    _check(A.mult(x, y).getData(), oracle.Csr(n, m, rp, ci, va2).mult(xh), rp, ci, va2, xh,
+           exact=A.format()[0] == "sell")
+    A.set(5, int(ci[rp[5]]), -3.0)
+    va3 = va2.copy()
+    va3[rp[5]] = -3.0
+    _check(A.mult(x, y).getData(), oracle.Csr(n, m, rp, ci, va3).mult(xh), rp, ci, va3, xh,
+           exact=A.format()[0] == "sell")
