@@ -4,10 +4,11 @@
 // inner Jacobi-CG on r / ||r|| to a relative tolerance, FP64 update
 // x += ||r|| d), but every active system advances in the same launches.
 //
-// Work layout: system s is cut into nb_s row tiles of rows_per_wg rows, and
-// each tile into SpMV row-blocks (spmv_rowblock.hpp) that never cross a tile
-// boundary; a device work table maps each workgroup of a launch to
-// (slot, tile, the slot's partial-sum offset).  One batch iteration is three
+// Work layout: system s is cut into nb_s row tiles of rows_per_wg rows (a
+// multiple of 256, so of the 64-row slices of the system's SELL-64 copy,
+// spmv_rowblock.hpp); a device work table maps each workgroup of a launch to
+// (slot, tile, the slot's partial-sum offset), and the workgroup's four
+// wavefronts take the tile's slices in turn.  One batch iteration is three
 // launches over the whole table:
 //   B1  phase INNER: q = A32 p and p.q per slot -> alpha
 //       phase OUTER: r = b - A64 x (MTJ's multAdd order) and ||r||, then the
@@ -52,11 +53,12 @@ struct BSlot {
   const int32_t *rp;
   const int32_t *ci;
   const double *v64;
-  const float *v32;
   const float *inv;
   const double *b;
-  const int2 *info;        // row-blocks (first row, first entry), tile-aligned
-  const int32_t *tile_blk; // first row-block of each tile, nb + 1 entries
+  const int32_t *sptr;     // SELL-64 copy: slice offsets, columns, values
+  const int32_t *scol;
+  const double *sv64;
+  const float *sv32;
   double *x, *r;
   float *rr, *d, *p, *q, *z;
   int64_t rid;             // request id (host bookkeeping)
@@ -125,29 +127,27 @@ __device__ bool slot_sum(dd (&v)[K], BSlot &S, const BWork &w, const BDev &d, do
        i < i##_end; i += BS)
 
 __global__ __launch_bounds__(BS) void k_batch_b1(BDev d) {
-  __shared__ double prod[SPMV_CAP];
   if ((int)blockIdx.x >= *d.nwork) return;
   const BWork w = d.work[blockIdx.x];
   BSlot &S = d.slots[w.slot];
   if (!slot_live(S)) return;
   const int phase = S.phase;
   if (phase == PH_START) return;
-  const int blk0 = S.tile_blk[w.tile], blk1 = S.tile_blk[w.tile + 1];
+  const int per_tile = S.rows_per_wg / SELL_C;
+  const int nslice = (S.n + SELL_C - 1) / SELL_C;
+  const int sl_end = min(nslice, (w.tile + 1) * per_tile);
+  const int wave = threadIdx.x / SELL_C, lane = threadIdx.x % SELL_C;
   dd acc0{0.0, 0.0}, acc1{0.0, 0.0};
   if (phase == PH_INNER) {
-    // q = A32 p with p.q (row-block SpMV, FP32 values and vectors)
-    for (int blk = blk0; blk < blk1; ++blk) {
-      if (blk > blk0) __syncthreads();
-      spmv_rowblock<float, float, SPMV_DOT>(blk, reinterpret_cast<float *>(prod), S.info, S.rp, S.ci, S.v32,
-                                            S.p, S.q, nullptr, 1.0, 1.0, S.p, acc0, acc1);
-    }
+    // q = A32 p with p.q (SELL-64 rows, FP32 values and vectors)
+    for (int sl = w.tile * per_tile + wave; sl < sl_end; sl += BS / SELL_C)
+      sell_row<float, float, SPMV_DOT>(sl, lane, S.n, S.sptr, S.scol, S.sv32, S.p, S.q, nullptr, 1.0, 1.0, S.p,
+                                       acc0, acc1);
   } else {
     // r = b + (-1) * (A64 x): MTJ's multAdd order, with r.r
-    for (int blk = blk0; blk < blk1; ++blk) {
-      if (blk > blk0) __syncthreads();
-      spmv_rowblock<double, double, SPMV_RESID>(blk, prod, S.info, S.rp, S.ci, S.v64, S.x, S.r, S.b, -1.0,
-                                                1.0, nullptr, acc0, acc1);
-    }
+    for (int sl = w.tile * per_tile + wave; sl < sl_end; sl += BS / SELL_C)
+      sell_row<double, double, SPMV_RESID>(sl, lane, S.n, S.sptr, S.scol, S.sv64, S.x, S.r, S.b, -1.0, 1.0,
+                                           nullptr, acc0, acc1);
     acc0 = acc1;
   }
   dd v[1] = {acc0};
@@ -247,14 +247,19 @@ __global__ __launch_bounds__(BS) void k_batch_b3(BDev d) {
   if (act == A3_PROMOTE && w.tile == 0 && threadIdx.x == 0) S.outer++;
 }
 
-__global__ void k_invdiag_f32(int32_t n, const int32_t *rp, const int32_t *ci, const double *v, float *inv,
-                              float *v32, int64_t nnz) {
-  int64_t t = blockIdx.x * (int64_t)BS + threadIdx.x;
-  for (int64_t k = t; k < nnz; k += (int64_t)gridDim.x * BS) v32[k] = (float)v[k];
-  for (int64_t i = t; i < n; i += (int64_t)gridDim.x * BS) {
+// Admission: the FP32 inverse diagonal and the row's lane of the SELL-64
+// copy (columns, FP64 and FP32 values; padding was preset to -1 / 0).
+__global__ void k_batch_setup(int32_t n, const int32_t *rp, const int32_t *ci, const double *v,
+                              const int32_t *sptr, float *inv, int32_t *scol, double *sv64, float *sv32) {
+  for (int32_t i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+    const int32_t off = sptr[i / SELL_C] + i % SELL_C;
     double dg = 0;
-    for (int32_t k = rp[i]; k < rp[i + 1]; ++k)
-      if (ci[k] == (int32_t)i) dg = v[k];
+    for (int32_t k = 0, j = rp[i]; j < rp[i + 1]; ++k, ++j) {
+      if (ci[j] == i) dg = v[j];
+      scol[off + k * SELL_C] = ci[j];
+      sv64[off + k * SELL_C] = v[j];
+      sv32[off + k * SELL_C] = (float)v[j];
+    }
     inv[i] = (float)(1.0 / dg);
   }
 }
@@ -314,8 +319,8 @@ struct Request {
   int32_t n;
   std::vector<int32_t> rp, ci;
   std::vector<double> val, b;
-  std::vector<int2> info;       // tile-aligned row-blocks
-  std::vector<int32_t> tile_blk;
+  std::vector<int32_t> sptr;    // SELL-64 slice offsets
+  int32_t nb = 0;               // tiles
   double rtol;
   int32_t max_outer, max_inner;
   double inner_rtol;
@@ -337,41 +342,26 @@ struct HostSlot {
 inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 size_t req_bytes(const Request &R) {
-  const size_t n = (size_t)R.n, nnz = R.ci.size();
-  return al256(4 * (n + 1)) + al256(4 * nnz) + al256(8 * nnz) + al256(4 * nnz) + al256(4 * n) + 3 * al256(8 * n) +
-         5 * al256(4 * n) + al256(sizeof(int2) * R.info.size()) + al256(4 * R.tile_blk.size());
+  const size_t n = (size_t)R.n, nnz = R.ci.size(), sell = (size_t)R.sptr.back();
+  return al256(4 * (n + 1)) + al256(4 * nnz) + al256(8 * nnz) + al256(4 * n) + 3 * al256(8 * n) +
+         5 * al256(4 * n) + al256(4 * R.sptr.size()) + al256(4 * sell) + al256(8 * sell) + al256(4 * sell);
 }
 
-// Row-blocks of each tile of `rows` rows, cut as for the CSR products
-// (at most SPMV_CAP entries and SPMV_MAX_ROWS rows, a longer row alone).
-void build_tile_blocks(Request &R, int rows) {
+// Slice offsets of the system's SELL-64 copy (each slice as wide as its
+// longest row) and its tile count for tiles of `rows` rows.
+void build_sell(Request &R, int rows) {
   const int32_t n = R.n;
-  const int nb = (n + rows - 1) / rows;
-  R.info.clear();
-  R.tile_blk.assign((size_t)nb + 1, 0);
-  R.info.push_back(make_int2(0, 0));
-  for (int t = 0; t < nb; ++t) {
-    R.tile_blk[t] = (int32_t)R.info.size() - 1;
-    const int32_t end = std::min(n, (t + 1) * rows);
-    int32_t r = t * rows;
-    while (r < end) {
-      const int32_t len = R.rp[r + 1] - R.rp[r];
-      if (len > SPMV_CAP) {
-        ++r;
-      } else {
-        int32_t acc = 0, cnt = 0;
-        while (r < end && cnt < SPMV_MAX_ROWS) {
-          const int32_t l = R.rp[r + 1] - R.rp[r];
-          if (acc + l > SPMV_CAP) break;
-          acc += l;
-          ++r;
-          ++cnt;
-        }
-      }
-      R.info.push_back(make_int2(r, R.rp[r]));
-    }
+  const int32_t nslice = (n + SELL_C - 1) / SELL_C;
+  R.sptr.assign((size_t)nslice + 1, 0);
+  int64_t total = 0;
+  for (int32_t sl = 0; sl < nslice; ++sl) {
+    int32_t w = 0;
+    for (int32_t r = sl * SELL_C; r < std::min(n, (sl + 1) * SELL_C); ++r) w = std::max(w, R.rp[r + 1] - R.rp[r]);
+    total += (int64_t)w * SELL_C;
+    if (total > INT32_MAX) fail(MTJG_E_UNSUPPORTED, "system too large for the batch engine");
+    R.sptr[(size_t)sl + 1] = (int32_t)total;
   }
-  R.tile_blk[nb] = (int32_t)R.info.size() - 1;
+  R.nb = (n + rows - 1) / rows;
 }
 
 }  // namespace
@@ -502,7 +492,7 @@ int admit(mtjg_batch B, std::vector<int> &new_slots) {
         break;
       }
     if (slot < 0) break;
-    const int nb = (int)R.tile_blk.size() - 1;
+    const int nb = R.nb;
     if (used + nb > B->max_work) break;
     const int64_t nnz = (int64_t)R.ci.size();
     const size_t bytes = req_bytes(R);
@@ -517,7 +507,6 @@ int admit(mtjg_batch B, std::vector<int> &new_slots) {
     int32_t *rp = (int32_t *)take(4 * (size_t)(R.n + 1));
     int32_t *ci = (int32_t *)take(4 * (size_t)nnz);
     double *v64 = (double *)take(8 * (size_t)nnz);
-    float *v32 = (float *)take(4 * (size_t)nnz);
     float *inv = (float *)take(4 * (size_t)R.n);
     double *b = (double *)take(8 * (size_t)R.n);
     double *x = (double *)take(8 * (size_t)R.n);
@@ -527,26 +516,34 @@ int admit(mtjg_batch B, std::vector<int> &new_slots) {
     float *pv = (float *)take(4 * (size_t)R.n);
     float *qv = (float *)take(4 * (size_t)R.n);
     float *zv = (float *)take(4 * (size_t)R.n);
-    int2 *info = (int2 *)take(sizeof(int2) * R.info.size());
-    int32_t *tb = (int32_t *)take(4 * R.tile_blk.size());
+    const size_t sell = (size_t)R.sptr.back();
+    int32_t *sptr = (int32_t *)take(4 * R.sptr.size());
+    int32_t *scol = (int32_t *)take(4 * sell);
+    double *sv64 = (double *)take(8 * sell);
+    float *sv32 = (float *)take(4 * sell);
     HIPCHK(hipMemcpyAsync(rp, R.rp.data(), 4 * (size_t)(R.n + 1), hipMemcpyHostToDevice, cs));
     HIPCHK(hipMemcpyAsync(ci, R.ci.data(), 4 * (size_t)nnz, hipMemcpyHostToDevice, cs));
     HIPCHK(hipMemcpyAsync(v64, R.val.data(), 8 * (size_t)nnz, hipMemcpyHostToDevice, cs));
     HIPCHK(hipMemcpyAsync(b, R.b.data(), 8 * (size_t)R.n, hipMemcpyHostToDevice, cs));
-    HIPCHK(hipMemcpyAsync(info, R.info.data(), sizeof(int2) * R.info.size(), hipMemcpyHostToDevice, cs));
-    HIPCHK(hipMemcpyAsync(tb, R.tile_blk.data(), 4 * R.tile_blk.size(), hipMemcpyHostToDevice, cs));
+    HIPCHK(hipMemcpyAsync(sptr, R.sptr.data(), 4 * R.sptr.size(), hipMemcpyHostToDevice, cs));
+    if (sell) {
+      HIPCHK(hipMemsetAsync(scol, 0xFF, 4 * sell, cs));  // padding column -1
+      HIPCHK(hipMemsetAsync(sv64, 0, 8 * sell, cs));
+      HIPCHK(hipMemsetAsync(sv32, 0, 4 * sell, cs));
+    }
     k_zero_f64<<<grid_for(R.n), BS, 0, cs>>>(R.n, x);
-    k_invdiag_f32<<<grid_for(std::max<int64_t>(nnz, R.n)), BS, 0, cs>>>(R.n, rp, ci, v64, inv, v32, nnz);
+    k_batch_setup<<<grid_for(R.n), BS, 0, cs>>>(R.n, rp, ci, v64, sptr, inv, scol, sv64, sv32);
     HIPCHK(hipGetLastError());
     BSlot S{};
     S.rp = rp;
     S.ci = ci;
     S.v64 = v64;
-    S.v32 = v32;
+    S.sptr = sptr;
+    S.scol = scol;
+    S.sv64 = sv64;
+    S.sv32 = sv32;
     S.inv = inv;
     S.b = b;
-    S.info = info;
-    S.tile_blk = tb;
     S.x = x;
     S.r = r;
     S.rr = rr;
@@ -718,8 +715,8 @@ extern "C" int mtjg_batch_submit(mtjg_batch B, int32_t n, int64_t nnz, const int
   Request R;
   R.n = n;
   R.rp.assign(row_pointer, row_pointer + n + 1);
-  build_tile_blocks(R, B->rows_per_wg);
-  const int nb = (int)R.tile_blk.size() - 1;
+  build_sell(R, B->rows_per_wg);
+  const int nb = R.nb;
   R.ci.assign(column_index, column_index + nnz);
   if (nb > B->max_work || req_bytes(R) > B->arena.size)
     fail(MTJG_E_OUT_OF_MEMORY, "system larger than the batch can ever hold");

@@ -133,12 +133,17 @@ __global__ __launch_bounds__(BS) void k_sell_dot(int nslice, int n, const int32_
   if (grid_sum_dd<2>(v, red, tot) && threadIdx.x == 0) spmv_finish(st, fin, tot[0], tot[1]);
 }
 
-// Refills the SELL-64 values from the CSR values (one thread per row).
-__global__ void k_sell_fill(int32_t nr, const int32_t *rp, const int32_t *sptr, const double *val, double *sval) {
+// Copies each row's values (and, with scol, its columns) from the CSR
+// arrays into its lane of the SELL-64 copy (one thread per row).
+__global__ void k_sell_fill(int32_t nr, const int32_t *rp, const int32_t *ci, const int32_t *sptr,
+                            const double *val, int32_t *scol, double *sval) {
   const int32_t row = blockIdx.x * BS + threadIdx.x;
   if (row >= nr) return;
   const int32_t off = sptr[row / SELL_C] + row % SELL_C;
-  for (int32_t k = 0, j = rp[row]; j < rp[row + 1]; ++k, ++j) sval[off + k * SELL_C] = val[j];
+  for (int32_t k = 0, j = rp[row]; j < rp[row + 1]; ++k, ++j) {
+    sval[off + k * SELL_C] = val[j];
+    if (scol) scol[off + k * SELL_C] = ci[j];
+  }
 }
 
 __global__ void k_to_f32(int64_t n, const double *a, float *b) {
@@ -150,7 +155,7 @@ __global__ void k_to_f32(int64_t n, const double *a, float *b) {
 static void sell_ready(mtjg_csr A, bool fp32) {
   hipStream_t s = A->ctx->stream;
   if (A->sell_stale) {
-    k_sell_fill<<<(A->nr + BS - 1) / BS, BS, 0, s>>>(A->nr, A->rp, A->sptr, A->val, A->sval);
+    k_sell_fill<<<(A->nr + BS - 1) / BS, BS, 0, s>>>(A->nr, A->rp, A->ci, A->sptr, A->val, nullptr, A->sval);
     HIPCHK(hipGetLastError());
     A->sell_stale = false;
   }
@@ -276,8 +281,9 @@ static void csr_free(mtjg_csr_s *A) {
 // Builds the SELL-64 copy when its padding is at most a quarter of the
 // stored entries (stencils and other regular patterns; MTJG_SPMV_FORMAT=csr
 // or =sell forces the choice).  Irregular row lengths keep the row-block
-// products only.
-static void sell_build(mtjg_csr_s *A, const int32_t *ci, const double *val) {
+// products only.  The slice widths come from the host row pointers; the
+// device fills the copy from its own CSR arrays.
+static void sell_build(mtjg_csr_s *A) {
   const int32_t nr = A->nr;
   if (nr == 0) return;
   int mode = 0;  // 0 auto, 1 never, 2 always
@@ -298,22 +304,16 @@ static void sell_build(mtjg_csr_s *A, const int32_t *ci, const double *val) {
     sptr[(size_t)sl + 1] = (int32_t)total;
   }
   if (mode == 0 && (double)total > 1.25 * (double)A->nnz + SELL_C) return;
-  std::vector<int32_t> scol((size_t)std::max<int64_t>(total, 1), -1);
-  std::vector<double> sval((size_t)std::max<int64_t>(total, 1), 0.0);
-  for (int32_t r = 0; r < nr; ++r) {
-    const int64_t off = sptr[(size_t)(r / SELL_C)] + r % SELL_C;
-    for (int32_t k = 0, j = A->h_rp[r]; j < A->h_rp[r + 1]; ++k, ++j) {
-      scol[(size_t)(off + (int64_t)k * SELL_C)] = ci[j];
-      sval[(size_t)(off + (int64_t)k * SELL_C)] = val[j];
-    }
-  }
   hipStream_t s = A->ctx->stream;
+  const size_t t1 = (size_t)std::max<int64_t>(total, 1);
   HIPCHK(hipMalloc(&A->sptr, sizeof(int32_t) * sptr.size()));
-  HIPCHK(hipMalloc(&A->scol, sizeof(int32_t) * scol.size()));
-  HIPCHK(hipMalloc(&A->sval, sizeof(double) * sval.size()));
+  HIPCHK(hipMalloc(&A->scol, sizeof(int32_t) * t1));
+  HIPCHK(hipMalloc(&A->sval, sizeof(double) * t1));
   HIPCHK(hipMemcpyAsync(A->sptr, sptr.data(), sizeof(int32_t) * sptr.size(), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(A->scol, scol.data(), sizeof(int32_t) * scol.size(), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(A->sval, sval.data(), sizeof(double) * sval.size(), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemsetAsync(A->scol, 0xFF, sizeof(int32_t) * t1, s));  // padding column -1
+  HIPCHK(hipMemsetAsync(A->sval, 0, sizeof(double) * t1, s));
+  k_sell_fill<<<(nr + BS - 1) / BS, BS, 0, s>>>(nr, A->rp, A->ci, A->sptr, A->val, A->scol, A->sval);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s));
   A->nslice = nslice;
   A->h_sptr_total = total;
@@ -350,7 +350,7 @@ static mtjg_csr_s *csr_upload(mtjg_ctx ctx, int32_t nr, int32_t nc, std::vector<
     HIPCHK(hipMemcpyAsync(A->rb, rb.data(), sizeof(int2) * rb.size(), hipMemcpyHostToDevice,
                           ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    sell_build(A, ci, val);
+    sell_build(A);
   } catch (...) {
     csr_free(A);
     throw;

@@ -86,10 +86,17 @@ def test_admission_waits_for_memory(ctx):
 
 
 def S_bytes(sysm):
+    """Arena bytes of one admitted system: CSR, FP32 inverse diagonal, FP64
+    b x r, FP32 rr d p q z, and the SELL-64 copy (slice offsets, columns,
+    FP64 and FP32 values, each slice as wide as its longest row)."""
     n, rp, ci, va, b = sysm
     nnz = len(ci)
     al = lambda v: (v + 255) // 256 * 256
-    return al(4 * (n + 1)) + al(4 * nnz) + al(8 * nnz) + al(4 * nnz) + al(4 * n) + 3 * al(8 * n) + 5 * al(4 * n)
+    lens = np.diff(rp)
+    nslice = (n + 63) // 64
+    sell = sum(int(lens[64 * s:64 * s + 64].max()) * 64 for s in range(nslice))
+    return (al(4 * (n + 1)) + al(4 * nnz) + al(8 * nnz) + al(4 * n) + 3 * al(8 * n) + 5 * al(4 * n)
+            + al(4 * (nslice + 1)) + al(4 * sell) + al(8 * sell) + al(4 * sell))
 
 
 def test_nan_system_is_isolated(ctx):
