@@ -196,6 +196,17 @@ int mtjg_prec_ilu_create(mtjg_ctx ctx, mtjg_csr LU, mtjg_prec *out);
  * (S/SSOR.java:143-194); transApply is apply (S/SSOR.java:196-199). */
 int mtjg_prec_ssor_create(mtjg_ctx ctx, mtjg_csr F, int32_t reverse,
                           double omega_f, double omega_r, mtjg_prec *out);
+/* new ICC(CompRowMatrix R) (S/ICC.java:57-63): incomplete Cholesky without
+ * fill-in; R supplies the size (square, else MTJG_E_INVALID_ARGUMENT).
+ * setMatrix(A) takes A's pattern and values and factors as
+ * S/ICC.java:90-136 does, bit for bit (the factorization runs on the host:
+ * each row is eliminated against the preceding rows its fill reaches);
+ * MTJG_E_MISSING_DIAGONAL, or MTJG_E_ZERO_DIAGONAL when a row's pivot is
+ * zero before its square root, with *bad_row.  A negative pivot gives NaN,
+ * as in MTJ.  apply(b, x) is R'y = b then R x = y on the device
+ * (S/ICC.java:65-71); transApply is apply (S/ICC.java:73-75); b and x may be
+ * the same vector. */
+int mtjg_prec_icc_create(mtjg_ctx ctx, mtjg_csr R, mtjg_prec *out);
 /* setOmega(omegaF, omegaR) (S/SSOR.java:116-124). */
 int mtjg_prec_ssor_set_omega(mtjg_prec M, double omega_f, double omega_r);
 /* setMatrix(A) (S/Preconditioner.java:63; S/DiagonalPreconditioner.java:

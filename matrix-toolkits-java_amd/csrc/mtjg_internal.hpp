@@ -198,7 +198,7 @@ mtjg_csr_s *csr_transpose(mtjg_csr A);
 void csr_ensure_fp32(mtjg_csr A);
 
 // ---- preconditioners (precond.hip) ----
-enum PrecKind : int { PREC_IDENTITY = 0, PREC_DIAGONAL = 1, PREC_ILU = 2, PREC_SSOR = 3 };
+enum PrecKind : int { PREC_IDENTITY = 0, PREC_DIAGONAL = 1, PREC_ILU = 2, PREC_SSOR = 3, PREC_ICC = 4 };
 // Applies M (identity when M is null) to n entries; kernels skip work once st's
 // status leaves running (st may be null).
 void prec_apply_dev(mtjg_prec M, int64_t n, const double *b, double *x, hipStream_t s, KState *st);

@@ -1,7 +1,8 @@
-// Preconditioners on the device: DiagonalPreconditioner (Jacobi), ILU(0) and
-// SSOR (S/DiagonalPreconditioner.java, S/ILU.java, S/UnitLowerCompRowMatrix.java,
-// S/UpperCompRowMatrix.java, S/SSOR.java).  SSOR's two sweeps use the same
-// level sets as ILU's triangular solves.
+// Preconditioners on the device: DiagonalPreconditioner (Jacobi), ILU(0),
+// SSOR and ICC (S/DiagonalPreconditioner.java, S/ILU.java,
+// S/UnitLowerCompRowMatrix.java, S/UpperCompRowMatrix.java, S/SSOR.java,
+// S/ICC.java).  SSOR's two sweeps and ICC's two triangular solves use the
+// same level sets as ILU's triangular solves.
 //
 // ILU(0) is level scheduled.  Rows are grouped on the host into dependency
 // levels (a row's level is one more than the highest level among the rows its
@@ -16,9 +17,19 @@
 // the first entry of row k+1; this implementation stops at the end of row k.
 // The two agree on every pattern where row k+1 has a strictly-lower entry,
 // which includes all structurally symmetric patterns.
+//
+// ICC's factorization is not level scheduled: MTJ eliminates each row k
+// against every earlier row through a dense work row that keeps the fill
+// produced inside row k and drops it only when the row is stored
+// (S/ICC.java:97-133), so row k depends on every row its fill reaches, which
+// on a mesh is the whole preceding band, and the rows form one chain.  The
+// factor is computed on the host during setMatrix, visiting only the rows
+// that can contribute (icc_factor_host), and both triangular solves of every
+// apply run on the device.
 #include <algorithm>
 #include <cstring>
 
+#include "icc_factor.hpp"
 #include "mtjg_internal.hpp"
 #include "reduce.hpp"
 
@@ -41,7 +52,7 @@ struct mtjg_prec_s {
   int32_t *rp = nullptr, *ci = nullptr, *diag = nullptr;
   double *lu = nullptr;
   double *y = nullptr;  // temporary of apply
-  std::vector<int32_t> h_rp, h_ci;
+  std::vector<int32_t> h_rp, h_ci, h_diag;
   Levels lev_l, lev_u;
   // transposed factor (transApply), built on first use
   bool trans_ready = false;
@@ -462,7 +473,8 @@ static void take_pattern(mtjg_prec M, mtjg_csr A, int32_t *bad_row, const char *
       M->h_rp.clear();
       fail(MTJG_E_MISSING_DIAGONAL, missing + std::to_string(b + 1));
     }
-    std::vector<int32_t> hd((size_t)n);
+    std::vector<int32_t> &hd = M->h_diag;
+    hd.assign((size_t)n, 0);
     if (n) HIPCHK(hipMemcpyAsync(hd.data(), M->diag, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     // L levels: dependencies are the strictly-lower entries [rp[i], diag[i])
@@ -507,10 +519,41 @@ static void ilu_set_matrix(mtjg_prec M, mtjg_csr A, int32_t *bad_row) {
     k_gather<<<grid_for(M->nnz), BS, 0, s>>>(M->nnz, M->tperm, M->lu, M->tval);
 }
 
-// Transposed factor for transApply (S/ILU.java:72-78): CSR of (LU)' with
-// each transposed row in ascending original-row order, a gather map from the
-// factor storage, and the level sets of U' (forward) and L' (backward).
-static void ilu_build_transpose(mtjg_prec M) {
+static void build_transpose(mtjg_prec M, bool lower_levels);
+
+// ICC.setMatrix: R.set(A), then factor() (S/ICC.java:80-84).
+static void icc_set_matrix(mtjg_prec M, mtjg_csr A, int32_t *bad_row) {
+  hipStream_t s = M->ctx->stream;
+  M->factored = false;
+  take_pattern(M, A, bad_row, "Missing diagonal entry on row ");
+  std::vector<double> data((size_t)M->nnz);
+  if (M->nnz)
+    HIPCHK(hipMemcpyAsync(data.data(), M->lu, sizeof(double) * (size_t)M->nnz, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  int32_t bad = -1;
+  int rc = icc_factor_host(M->n, M->h_rp, M->h_ci, M->h_diag, data, &bad);
+  if (rc) {
+    if (bad_row) *bad_row = bad;
+    if (rc == 1)
+      fail(MTJG_E_ZERO_PIVOT, "Zero pivot encountered on row " + std::to_string(bad + 1) + " during ICC process");
+    fail(MTJG_E_ZERO_DIAGONAL,
+         "Zero diagonal entry encountered on row " + std::to_string(bad + 1) + " during ICC process");
+  }
+  if (M->nnz) {
+    HIPCHK(hipMemcpyAsync(M->lu, data.data(), sizeof(double) * (size_t)M->nnz, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  if (!M->trans_ready) build_transpose(M, false);
+  else if (M->nnz) k_gather<<<grid_for(M->nnz), BS, 0, s>>>(M->nnz, M->tperm, M->lu, M->tval);
+  HIPCHK(hipGetLastError());
+  M->factored = true;
+}
+
+// Transposed factor for ILU's transApply (S/ILU.java:72-78) and ICC's apply
+// (S/ICC.java:65-67): CSR of the factor's transpose with each transposed row
+// in ascending original-row order, a gather map from the factor storage, and
+// the level sets of U' (forward) and, for ILU, L' (backward).
+static void build_transpose(mtjg_prec M, bool lower_levels) {
   mtjg_ctx ctx = M->ctx;
   hipStream_t s = ctx->stream;
   const int32_t n = M->n;
@@ -545,11 +588,13 @@ static void ilu_build_transpose(mtjg_prec M) {
     end[(size_t)c] = tdiag[(size_t)c];
   }
   build_levels(ctx, n, trp, tci, beg, end, true, M->lev_ut);
-  for (int32_t c = 0; c < n; ++c) {
-    beg[(size_t)c] = tdiag[(size_t)c] + 1;
-    end[(size_t)c] = trp[c + 1];
+  if (lower_levels) {
+    for (int32_t c = 0; c < n; ++c) {
+      beg[(size_t)c] = tdiag[(size_t)c] + 1;
+      end[(size_t)c] = trp[c + 1];
+    }
+    build_levels(ctx, n, trp, tci, beg, end, false, M->lev_lt);
   }
-  build_levels(ctx, n, trp, tci, beg, end, false, M->lev_lt);
   if (nnz) k_gather<<<grid_for(nnz), BS, 0, s>>>(nnz, M->tperm, M->lu, M->tval);
   HIPCHK(hipGetLastError());
   M->trans_ready = true;
@@ -564,6 +609,16 @@ void prec_apply_dev(mtjg_prec M, int64_t n, const double *b, double *x, hipStrea
   } else if (M->kind == PREC_SSOR) {
     if (!M->factored) fail(MTJG_E_INVALID_ARGUMENT, "SSOR.setMatrix must precede apply");
     run_ssor(M, s, b, x);
+  } else if (M->kind == PREC_ICC) {
+    if (!M->factored) fail(MTJG_E_INVALID_ARGUMENT, "ICC.setMatrix must precede apply");
+    // Rt.transSolve(b, y); Rt.solve(y, x) (S/ICC.java:65-71)
+    if (M->level_launch) {
+      run_levels<TRSV_COL_DIV>(M->lev_ut, s, M->trp, M->tci, M->tdiag, M->tval, b, M->y);
+      run_levels<TRSV_ROW_DIV>(M->lev_u, s, M->rp, M->ci, M->diag, M->lu, M->y, x);
+    } else {
+      run_syncfree<TRSV_COL_DIV>(M, M->lev_ut, 2, s, M->trp, M->tci, M->tdiag, M->tval, b, M->y);
+      run_syncfree<TRSV_ROW_DIV>(M, M->lev_u, 1, s, M->rp, M->ci, M->diag, M->lu, M->y, x);
+    }
   } else {
     if (!M->factored) fail(MTJG_E_INVALID_ARGUMENT, "ILU.setMatrix must precede apply");
     // L.solve(b, y); U.solve(y, x) (S/ILU.java:64-70)
@@ -584,7 +639,7 @@ void prec_trans_apply_dev(mtjg_prec M, const double *b, double *x, hipStream_t s
     return;
   }
   if (!M->factored) fail(MTJG_E_INVALID_ARGUMENT, "ILU.setMatrix must precede transApply");
-  if (!M->trans_ready) ilu_build_transpose(M);
+  if (!M->trans_ready) build_transpose(M, true);
   // U.transSolve(b, y); L.transSolve(y, x) (S/ILU.java:72-78)
   if (M->level_launch) {
     run_levels<TRSV_COL_DIV>(M->lev_ut, s, M->trp, M->tci, M->tdiag, M->tval, b, M->y);
@@ -598,13 +653,13 @@ void prec_trans_apply_dev(mtjg_prec M, const double *b, double *x, hipStream_t s
 // Raises if a single-launch triangular solve ran out of its poll budget
 // since the last check (synchronises the stream).
 void prec_check(mtjg_prec M) {
-  if (!M || M->kind != PREC_ILU || !M->sf_err) return;
+  if (!M || (M->kind != PREC_ILU && M->kind != PREC_ICC) || !M->sf_err) return;
   int32_t e = 0;
   HIPCHK(hipMemcpyAsync(&e, M->sf_err, sizeof(int32_t), hipMemcpyDeviceToHost, M->ctx->stream));
   HIPCHK(hipStreamSynchronize(M->ctx->stream));
   if (e) {
     HIPCHK(hipMemsetAsync(M->sf_err, 0, sizeof(int32_t), M->ctx->stream));
-    fail(MTJG_E_DEVICE, "ILU triangular solve exceeded its dependency poll budget");
+    fail(MTJG_E_DEVICE, "triangular solve exceeded its dependency poll budget");
   }
 }
 
@@ -642,6 +697,27 @@ extern "C" int mtjg_prec_ilu_create(mtjg_ctx ctx, mtjg_csr LU, mtjg_prec *out) {
   M->ctx = ctx;
   M->kind = PREC_ILU;
   M->n = LU->nr;
+  HIPCHK(hipMalloc(&M->y, sizeof(double) * (size_t)std::max(M->n, 1)));
+  HIPCHK(hipMemsetAsync(M->y, 0, sizeof(double) * (size_t)std::max(M->n, 1), ctx->stream));
+  HIPCHK(hipMalloc(&M->bad, sizeof(int32_t)));
+  HIPCHK(hipMalloc(&M->sf_next, sizeof(uint32_t) * 4));
+  HIPCHK(hipMalloc(&M->sf_err, sizeof(int32_t)));
+  HIPCHK(hipMemsetAsync(M->sf_next, 0, sizeof(uint32_t) * 4, ctx->stream));
+  HIPCHK(hipMemsetAsync(M->sf_err, 0, sizeof(int32_t), ctx->stream));
+  M->level_launch = true;
+  if (const char *e = getenv("MTJG_ILU_SYNCFREE")) M->level_launch = atoi(e) == 0;
+  *out = M;
+  MTJG_API_END
+}
+
+extern "C" int mtjg_prec_icc_create(mtjg_ctx ctx, mtjg_csr R, mtjg_prec *out) {
+  MTJG_API_BEGIN
+  if (!ctx || !R || !out) fail(MTJG_E_NULL, "null argument");
+  if (R->nr != R->nc) fail(MTJG_E_INVALID_ARGUMENT, "ICC only applies to square matrices");
+  auto *M = new mtjg_prec_s();
+  M->ctx = ctx;
+  M->kind = PREC_ICC;
+  M->n = R->nr;
   HIPCHK(hipMalloc(&M->y, sizeof(double) * (size_t)std::max(M->n, 1)));
   HIPCHK(hipMemsetAsync(M->y, 0, sizeof(double) * (size_t)std::max(M->n, 1), ctx->stream));
   HIPCHK(hipMalloc(&M->bad, sizeof(int32_t)));
@@ -706,6 +782,9 @@ extern "C" int mtjg_prec_set_matrix(mtjg_prec M, mtjg_csr A, int32_t *bad_row) {
   } else if (M->kind == PREC_ILU) {
     if (A->nr != M->n || A->nc != M->n) fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "matrix size differs from ILU size");
     ilu_set_matrix(M, A, bad_row);
+  } else if (M->kind == PREC_ICC) {
+    if (A->nr != M->n || A->nc != M->n) fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "matrix size differs from ICC size");
+    icc_set_matrix(M, A, bad_row);
   } else if (M->kind == PREC_SSOR) {
     if (A->nr != M->n || A->nc != M->n) fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "matrix size differs from SSOR size");
     M->factored = false;

@@ -21,11 +21,12 @@ from . import _lib as L
 from ._lib import load
 
 __all__ = [
-    "Context", "DenseVector", "CompRowMatrix", "DiagonalPreconditioner", "ILU",
-    "DefaultIterationMonitor", "CG", "BiCGstab", "GMRES", "IR", "MixedCG", "Norm",
-    "Reason", "IterativeSolverNotConvergedException", "IndexOutOfBoundsException",
+    "Context", "DenseVector", "CompRowMatrix", "DiagonalPreconditioner", "ILU", "ICC", "SSOR",
+    "DefaultIterationMonitor", "CG", "BiCGstab", "BiCG", "CGS", "QMR", "GMRES", "IR", "MixedCG",
+    "Norm", "Reason", "IterativeSolverNotConvergedException", "IndexOutOfBoundsException",
     "IllegalArgumentException", "MatrixSingularException", "MtjgError",
-    "UnsupportedOperationException", "default_context",
+    "UnsupportedOperationException", "IOException", "ZeroDiagonalException",
+    "MissingDiagonalException", "ZeroPivotException", "default_context",
 ]
 
 
@@ -462,6 +463,18 @@ class ILU(_Preconditioner):
         self._lib = self.ctx._lib
         h = C.c_void_p()
         _ck(self._lib.mtjg_prec_ilu_create(self.ctx.handle, LU.handle, C.byref(h)))
+        self.handle = h
+
+
+class ICC(_Preconditioner):
+    """S/ICC.java: incomplete Cholesky without fill-in, in the pattern of the
+    matrix given to setMatrix.  transApply is apply."""
+
+    def __init__(self, R, ctx=None):
+        self.ctx = ctx or R.ctx
+        self._lib = self.ctx._lib
+        h = C.c_void_p()
+        _ck(self._lib.mtjg_prec_icc_create(self.ctx.handle, R.handle, C.byref(h)))
         self.handle = h
 
 

@@ -93,6 +93,7 @@ PROTOTYPES = [
     ("mtjg_csr_row_histogram", C.c_int, [P, PI64, C.c_int]),
     ("mtjg_prec_diagonal_create", C.c_int, [P, I32, PP]),
     ("mtjg_prec_ilu_create", C.c_int, [P, P, PP]),
+    ("mtjg_prec_icc_create", C.c_int, [P, P, PP]),
     ("mtjg_prec_ssor_create", C.c_int, [P, P, I32, D, D, PP]),
     ("mtjg_prec_ssor_set_omega", C.c_int, [P, D, D]),
     ("mtjg_prec_set_matrix", C.c_int, [P, P, PI32]),

@@ -271,6 +271,68 @@ int oc_prec_ilu(oc_prec *p, int n, const int32_t *rp, const int32_t *ci,
     return OC_OK;
 }
 
+/* S/ICC.java:80-139, restated literally: for each row k, expand row k of R
+ * (still A's values) into the dense Rk; for every i < k in turn take
+ * Rki = Rk[i] / R_ii and, unless it is zero, subtract Rki * R_ij from Rk at
+ * every strictly-upper entry (i, j) of the stored rows; then store
+ * Rk[j] / sqrt(Rk[k]) over row k's upper pattern.  Fill produced in Rk is
+ * used within the row and dropped on storing.  Work is O(n^2) as in MTJ. */
+int oc_prec_icc(oc_prec *p, int n, const int32_t *rp, const int32_t *ci,
+                const double *a, int *bad_row) {
+    memset(p, 0, sizeof(*p));
+    p->kind = OC_PREC_ICC;
+    p->n = n;
+    int32_t nnz = rp[n];
+    p->rp = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
+    p->ci = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1));
+    p->lu = (double *)malloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
+    p->diagind = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    p->y = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    memcpy(p->rp, rp, sizeof(int32_t) * (size_t)(n + 1));
+    memcpy(p->ci, ci, sizeof(int32_t) * (size_t)nnz);
+    memcpy(p->lu, a, sizeof(double) * (size_t)nnz);
+    const int32_t *colind = p->ci, *rowptr = p->rp;
+    int32_t *diagind = p->diagind;
+    double *data = p->lu;
+
+    /* findDiagonalIndices, S/ICC.java:141-155 */
+    for (int k = 0; k < n; ++k) {
+        diagind[k] = bsearch_range(colind, k, rowptr[k], rowptr[k + 1]);
+        if (diagind[k] < 0) {
+            if (bad_row) *bad_row = k;
+            return OC_MISSING_DIAGONAL;
+        }
+    }
+    double *Rk = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    int rc = OC_OK;
+    for (int k = 0; k < n && rc == OC_OK; ++k) {
+        for (int i = 0; i < n; ++i) Rk[i] = 0;
+        for (int32_t i = rowptr[k]; i < rowptr[k + 1]; ++i) Rk[colind[i]] = data[i];
+        for (int i = 0; i < k; ++i) {
+            double Rii = data[diagind[i]];
+            if (Rii == 0) {
+                if (bad_row) *bad_row = i;
+                rc = OC_ZERO_PIVOT;
+                break;
+            }
+            double Rki = Rk[i] / Rii;
+            if (Rki == 0) continue;
+            for (int32_t j = diagind[i] + 1; j < rowptr[i + 1]; ++j)
+                Rk[colind[j]] -= Rki * data[j];
+        }
+        if (rc != OC_OK) break;
+        if (Rk[k] == 0) {
+            if (bad_row) *bad_row = k;
+            rc = OC_ZERO_DIAGONAL;
+            break;
+        }
+        double sqRkk = sqrt(Rk[k]);
+        for (int32_t i = diagind[k]; i < rowptr[k + 1]; ++i) data[i] = Rk[colind[i]] / sqRkk;
+    }
+    free(Rk);
+    return rc;
+}
+
 /* S/UnitLowerCompRowMatrix.java:48-67: x_i = b_i - sum_{j<diag} L_ij x_j. */
 void oc_unit_lower_solve(int n, const int32_t *rp, const int32_t *ci,
                          const double *a, const int32_t *diagind,
@@ -386,15 +448,19 @@ void oc_prec_apply(const oc_prec *p, const double *b, double *x) {
         oc_dv_set(n, b, x);
     } else if (p->kind == OC_PREC_DIAGONAL) {
         for (int i = 0; i < n; ++i) x[i] = b[i] * p->invdiag[i];
+    } else if (p->kind == OC_PREC_ICC) {
+        /* R'y = b, then R x = y (S/ICC.java:65-71) */
+        oc_upper_trans_solve(n, p->rp, p->ci, p->lu, p->diagind, b, p->y);
+        oc_upper_solve(n, p->rp, p->ci, p->lu, p->diagind, p->y, x);
     } else {
         oc_unit_lower_solve(n, p->rp, p->ci, p->lu, p->diagind, b, p->y);
         oc_upper_solve(n, p->rp, p->ci, p->lu, p->diagind, p->y, x);
     }
 }
 
-/* transApply: identity and Jacobi are symmetric, SSOR assumes a symmetric
- * matrix and applies itself (S/SSOR.java:196-199); ILU is U' then L'
- * (S/ILU.java:72-78). */
+/* transApply: identity and Jacobi are symmetric, SSOR and ICC assume a
+ * symmetric matrix and apply themselves (S/SSOR.java:196-199,
+ * S/ICC.java:73-75); ILU is U' then L' (S/ILU.java:72-78). */
 void oc_prec_trans_apply(const oc_prec *p, const double *b, double *x) {
     int n = p->n;
     if (p->kind != OC_PREC_ILU) {

@@ -88,7 +88,7 @@ void oc_monitor_init(oc_monitor *m, int max_iter, double rtol, double atol,
 int oc_monitor_converged(oc_monitor *m, double r);
 
 /* ---- Preconditioners ------------------------------------------------------ */
-enum { OC_PREC_IDENTITY = 0, OC_PREC_DIAGONAL = 1, OC_PREC_ILU = 2, OC_PREC_SSOR = 3 };
+enum { OC_PREC_IDENTITY = 0, OC_PREC_DIAGONAL = 1, OC_PREC_ILU = 2, OC_PREC_SSOR = 3, OC_PREC_ICC = 4 };
 
 typedef struct {
     int kind;
@@ -116,6 +116,13 @@ int oc_prec_ilu(oc_prec *p, int n, const int32_t *rp, const int32_t *ci,
 int oc_prec_ssor(oc_prec *p, int n, const int32_t *rp, const int32_t *ci,
                  const double *a, int reverse, double omega_f, double omega_r,
                  int *bad_row);
+/* ICC(R) + setMatrix(A) (S/ICC.java:57-139): the factor R (upper part in
+ * place, lower part keeps A's values) over A's pattern.  OC_MISSING_DIAGONAL,
+ * OC_ZERO_PIVOT (a zero R_ii, S/ICC.java:108-110) or OC_ZERO_DIAGONAL (the
+ * pivot of row k is zero before its square root, S/ICC.java:127-130), row in
+ * *bad_row. */
+int oc_prec_icc(oc_prec *p, int n, const int32_t *rp, const int32_t *ci,
+                const double *a, int *bad_row);
 void oc_prec_apply(const oc_prec *p, const double *b, double *x);
 void oc_prec_trans_apply(const oc_prec *p, const double *b, double *x);
 void oc_prec_free(oc_prec *p);

@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "build", "libmtjoracle.so")
 
 OK, ITERATIONS, DIVERGENCE, BREAKDOWN = 0, 1, 2, 3
 ZERO_DIAGONAL, MISSING_DIAGONAL, ZERO_PIVOT, SINGULAR, BAD_ARG = 10, 11, 12, 13, 20
-PREC_IDENTITY, PREC_DIAGONAL, PREC_ILU, PREC_SSOR = 0, 1, 2, 3
+PREC_IDENTITY, PREC_DIAGONAL, PREC_ILU, PREC_SSOR, PREC_ICC = 0, 1, 2, 3, 4
 
 
 class Monitor(C.Structure):
@@ -67,6 +67,7 @@ def load():
         lib.oc_prec_identity.argtypes = [C.POINTER(Prec), C.c_int]
         lib.oc_prec_diagonal.argtypes = [C.POINTER(Prec), C.c_int, P, P, P, C.POINTER(C.c_int)]
         lib.oc_prec_ilu.argtypes = [C.POINTER(Prec), C.c_int, P, P, P, C.POINTER(C.c_int)]
+        lib.oc_prec_icc.argtypes = [C.POINTER(Prec), C.c_int, P, P, P, C.POINTER(C.c_int)]
         lib.oc_prec_ssor.argtypes = [C.POINTER(Prec), C.c_int, P, P, P, C.c_int, C.c_double, C.c_double,
                                      C.POINTER(C.c_int)]
         lib.oc_prec_apply.argtypes = [C.POINTER(Prec), P, P]
@@ -131,7 +132,7 @@ class Csr:
 
 
 class Precond:
-    """Identity / DiagonalPreconditioner / ILU / SSOR restatement."""
+    """Identity / DiagonalPreconditioner / ILU / SSOR / ICC restatement."""
 
     def __init__(self, kind, A=None, n=None, reverse=True, omega_f=1.0, omega_r=1.0):
         self.s = Prec()
@@ -146,6 +147,9 @@ class Precond:
         elif kind == PREC_SSOR:
             self.code = lib.oc_prec_ssor(C.byref(self.s), A.nr, _p(A.rp), _p(A.ci), _p(A.va), int(reverse),
                                          float(omega_f), float(omega_r), C.byref(self.bad))
+        elif kind == PREC_ICC:
+            self.code = lib.oc_prec_icc(C.byref(self.s), A.nr, _p(A.rp), _p(A.ci), _p(A.va),
+                                        C.byref(self.bad))
         else:
             self.code = lib.oc_prec_ilu(C.byref(self.s), A.nr, _p(A.rp), _p(A.ci), _p(A.va),
                                         C.byref(self.bad))
