@@ -207,6 +207,30 @@ int mtjg_prec_ssor_create(mtjg_ctx ctx, mtjg_csr F, int32_t reverse,
  * (S/ICC.java:65-71); transApply is apply (S/ICC.java:73-75); b and x may be
  * the same vector. */
 int mtjg_prec_icc_create(mtjg_ctx ctx, mtjg_csr R, mtjg_prec *out);
+/* new AMG(omegaPreF, omegaPreR, omegaPostF, omegaPostR, nu1, nu2, gamma,
+ * min, omega) with reverse = 1 (S/AMG.java:140-156; AMG() is
+ * (1, 1.85, 1.85, 1, 1, 1, 1, 40, 2/3), :196-205), or
+ * AMG(omegaPre, omegaPost, nu1, nu2, gamma, min, omega) with reverse = 0 and
+ * omegaPreF = omegaPreR = omegaPre, omegaPostF = omegaPostR = omegaPost
+ * (:179-194): smoothed-aggregation algebraic multigrid with SSOR smoothing.
+ * setMatrix(A) builds the hierarchy on the host bit for bit as
+ * S/AMG.java:227-289 does and uploads it; A must be square
+ * (MTJG_E_INVALID_ARGUMENT); MTJG_E_MISSING_DIAGONAL with *bad_row for a
+ * level-0 row without a diagonal entry; MTJG_E_INVALID_ARGUMENT for a
+ * relaxation outside [0, 2] once smoothers are built; MTJG_E_UNSUPPORTED when
+ * a strong neighbourhood would form a java.util.HashSet tree bin, the
+ * hierarchy exceeds 100 levels, or the coarsest level exceeds 4096 rows.
+ * apply(b, x) runs one cycle starting from x's incoming values
+ * (S/AMG.java:207-214); transApply uses the transposed coarse solve; both fail
+ * with MTJG_E_SINGULAR when the coarsest matrix is singular, leaving x as it
+ * was.  b and x may be the same vector. */
+int mtjg_prec_amg_create(mtjg_ctx ctx, double omega_pre_f, double omega_pre_r,
+                         double omega_post_f, double omega_post_r, int32_t nu1,
+                         int32_t nu2, int32_t gamma, int32_t min, double omega,
+                         int32_t reverse, mtjg_prec *out);
+/* The hierarchy built by the last setMatrix: *levels, and the first
+ * max_sizes level sizes (finest first) into sizes (may be NULL). */
+int mtjg_prec_amg_levels(mtjg_prec M, int32_t *levels, int32_t *sizes, int32_t max_sizes);
 /* setOmega(omegaF, omegaR) (S/SSOR.java:116-124). */
 int mtjg_prec_ssor_set_omega(mtjg_prec M, double omega_f, double omega_r);
 /* setMatrix(A) (S/Preconditioner.java:63; S/DiagonalPreconditioner.java:

@@ -198,7 +198,8 @@ mtjg_csr_s *csr_transpose(mtjg_csr A);
 void csr_ensure_fp32(mtjg_csr A);
 
 // ---- preconditioners (precond.hip) ----
-enum PrecKind : int { PREC_IDENTITY = 0, PREC_DIAGONAL = 1, PREC_ILU = 2, PREC_SSOR = 3, PREC_ICC = 4 };
+enum PrecKind : int { PREC_IDENTITY = 0, PREC_DIAGONAL = 1, PREC_ILU = 2, PREC_SSOR = 3, PREC_ICC = 4,
+                      PREC_AMG = 5 };
 // Applies M (identity when M is null) to n entries; kernels skip work once st's
 // status leaves running (st may be null).
 void prec_apply_dev(mtjg_prec M, int64_t n, const double *b, double *x, hipStream_t s, KState *st);
@@ -207,6 +208,20 @@ void prec_trans_apply_dev(mtjg_prec M, const double *b, double *x, hipStream_t s
 // (synchronises the stream).
 void prec_check(mtjg_prec M);
 int prec_kind(mtjg_prec M);
+
+// ---- algebraic multigrid (amg.hip) ----
+namespace amg {
+struct Params;
+}
+struct AmgDevice;
+// Builds the hierarchy of A on the host and uploads it (AMG.setMatrix).
+AmgDevice *amg_create_device(mtjg_ctx ctx, mtjg_csr A, const amg::Params &p, int32_t *bad_row);
+// One cycle from x's incoming values: x = cycle(b, x); transpose selects the
+// transposed coarse solve (transApply).
+void amg_apply_device(AmgDevice *H, const double *b, double *x, bool transpose, hipStream_t s);
+void amg_destroy_device(AmgDevice *H);
+// Level sizes (finest first); returns the number of levels.
+int32_t amg_level_sizes(const AmgDevice *H, int32_t *sizes, int32_t cap);
 const double *prec_invdiag(mtjg_prec M);
 int32_t prec_n(mtjg_prec M);
 

@@ -1,8 +1,9 @@
 // Preconditioners on the device: DiagonalPreconditioner (Jacobi), ILU(0),
 // SSOR and ICC (S/DiagonalPreconditioner.java, S/ILU.java,
 // S/UnitLowerCompRowMatrix.java, S/UpperCompRowMatrix.java, S/SSOR.java,
-// S/ICC.java).  SSOR's two sweeps and ICC's two triangular solves use the
-// same level sets as ILU's triangular solves.
+// S/ICC.java), and the entry points of the algebraic multigrid preconditioner
+// (S/AMG.java), whose cycle lives in amg.hip.  SSOR's two sweeps and ICC's two
+// triangular solves use the same level sets as ILU's triangular solves.
 //
 // ILU(0) is level scheduled.  Rows are grouped on the host into dependency
 // levels (a row's level is one more than the highest level among the rows its
@@ -29,6 +30,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "amg_setup.hpp"
 #include "icc_factor.hpp"
 #include "mtjg_internal.hpp"
 #include "reduce.hpp"
@@ -67,6 +69,9 @@ struct mtjg_prec_s {
   double omega_f = 1, omega_r = 1;
   bool reverse = true;
   double *xx = nullptr;
+  // AMG: parameters and the device hierarchy built by setMatrix
+  mtjg::amg::Params amg_params;
+  mtjg::AmgDevice *amg = nullptr;
 };
 
 namespace mtjg {
@@ -609,6 +614,9 @@ void prec_apply_dev(mtjg_prec M, int64_t n, const double *b, double *x, hipStrea
   } else if (M->kind == PREC_SSOR) {
     if (!M->factored) fail(MTJG_E_INVALID_ARGUMENT, "SSOR.setMatrix must precede apply");
     run_ssor(M, s, b, x);
+  } else if (M->kind == PREC_AMG) {
+    if (!M->amg) fail(MTJG_E_INVALID_ARGUMENT, "AMG.setMatrix must precede apply");
+    amg_apply_device(M->amg, b, x, false, s);
   } else if (M->kind == PREC_ICC) {
     if (!M->factored) fail(MTJG_E_INVALID_ARGUMENT, "ICC.setMatrix must precede apply");
     // Rt.transSolve(b, y); Rt.solve(y, x) (S/ICC.java:65-71)
@@ -634,6 +642,11 @@ void prec_apply_dev(mtjg_prec M, int64_t n, const double *b, double *x, hipStrea
 }
 
 void prec_trans_apply_dev(mtjg_prec M, const double *b, double *x, hipStream_t s) {
+  if (M->kind == PREC_AMG) {
+    if (!M->amg) fail(MTJG_E_INVALID_ARGUMENT, "AMG.setMatrix must precede transApply");
+    amg_apply_device(M->amg, b, x, true, s);
+    return;
+  }
   if (M->kind != PREC_ILU) {
     prec_apply_dev(M, M->n, b, x, s, nullptr);
     return;
@@ -731,6 +744,39 @@ extern "C" int mtjg_prec_icc_create(mtjg_ctx ctx, mtjg_csr R, mtjg_prec *out) {
   MTJG_API_END
 }
 
+extern "C" int mtjg_prec_amg_create(mtjg_ctx ctx, double omega_pre_f, double omega_pre_r, double omega_post_f,
+                                    double omega_post_r, int32_t nu1, int32_t nu2, int32_t gamma, int32_t min,
+                                    double omega, int32_t reverse, mtjg_prec *out) {
+  MTJG_API_BEGIN
+  if (!ctx || !out) fail(MTJG_E_NULL, "null argument");
+  auto *M = new mtjg_prec_s();
+  M->ctx = ctx;
+  M->kind = PREC_AMG;
+  M->n = 0;
+  amg::Params &p = M->amg_params;
+  p.omega_pre_f = omega_pre_f;
+  p.omega_pre_r = omega_pre_r;
+  p.omega_post_f = omega_post_f;
+  p.omega_post_r = omega_post_r;
+  p.nu1 = nu1;
+  p.nu2 = nu2;
+  p.gamma = gamma;
+  p.min = min;
+  p.omega = omega;
+  p.reverse = reverse != 0;
+  *out = M;
+  MTJG_API_END
+}
+
+extern "C" int mtjg_prec_amg_levels(mtjg_prec M, int32_t *levels, int32_t *sizes, int32_t max_sizes) {
+  MTJG_API_BEGIN
+  if (!M || !levels) fail(MTJG_E_NULL, "null argument");
+  if (M->kind != PREC_AMG) fail(MTJG_E_INVALID_ARGUMENT, "not an AMG preconditioner");
+  int32_t tmp = 0;
+  *levels = M->amg ? amg_level_sizes(M->amg, sizes ? sizes : &tmp, sizes ? max_sizes : 0) : 0;
+  MTJG_API_END
+}
+
 extern "C" int mtjg_prec_ssor_create(mtjg_ctx ctx, mtjg_csr F, int32_t reverse, double omega_f, double omega_r,
                                      mtjg_prec *out) {
   MTJG_API_BEGIN
@@ -782,6 +828,13 @@ extern "C" int mtjg_prec_set_matrix(mtjg_prec M, mtjg_csr A, int32_t *bad_row) {
   } else if (M->kind == PREC_ILU) {
     if (A->nr != M->n || A->nc != M->n) fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "matrix size differs from ILU size");
     ilu_set_matrix(M, A, bad_row);
+  } else if (M->kind == PREC_AMG) {
+    if (A->nr != A->nc) fail(MTJG_E_INVALID_ARGUMENT, "AMG only applies to square matrices");
+    amg_destroy_device(M->amg);
+    M->amg = nullptr;
+    M->n = 0;
+    M->amg = amg_create_device(M->ctx, A, M->amg_params, bad_row);
+    M->n = A->nr;
   } else if (M->kind == PREC_ICC) {
     if (A->nr != M->n || A->nc != M->n) fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "matrix size differs from ICC size");
     icc_set_matrix(M, A, bad_row);
@@ -796,6 +849,7 @@ extern "C" int mtjg_prec_set_matrix(mtjg_prec M, mtjg_csr A, int32_t *bad_row) {
 
 static void check_apply(mtjg_prec M, mtjg_vec b, mtjg_vec x) {
   if (!M || !b || !x) fail(MTJG_E_NULL, "null argument");
+  if (M->kind == PREC_AMG && !M->amg) fail(MTJG_E_INVALID_ARGUMENT, "AMG.setMatrix must precede apply");
   if (b->dtype != MTJG_F64 || x->dtype != MTJG_F64)
     fail(MTJG_E_INVALID_ARGUMENT, "Vector must be DenseVectors");
   if (b->n != M->n || x->n != M->n) fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "vector size differs from preconditioner size");
@@ -829,6 +883,7 @@ extern "C" int mtjg_prec_destroy(mtjg_prec M) {
   (void)hipFree(M->sf_next);
   (void)hipFree(M->sf_err);
   (void)hipFree(M->xx);
+  amg_destroy_device(M->amg);
   ilu_free_storage(M);
   delete M;
   MTJG_API_END

@@ -21,7 +21,7 @@ from . import _lib as L
 from ._lib import load
 
 __all__ = [
-    "Context", "DenseVector", "CompRowMatrix", "DiagonalPreconditioner", "ILU", "ICC", "SSOR",
+    "Context", "DenseVector", "CompRowMatrix", "DiagonalPreconditioner", "ILU", "ICC", "SSOR", "AMG",
     "DefaultIterationMonitor", "CG", "BiCGstab", "BiCG", "CGS", "QMR", "GMRES", "IR", "MixedCG",
     "Norm", "Reason", "IterativeSolverNotConvergedException", "IndexOutOfBoundsException",
     "IllegalArgumentException", "MatrixSingularException", "MtjgError",
@@ -476,6 +476,43 @@ class ICC(_Preconditioner):
         h = C.c_void_p()
         _ck(self._lib.mtjg_prec_icc_create(self.ctx.handle, R.handle, C.byref(h)))
         self.handle = h
+
+
+class AMG(_Preconditioner):
+    """S/AMG.java: smoothed-aggregation algebraic multigrid with SSOR smoothing.
+
+    AMG() uses MTJ's defaults (1, 1.85, 1.85, 1, 1, 1, 1, 40, 2/3);
+    AMG(omegaPreF, omegaPreR, omegaPostF, omegaPostR, nu1, nu2, gamma, min,
+    omega) runs reverse (symmetric) sweeps; AMG(omegaPre, omegaPost, nu1, nu2,
+    gamma, min, omega) runs forward sweeps only.  The hierarchy is built by
+    setMatrix; apply starts its cycle from x's incoming values."""
+
+    def __init__(self, *params, ctx=None):
+        if not params:
+            params = (1.0, 1.85, 1.85, 1.0, 1, 1, 1, 40, 2.0 / 3)
+        if len(params) == 7:
+            w1, w2, nu1, nu2, gamma, mn, omega = params
+            full, reverse = (w1, w1, w2, w2, nu1, nu2, gamma, mn, omega), 0
+        elif len(params) == 9:
+            full, reverse = tuple(params), 1
+        else:
+            raise TypeError("AMG takes 0, 7 or 9 parameters")
+        self.ctx = ctx or default_context()
+        self._lib = self.ctx._lib
+        wpf, wpr, wqf, wqr, nu1, nu2, gamma, mn, omega = full
+        h = C.c_void_p()
+        _ck(self._lib.mtjg_prec_amg_create(self.ctx.handle, float(wpf), float(wpr), float(wqf), float(wqr),
+                                           int(nu1), int(nu2), int(gamma), int(mn), float(omega), reverse,
+                                           C.byref(h)))
+        self.handle = h
+
+    def levels(self):
+        """Sizes of the hierarchy built by the last setMatrix, finest first."""
+        m = C.c_int32(0)
+        _ck(self._lib.mtjg_prec_amg_levels(self.handle, C.byref(m), None, 0))
+        sizes = (C.c_int32 * max(m.value, 1))()
+        _ck(self._lib.mtjg_prec_amg_levels(self.handle, C.byref(m), sizes, m.value))
+        return [int(sizes[k]) for k in range(m.value)]
 
 
 class SSOR(_Preconditioner):

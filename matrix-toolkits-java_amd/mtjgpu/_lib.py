@@ -94,6 +94,8 @@ PROTOTYPES = [
     ("mtjg_prec_diagonal_create", C.c_int, [P, I32, PP]),
     ("mtjg_prec_ilu_create", C.c_int, [P, P, PP]),
     ("mtjg_prec_icc_create", C.c_int, [P, P, PP]),
+    ("mtjg_prec_amg_create", C.c_int, [P, D, D, D, D, I32, I32, I32, I32, D, I32, PP]),
+    ("mtjg_prec_amg_levels", C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), I32]),
     ("mtjg_prec_ssor_create", C.c_int, [P, P, I32, D, D, PP]),
     ("mtjg_prec_ssor_set_omega", C.c_int, [P, D, D]),
     ("mtjg_prec_set_matrix", C.c_int, [P, P, PI32]),

@@ -442,7 +442,9 @@ static void ssor_apply(const oc_prec *p, const double *bd, double *xd) {
 
 void oc_prec_apply(const oc_prec *p, const double *b, double *x) {
     int n = p->n;
-    if (p->kind == OC_PREC_SSOR) {
+    if (p->kind == OC_PREC_AMG) {
+        oc_amg_apply(p->amg, b, x, 0);
+    } else if (p->kind == OC_PREC_SSOR) {
         ssor_apply(p, b, x);
     } else if (p->kind == OC_PREC_IDENTITY) {
         oc_dv_set(n, b, x);
@@ -463,7 +465,9 @@ void oc_prec_apply(const oc_prec *p, const double *b, double *x) {
  * S/ICC.java:73-75); ILU is U' then L' (S/ILU.java:72-78). */
 void oc_prec_trans_apply(const oc_prec *p, const double *b, double *x) {
     int n = p->n;
-    if (p->kind != OC_PREC_ILU) {
+    if (p->kind == OC_PREC_AMG) {
+        oc_amg_apply(p->amg, b, x, 1);
+    } else if (p->kind != OC_PREC_ILU) {
         oc_prec_apply(p, b, x);
     } else {
         oc_upper_trans_solve(n, p->rp, p->ci, p->lu, p->diagind, b, p->y);

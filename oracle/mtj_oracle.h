@@ -88,7 +88,10 @@ void oc_monitor_init(oc_monitor *m, int max_iter, double rtol, double atol,
 int oc_monitor_converged(oc_monitor *m, double r);
 
 /* ---- Preconditioners ------------------------------------------------------ */
-enum { OC_PREC_IDENTITY = 0, OC_PREC_DIAGONAL = 1, OC_PREC_ILU = 2, OC_PREC_SSOR = 3, OC_PREC_ICC = 4 };
+enum { OC_PREC_IDENTITY = 0, OC_PREC_DIAGONAL = 1, OC_PREC_ILU = 2, OC_PREC_SSOR = 3, OC_PREC_ICC = 4,
+       OC_PREC_AMG = 5 };
+
+struct oc_amg_s;
 
 typedef struct {
     int kind;
@@ -101,6 +104,7 @@ typedef struct {
     double omega_f, omega_r; /* SSOR relaxation of the forward / reverse sweep */
     int reverse;         /* SSOR: run the reverse sweep */
     double *xx;          /* SSOR: half iterate */
+    struct oc_amg_s *amg; /* AMG: the hierarchy (owned by the caller) */
 } oc_prec;
 
 void oc_prec_identity(oc_prec *p, int n);
@@ -129,6 +133,54 @@ void oc_prec_free(oc_prec *p);
 /* Accessors for tests: the factored values and diagonal index (ILU). */
 const double *oc_prec_ilu_values(const oc_prec *p);
 const int32_t *oc_prec_ilu_diagind(const oc_prec *p);
+
+/* ---- AMG (S/AMG.java) ---------------------------------------------------- */
+/* Parameters (S/AMG.java:113-205), then the hierarchy built by setMatrix:
+ * m levels; levels 0..m-2 carry A_k (CSR), the aggregate of each node
+ * (pt_k, -1 for none), the interpolation I_k (n_k x n_{k+1}, compressed
+ * columns, rows ascending) and one SSOR relaxation (the pre- and
+ * post-smoothers differ only in their relaxations); the coarsest level a
+ * column-major dense LU with 1-based pivots. */
+typedef struct oc_amg_s {
+    double omega_pre_f, omega_pre_r, omega_post_f, omega_post_r, omega;
+    int nu1, nu2, gamma, min_size, reverse;
+    int m;
+    int32_t *n;
+    int32_t **arp, **aci;
+    double **ava;
+    int32_t **pt;
+    int32_t **icp, **iri;
+    double **iva;
+    oc_prec *ssor;
+    double *lu;
+    int32_t *piv;
+    int singular;
+    double **u, **f, **r;
+    int transpose, error;
+} oc_amg;
+
+void oc_amg_params(oc_amg *g, double omega_pre_f, double omega_pre_r, double omega_post_f,
+                   double omega_post_r, int nu1, int nu2, int gamma, int min_size, double omega,
+                   int reverse);
+/* AMG.setMatrix (S/AMG.java:227-289): OC_OK, OC_MISSING_DIAGONAL (row and
+ * level in *bad_row, *bad_level), OC_BAD_ARG (a relaxation outside [0, 2],
+ * checked when SSOR levels are built), OC_UNSUPPORTED (a strong
+ * neighbourhood whose HashSet would hold a tree bin, or 100 levels). */
+int oc_amg_setup(oc_amg *g, int n, const int32_t *rp, const int32_t *ci, const double *a,
+                 int *bad_row, int *bad_level);
+/* apply (transpose = 0) / transApply (1) (S/AMG.java:207-225): OC_OK, or
+ * OC_SINGULAR when the coarsest matrix is singular (x left unchanged). */
+int oc_amg_apply(oc_amg *g, const double *b, double *x, int transpose);
+void oc_amg_free(oc_amg *g);
+/* An AMG as a Preconditioner for the restated solvers. */
+void oc_prec_amg(oc_prec *p, oc_amg *g);
+/* Iteration order of a java.util.HashSet<Integer> filled with vals[0..cnt)
+ * in that order: OC_OK, or OC_UNSUPPORTED when a tree bin would form. */
+int oc_java_hashset_order(const int32_t *vals, int cnt, int32_t *order);
+/* Reference LAPACK dgetf2 (returns info) and dgetrs (trans 0 / 1), n x n,
+ * column-major, pivots 1-based. */
+int oc_dgetf2(int n, double *a, int32_t *piv);
+void oc_dgetrs(int trans, int n, const double *a, const int32_t *piv, double *b);
 
 /* Triangular views over an ILU factor (S/UnitLowerCompRowMatrix.java,
  * S/UpperCompRowMatrix.java). */

@@ -13,8 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "build", "libmtjoracle.so")
 
 OK, ITERATIONS, DIVERGENCE, BREAKDOWN = 0, 1, 2, 3
-ZERO_DIAGONAL, MISSING_DIAGONAL, ZERO_PIVOT, SINGULAR, BAD_ARG = 10, 11, 12, 13, 20
-PREC_IDENTITY, PREC_DIAGONAL, PREC_ILU, PREC_SSOR, PREC_ICC = 0, 1, 2, 3, 4
+ZERO_DIAGONAL, MISSING_DIAGONAL, ZERO_PIVOT, SINGULAR, BAD_ARG, UNSUPPORTED = 10, 11, 12, 13, 20, 22
+PREC_IDENTITY, PREC_DIAGONAL, PREC_ILU, PREC_SSOR, PREC_ICC, PREC_AMG = 0, 1, 2, 3, 4, 5
 
 
 class Monitor(C.Structure):
@@ -27,7 +27,22 @@ class Prec(C.Structure):
     _fields_ = [("kind", C.c_int), ("n", C.c_int), ("invdiag", C.c_void_p), ("rp", C.c_void_p),
                 ("ci", C.c_void_p), ("lu", C.c_void_p), ("diagind", C.c_void_p),
                 ("y", C.c_void_p), ("omega_f", C.c_double), ("omega_r", C.c_double),
-                ("reverse", C.c_int), ("xx", C.c_void_p)]
+                ("reverse", C.c_int), ("xx", C.c_void_p), ("amg", C.c_void_p)]
+
+
+_PI = C.POINTER(C.c_int32)
+_PD = C.POINTER(C.c_double)
+
+
+class AmgS(C.Structure):
+    _fields_ = [("omega_pre_f", C.c_double), ("omega_pre_r", C.c_double), ("omega_post_f", C.c_double),
+                ("omega_post_r", C.c_double), ("omega", C.c_double), ("nu1", C.c_int), ("nu2", C.c_int),
+                ("gamma", C.c_int), ("min_size", C.c_int), ("reverse", C.c_int), ("m", C.c_int),
+                ("n", _PI), ("arp", C.POINTER(_PI)), ("aci", C.POINTER(_PI)), ("ava", C.POINTER(_PD)),
+                ("pt", C.POINTER(_PI)), ("icp", C.POINTER(_PI)), ("iri", C.POINTER(_PI)),
+                ("iva", C.POINTER(_PD)), ("ssor", C.c_void_p), ("lu", _PD), ("piv", _PI),
+                ("singular", C.c_int), ("u", C.POINTER(_PD)), ("f", C.POINTER(_PD)), ("r", C.POINTER(_PD)),
+                ("transpose", C.c_int), ("error", C.c_int)]
 
 
 _lib = None
@@ -68,6 +83,16 @@ def load():
         lib.oc_prec_diagonal.argtypes = [C.POINTER(Prec), C.c_int, P, P, P, C.POINTER(C.c_int)]
         lib.oc_prec_ilu.argtypes = [C.POINTER(Prec), C.c_int, P, P, P, C.POINTER(C.c_int)]
         lib.oc_prec_icc.argtypes = [C.POINTER(Prec), C.c_int, P, P, P, C.POINTER(C.c_int)]
+        lib.oc_amg_params.argtypes = [C.POINTER(AmgS), D, D, D, D, C.c_int, C.c_int, C.c_int, C.c_int, D,
+                                      C.c_int]
+        lib.oc_amg_setup.argtypes = [C.POINTER(AmgS), C.c_int, P, P, P, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)]
+        lib.oc_amg_apply.argtypes = [C.POINTER(AmgS), P, P, C.c_int]
+        lib.oc_amg_free.argtypes = [C.POINTER(AmgS)]
+        lib.oc_prec_amg.argtypes = [C.POINTER(Prec), C.POINTER(AmgS)]
+        lib.oc_java_hashset_order.argtypes = [P, C.c_int, P]
+        lib.oc_dgetf2.argtypes = [C.c_int, P, P]
+        lib.oc_dgetrs.argtypes = [C.c_int, C.c_int, P, P, P]
         lib.oc_prec_ssor.argtypes = [C.POINTER(Prec), C.c_int, P, P, P, C.c_int, C.c_double, C.c_double,
                                      C.POINTER(C.c_int)]
         lib.oc_prec_apply.argtypes = [C.POINTER(Prec), P, P]
@@ -180,6 +205,106 @@ class Precond:
             load().oc_prec_free(C.byref(self.s))
         except Exception:
             pass
+
+
+class Amg:
+    """S/AMG.java restatement.  Amg(A) uses AMG()'s defaults; params are
+    (omegaPreF, omegaPreR, omegaPostF, omegaPostR, nu1, nu2, gamma, min,
+    omega) with reverse=True (the nine-argument constructor) or
+    (omegaPre, omegaPost, nu1, nu2, gamma, min, omega) with reverse=False."""
+
+    def __init__(self, A, *params):
+        if not params:
+            params = (1.0, 1.85, 1.85, 1.0, 1, 1, 1, 40, 2.0 / 3)
+        if len(params) == 7:
+            wpre, wpost, nu1, nu2, gamma, mn, omega = params
+            full, reverse = (wpre, wpre, wpost, wpost, nu1, nu2, gamma, mn, omega), 0
+        else:
+            full, reverse = tuple(params), 1
+        self.g = AmgS()
+        self.bad = C.c_int(-1)
+        self.bad_level = C.c_int(-1)
+        lib = load()
+        wpf, wpr, wqf, wqr, nu1, nu2, gamma, mn, omega = full
+        lib.oc_amg_params(C.byref(self.g), float(wpf), float(wpr), float(wqf), float(wqr), int(nu1), int(nu2),
+                          int(gamma), int(mn), float(omega), reverse)
+        self.A = A
+        self.code = lib.oc_amg_setup(C.byref(self.g), A.nr, _p(A.rp), _p(A.ci), _p(A.va), C.byref(self.bad),
+                                     C.byref(self.bad_level))
+
+    def levels(self):
+        return [int(self.g.n[k]) for k in range(self.g.m)]
+
+    def matrix(self, k):
+        """(rowptr, colind, values) of A_k."""
+        n = self.g.n[k]
+        nnz = self.g.arp[k][n]
+        return (np.ctypeslib.as_array(self.g.arp[k], (n + 1,)).copy(),
+                np.ctypeslib.as_array(self.g.aci[k], (max(nnz, 1),))[:nnz].copy(),
+                np.ctypeslib.as_array(self.g.ava[k], (max(nnz, 1),))[:nnz].copy())
+
+    def interpolation(self, k):
+        """(colptr, rowind, values) of I_k (n_k x n_{k+1})."""
+        c = self.g.n[k + 1]
+        nnz = self.g.icp[k][c]
+        return (np.ctypeslib.as_array(self.g.icp[k], (c + 1,)).copy(),
+                np.ctypeslib.as_array(self.g.iri[k], (max(nnz, 1),))[:nnz].copy(),
+                np.ctypeslib.as_array(self.g.iva[k], (max(nnz, 1),))[:nnz].copy())
+
+    def aggregates(self, k):
+        return np.ctypeslib.as_array(self.g.pt[k], (self.g.n[k],)).copy()
+
+    def coarse_lu(self):
+        nc = self.g.n[self.g.m - 1]
+        lu = np.ctypeslib.as_array(self.g.lu, (nc * nc + 1,))[:nc * nc].reshape(nc, nc, order="F").copy()
+        piv = np.ctypeslib.as_array(self.g.piv, (max(nc, 1),))[:nc].copy()
+        return lu, piv, bool(self.g.singular)
+
+    def apply(self, b, x0=None, transpose=False):
+        x = np.zeros(self.g.n[0]) if x0 is None else np.array(x0, dtype=float)
+        rc = load().oc_amg_apply(C.byref(self.g), _p(np.ascontiguousarray(b, float)), _p(x), int(transpose))
+        return rc, x
+
+    def precond(self):
+        """This AMG as a Precond for solve()."""
+        P = Precond.__new__(Precond)
+        P.s = Prec()
+        P.bad = C.c_int(-1)
+        load().oc_prec_amg(C.byref(P.s), C.byref(self.g))
+        P.code = OK
+        P.A = self.A
+        P._owner = self
+        return P
+
+    def __del__(self):
+        try:
+            load().oc_amg_free(C.byref(self.g))
+        except Exception:
+            pass
+
+
+def java_hashset_order(vals):
+    """Iteration order of a java.util.HashSet<Integer> filled with vals in order."""
+    v = np.ascontiguousarray(vals, np.int32)
+    out = np.zeros(max(len(v), 1), np.int32)
+    rc = load().oc_java_hashset_order(_p(v), len(v), _p(out))
+    return rc, out[:len(v)]
+
+
+def dgetf2(a):
+    """Reference LAPACK dgetf2 on a copy of the square matrix a: (info, lu, piv 1-based)."""
+    n = a.shape[0]
+    lu = np.asfortranarray(np.array(a, dtype=float))
+    piv = np.zeros(max(n, 1), np.int32)
+    info = load().oc_dgetf2(n, _p(lu), _p(piv))
+    return info, lu, piv[:n]
+
+
+def dgetrs(lu, piv, b, trans=False):
+    n = lu.shape[0]
+    x = np.array(b, dtype=float)
+    load().oc_dgetrs(int(trans), n, _p(np.asfortranarray(lu)), _p(np.ascontiguousarray(piv, np.int32)), _p(x))
+    return x
 
 
 def monitor(max_iter=100000, rtol=1e-5, atol=1e-50, dtol=1e5):
