@@ -129,7 +129,47 @@ SOLVE_WORKLOADS = {
                           "vk", "mixed", "diag", 1e-10, False),
     "cg_varcoef_192": ("3D variable-kappa Poisson 192^3, FP64 Jacobi-CG to 1e-10 (oracle of the mixed solve)",
                        "vk", "cg", "diag", 1e-10, False),
+    "bicg_ilu_convdiff_64": ("3D upwind convection-diffusion 64^3 (Pe 10), BiCG + ILU(0) to rtol 1e-8",
+                             "cd64", "bicg", "ilu", 1e-8, True),
+    "cgs_ilu_convdiff_64": ("3D upwind convection-diffusion 64^3 (Pe 10), CGS + ILU(0) to rtol 1e-8",
+                            "cd64", "cgs", "ilu", 1e-8, True),
+    "qmr_ilu_convdiff_64": ("3D upwind convection-diffusion 64^3 (Pe 10), QMR + ILU(0) to rtol 1e-8",
+                            "cd64", "qmr", "ilu", 1e-8, True),
+    "cg_icc_poisson2d_256": ("2D 5-point Poisson 256^2, CG + ICC to rtol 1e-8", "p2d", "cg", "icc", 1e-8, True),
+    "mg_amg_poisson2d_256": ("2D 5-point Poisson 256^2, AMG() cycles x = M(b, x) as a multigrid iteration "
+                             "to rtol 1e-8", "p2d", "mg", "amg", 1e-8, True),
 }
+
+
+def _solve_system(gen):
+    from mtjgpu import problems as P
+    return {"p2d": lambda: P.poisson2d(256), "p3d": lambda: P.poisson3d(256),
+            "cd": lambda: P.convdiff3d(128, peclet=10.0, seed=1),
+            "cd64": lambda: P.convdiff3d(64, peclet=10.0, seed=1),
+            "vk": lambda: P.varcoef_poisson(3, 192, seed=1)}[gen]()
+
+
+def _mg_gpu(M, Mp, A, b, x, bh, rtol, max_cycles=200):
+    """x = M(b, x) until ||b - A x|| < rtol ||b||; returns the cycle count."""
+    r = M.DenseVector(len(bh), x.ctx)
+    nb = float(np.linalg.norm(bh))
+    for it in range(1, max_cycles + 1):
+        Mp.apply(b, x)
+        r.set(b)
+        A.multAdd(-1.0, x, r)
+        if r.norm(M.Norm.Two) < rtol * nb:
+            return it
+    return max_cycles
+
+
+def _mg_oracle(oracle, O, g, bh, rtol, max_cycles=200):
+    x = np.zeros(O.nr)
+    nb = float(np.linalg.norm(bh))
+    for it in range(1, max_cycles + 1):
+        _, x = g.apply(bh, x)
+        if np.linalg.norm(O.mult_add(-1.0, x, bh)) < rtol * nb:
+            return it, x
+    return max_cycles, x
 
 
 def solve_mode(name, repeats):
@@ -139,9 +179,7 @@ def solve_mode(name, repeats):
     import mtjgpu as M
     from mtjgpu import problems as P
     desc, gen, method, prec, rtol, with_oracle = SOLVE_WORKLOADS[name]
-    sysm = {"p2d": lambda: P.poisson2d(256), "p3d": lambda: P.poisson3d(256),
-            "cd": lambda: P.convdiff3d(128, peclet=10.0, seed=1),
-            "vk": lambda: P.varcoef_poisson(3, 192, seed=1)}[gen]()
+    sysm = _solve_system(gen)
     n, m, rp, ci, va = sysm
     import scipy.sparse as sp
     Ah = sp.csr_matrix((va, ci, rp), shape=(n, n))
@@ -151,30 +189,38 @@ def solve_mode(name, repeats):
     A = M.CompRowMatrix(n, m, rp, ci, va, ctx=ctx)
     b = M.DenseVector(bh, ctx)
     x = M.DenseVector(n, ctx)
-    cls = {"cg": M.CG, "bicgstab": M.BiCGstab, "gmres": M.GMRES, "mixed": M.MixedCG}[method]
-    S = cls(x)
-    if prec == "diag":
-        Mp = M.DiagonalPreconditioner(n, ctx)
-    else:
-        Mp = M.ILU(A)
+    S = None
+    if method != "mg":
+        cls = {"cg": M.CG, "bicgstab": M.BiCGstab, "gmres": M.GMRES, "mixed": M.MixedCG, "bicg": M.BiCG,
+               "cgs": M.CGS, "qmr": M.QMR}[method]
+        S = cls(x)
+    Mp = {"diag": lambda: M.DiagonalPreconditioner(n, ctx), "ilu": lambda: M.ILU(A), "icc": lambda: M.ICC(A),
+          "amg": lambda: M.AMG(ctx=ctx)}[prec]()
     t0 = time.perf_counter()
     Mp.setMatrix(A)
+    ctx.synchronize()
     setup = time.perf_counter() - t0
-    if method != "mixed":
-        S.setPreconditioner(Mp)
-    S.setIterationMonitor(M.DefaultIterationMonitor(100000 if method != "mixed" else 100, rtol))
+    if S is not None:
+        if method != "mixed":
+            S.setPreconditioner(Mp)
+        S.setIterationMonitor(M.DefaultIterationMonitor(100000 if method != "mixed" else 100, rtol))
     times = []
     for _ in range(repeats):
         x.zero()
         ctx.synchronize()
         t0 = time.perf_counter()
-        S.solve(A, b, x)
+        if S is None:
+            iters = _mg_gpu(M, Mp, A, b, x, bh, rtol)
+        else:
+            S.solve(A, b, x)
         ctx.synchronize()
         times.append(time.perf_counter() - t0)
     xs = x.getData()
+    if S is not None:
+        iters = S.getIterationMonitor().iterations()
     out = {"mode": "solve", "workload": name, "config": desc, "n": n, "nnz": len(va),
            "method": method, "preconditioner": prec, "rtol": rtol,
-           "iterations": S.getIterationMonitor().iterations(),
+           "iterations": iters,
            "gpu_s": round(min(times), 5), "gpu_s_all": [round(t, 5) for t in times],
            "setup_s": round(setup, 4),
            "true_rel_residual": float(np.linalg.norm(bh - Ah @ xs) / np.linalg.norm(bh)),
@@ -185,11 +231,87 @@ def solve_mode(name, repeats):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
         O = oracle.Csr(n, m, rp, ci, va)
-        op = oracle.Precond(oracle.PREC_DIAGONAL if prec == "diag" else oracle.PREC_ILU, O)
         t0 = time.perf_counter()
-        code, xo, it_o, _ = oracle.solve(method, O, bh, np.zeros(n), op, oracle.monitor(100000, rtol))
+        if prec == "amg":
+            g = oracle.Amg(O)
+        else:
+            op = oracle.Precond({"diag": oracle.PREC_DIAGONAL, "ilu": oracle.PREC_ILU,
+                                 "icc": oracle.PREC_ICC}[prec], O)
+        setup_o = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        if method == "mg":
+            it_o, xo = _mg_oracle(oracle, O, g, bh, rtol)
+        else:
+            code, xo, it_o, _ = oracle.solve(method, O, bh, np.zeros(n), op, oracle.monitor(100000, rtol))
         out["oracle"] = {"kind": "port", "cores": 1, "iterations": it_o, "cpu_s": round(time.perf_counter() - t0, 4),
-                         "rel_diff_x": float(np.linalg.norm(xs - xo) / np.linalg.norm(xo))}
+                         "setup_s": round(setup_o, 4),
+                         "rel_diff_x": float(np.linalg.norm(xs - xo) / np.linalg.norm(xo)),
+                         "bitwise_x": bool(xs.tobytes() == xo.tobytes())}
+    print(json.dumps(out))
+    return 0
+
+
+PRECOND_WORKLOADS = {
+    "p2d_256": ("2D 5-point Poisson 256^2", lambda P: P.poisson2d(256)),
+    "p3d_64": ("3D 7-point Poisson 64^3", lambda P: P.poisson3d(64)),
+    "cd_64": ("3D upwind convection-diffusion 64^3 (Pe 10)", lambda P: P.convdiff3d(64, peclet=10.0, seed=1)),
+}
+
+
+def precond_mode(prec, wl, steps):
+    """One preconditioner apply on the device, timed with HIP events over
+    `steps` applies, beside the oracle's apply on one core; the two results
+    are compared bit for bit.  SSOR and AMG start from x's incoming values,
+    so every apply starts from the same x (copied in, inside the timing)."""
+    import mtjgpu as M
+    from mtjgpu import problems as P
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    desc, gen = PRECOND_WORKLOADS[wl]
+    n, m, rp, ci, va = gen(P)
+    ctx = M.Context(0)
+    A = M.CompRowMatrix(n, m, rp, ci, va, ctx=ctx)
+    Mp = {"ilu": lambda: M.ILU(A), "ssor": lambda: M.SSOR(A), "icc": lambda: M.ICC(A),
+          "amg": lambda: M.AMG(ctx=ctx)}[prec]()
+    t0 = time.perf_counter()
+    Mp.setMatrix(A)
+    ctx.synchronize()
+    setup = time.perf_counter() - t0
+    bh = P.uniform(n, 4)
+    x0 = P.uniform(n, 5)
+    b = M.DenseVector(bh, ctx)
+    xs = M.DenseVector(x0, ctx)
+    x = M.DenseVector(n, ctx)
+    for _ in range(2):
+        x.set(xs)
+        Mp.apply(b, x)
+    ctx.synchronize()
+    timer = HipTimer(ctx.stream)
+    timer.start()
+    for _ in range(steps):
+        x.set(xs)
+        Mp.apply(b, x)
+    ms = timer.stop_ms() / steps
+    xg = x.getData()
+    O = oracle.Csr(n, m, rp, ci, va)
+    t0 = time.perf_counter()
+    if prec == "amg":
+        g = oracle.Amg(O)
+    else:
+        op = oracle.Precond({"ilu": oracle.PREC_ILU, "ssor": oracle.PREC_SSOR, "icc": oracle.PREC_ICC}[prec], O)
+    setup_o = time.perf_counter() - t0
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        xo = g.apply(bh, x0)[1] if prec == "amg" else op.apply(bh, x0)
+    cpu_ms = (time.perf_counter() - t0) / reps * 1e3
+    out = {"mode": "precond", "preconditioner": prec, "workload": wl, "config": desc, "n": n, "nnz": len(va),
+           "apply_ms": round(ms, 4), "setup_s": round(setup, 4),
+           "cpu_baseline": {"apply_ms": round(cpu_ms, 3), "setup_s": round(setup_o, 4), "cores": 1,
+                            "kind": "port"},
+           "speedup_apply": round(cpu_ms / ms, 2), "bitwise_equal": bool(xg.tobytes() == xo.tobytes())}
+    if prec == "amg":
+        out["levels"] = Mp.levels()
     print(json.dumps(out))
     return 0
 
@@ -283,7 +405,9 @@ def main():
     ap.add_argument("--workload", default="poisson3d_256")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", choices=["spmv", "copy", "solve", "serve", "mmload"], default="spmv")
+    ap.add_argument("--mode", choices=["spmv", "copy", "solve", "serve", "mmload", "precond"], default="spmv")
+    ap.add_argument("--prec", default="ilu", choices=["ilu", "ssor", "icc", "amg"])
+    ap.add_argument("--prec-workload", default="p2d_256", choices=sorted(PRECOND_WORKLOADS))
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--serve-requests", type=int, default=256,
@@ -303,6 +427,8 @@ def main():
         return solve_mode(args.solve, args.repeats)
     if args.mode == "serve":
         return serve_mode(args.requests, args.lam, args.seed)
+    if args.mode == "precond":
+        return precond_mode(args.prec, args.prec_workload, min(args.steps, 50))
     if args.mode == "mmload":
         return mmload_mode(args.grid, args.repeats, 0.0 if args.no_cpu else args.cpu_seconds)
 

@@ -30,6 +30,7 @@
 
 #include "amg_setup.hpp"
 #include "mtjg_internal.hpp"
+#include "rowdot.hpp"
 
 namespace mtjg {
 
@@ -62,9 +63,7 @@ __global__ __launch_bounds__(BS) void k_amg_restrict(int32_t nc, const int32_t *
                                                      const double *__restrict__ r, double *__restrict__ fc) {
   const int32_t i = blockIdx.x * BS + threadIdx.x;
   if (i >= nc) return;
-  double dot = 0;
-  for (int32_t e = ptr[i]; e < ptr[i + 1]; ++e) dot += val[e] * r[idx[e]];
-  fc[i] = dot;
+  fc[i] = rowdot<false>(0.0, ptr[i], ptr[i + 1] - ptr[i], idx, val, r);
 }
 
 // u[i] += I(i, j) * u_c[j] for the entries of fine row i in ascending j
@@ -75,9 +74,7 @@ __global__ __launch_bounds__(BS) void k_amg_prolong(int32_t nf, const int32_t *_
                                                     const double *__restrict__ uc, double *__restrict__ u) {
   const int32_t i = blockIdx.x * BS + threadIdx.x;
   if (i >= nf) return;
-  double s = u[i];
-  for (int32_t e = ptr[i]; e < ptr[i + 1]; ++e) s = s + val[e] * uc[idx[e]];
-  u[i] = s;
+  u[i] = rowdot<false>(u[i], ptr[i], ptr[i + 1] - ptr[i], idx, val, uc);
 }
 
 // r = f + (-1) (A u) with each row's dot summed sequentially in storage
@@ -90,8 +87,7 @@ __global__ __launch_bounds__(BS) void k_amg_resid(int32_t n, const int32_t *__re
                                                   double *__restrict__ r) {
   const int32_t i = blockIdx.x * BS + threadIdx.x;
   if (i >= n) return;
-  double dot = 0;
-  for (int32_t e = rp[i]; e < rp[i + 1]; ++e) dot += va[e] * u[ci[e]];
+  const double dot = rowdot<false>(0.0, rp[i], rp[i + 1] - rp[i], ci, va, u);
   r[i] = f[i] + -1.0 * dot;
 }
 
