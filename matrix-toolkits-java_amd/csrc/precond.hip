@@ -50,12 +50,30 @@ struct LevelRun {
   bool fused;
 };
 
+// The entries one sweep over a level set reads, copied in level order: for
+// level position t, desc[t] = {row, first packed entry, entry count, count of
+// strictly-lower entries (SSOR)}, the packed entries in the order the row
+// consumes them (columns pci, values pval gathered from the factor through
+// pmap), and the row's diagonal value pdv[t] (through pdmap).  A row then
+// costs one coalesced descriptor load, one mostly coalesced load of its
+// entries and the x gathers, instead of four dependent random loads.
+struct Packed {
+  int form = -1;                // TRSV_* form or PACK_SSOR; -1: not built
+  uint64_t version = ~0ull;     // factor values gathered at this version
+  int64_t nnz = 0;
+  int4 *desc = nullptr;
+  int32_t *pci = nullptr, *pmap = nullptr, *pdmap = nullptr;
+  double *pval = nullptr, *pdv = nullptr;
+};
+
 struct Levels {
   int32_t nlev = 0;
-  std::vector<int32_t> ptr;  // host copy of level offsets [nlev + 1]
-  int32_t *dptr = nullptr;   // device copy of the level offsets
-  int32_t *rows = nullptr;   // device: rows sorted by level
+  std::vector<int32_t> ptr;     // host copy of level offsets [nlev + 1]
+  int32_t *dptr = nullptr;      // device copy of the level offsets
+  int32_t *rows = nullptr;      // device: rows sorted by level
+  std::vector<int32_t> h_rows;  // host copy of rows
   std::vector<LevelRun> runs;
+  Packed pk;
 };
 
 struct mtjg_prec_s {
@@ -73,10 +91,12 @@ struct mtjg_prec_s {
   double *y = nullptr;  // temporary of apply
   std::vector<int32_t> h_rp, h_ci, h_diag;
   Levels lev_l, lev_u;
+  uint64_t vals_version = 0;  // bumped whenever lu or tval change
   // transposed factor (transApply), built on first use
   bool trans_ready = false;
   int32_t *trp = nullptr, *tci = nullptr, *tdiag = nullptr, *tperm = nullptr;
   double *tval = nullptr;
+  std::vector<int32_t> h_trp, h_tci, h_tdiag;
   Levels lev_ut, lev_lt;
   // single-launch solves: ticket counters [4] and the poll-budget error flag
   uint32_t *sf_next = nullptr;
@@ -188,34 +208,38 @@ enum TrsvForm : int {
   TRSV_COL_UNIT_REV = 3  // s = b_i; s -= a x descending; x_i = s   (L'.solve)
 };
 
-// Row i of a triangular solve in MTJ's per-row order.
+constexpr int PACK_SSOR = 4;
+
+// Level position t of a triangular solve over a packed level set, in MTJ's
+// per-row order (the transposed unit-lower form's entries are packed in its
+// descending order).
 template <int FORM>
-__device__ __forceinline__ void trsv_row(int32_t i, const int32_t *rp, const int32_t *ci, const int32_t *diag,
-                                         const double *a, const double *b, double *x) {
-  const int32_t r0 = rp[i], r1 = rp[i + 1], d = diag[i];
+__device__ __forceinline__ void trsv_row(int32_t t, const int4 *__restrict__ desc, const int32_t *__restrict__ pci,
+                                         const double *__restrict__ pval, const double *__restrict__ pdv,
+                                         const double *b, double *x) {
+  const int4 d = desc[t];
+  const int32_t i = d.x;
   if (FORM == TRSV_ROW_UNIT) {
-    const double sum = rowdot<false>(0.0, r0, d - r0, ci, a, x);
+    const double sum = rowdot<false>(0.0, d.y, d.z, pci, pval, x);
     x[i] = b[i] - sum;
   } else if (FORM == TRSV_ROW_DIV) {
-    const double sum = rowdot<false>(0.0, d + 1, r1 - d - 1, ci, a, x);
-    x[i] = (b[i] - sum) / a[d];
+    const double sum = rowdot<false>(0.0, d.y, d.z, pci, pval, x);
+    x[i] = (b[i] - sum) / pdv[t];
   } else if (FORM == TRSV_COL_DIV) {
-    const double s = rowdot<true>(b[i], r0, d - r0, ci, a, x);
-    x[i] = s / a[d];
+    x[i] = rowdot<true>(b[i], d.y, d.z, pci, pval, x) / pdv[t];
   } else {
-    x[i] = rowdot<true, true>(b[i], r1 - 1, r1 - 1 - d, ci, a, x);
+    x[i] = rowdot<true>(b[i], d.y, d.z, pci, pval, x);
   }
 }
 
-// One level of a triangular solve; one thread per row.
+// One level of a triangular solve (positions [t0, t0 + cnt)); a thread per row.
 template <int FORM>
-__global__ __launch_bounds__(BS) void k_trsv_level(int32_t cnt, const int32_t *rows,
-                                                   const int32_t *rp, const int32_t *ci,
-                                                   const int32_t *diag, const double *a,
-                                                   const double *b, double *x) {
+__global__ __launch_bounds__(BS) void k_trsv_level(int32_t cnt, int32_t t0, const int4 *desc, const int32_t *pci,
+                                                   const double *pval, const double *pdv, const double *b,
+                                                   double *x) {
   int32_t t = blockIdx.x * BS + threadIdx.x;
   if (t >= cnt) return;
-  trsv_row<FORM>(rows[t], rp, ci, diag, a, b, x);
+  trsv_row<FORM>(t0 + t, desc, pci, pval, pdv, b, x);
 }
 
 // Levels [l0, l1) of a triangular solve in one workgroup: each level has at
@@ -228,52 +252,52 @@ constexpr int FUSE_BS = 512;
 constexpr int FUSE_MAX = FUSE_BS;  // widest level a fused run takes: one pass
 template <int FORM>
 __global__ __launch_bounds__(FUSE_BS) void k_trsv_fused(int32_t l0, int32_t l1, const int32_t *lptr,
-                                                        const int32_t *rows, const int32_t *rp, const int32_t *ci,
-                                                        const int32_t *diag, const double *a, const double *b,
-                                                        double *x) {
+                                                        const int4 *desc, const int32_t *pci, const double *pval,
+                                                        const double *pdv, const double *b, double *x) {
   for (int32_t l = l0; l < l1; ++l) {
     const int32_t t = lptr[l] + (int32_t)threadIdx.x;
-    if (t < lptr[l + 1]) trsv_row<FORM>(rows[t], rp, ci, diag, a, b, x);
+    if (t < lptr[l + 1]) trsv_row<FORM>(t, desc, pci, pval, pdv, b, x);
     __syncthreads();
   }
 }
 
-// Row i of an SSOR sweep (S/SSOR.java:143-194) in MTJ's order: sigma adds
-// the strictly-lower entries against xx and the strictly-upper ones against
-// xd, then sigma = (b_i - sigma) / a_ii.  The forward sweep writes
-// xx_i = xd_i + omega (sigma - xd_i) (levels of the lower triangle; xd still
-// holds x's incoming values), the backward sweep
+// Level position t of an SSOR sweep (S/SSOR.java:143-194) over a packed
+// level set, in MTJ's order: sigma adds the strictly-lower entries against xx
+// and the strictly-upper ones against xd, then sigma = (b_i - sigma) / a_ii.
+// The forward sweep writes xx_i = xd_i + omega (sigma - xd_i) (levels of the
+// lower triangle; xd still holds x's incoming values), the backward sweep
 // xd_i = xx_i + omega (sigma - xx_i) (levels of the upper triangle, from the
 // last row up).
 template <bool FORWARD>
-__device__ __forceinline__ void ssor_row(int32_t i, const int32_t *rp, const int32_t *ci, const int32_t *diag,
-                                         const double *a, const double *b, double *xd, double *xx,
-                                         double omega) {
-  const int32_t r0 = rp[i], r1 = rp[i + 1], d = diag[i];
-  double sigma = rowdot<false>(0.0, r0, d - r0, ci, a, xx);
-  sigma = rowdot<false>(sigma, d + 1, r1 - d - 1, ci, a, xd);
-  sigma = (b[i] - sigma) / a[d];
+__device__ __forceinline__ void ssor_row(int32_t t, const int4 *__restrict__ desc, const int32_t *__restrict__ pci,
+                                         const double *__restrict__ pval, const double *__restrict__ pdv,
+                                         const double *b, double *xd, double *xx, double omega) {
+  const int4 d = desc[t];
+  const int32_t i = d.x;
+  double sigma = rowdot<false>(0.0, d.y, d.w, pci, pval, xx);
+  sigma = rowdot<false>(sigma, d.y + d.w, d.z - d.w, pci, pval, xd);
+  sigma = (b[i] - sigma) / pdv[t];
   if (FORWARD) xx[i] = xd[i] + omega * (sigma - xd[i]);
   else xd[i] = xx[i] + omega * (sigma - xx[i]);
 }
 
 template <bool FORWARD>
-__global__ __launch_bounds__(BS) void k_ssor_level(int32_t cnt, const int32_t *rows, const int32_t *rp,
-                                                   const int32_t *ci, const int32_t *diag, const double *a,
-                                                   const double *b, double *xd, double *xx, double omega) {
+__global__ __launch_bounds__(BS) void k_ssor_level(int32_t cnt, int32_t t0, const int4 *desc, const int32_t *pci,
+                                                   const double *pval, const double *pdv, const double *b,
+                                                   double *xd, double *xx, double omega) {
   int32_t t = blockIdx.x * BS + threadIdx.x;
   if (t >= cnt) return;
-  ssor_row<FORWARD>(rows[t], rp, ci, diag, a, b, xd, xx, omega);
+  ssor_row<FORWARD>(t0 + t, desc, pci, pval, pdv, b, xd, xx, omega);
 }
 
 template <bool FORWARD>
 __global__ __launch_bounds__(FUSE_BS) void k_ssor_fused(int32_t l0, int32_t l1, const int32_t *lptr,
-                                                        const int32_t *rows, const int32_t *rp, const int32_t *ci,
-                                                        const int32_t *diag, const double *a, const double *b,
-                                                        double *xd, double *xx, double omega) {
+                                                        const int4 *desc, const int32_t *pci, const double *pval,
+                                                        const double *pdv, const double *b, double *xd, double *xx,
+                                                        double omega) {
   for (int32_t l = l0; l < l1; ++l) {
     const int32_t t = lptr[l] + (int32_t)threadIdx.x;
-    if (t < lptr[l + 1]) ssor_row<FORWARD>(rows[t], rp, ci, diag, a, b, xd, xx, omega);
+    if (t < lptr[l + 1]) ssor_row<FORWARD>(t, desc, pci, pval, pdv, b, xd, xx, omega);
     __syncthreads();
   }
 }
@@ -397,6 +421,7 @@ static void build_levels(mtjg_ctx ctx, int32_t n, const std::vector<int32_t> &rp
   std::vector<int32_t> pos(L.ptr.begin(), L.ptr.end() - (L.nlev > 0 ? 1 : 0));
   std::vector<int32_t> rows((size_t)std::max(n, 1));
   for (int32_t i = 0; i < n; ++i) rows[(size_t)pos[(size_t)lev[(size_t)i]]++] = i;
+  L.h_rows.assign(rows.begin(), rows.begin() + n);
   if (L.rows) (void)hipFree(L.rows);
   if (L.dptr) (void)hipFree(L.dptr);
   HIPCHK(hipMalloc(&L.rows, sizeof(int32_t) * rows.size()));
@@ -422,15 +447,86 @@ static void build_levels(mtjg_ctx ctx, int32_t n, const std::vector<int32_t> &rp
   HIPCHK(hipStreamSynchronize(ctx->stream));
 }
 
+static void free_pack(Packed &P) {
+  for (void *q : {(void *)P.desc, (void *)P.pci, (void *)P.pmap, (void *)P.pdmap, (void *)P.pval, (void *)P.pdv})
+    (void)hipFree(q);
+  P = Packed();
+}
+
+// Builds (once per pattern and form) and refreshes (whenever the values
+// changed) the packed copy of level set L over the structure (rp, ci, diag),
+// whose values are `a`.
+static void ensure_pack(mtjg_prec M, Levels &L, int form, const std::vector<int32_t> &rp,
+                        const std::vector<int32_t> &ci, const std::vector<int32_t> &diag, const double *a,
+                        hipStream_t s) {
+  Packed &P = L.pk;
+  const int32_t n = (int32_t)L.h_rows.size();
+  if (P.form != form) {
+    free_pack(P);
+    std::vector<int4> desc((size_t)std::max(n, 1));
+    std::vector<int32_t> pci, pmap, pdmap((size_t)std::max(n, 1));
+    pci.reserve(ci.size());
+    pmap.reserve(ci.size());
+    auto push = [&](int32_t j) {
+      pci.push_back(ci[(size_t)j]);
+      pmap.push_back(j);
+    };
+    for (int32_t t = 0; t < n; ++t) {
+      const int32_t i = L.h_rows[(size_t)t], r0 = rp[(size_t)i], r1 = rp[(size_t)i + 1], d = diag[(size_t)i];
+      const int32_t e0 = (int32_t)pci.size();
+      int32_t nlo = 0;
+      if (form == TRSV_ROW_UNIT || form == TRSV_COL_DIV) {
+        for (int32_t j = r0; j < d; ++j) push(j);
+      } else if (form == TRSV_ROW_DIV) {
+        for (int32_t j = d + 1; j < r1; ++j) push(j);
+      } else if (form == TRSV_COL_UNIT_REV) {
+        for (int32_t j = r1 - 1; j > d; --j) push(j);
+      } else {  // PACK_SSOR: the row's strictly-lower, then strictly-upper entries
+        for (int32_t j = r0; j < d; ++j) push(j);
+        nlo = d - r0;
+        for (int32_t j = d + 1; j < r1; ++j) push(j);
+      }
+      desc[(size_t)t] = make_int4(i, e0, (int32_t)pci.size() - e0, nlo);
+      pdmap[(size_t)t] = d;
+    }
+    P.nnz = (int64_t)pci.size();
+    const size_t m1 = (size_t)std::max<int64_t>(P.nnz, 1);
+    HIPCHK(hipMalloc(&P.desc, sizeof(int4) * desc.size()));
+    HIPCHK(hipMalloc(&P.pdmap, sizeof(int32_t) * pdmap.size()));
+    HIPCHK(hipMalloc(&P.pdv, sizeof(double) * pdmap.size()));
+    HIPCHK(hipMalloc(&P.pci, sizeof(int32_t) * m1));
+    HIPCHK(hipMalloc(&P.pmap, sizeof(int32_t) * m1));
+    HIPCHK(hipMalloc(&P.pval, sizeof(double) * m1));
+    HIPCHK(hipMemcpyAsync(P.desc, desc.data(), sizeof(int4) * desc.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(P.pdmap, pdmap.data(), sizeof(int32_t) * pdmap.size(), hipMemcpyHostToDevice, s));
+    if (P.nnz) {
+      HIPCHK(hipMemcpyAsync(P.pci, pci.data(), sizeof(int32_t) * (size_t)P.nnz, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(P.pmap, pmap.data(), sizeof(int32_t) * (size_t)P.nnz, hipMemcpyHostToDevice, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));  // host staging vectors go out of scope
+    P.form = form;
+    P.version = ~0ull;
+  }
+  if (P.version != M->vals_version) {
+    if (P.nnz) k_gather<<<grid_for(P.nnz), BS, 0, s>>>(P.nnz, P.pmap, a, P.pval);
+    if (n) k_gather<<<grid_for(n), BS, 0, s>>>(n, P.pdmap, a, P.pdv);
+    HIPCHK(hipGetLastError());
+    P.version = M->vals_version;
+  }
+}
+
 template <int FORM>
-static void run_levels(const Levels &L, hipStream_t s, const int32_t *rp, const int32_t *ci,
-                       const int32_t *diag, const double *a, const double *b, double *x) {
+static void run_levels(mtjg_prec M, Levels &L, hipStream_t s, const std::vector<int32_t> &rp,
+                       const std::vector<int32_t> &ci, const std::vector<int32_t> &diag, const double *a,
+                       const double *b, double *x) {
+  ensure_pack(M, L, FORM, rp, ci, diag, a, s);
+  const Packed &P = L.pk;
   for (const LevelRun &r : L.runs) {
     if (r.fused) {
-      k_trsv_fused<FORM><<<1, FUSE_BS, 0, s>>>(r.l0, r.l1, L.dptr, L.rows, rp, ci, diag, a, b, x);
+      k_trsv_fused<FORM><<<1, FUSE_BS, 0, s>>>(r.l0, r.l1, L.dptr, P.desc, P.pci, P.pval, P.pdv, b, x);
     } else {
       const int32_t cnt = L.ptr[r.l0 + 1] - L.ptr[r.l0];
-      k_trsv_level<FORM><<<(cnt + BS - 1) / BS, BS, 0, s>>>(cnt, L.rows + L.ptr[r.l0], rp, ci, diag, a, b, x);
+      k_trsv_level<FORM><<<(cnt + BS - 1) / BS, BS, 0, s>>>(cnt, L.ptr[r.l0], P.desc, P.pci, P.pval, P.pdv, b, x);
     }
   }
   HIPCHK(hipGetLastError());
@@ -457,21 +553,23 @@ static void run_syncfree(mtjg_prec M, const Levels &L, int slot, hipStream_t s, 
 // xx = x is not needed: the forward sweep writes every xx_i before any row
 // reads it.
 template <bool FORWARD>
-static void ssor_sweep(mtjg_prec M, const Levels &L, hipStream_t s, const double *b, double *x, double omega) {
+static void ssor_sweep(mtjg_prec M, Levels &L, hipStream_t s, const double *b, double *x, double omega) {
+  ensure_pack(M, L, PACK_SSOR, M->h_rp, M->h_ci, M->h_diag, M->lu, s);
+  const Packed &P = L.pk;
   for (const LevelRun &r : L.runs) {
     if (r.fused) {
-      k_ssor_fused<FORWARD><<<1, FUSE_BS, 0, s>>>(r.l0, r.l1, L.dptr, L.rows, M->rp, M->ci, M->diag, M->lu, b, x,
-                                                  M->xx, omega);
+      k_ssor_fused<FORWARD><<<1, FUSE_BS, 0, s>>>(r.l0, r.l1, L.dptr, P.desc, P.pci, P.pval, P.pdv, b, x, M->xx,
+                                                  omega);
     } else {
       const int32_t cnt = L.ptr[r.l0 + 1] - L.ptr[r.l0];
-      k_ssor_level<FORWARD><<<(cnt + BS - 1) / BS, BS, 0, s>>>(cnt, L.rows + L.ptr[r.l0], M->rp, M->ci, M->diag,
-                                                               M->lu, b, x, M->xx, omega);
+      k_ssor_level<FORWARD><<<(cnt + BS - 1) / BS, BS, 0, s>>>(cnt, L.ptr[r.l0], P.desc, P.pci, P.pval, P.pdv, b,
+                                                               x, M->xx, omega);
     }
   }
 }
 
 static void run_ssor(mtjg_prec M, hipStream_t s, const double *b, double *x) {
-  const Levels &L = M->lev_l, &U = M->lev_u;
+  Levels &L = M->lev_l, &U = M->lev_u;
   ssor_sweep<true>(M, L, s, b, x, M->omega_f);
   if (!M->reverse) {
     if (M->n) HIPCHK(hipMemcpyAsync(x, M->xx, sizeof(double) * (size_t)M->n, hipMemcpyDeviceToDevice, s));
@@ -488,6 +586,8 @@ static void free_levels(Levels &L) {
   L.rows = nullptr;
   L.dptr = nullptr;
   L.runs.clear();
+  L.h_rows.clear();
+  free_pack(L.pk);
   L.nlev = 0;
   L.ptr.clear();
 }
@@ -504,6 +604,9 @@ static void ilu_free_storage(mtjg_prec M) {
   (void)hipFree(M->tval);
   M->rp = M->ci = M->diag = M->trp = M->tci = M->tdiag = M->tperm = nullptr;
   M->lu = M->tval = nullptr;
+  M->h_trp.clear();
+  M->h_tci.clear();
+  M->h_tdiag.clear();
   free_levels(M->lev_l);
   free_levels(M->lev_u);
   free_levels(M->lev_ut);
@@ -586,6 +689,7 @@ static void take_pattern(mtjg_prec M, mtjg_csr A, int32_t *bad_row, const char *
   }
   if (A->nnz)
     HIPCHK(hipMemcpyAsync(M->lu, A->val, sizeof(double) * (size_t)A->nnz, hipMemcpyDeviceToDevice, s));
+  M->vals_version++;
 }
 
 // ILU.setMatrix: LU.set(A), then factor() (S/ILU.java:86-125).
@@ -610,6 +714,7 @@ static void ilu_set_matrix(mtjg_prec M, mtjg_csr A, int32_t *bad_row) {
   M->factored = true;
   if (M->trans_ready && M->tval)
     k_gather<<<grid_for(M->nnz), BS, 0, s>>>(M->nnz, M->tperm, M->lu, M->tval);
+  M->vals_version++;
 }
 
 static void build_transpose(mtjg_prec M, bool lower_levels);
@@ -639,6 +744,7 @@ static void icc_set_matrix(mtjg_prec M, mtjg_csr A, int32_t *bad_row) {
   if (!M->trans_ready) build_transpose(M, false);
   else if (M->nnz) k_gather<<<grid_for(M->nnz), BS, 0, s>>>(M->nnz, M->tperm, M->lu, M->tval);
   HIPCHK(hipGetLastError());
+  M->vals_version++;
   M->factored = true;
 }
 
@@ -690,7 +796,11 @@ static void build_transpose(mtjg_prec M, bool lower_levels) {
   }
   if (nnz) k_gather<<<grid_for(nnz), BS, 0, s>>>(nnz, M->tperm, M->lu, M->tval);
   HIPCHK(hipGetLastError());
+  M->h_trp = std::move(trp);
+  M->h_tci = std::move(tci);
+  M->h_tdiag = std::move(tdiag);
   M->trans_ready = true;
+  M->vals_version++;
 }
 
 void prec_apply_dev(mtjg_prec M, int64_t n, const double *b, double *x, hipStream_t s, KState *st) {
@@ -709,8 +819,8 @@ void prec_apply_dev(mtjg_prec M, int64_t n, const double *b, double *x, hipStrea
     if (!M->factored) fail(MTJG_E_INVALID_ARGUMENT, "ICC.setMatrix must precede apply");
     // Rt.transSolve(b, y); Rt.solve(y, x) (S/ICC.java:65-71)
     if (M->level_launch) {
-      run_levels<TRSV_COL_DIV>(M->lev_ut, s, M->trp, M->tci, M->tdiag, M->tval, b, M->y);
-      run_levels<TRSV_ROW_DIV>(M->lev_u, s, M->rp, M->ci, M->diag, M->lu, M->y, x);
+      run_levels<TRSV_COL_DIV>(M, M->lev_ut, s, M->h_trp, M->h_tci, M->h_tdiag, M->tval, b, M->y);
+      run_levels<TRSV_ROW_DIV>(M, M->lev_u, s, M->h_rp, M->h_ci, M->h_diag, M->lu, M->y, x);
     } else {
       run_syncfree<TRSV_COL_DIV>(M, M->lev_ut, 2, s, M->trp, M->tci, M->tdiag, M->tval, b, M->y);
       run_syncfree<TRSV_ROW_DIV>(M, M->lev_u, 1, s, M->rp, M->ci, M->diag, M->lu, M->y, x);
@@ -719,8 +829,8 @@ void prec_apply_dev(mtjg_prec M, int64_t n, const double *b, double *x, hipStrea
     if (!M->factored) fail(MTJG_E_INVALID_ARGUMENT, "ILU.setMatrix must precede apply");
     // L.solve(b, y); U.solve(y, x) (S/ILU.java:64-70)
     if (M->level_launch) {
-      run_levels<TRSV_ROW_UNIT>(M->lev_l, s, M->rp, M->ci, M->diag, M->lu, b, M->y);
-      run_levels<TRSV_ROW_DIV>(M->lev_u, s, M->rp, M->ci, M->diag, M->lu, M->y, x);
+      run_levels<TRSV_ROW_UNIT>(M, M->lev_l, s, M->h_rp, M->h_ci, M->h_diag, M->lu, b, M->y);
+      run_levels<TRSV_ROW_DIV>(M, M->lev_u, s, M->h_rp, M->h_ci, M->h_diag, M->lu, M->y, x);
     } else {
       run_syncfree<TRSV_ROW_UNIT>(M, M->lev_l, 0, s, M->rp, M->ci, M->diag, M->lu, b, M->y);
       run_syncfree<TRSV_ROW_DIV>(M, M->lev_u, 1, s, M->rp, M->ci, M->diag, M->lu, M->y, x);
@@ -743,8 +853,8 @@ void prec_trans_apply_dev(mtjg_prec M, const double *b, double *x, hipStream_t s
   if (!M->trans_ready) build_transpose(M, true);
   // U.transSolve(b, y); L.transSolve(y, x) (S/ILU.java:72-78)
   if (M->level_launch) {
-    run_levels<TRSV_COL_DIV>(M->lev_ut, s, M->trp, M->tci, M->tdiag, M->tval, b, M->y);
-    run_levels<TRSV_COL_UNIT_REV>(M->lev_lt, s, M->trp, M->tci, M->tdiag, M->tval, M->y, x);
+    run_levels<TRSV_COL_DIV>(M, M->lev_ut, s, M->h_trp, M->h_tci, M->h_tdiag, M->tval, b, M->y);
+    run_levels<TRSV_COL_UNIT_REV>(M, M->lev_lt, s, M->h_trp, M->h_tci, M->h_tdiag, M->tval, M->y, x);
   } else {
     run_syncfree<TRSV_COL_DIV>(M, M->lev_ut, 2, s, M->trp, M->tci, M->tdiag, M->tval, b, M->y);
     run_syncfree<TRSV_COL_UNIT_REV>(M, M->lev_lt, 3, s, M->trp, M->tci, M->tdiag, M->tval, M->y, x);
