@@ -140,6 +140,7 @@ __global__ __launch_bounds__(BS) void k_sell_dot(int nslice, int n, const int32_
   if (st->status != ST_RUNNING) return;
   dd acc0{0.0, 0.0}, acc1{0.0, 0.0};
   constexpr int WPB = BS / SELL_C;
+  // (the XCD run mapping of k_sell measured 1.4% slower on this grid-stride walk)
   for (int slice = blockIdx.x * WPB + (int)threadIdx.x / SELL_C; slice < nslice; slice += gridDim.x * WPB)
     sell_row<VT, XT, SPMV_DOT>(slice, threadIdx.x % SELL_C, n, sptr, scol, sval, x, y, nullptr, 1.0, 1.0, w,
                                acc0, acc1);
