@@ -258,7 +258,7 @@ PRECOND_WORKLOADS = {
 }
 
 
-def precond_mode(prec, wl, steps):
+def precond_mode(prec, wl, steps, with_cpu=True):
     """One preconditioner apply on the device, timed with HIP events over
     `steps` applies, beside the oracle's apply on one core; the two results
     are compared bit for bit.  SSOR and AMG start from x's incoming values,
@@ -293,6 +293,13 @@ def precond_mode(prec, wl, steps):
         Mp.apply(b, x)
     ms = timer.stop_ms() / steps
     xg = x.getData()
+    if not with_cpu:
+        out = {"mode": "precond", "preconditioner": prec, "workload": wl, "config": desc, "n": n, "nnz": len(va),
+               "apply_ms": round(ms, 4), "setup_s": round(setup, 4)}
+        if prec == "amg":
+            out["levels"] = Mp.levels()
+        print(json.dumps(out))
+        return 0
     O = oracle.Csr(n, m, rp, ci, va)
     t0 = time.perf_counter()
     if prec == "amg":
@@ -428,7 +435,7 @@ def main():
     if args.mode == "serve":
         return serve_mode(args.requests, args.lam, args.seed)
     if args.mode == "precond":
-        return precond_mode(args.prec, args.prec_workload, min(args.steps, 50))
+        return precond_mode(args.prec, args.prec_workload, min(args.steps, 50), not args.no_cpu)
     if args.mode == "mmload":
         return mmload_mode(args.grid, args.repeats, 0.0 if args.no_cpu else args.cpu_seconds)
 
