@@ -134,3 +134,43 @@ def test_ilu_reference_known_answers(ctx, name):
         b = d @ np.full(n, fx["x"])
         x = ilu.apply(M.DenseVector(b, ctx), M.DenseVector(n, ctx)).getData()
         assert np.linalg.norm(b - d @ x) <= fx["residual_tol"]
+
+
+def _wide_level_systems():
+    # 40^3 conv-diff: levels up to ~1,200 rows wide (launched on their own),
+    # rows of at most 3 strictly-lower / upper entries (slab levels)
+    yield "convdiff_40", P.convdiff3d(40, peclet=10.0, seed=5)
+    # random SPD, ~20 entries per row: wide levels of long rows (row-major)
+    n = 20000
+    rng = np.random.default_rng(9)
+    d = sp.random(n, n, density=10.0 / n, random_state=9, format="csr")
+    d = d + d.T + sp.diags(rng.random(n) + 30.0)
+    d = sp.csr_matrix(d)
+    d.sort_indices()
+    yield "random_spd_20k", (n, n, d.indptr.astype(np.int32), d.indices.astype(np.int32), d.data)
+
+
+@pytest.mark.parametrize("name,sysm", list(_wide_level_systems()))
+def test_wide_levels_bitwise(ctx, name, sysm):
+    """Levels wider than one fused workgroup run as launches of their own, in
+    both packed layouts; ILU apply/transApply, ICC and SSOR stay bitwise."""
+    n = sysm[0]
+    A = M.CompRowMatrix(*sysm, ctx=ctx)
+    O = oracle.Csr(*sysm)
+    bh = P.uniform(n, 11)
+    x0 = P.uniform(n, 12)
+    b = M.DenseVector(bh, ctx)
+    ilu = M.ILU(A)
+    ilu.setMatrix(A)
+    Oi = oracle.Precond(oracle.PREC_ILU, O)
+    np.testing.assert_array_equal(ilu.apply(b, M.DenseVector(n, ctx)).getData(), Oi.apply(bh))
+    np.testing.assert_array_equal(ilu.transApply(b, M.DenseVector(n, ctx)).getData(), Oi.trans_apply(bh))
+    ssor = M.SSOR(A, omegaF=1.2, omegaR=0.8)
+    ssor.setMatrix(A)
+    Os = oracle.Precond(oracle.PREC_SSOR, O, omega_f=1.2, omega_r=0.8)
+    np.testing.assert_array_equal(ssor.apply(b, M.DenseVector(x0, ctx)).getData(), Os.apply(bh, x0))
+    if name.startswith("random_spd"):
+        icc = M.ICC(A)
+        icc.setMatrix(A)
+        Oc = oracle.Precond(oracle.PREC_ICC, O)
+        np.testing.assert_array_equal(icc.apply(b, M.DenseVector(n, ctx)).getData(), Oc.apply(bh))
