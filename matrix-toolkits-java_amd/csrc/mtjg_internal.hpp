@@ -192,6 +192,9 @@ struct SpmvArgs {
   GridRed red{};             // SPMV_DOT: two-level reduction workspace
 };
 
+// Reduction workspace columns (GridRed::cap) an SPMV_DOT product of A needs.
+int64_t spmv_dot_red_cap(mtjg_csr A);
+
 void spmv_f64(mtjg_csr A, const double *x, double *y, const SpmvArgs &a, hipStream_t s);
 void spmv_f32(mtjg_csr A, const float *x, float *y, const SpmvArgs &a, hipStream_t s);
 mtjg_csr_s *csr_transpose(mtjg_csr A);

@@ -602,10 +602,9 @@ void free_red(GridRed &r) {
 // the vector kernels' vgrid(n) and the fused product's one workgroup per
 // row-block.  Counters start at zero and are left at zero by every launch.
 // Every reducing kernel of a solve runs at most VGRID_MAX (vector kernels)
-// or 2048 (fused products, row-block or SELL) workgroups.
+// workgroups; the fused product's need comes from spmv_dot_red_cap.
 void ensure_red(mtjg_solver S, mtjg_csr A) {
-  (void)A;
-  const int need = std::max(VGRID_MAX, 2048);
+  const int need = (int)std::max<int64_t>(VGRID_MAX, spmv_dot_red_cap(A));
   if (need <= S->red.cap) return;
   HIPCHK(hipStreamSynchronize(S->ctx->stream));
   free_red(S->red);

@@ -130,8 +130,9 @@ __global__ __launch_bounds__(BS) void k_sell(int nslice, int n, const int32_t *_
                          acc0, acc1);
 }
 
-// Fused solver product from the SELL-64 copy (see k_spmv_dot): at most
-// SPMV_DOT_GRID workgroups walk the slices in grid-stride order.
+// Fused solver product from the SELL-64 copy (see k_spmv_dot) for grids of
+// at most SPMV_DOT_GRID workgroups (larger ones use k_sell_part + k_sell_fin):
+// the workgroups walk the slices in grid-stride order.
 template <typename VT, typename XT>
 __global__ __launch_bounds__(BS) void k_sell_dot(int nslice, int n, const int32_t *__restrict__ sptr,
                                                  const int32_t *__restrict__ scol, const VT *__restrict__ sval,
@@ -145,6 +146,52 @@ __global__ __launch_bounds__(BS) void k_sell_dot(int nslice, int n, const int32_
     sell_row<VT, XT, SPMV_DOT>(slice, threadIdx.x % SELL_C, n, sptr, scol, sval, x, y, nullptr, 1.0, 1.0, w,
                                acc0, acc1);
   dd v[2] = {acc0, acc1};
+  double tot[2];
+  if (grid_sum_dd<2>(v, red, tot) && threadIdx.x == 0) spmv_finish(st, fin, tot[0], tot[1]);
+}
+
+// Split form of k_sell_dot for grids above SPMV_DOT_GRID: the product runs
+// k_sell's grid (one workgroup per four slices, XCD run mapping) and each
+// workgroup stores its two compensated partial sums, without a ticket, to
+// red.partials[j * red.cap + off + blockIdx.x]; k_sell_fin then adds them.
+// A per-workgroup ticket waits on a device-scope atomic round trip while the
+// workgroup holds its CU slot, which is what made the ticketed full grid slow.
+template <typename VT, typename XT>
+__global__ __launch_bounds__(BS) void k_sell_part(int nslice, int n, const int32_t *__restrict__ sptr,
+                                                  const int32_t *__restrict__ scol, const VT *__restrict__ sval,
+                                                  const XT *__restrict__ x, XT *y, const XT *w, const KState *st,
+                                                  double *part, int cap) {
+  if (st->status != ST_RUNNING) return;
+  int blk = blockIdx.x;
+  const int full = (int)(gridDim.x / (8 * SELL_XCD_RUN)) * (8 * SELL_XCD_RUN);
+  if (blk < full) {
+    const int xcd = blk % 8, j = blk / 8;
+    blk = ((j / SELL_XCD_RUN) * 8 + xcd) * SELL_XCD_RUN + j % SELL_XCD_RUN;
+  }
+  const int slice = blk * (BS / SELL_C) + (int)threadIdx.x / SELL_C;
+  dd v[2] = {{0.0, 0.0}, {0.0, 0.0}};
+  if (slice < nslice)
+    sell_row<VT, XT, SPMV_DOT>(slice, threadIdx.x % SELL_C, n, sptr, scol, sval, x, y, nullptr, 1.0, 1.0, w, v[0],
+                               v[1]);
+  block_sum_dd<2>(v);
+  if (threadIdx.x == 0) {
+    part[0 * (size_t)cap + blockIdx.x] = v[0].hi;
+    part[1 * (size_t)cap + blockIdx.x] = v[0].lo;
+    part[2 * (size_t)cap + blockIdx.x] = v[1].hi;
+    part[3 * (size_t)cap + blockIdx.x] = v[1].lo;
+  }
+}
+
+// Adds k_sell_part's np partials (RED_GROUP per workgroup, then the grid
+// reduction, whose block partials sit below `part`) and runs the epilogue.
+__global__ __launch_bounds__(BS) void k_sell_fin(int np, const double *part, KState *st, int fin, GridRed red) {
+  if (st->status != ST_RUNNING) return;
+  const int i = blockIdx.x * BS + (int)threadIdx.x;
+  dd v[2] = {{0.0, 0.0}, {0.0, 0.0}};
+  if (i < np) {
+    v[0] = dd{part[0 * (size_t)red.cap + i], part[1 * (size_t)red.cap + i]};
+    v[1] = dd{part[2 * (size_t)red.cap + i], part[3 * (size_t)red.cap + i]};
+  }
   double tot[2];
   if (grid_sum_dd<2>(v, red, tot) && threadIdx.x == 0) spmv_finish(st, fin, tot[0], tot[1]);
 }
@@ -183,6 +230,18 @@ static void sell_ready(mtjg_csr A, bool fp32) {
   }
 }
 
+// Workgroups of k_sell_fin for a k_sell_part grid.
+static int sell_fin_grid(int grid) { return (grid + BS - 1) / BS; }
+
+int64_t spmv_dot_red_cap(mtjg_csr A) {
+  int64_t need = SPMV_DOT_GRID;
+  if (A->sell) {
+    const int grid = (A->nslice + BS / SELL_C - 1) / (BS / SELL_C);
+    if (grid > SPMV_DOT_GRID) need = std::max<int64_t>(need, (int64_t)sell_fin_grid(grid) + grid);
+  }
+  return need;
+}
+
 template <typename VT, typename XT>
 static void sell_launch(mtjg_csr A, const VT *sval, const XT *x, XT *y, const SpmvArgs &a, hipStream_t s) {
   const int grid = (A->nslice + BS / SELL_C - 1) / (BS / SELL_C);
@@ -201,6 +260,16 @@ static void sell_launch(mtjg_csr A, const VT *sval, const XT *x, XT *y, const Sp
                                                        a.alpha, a.inv_alpha);
       break;
     case SPMV_DOT: {
+      if (grid > SPMV_DOT_GRID) {  // 256^3 Poisson CG: 0.310 -> 0.288 s
+        const int f = sell_fin_grid(grid);
+        if ((int64_t)f + grid > a.red.cap) fail(MTJG_E_INVALID_ARGUMENT, "reduction workspace smaller than the grid");
+        double *part = a.red.partials + f;
+        k_sell_part<VT, XT><<<grid, BS, 0, s>>>(A->nslice, A->nr, A->sptr, A->scol, sval, x, y, (const XT *)a.w,
+                                                a.st, part, a.red.cap);
+        HIPCHK(hipGetLastError());
+        k_sell_fin<<<f, BS, 0, s>>>(grid, part, a.st, a.fin, a.red);
+        break;
+      }
       const int g = std::min(grid, SPMV_DOT_GRID);
       if (g > a.red.cap) fail(MTJG_E_INVALID_ARGUMENT, "reduction workspace smaller than the grid");
       k_sell_dot<VT, XT><<<g, BS, 0, s>>>(A->nslice, A->nr, A->sptr, A->scol, sval, x, y, (const XT *)a.w, a.st,
