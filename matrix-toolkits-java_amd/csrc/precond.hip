@@ -42,13 +42,18 @@
 #include "rowdot.hpp"
 
 // A run of levels [l0, l1) launched together: `fused` runs go to one
-// workgroup that walks the levels with a barrier between them (levels no
-// wider than FUSE_MAX rows); other runs are a single level launched over the
-// whole device.
+// workgroup, either as an LDS chunk (`lds`, slab levels, see k_sweep_lds) or
+// as a walk over narrow row-major levels with a barrier between them; other
+// runs are a single level launched over the whole device.
 struct LevelRun {
   int32_t l0, l1;
   bool fused;
-  bool slab = false;  // levels stored as slabs (set once the packed copy exists)
+  bool slab = false;  // levels stored as slabs
+  // LDS chunks: positions [t0, t1), imports imp[imp0, imp0 + nimp), widest
+  // row wmax, steps[s0, s0 + nsteps)
+  bool lds = false;
+  int32_t t0 = 0, t1 = 0, nimp = 0, wmax = 0, nsteps = 0;
+  int64_t imp0 = 0, s0 = 0;
 };
 
 // The entries one sweep over a level set reads, copied in level order.  Level
@@ -78,6 +83,16 @@ struct Packed {
   std::vector<uint8_t> slab;   // per level: stored as a slab
   int32_t *eci = nullptr, *emap = nullptr;
   double *eval = nullptr;
+  // LDS chunks: per slot of their levels the LDS index of the value the slot
+  // multiplies (-1 none, -2 a fixed SSOR operand whose product the sweep's
+  // prep stores in fval), the imported global indices, the fixed slots, and
+  // per-apply level-order copies of b and of SSOR's own operand
+  bool any_lds = false;
+  int32_t *lci = nullptr, *imp = nullptr;
+  int4 *steps = nullptr;  // per step {first position - t0, rows | width << 16 | barrier << 24, slot base, stride}
+  int64_t *fixslot = nullptr;
+  int64_t nfix = 0;
+  double *fval = nullptr, *bp = nullptr, *own = nullptr;
 };
 
 struct Levels {
@@ -86,7 +101,6 @@ struct Levels {
   int32_t *dptr = nullptr;      // device copy of the level offsets
   int32_t *rows = nullptr;      // device: rows sorted by level
   std::vector<int32_t> h_rows;  // host copy of rows
-  std::vector<LevelRun> runs;
   Packed pk;
 };
 
@@ -222,7 +236,8 @@ enum TrsvForm : int {
   TRSV_COL_UNIT_REV = 3  // s = b_i; s -= a x descending; x_i = s   (L'.solve)
 };
 
-constexpr int PACK_SSOR = 4;
+constexpr int PACK_SSOR = 4;      // forward sweeps (lower level set)
+constexpr int PACK_SSOR_BWD = 5;  // reverse sweeps (upper level set)
 
 // Sums over the entry slots of one packed row (slot k at base + k * stride,
 // w slots, column -1 after the row's last entry) in slot order, loads issued
@@ -318,17 +333,15 @@ __global__ __launch_bounds__(BS) void k_trsv_level(int32_t cnt, int32_t t0, int6
 
 constexpr int FUSE_BS = 512;
 constexpr int FUSE_MAX = FUSE_BS;  // widest level a fused run takes: one pass
-template <int FORM, bool SLAB>
+// A run of narrow row-major levels on one workgroup, a barrier between
+// levels (slab levels run in LDS chunks instead, see k_sweep_lds).
+template <int FORM>
 __global__ __launch_bounds__(FUSE_BS) void k_trsv_fused(int32_t l0, int32_t l1, const int32_t *lptr,
-                                                        const int64_t *eoff, const int32_t *width, const int4 *desc,
-                                                        const int32_t *prow, const double *pdv, const int32_t *eci,
+                                                        const int4 *desc, const double *pdv, const int32_t *eci,
                                                         const double *eval, const double *b, double *x) {
   for (int32_t l = l0; l < l1; ++l) {
     const int32_t t0 = lptr[l], cnt = lptr[l + 1] - t0;
-    if ((int32_t)threadIdx.x < cnt) {
-      if (SLAB) trsv_row_slab<FORM>(t0, threadIdx.x, cnt, eoff[l], width[l], prow, pdv, eci, eval, b, x);
-      else trsv_row_desc<FORM>(t0 + (int32_t)threadIdx.x, desc, pdv, eci, eval, b, x);
-    }
+    if ((int32_t)threadIdx.x < cnt) trsv_row_desc<FORM>(t0 + (int32_t)threadIdx.x, desc, pdv, eci, eval, b, x);
     __syncthreads();
   }
 }
@@ -399,22 +412,175 @@ __global__ __launch_bounds__(BS) void k_ssor_level(int32_t cnt, int32_t t0, int6
   else ssor_row_desc<FORWARD>(t0 + tl, desc, pdv, eci, eval, b, xd, xx, omega);
 }
 
-template <bool FORWARD, bool SLAB>
-__global__ __launch_bounds__(FUSE_BS) void k_ssor_fused(int32_t l0, int32_t l1, const int32_t *lptr,
-                                                        const int64_t *eoff, const int32_t *width, const int4 *desc,
-                                                        const int32_t *prow, const int32_t *pnlo, const double *pdv,
-                                                        const int32_t *eci, const double *eval, const double *b,
-                                                        double *xd, double *xx, double omega) {
+template <bool FORWARD>
+__global__ __launch_bounds__(FUSE_BS) void k_ssor_fused(int32_t l0, int32_t l1, const int32_t *lptr, const int4 *desc,
+                                                        const double *pdv, const int32_t *eci, const double *eval,
+                                                        const double *b, double *xd, double *xx, double omega) {
   for (int32_t l = l0; l < l1; ++l) {
     const int32_t t0 = lptr[l], cnt = lptr[l + 1] - t0;
-    if ((int32_t)threadIdx.x < cnt) {
-      if (SLAB)
-        ssor_row_slab<FORWARD>(t0, threadIdx.x, cnt, eoff[l], width[l], prow, pnlo, pdv, eci, eval, b, xd, xx,
-                               omega);
-      else
-        ssor_row_desc<FORWARD>(t0 + (int32_t)threadIdx.x, desc, pdv, eci, eval, b, xd, xx, omega);
-    }
+    if ((int32_t)threadIdx.x < cnt)
+      ssor_row_desc<FORWARD>(t0 + (int32_t)threadIdx.x, desc, pdv, eci, eval, b, xd, xx, omega);
     __syncthreads();
+  }
+}
+
+// ---- fused sweeps through LDS -----------------------------------------------
+//
+// A chunk of consecutive slab levels runs on one workgroup whose LDS holds the chunk's results (LDS index t - t0 for
+// position t) followed by the values it reads from outside the chunk (its
+// imports, loaded once at the start).  Every operand a row waits for is then
+// an LDS read, and each level's own loads (the row's slots, its diagonal, b
+// and, for SSOR, its own incoming value and the precomputed products of the
+// fixed operands) are issued one level ahead, before the barrier that ends
+// the previous level, so a level costs an LDS round trip and a barrier rather
+// than the L2 store-to-load round trip.  The results leave LDS at the end of
+// the chunk.  Every row still sums its slots in MTJ's order with the same
+// operations, so the output is bitwise the launched kernels'.
+// Widest level of at most four slots an LDS chunk takes, in steps of FUSE_BS
+// rows (wider levels of up to eight slots: FUSE_MAX).  One compute unit walks
+// a chunk, so a wide level is better launched over the device: measured on
+// 3-D Poisson 64^3, ILU's apply took 1.27 / 1.02 / 1.39 ms with 512 / 2048 /
+// 8192-row levels in chunks, SSOR's (six slots) 1.33 / 1.40 / 2.12 ms.
+constexpr int LDS_LEVEL_MAX = 2048;
+constexpr int LDS_PAD = LDS_LEVEL_MAX;  // slots a stage may read past the last level
+constexpr int LDS_DEPTH = 4;           // steps whose loads are in flight
+
+// One step's operands for the calling thread's row, W slot registers.
+template <int W>
+struct SweepStage {
+  int32_t tb, cnt, act, w, bar;  // first position (relative to the chunk), rows, rows computed, slots, barrier
+  double bp, d, own;
+  int32_t c[W];
+  double v[W];
+};
+
+// Step li of the chunk: its meta from LDS, then the row's operands.  Lanes
+// past the step's rows load the last row's, and slots past the level's width
+// load the last slot (or, for a level without slots, a slot no further than
+// LDS_PAD past the end), so every load is in bounds and the loads of a stage
+// are the same on every path.
+template <int KIND, int W>
+__device__ __forceinline__ void sweep_stage_load(SweepStage<W> &S, const int4 *lm, int32_t li, int32_t t0,
+                                                 const int32_t *lci, const double *vals, const double *pdv,
+                                                 const double *bp, const double *own) {
+  const int4 m = lm[li];
+  S.tb = __builtin_amdgcn_readfirstlane(m.x);
+  const int32_t y = __builtin_amdgcn_readfirstlane(m.y);
+  S.cnt = y & 0xFFFF;
+  S.w = (y >> 16) & 0xFF;
+  S.bar = (y >> 24) & 1;
+  S.act = (y >> 25) ? 0 : S.cnt;  // padding steps repeat the last step's loads and compute nothing
+  const int32_t base = __builtin_amdgcn_readfirstlane(m.z), stride = __builtin_amdgcn_readfirstlane(m.w);
+  const int32_t tl = min((int32_t)threadIdx.x, S.cnt - 1);
+  const int32_t t = t0 + S.tb + tl;
+  S.bp = bp[t];
+  S.d = (KIND == TRSV_ROW_UNIT || KIND == TRSV_COL_UNIT_REV) ? 1.0 : pdv[t];
+  S.own = (KIND >= PACK_SSOR) ? own[t] : 0.0;
+  const int32_t klast = max(S.w - 1, 0);
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    const int64_t e = (int64_t)base + (int64_t)min(k, klast) * stride + tl;
+    S.c[k] = lci[e];
+    S.v[k] = vals[e];
+  }
+}
+
+template <int KIND, int W>
+__device__ __forceinline__ void sweep_stage_row(const SweepStage<W> &S, double *lx, double omega) {
+  if ((int32_t)threadIdx.x >= S.act) return;
+  double xv[W];
+#pragma unroll
+  for (int k = 0; k < W; ++k)
+    if (k < S.w && S.c[k] >= 0) xv[k] = lx[S.c[k]];
+  double r;
+  if (KIND >= PACK_SSOR) {
+    double sigma = 0;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      if (k >= S.w) break;
+      if (S.c[k] >= 0) sigma += S.v[k] * xv[k];
+      else if (S.c[k] == -2) sigma += S.v[k];
+    }
+    sigma = (S.bp - sigma) / S.d;
+    r = S.own + omega * (sigma - S.own);
+  } else if (KIND == TRSV_ROW_UNIT || KIND == TRSV_ROW_DIV) {
+    double sum = 0;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      if (k >= S.w) break;
+      if (S.c[k] >= 0) sum += S.v[k] * xv[k];
+    }
+    r = (KIND == TRSV_ROW_UNIT) ? S.bp - sum : (S.bp - sum) / S.d;
+  } else {
+    double sum = S.bp;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      if (k >= S.w) break;
+      if (S.c[k] >= 0) sum -= S.v[k] * xv[k];
+    }
+    r = (KIND == TRSV_COL_DIV) ? sum / S.d : sum;
+  }
+  lx[S.tb + (int32_t)threadIdx.x] = r;
+}
+
+// A chunk is a list of steps of at most FUSE_BS rows: a level of up to
+// LDS_LEVEL_MAX rows is cut into consecutive steps, the last of which ends
+// with the barrier (rows of one level never read each other).  LDS: the
+// chunk's results, its imports, then the steps' meta (at an even double
+// offset).  LDS_DEPTH stage buffers are used in rotation: stage j always
+// holds a step congruent to j, so no register copies, which would wait for
+// the loads in flight; a step's buffer is refilled with the step LDS_DEPTH
+// ahead right after the step is computed.  The step list is padded to a
+// multiple of LDS_DEPTH, so the unrolled body has no exit: an exit inside it
+// would give the loop head a path on which some stages were not refilled,
+// and the compiler would then wait for every load at the head.
+template <int KIND, int W>
+__global__ __launch_bounds__(FUSE_BS) void k_sweep_lds(int32_t nsteps, int32_t t0, int32_t t1, const int4 *steps,
+                                                       const int32_t *lci, const double *vals, const double *pdv,
+                                                       const double *bp, const double *own, const int32_t *imp,
+                                                       int32_t nimp, const int32_t *prow, double *xdep,
+                                                       double omega) {
+  extern __shared__ double lx[];
+  const int32_t nrows = t1 - t0;
+  int4 *lm = (int4 *)(lx + ((nrows + nimp + 1) & ~1));
+  for (int32_t k = threadIdx.x; k < nimp; k += FUSE_BS) lx[nrows + k] = xdep[imp[k]];
+  for (int32_t k = threadIdx.x; k < nsteps; k += FUSE_BS) lm[k] = steps[k];
+  __syncthreads();
+  SweepStage<W> st[LDS_DEPTH];
+#pragma unroll
+  for (int j = 0; j < LDS_DEPTH; ++j)
+    sweep_stage_load<KIND, W>(st[j], lm, min(j, nsteps - 1), t0, lci, vals, pdv, bp, own);
+  // The prologue's loads are issued in whatever order the scheduler picks;
+  // waiting for them once here keeps that order from leaking into the waits
+  // the compiler places inside the loop.
+  __builtin_amdgcn_s_waitcnt(0);
+  for (int32_t li = 0; li < nsteps; li += LDS_DEPTH) {
+#pragma unroll
+    for (int j = 0; j < LDS_DEPTH; ++j) {
+      const int32_t bar = st[j].bar;
+      sweep_stage_row<KIND, W>(st[j], lx, omega);
+      sweep_stage_load<KIND, W>(st[j], lm, min(li + j + LDS_DEPTH, nsteps - 1), t0, lci, vals, pdv, bp, own);
+      if (bar) __syncthreads();
+    }
+  }
+  __syncthreads();
+  for (int32_t t = t0 + threadIdx.x; t < t1; t += FUSE_BS) xdep[prow[t]] = lx[t - t0];
+}
+
+// Per-sweep preparation of the LDS chunks: b and (SSOR) the row's own fixed
+// operand in level order, and the products of the fixed SSOR slots.
+__global__ void k_sweep_prep(int32_t n, const int32_t *prow, const double *b, double *bp, const double *fixed,
+                             double *own, int64_t nfix, const int64_t *fixslot, const int32_t *eci,
+                             const double *eval, double *fval) {
+  const int64_t stride = (int64_t)gridDim.x * BS;
+  for (int64_t t = blockIdx.x * (int64_t)BS + threadIdx.x; t < n; t += stride) {
+    const int32_t i = prow[t];
+    bp[t] = b[i];
+    if (own) own[t] = fixed[i];
+  }
+  for (int64_t j = blockIdx.x * (int64_t)BS + threadIdx.x; j < nfix; j += stride) {
+    const int64_t e = fixslot[j];
+    fval[e] = eval[e] * fixed[eci[e]];
   }
 }
 
@@ -546,20 +712,6 @@ static void build_levels(mtjg_ctx ctx, int32_t n, const std::vector<int32_t> &rp
   HIPCHK(hipMalloc(&L.dptr, sizeof(int32_t) * L.ptr.size()));
   HIPCHK(hipMemcpyAsync(L.dptr, L.ptr.data(), sizeof(int32_t) * L.ptr.size(), hipMemcpyHostToDevice,
                         ctx->stream));
-  // runs: maximal stretches of at least two levels no wider than FUSE_MAX
-  // become one fused launch; every other level is a launch of its own
-  L.runs.clear();
-  for (int32_t l = 0; l < L.nlev;) {
-    int32_t e = l;
-    while (e < L.nlev && L.ptr[e + 1] - L.ptr[e] <= FUSE_MAX) ++e;
-    if (e - l >= 2) {
-      L.runs.push_back({l, e, true});
-      l = e;
-    } else {
-      L.runs.push_back({l, l + 1, false});
-      ++l;
-    }
-  }
   HIPCHK(hipStreamSynchronize(ctx->stream));
 }
 
@@ -572,7 +724,8 @@ __global__ void k_gather_pad(int64_t n, const int32_t *map, const double *src, d
 
 static void free_pack(Packed &P) {
   for (void *q : {(void *)P.deoff, (void *)P.dwidth, (void *)P.prow, (void *)P.pnlo, (void *)P.pdmap, (void *)P.pdv,
-                  (void *)P.desc, (void *)P.eci, (void *)P.emap, (void *)P.eval})
+                  (void *)P.desc, (void *)P.eci, (void *)P.emap, (void *)P.eval, (void *)P.lci, (void *)P.imp,
+                  (void *)P.fixslot, (void *)P.fval, (void *)P.bp, (void *)P.own, (void *)P.steps})
     (void)hipFree(q);
   P = Packed();
 }
@@ -583,6 +736,25 @@ static T *to_device(const std::vector<T> &v, hipStream_t s) {
   HIPCHK(hipMalloc(&d, sizeof(T) * std::max<size_t>(v.size(), 1)));
   if (!v.empty()) HIPCHK(hipMemcpyAsync(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s));
   return d;
+}
+
+// Doubles of LDS one fused chunk may use: the device's per-workgroup limit,
+// at most 160 KiB.
+static int32_t lds_cap_doubles() {
+  static int32_t cap = 0;
+  if (!cap) {
+    int dev = 0, v = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    cap = std::max(ROWDOT_K * FUSE_BS + FUSE_BS, std::min(v, 160 * 1024) / (int)sizeof(double));
+  }
+  return cap;
+}
+
+// LDS bytes of a chunk: results and imports (rounded to an even count of
+// doubles), then 16 bytes of meta per level.
+static size_t chunk_lds_bytes(int64_t rows, int64_t nimp, int64_t nlev) {
+  return sizeof(double) * (size_t)(((rows + nimp + 1) & ~1) + 2 * nlev);
 }
 
 // Builds (once per pattern and form) and refreshes (whenever the values
@@ -656,17 +828,140 @@ static void ensure_pack(mtjg_prec M, Levels &L, int form, const std::vector<int3
     }
     if (P.slots > INT32_MAX) fail(MTJG_E_UNSUPPORTED, "triangular sweep with more than 2^31 packed entries");
     P.slab.assign(slab.begin(), slab.end());
-    P.runs.clear();
-    for (const LevelRun &r : L.runs) {
-      if (!r.fused) {
-        P.runs.push_back({r.l0, r.l1, false, slab[(size_t)r.l0] != 0});
-        continue;
+    // Each level is launched on its own, joins a fused row-major run (at
+    // most FUSE_MAX rows, row-major) or an LDS chunk (a slab of at most
+    // LDS_LEVEL_MAX rows, or FUSE_MAX rows when its rows hold more than four
+    // slots, whose rows, operands and steps fit one chunk by themselves);
+    // stretches of at least two fused or LDS levels run together.
+    std::vector<int32_t> lci((size_t)P.slots + LDS_PAD, -1), imps;
+    std::vector<int64_t> fixslot;
+    std::vector<int4> steps;
+    {
+      const int32_t cap = lds_cap_doubles();
+      const size_t cap_bytes = (size_t)cap * sizeof(double);
+      std::vector<int32_t> pos((size_t)n), mark((size_t)n, -1), impidx((size_t)n, -1);
+      for (int32_t t = 0; t < n; ++t) pos[(size_t)L.h_rows[(size_t)t]] = t;
+      auto is_dep = [&](int32_t k, int32_t nlo) {
+        return form == PACK_SSOR ? k < nlo : form == PACK_SSOR_BWD ? k >= nlo : true;
+      };
+      auto slot = [&](int32_t l, int32_t k, int32_t t) {
+        return P.eoff[(size_t)l] + (int64_t)k * (L.ptr[l + 1] - L.ptr[l]) + (t - L.ptr[l]);
+      };
+      auto nsteps_of = [](int32_t cnt) { return (cnt + FUSE_BS - 1) / FUSE_BS; };
+      int32_t stamp = 0;
+      // distinct operands of level l outside position t0, marked with stamp
+      auto new_operands = [&](int32_t l, int32_t t0, std::vector<int32_t> &out) {
+        for (int32_t t = L.ptr[l]; t < L.ptr[l + 1]; ++t)
+          for (int32_t k = 0; k < desc[(size_t)t].z; ++k) {
+            if (!is_dep(k, pnlo[(size_t)t])) continue;
+            const int32_t col = eci[(size_t)slot(l, k, t)];
+            if (pos[(size_t)col] < t0 && mark[(size_t)col] != stamp) {
+              mark[(size_t)col] = stamp;
+              out.push_back(col);
+            }
+          }
+      };
+      enum { CAT_LAUNCH, CAT_FUSED, CAT_LDS };
+      std::vector<uint8_t> cat((size_t)std::max(L.nlev, 1), CAT_LAUNCH);
+      std::vector<int32_t> tmpimp;
+      for (int32_t l = 0; l < L.nlev; ++l) {
+        const int32_t cnt = L.ptr[l + 1] - L.ptr[l];
+        if (slab[(size_t)l] && cnt <= (P.width[(size_t)l] <= 4 ? LDS_LEVEL_MAX : FUSE_MAX)) {
+          tmpimp.clear();
+          ++stamp;
+          new_operands(l, L.ptr[l], tmpimp);
+          if (chunk_lds_bytes(cnt, (int64_t)tmpimp.size(), nsteps_of(cnt) + LDS_DEPTH - 1) <= cap_bytes)
+            cat[(size_t)l] = CAT_LDS;
+        } else if (!slab[(size_t)l] && cnt <= FUSE_MAX) {
+          cat[(size_t)l] = CAT_FUSED;
+        }
       }
-      for (int32_t l = r.l0; l < r.l1;) {
+      P.runs.clear();
+      for (int32_t l = 0; l < L.nlev;) {
         int32_t e = l + 1;
-        while (e < r.l1 && slab[(size_t)e] == slab[(size_t)l]) ++e;
-        P.runs.push_back({l, e, true, slab[(size_t)l] != 0});
-        l = e;
+        while (e < L.nlev && cat[(size_t)e] == cat[(size_t)l]) ++e;
+        if (cat[(size_t)l] == CAT_FUSED && e - l >= 2) {
+          P.runs.push_back({l, e, true, false});
+          l = e;
+          continue;
+        }
+        if (cat[(size_t)l] != CAT_LDS || e - l < 2) {
+          for (int32_t lv = l; lv < e; ++lv) P.runs.push_back({lv, lv + 1, false, slab[(size_t)lv] != 0});
+          l = e;
+          continue;
+        }
+        // LDS stretch [l, e): chunks of whole levels within the LDS budget
+        while (l < e) {
+          const int32_t c0 = l, t0 = L.ptr[l];
+          const size_t i0 = imps.size();
+          int32_t rows = 0, nst = 0;
+          ++stamp;
+          while (l < e) {
+            const int32_t cnt = L.ptr[l + 1] - L.ptr[l];
+            const size_t before = imps.size();
+            new_operands(l, t0, imps);
+            if (l > c0 && chunk_lds_bytes((int64_t)rows + cnt, (int64_t)(imps.size() - i0),
+                                          nst + nsteps_of(cnt) + LDS_DEPTH - 1) > cap_bytes) {
+              imps.resize(before);  // the level starts the next chunk
+              break;
+            }
+            rows += cnt;
+            nst += nsteps_of(cnt);
+            ++l;
+          }
+          LevelRun c{c0, l, true, true};
+          c.lds = true;
+          c.t0 = t0;
+          c.t1 = L.ptr[l];
+          c.imp0 = (int64_t)i0;
+          c.nimp = (int32_t)(imps.size() - i0);
+          c.s0 = (int64_t)steps.size();
+          for (int32_t lv = c0; lv < l; ++lv) {
+            const int32_t cnt = L.ptr[lv + 1] - L.ptr[lv], w = P.width[(size_t)lv];
+            c.wmax = std::max(c.wmax, w);
+            for (int32_t q = 0; q < nsteps_of(cnt); ++q) {
+              const int32_t rq = std::min(FUSE_BS, cnt - q * FUSE_BS), bar = (q == nsteps_of(cnt) - 1) ? 1 : 0;
+              steps.push_back(make_int4(L.ptr[lv] + q * FUSE_BS - t0, rq | (w << 16) | (bar << 24),
+                                        (int32_t)(P.eoff[(size_t)lv] + (int64_t)q * FUSE_BS), cnt));
+            }
+          }
+          // padding steps: the last step's loads again, flagged to compute nothing
+          while (((int64_t)steps.size() - c.s0) % LDS_DEPTH) {
+            int4 pad = steps.back();
+            pad.y = (pad.y & 0xFFFFFF) | (1 << 25);
+            steps.push_back(pad);
+          }
+          c.nsteps = (int32_t)((int64_t)steps.size() - c.s0);
+          for (size_t q = i0; q < imps.size(); ++q) impidx[(size_t)imps[q]] = rows + (int32_t)(q - i0);
+          for (int32_t lv = c0; lv < l; ++lv)
+            for (int32_t t = L.ptr[lv]; t < L.ptr[lv + 1]; ++t)
+              for (int32_t k = 0; k < desc[(size_t)t].z; ++k) {
+                const int64_t se = slot(lv, k, t);
+                if (!is_dep(k, pnlo[(size_t)t])) {
+                  lci[(size_t)se] = -2;
+                  fixslot.push_back(se);
+                  continue;
+                }
+                const int32_t col = eci[(size_t)se], pc = pos[(size_t)col];
+                lci[(size_t)se] = pc >= t0 ? pc - t0 : impidx[(size_t)col];
+              }
+          for (size_t q = i0; q < imps.size(); ++q) impidx[(size_t)imps[q]] = -1;
+          P.runs.push_back(c);
+        }
+      }
+    }
+    P.any_lds = false;
+    for (const LevelRun &r : P.runs) P.any_lds |= r.lds;
+    P.nfix = (int64_t)fixslot.size();
+    P.lci = to_device(lci, s);
+    P.imp = to_device(imps, s);
+    P.fixslot = to_device(fixslot, s);
+    P.steps = to_device(steps, s);
+    if (P.any_lds) {
+      HIPCHK(hipMalloc(&P.bp, sizeof(double) * (size_t)std::max(n, 1)));
+      if (form >= PACK_SSOR) {
+        HIPCHK(hipMalloc(&P.own, sizeof(double) * (size_t)std::max(n, 1)));
+        HIPCHK(hipMalloc(&P.fval, sizeof(double) * (size_t)(P.slots + LDS_PAD)));
       }
     }
     P.desc = to_device(desc, s);
@@ -678,17 +973,41 @@ static void ensure_pack(mtjg_prec M, Levels &L, int form, const std::vector<int3
     P.eci = to_device(eci, s);
     P.emap = to_device(emap, s);
     HIPCHK(hipMalloc(&P.pdv, sizeof(double) * (size_t)std::max(n, 1)));
-    HIPCHK(hipMalloc(&P.eval, sizeof(double) * (size_t)std::max<int64_t>(P.slots, 1)));
+    HIPCHK(hipMalloc(&P.eval, sizeof(double) * (size_t)(P.slots + LDS_PAD)));
     HIPCHK(hipStreamSynchronize(s));  // host staging vectors go out of scope
     P.form = form;
     P.version = ~0ull;
   }
   if (P.version != M->vals_version) {
     if (P.slots) k_gather_pad<<<grid_for(P.slots), BS, 0, s>>>(P.slots, P.emap, a, P.eval);
+    if (P.slots && P.fval) k_gather_pad<<<grid_for(P.slots), BS, 0, s>>>(P.slots, P.emap, a, P.fval);
     if (n) k_gather<<<grid_for(n), BS, 0, s>>>(n, P.pdmap, a, P.pdv);
     HIPCHK(hipGetLastError());
     P.version = M->vals_version;
   }
+}
+
+template <int KIND, int W>
+static void launch_sweep_lds_w(const LevelRun &r, const Levels &L, const Packed &P, const double *vals,
+                               double *xdep, double omega, hipStream_t s) {
+  static bool attr_set = false;  // per instantiation: allow LDS beyond the 64 KiB default
+  if (!attr_set) {
+    HIPCHK(hipFuncSetAttribute((const void *)k_sweep_lds<KIND, W>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               lds_cap_doubles() * (int)sizeof(double)));
+    attr_set = true;
+  }
+  (void)L;
+  k_sweep_lds<KIND, W><<<1, FUSE_BS, chunk_lds_bytes(r.t1 - r.t0, r.nimp, r.nsteps), s>>>(
+      r.nsteps, r.t0, r.t1, P.steps + r.s0, P.lci, vals, P.pdv, P.bp, P.own, P.imp + r.imp0, r.nimp, P.prow, xdep,
+      omega);
+}
+
+template <int KIND>
+static void launch_sweep_lds(const LevelRun &r, const Levels &L, const Packed &P, const double *vals, double *xdep,
+                             double omega, hipStream_t s) {
+  if (r.wmax <= 2) launch_sweep_lds_w<KIND, 2>(r, L, P, vals, xdep, omega, s);
+  else if (r.wmax <= 4) launch_sweep_lds_w<KIND, 4>(r, L, P, vals, xdep, omega, s);
+  else launch_sweep_lds_w<KIND, ROWDOT_K>(r, L, P, vals, xdep, omega, s);
 }
 
 template <int FORM>
@@ -697,14 +1016,14 @@ static void run_levels(mtjg_prec M, Levels &L, hipStream_t s, const std::vector<
                        const double *b, double *x) {
   ensure_pack(M, L, FORM, rp, ci, diag, a, s);
   const Packed &P = L.pk;
+  if (P.any_lds)
+    k_sweep_prep<<<grid_for(M->n), BS, 0, s>>>(M->n, P.prow, b, P.bp, nullptr, nullptr, 0, nullptr, nullptr, nullptr,
+                                               nullptr);
   for (const LevelRun &r : P.runs) {
-    if (r.fused) {
-      if (r.slab)
-        k_trsv_fused<FORM, true><<<1, FUSE_BS, 0, s>>>(r.l0, r.l1, L.dptr, P.deoff, P.dwidth, P.desc, P.prow, P.pdv,
-                                                       P.eci, P.eval, b, x);
-      else
-        k_trsv_fused<FORM, false><<<1, FUSE_BS, 0, s>>>(r.l0, r.l1, L.dptr, P.deoff, P.dwidth, P.desc, P.prow, P.pdv,
-                                                        P.eci, P.eval, b, x);
+    if (r.lds) {
+      launch_sweep_lds<FORM>(r, L, P, P.eval, x, 1.0, s);
+    } else if (r.fused) {
+      k_trsv_fused<FORM><<<1, FUSE_BS, 0, s>>>(r.l0, r.l1, L.dptr, P.desc, P.pdv, P.eci, P.eval, b, x);
     } else {
       const int32_t l = r.l0, cnt = L.ptr[l + 1] - L.ptr[l];
       if (r.slab)
@@ -742,16 +1061,20 @@ static void run_syncfree(mtjg_prec M, const Levels &L, int slot, hipStream_t s, 
 // reads it.
 template <bool FORWARD>
 static void ssor_sweep(mtjg_prec M, Levels &L, hipStream_t s, const double *b, double *x, double omega) {
-  ensure_pack(M, L, PACK_SSOR, M->h_rp, M->h_ci, M->h_diag, M->lu, s);
+  constexpr int KIND = FORWARD ? PACK_SSOR : PACK_SSOR_BWD;
+  ensure_pack(M, L, KIND, M->h_rp, M->h_ci, M->h_diag, M->lu, s);
   const Packed &P = L.pk;
+  // the forward sweep reads x (as xd) and writes xx; the reverse one reads xx
+  double *fixed = FORWARD ? x : M->xx, *xdep = FORWARD ? M->xx : x;
+  if (P.any_lds)
+    k_sweep_prep<<<grid_for(std::max<int64_t>(M->n, P.nfix)), BS, 0, s>>>(M->n, P.prow, b, P.bp, fixed, P.own,
+                                                                         P.nfix, P.fixslot, P.eci, P.eval, P.fval);
   for (const LevelRun &r : P.runs) {
-    if (r.fused) {
-      if (r.slab)
-        k_ssor_fused<FORWARD, true><<<1, FUSE_BS, 0, s>>>(r.l0, r.l1, L.dptr, P.deoff, P.dwidth, P.desc, P.prow,
-                                                          P.pnlo, P.pdv, P.eci, P.eval, b, x, M->xx, omega);
-      else
-        k_ssor_fused<FORWARD, false><<<1, FUSE_BS, 0, s>>>(r.l0, r.l1, L.dptr, P.deoff, P.dwidth, P.desc, P.prow,
-                                                           P.pnlo, P.pdv, P.eci, P.eval, b, x, M->xx, omega);
+    if (r.lds) {
+      launch_sweep_lds<KIND>(r, L, P, P.fval, xdep, omega, s);
+    } else if (r.fused) {
+      k_ssor_fused<FORWARD><<<1, FUSE_BS, 0, s>>>(r.l0, r.l1, L.dptr, P.desc, P.pdv, P.eci, P.eval, b, x, M->xx,
+                                                  omega);
     } else {
       const int32_t l = r.l0, cnt = L.ptr[l + 1] - L.ptr[l];
       if (r.slab)
@@ -783,7 +1106,6 @@ static void free_levels(Levels &L) {
   if (L.dptr) (void)hipFree(L.dptr);
   L.rows = nullptr;
   L.dptr = nullptr;
-  L.runs.clear();
   L.h_rows.clear();
   free_pack(L.pk);
   L.nlev = 0;

@@ -148,12 +148,18 @@ def _wide_level_systems():
     d = sp.csr_matrix(d)
     d.sort_indices()
     yield "random_spd_20k", (n, n, d.indptr.astype(np.int32), d.indices.astype(np.int32), d.data)
+    # 2-D Poisson 200^2: every level at most 200 rows, one fused run of 40,000
+    # rows, cut into LDS chunks that import the previous chunk's last levels
+    yield "poisson2d_200", P.poisson2d(200)
+    # 2-D variable-coefficient Poisson 190 x 170: the same with unequal sides
+    yield "varcoef2d_190x170", P.varcoef_poisson(2, 190, 170, seed=6)
 
 
 @pytest.mark.parametrize("name,sysm", list(_wide_level_systems()))
 def test_wide_levels_bitwise(ctx, name, sysm):
     """Levels wider than one fused workgroup run as launches of their own, in
-    both packed layouts; ILU apply/transApply, ICC and SSOR stay bitwise."""
+    both packed layouts, and long fused runs are cut into several LDS chunks;
+    ILU apply/transApply, ICC apply/transApply and SSOR stay bitwise."""
     n = sysm[0]
     A = M.CompRowMatrix(*sysm, ctx=ctx)
     O = oracle.Csr(*sysm)
@@ -169,8 +175,13 @@ def test_wide_levels_bitwise(ctx, name, sysm):
     ssor.setMatrix(A)
     Os = oracle.Precond(oracle.PREC_SSOR, O, omega_f=1.2, omega_r=0.8)
     np.testing.assert_array_equal(ssor.apply(b, M.DenseVector(x0, ctx)).getData(), Os.apply(bh, x0))
-    if name.startswith("random_spd"):
+    ssor1 = M.SSOR(A, reverse=False, omegaF=1.3, omegaR=1.3)
+    ssor1.setMatrix(A)
+    Os1 = oracle.Precond(oracle.PREC_SSOR, O, reverse=False, omega_f=1.3, omega_r=1.3)
+    np.testing.assert_array_equal(ssor1.apply(b, M.DenseVector(x0, ctx)).getData(), Os1.apply(bh, x0))
+    if not name.startswith("convdiff"):
         icc = M.ICC(A)
         icc.setMatrix(A)
         Oc = oracle.Precond(oracle.PREC_ICC, O)
         np.testing.assert_array_equal(icc.apply(b, M.DenseVector(n, ctx)).getData(), Oc.apply(bh))
+        np.testing.assert_array_equal(icc.transApply(b, M.DenseVector(n, ctx)).getData(), Oc.trans_apply(bh))
