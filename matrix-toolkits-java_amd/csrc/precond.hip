@@ -53,6 +53,7 @@ struct LevelRun {
   // row wmax, steps[s0, s0 + nsteps)
   bool lds = false;
   int32_t t0 = 0, t1 = 0, nimp = 0, wmax = 0, nsteps = 0;
+  bool grouped = false;  // some level's rows spread over several lanes
   int64_t imp0 = 0, s0 = 0;
 };
 
@@ -444,21 +445,26 @@ __global__ __launch_bounds__(FUSE_BS) void k_ssor_fused(int32_t l0, int32_t l1, 
 constexpr int LDS_LEVEL_MAX = 2048;
 constexpr int LDS_PAD = LDS_LEVEL_MAX;  // slots a stage may read past the last level
 constexpr int LDS_DEPTH = 4;           // steps whose loads are in flight
+constexpr int SWEEP_GROUP_MAX = 64;    // lanes per row at most (one wavefront)
+constexpr int LDS_GROUP_LANES = 1024;  // lanes of a level of grouped rows an LDS chunk takes
 
-// One step's operands for the calling thread's row, W slot registers.
+// One step's operands for the calling lane, W slot registers.  A row of a
+// grouped level (more than ROWDOT_K entries) spreads over G = 2^lg lanes:
+// lane g of the row holds its slots g, g + G, g + 2G, ...
 template <int W>
 struct SweepStage {
-  int32_t tb, cnt, act, w, bar;  // first position (relative to the chunk), rows, rows computed, slots, barrier
+  int32_t tb, cnt, act, w, bar, lg;  // first position (relative to the chunk), lanes, lanes computed,
+                                     // slots per lane, barrier after, log2 lanes per row
   double bp, d, own;
   int32_t c[W];
   double v[W];
 };
 
-// Step li of the chunk: its meta from LDS, then the row's operands.  Lanes
-// past the step's rows load the last row's, and slots past the level's width
-// load the last slot (or, for a level without slots, a slot no further than
-// LDS_PAD past the end), so every load is in bounds and the loads of a stage
-// are the same on every path.
+// Step li of the chunk: its meta from LDS, then the lane's operands.  Lanes
+// past the step's lanes load the last lane's, and slots past the level's
+// width load the last slot (or, for a level without slots, a slot no further
+// than LDS_PAD past the end), so every load is in bounds and the loads of a
+// stage are the same on every path.
 template <int KIND, int W>
 __device__ __forceinline__ void sweep_stage_load(SweepStage<W> &S, const int4 *lm, int32_t li, int32_t t0,
                                                  const int32_t *lci, const double *vals, const double *pdv,
@@ -469,10 +475,11 @@ __device__ __forceinline__ void sweep_stage_load(SweepStage<W> &S, const int4 *l
   S.cnt = y & 0xFFFF;
   S.w = (y >> 16) & 0xFF;
   S.bar = (y >> 24) & 1;
-  S.act = (y >> 25) ? 0 : S.cnt;  // padding steps repeat the last step's loads and compute nothing
+  S.act = ((y >> 25) & 1) ? 0 : S.cnt;  // padding steps repeat the last step's loads and compute nothing
+  S.lg = (y >> 26) & 7;
   const int32_t base = __builtin_amdgcn_readfirstlane(m.z), stride = __builtin_amdgcn_readfirstlane(m.w);
   const int32_t tl = min((int32_t)threadIdx.x, S.cnt - 1);
-  const int32_t t = t0 + S.tb + tl;
+  const int32_t t = t0 + S.tb + (tl >> S.lg);
   S.bp = bp[t];
   S.d = (KIND == TRSV_ROW_UNIT || KIND == TRSV_COL_UNIT_REV) ? 1.0 : pdv[t];
   S.own = (KIND >= PACK_SSOR) ? own[t] : 0.0;
@@ -485,64 +492,94 @@ __device__ __forceinline__ void sweep_stage_load(SweepStage<W> &S, const int4 *l
   }
 }
 
-template <int KIND, int W>
-__device__ __forceinline__ void sweep_stage_row(const SweepStage<W> &S, double *lx, double omega) {
-  if ((int32_t)threadIdx.x >= S.act) return;
-  double xv[W];
-#pragma unroll
-  for (int k = 0; k < W; ++k)
-    if (k < S.w && S.c[k] >= 0) xv[k] = lx[S.c[k]];
-  double r;
+// Marks an empty slot among a grouped row's products: a signalling NaN,
+// which no arithmetic result is.
+constexpr uint64_t SWEEP_NO_SLOT = 0x7FF4DEAD5EA1ED01ULL;
+
+// The row's result from its sum s, in the form's operation order.
+template <int KIND>
+__device__ __forceinline__ double sweep_finish(double s, double bp, double d, double own, double omega) {
   if (KIND >= PACK_SSOR) {
-    double sigma = 0;
-#pragma unroll
-    for (int k = 0; k < W; ++k) {
-      if (k >= S.w) break;
-      if (S.c[k] >= 0) sigma += S.v[k] * xv[k];
-      else if (S.c[k] == -2) sigma += S.v[k];
-    }
-    sigma = (S.bp - sigma) / S.d;
-    r = S.own + omega * (sigma - S.own);
-  } else if (KIND == TRSV_ROW_UNIT || KIND == TRSV_ROW_DIV) {
-    double sum = 0;
-#pragma unroll
-    for (int k = 0; k < W; ++k) {
-      if (k >= S.w) break;
-      if (S.c[k] >= 0) sum += S.v[k] * xv[k];
-    }
-    r = (KIND == TRSV_ROW_UNIT) ? S.bp - sum : (S.bp - sum) / S.d;
-  } else {
-    double sum = S.bp;
-#pragma unroll
-    for (int k = 0; k < W; ++k) {
-      if (k >= S.w) break;
-      if (S.c[k] >= 0) sum -= S.v[k] * xv[k];
-    }
-    r = (KIND == TRSV_COL_DIV) ? sum / S.d : sum;
+    const double sigma = (bp - s) / d;
+    return own + omega * (sigma - own);
   }
-  lx[S.tb + (int32_t)threadIdx.x] = r;
+  if (KIND == TRSV_ROW_UNIT) return bp - s;
+  if (KIND == TRSV_ROW_DIV) return (bp - s) / d;
+  if (KIND == TRSV_COL_DIV) return s / d;
+  return s;
 }
 
-// A chunk is a list of steps of at most FUSE_BS rows: a level of up to
-// LDS_LEVEL_MAX rows is cut into consecutive steps, the last of which ends
-// with the barrier (rows of one level never read each other).  LDS: the
-// chunk's results, its imports, then the steps' meta (at an even double
-// offset).  LDS_DEPTH stage buffers are used in rotation: stage j always
-// holds a step congruent to j, so no register copies, which would wait for
-// the loads in flight; a step's buffer is refilled with the step LDS_DEPTH
-// ahead right after the step is computed.  The step list is padded to a
-// multiple of LDS_DEPTH, so the unrolled body has no exit: an exit inside it
-// would give the loop head a path on which some stages were not refilled,
-// and the compiler would then wait for every load at the head.
+template <int KIND, int W>
+__device__ __forceinline__ void sweep_stage_row(const SweepStage<W> &S, double *lx, double *scr, double omega) {
+  constexpr bool SUB = (KIND == TRSV_COL_DIV || KIND == TRSV_COL_UNIT_REV);  // s = b, s -= a x
+  if ((int32_t)threadIdx.x >= S.act) return;
+  if (S.lg == 0) {
+    double xv[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k)
+      if (k < S.w && S.c[k] >= 0) xv[k] = lx[S.c[k]];
+    double s = SUB ? S.bp : 0.0;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      if (k >= S.w) break;
+      if (S.c[k] >= 0) {
+        if (SUB) s -= S.v[k] * xv[k];
+        else s += S.v[k] * xv[k];
+      } else if (KIND >= PACK_SSOR && S.c[k] == -2) {
+        s += S.v[k];
+      }
+    }
+    lx[S.tb + (int32_t)threadIdx.x] = sweep_finish<KIND>(s, S.bp, S.d, S.own, omega);
+    return;
+  }
+  // grouped row: every lane stores its products (slot g + kk * G at index
+  // g + kk * G of the row's scratch), then the row's first lane adds them in
+  // slot order.  The row's lanes share a wavefront, whose LDS accesses are
+  // performed in order, so no barrier is needed between the two.
+  const int32_t G = 1 << S.lg, g = (int32_t)threadIdx.x & (G - 1);
+  double *pr = scr + ((int32_t)threadIdx.x - g) * W;
+#pragma unroll
+  for (int kk = 0; kk < W; ++kk) {
+    if (kk >= S.w) break;
+    const int32_t c = S.c[kk];
+    double p = __longlong_as_double((long long)SWEEP_NO_SLOT);
+    if (c >= 0) p = S.v[kk] * lx[c];
+    else if (c == -2) p = S.v[kk];
+    pr[g + kk * G] = p;
+  }
+  if (g != 0) return;
+  double s = SUB ? S.bp : 0.0;
+  for (int32_t k = 0; k < S.w * G; ++k) {
+    const double p = pr[k];
+    if ((uint64_t)__double_as_longlong(p) == SWEEP_NO_SLOT) continue;
+    if (SUB) s -= p;
+    else s += p;
+  }
+  lx[S.tb + ((int32_t)threadIdx.x >> S.lg)] = sweep_finish<KIND>(s, S.bp, S.d, S.own, omega);
+}
+
+// A chunk is a list of steps of at most FUSE_BS lanes: a level is cut into
+// consecutive steps of whole rows, the last of which ends with the barrier
+// (rows of one level never read each other).  LDS: the chunk's results, its
+// imports, the steps' meta (at an even double offset), then FUSE_BS * W doubles of
+// product scratch for grouped rows.  LDS_DEPTH stage buffers are used in
+// rotation: stage j always holds a step congruent to j, so no register
+// copies, which would wait for the loads in flight; a step's buffer is
+// refilled with the step LDS_DEPTH ahead right after the step is computed.
+// The step list is padded to a multiple of LDS_DEPTH, so the unrolled body
+// has no exit: an exit inside it would give the loop head a path on which
+// some stages were not refilled, and the compiler would then wait for every
+// load at the head.
 template <int KIND, int W>
 __global__ __launch_bounds__(FUSE_BS) void k_sweep_lds(int32_t nsteps, int32_t t0, int32_t t1, const int4 *steps,
-                                                       const int32_t *lci, const double *vals, const double *pdv,
-                                                       const double *bp, const double *own, const int32_t *imp,
-                                                       int32_t nimp, const int32_t *prow, double *xdep,
-                                                       double omega) {
+                                                   const int32_t *lci, const double *vals, const double *pdv,
+                                                   const double *bp, const double *own, const int32_t *imp,
+                                                   int32_t nimp, const int32_t *prow, double *xdep, double omega) {
   extern __shared__ double lx[];
   const int32_t nrows = t1 - t0;
-  int4 *lm = (int4 *)(lx + ((nrows + nimp + 1) & ~1));
+  const int32_t moff = (nrows + nimp + 1) & ~1;
+  int4 *lm = (int4 *)(lx + moff);
+  double *scr = lx + moff + 2 * nsteps;
   for (int32_t k = threadIdx.x; k < nimp; k += FUSE_BS) lx[nrows + k] = xdep[imp[k]];
   for (int32_t k = threadIdx.x; k < nsteps; k += FUSE_BS) lm[k] = steps[k];
   __syncthreads();
@@ -558,7 +595,7 @@ __global__ __launch_bounds__(FUSE_BS) void k_sweep_lds(int32_t nsteps, int32_t t
 #pragma unroll
     for (int j = 0; j < LDS_DEPTH; ++j) {
       const int32_t bar = st[j].bar;
-      sweep_stage_row<KIND, W>(st[j], lx, omega);
+      sweep_stage_row<KIND, W>(st[j], lx, scr, omega);
       sweep_stage_load<KIND, W>(st[j], lm, min(li + j + LDS_DEPTH, nsteps - 1), t0, lci, vals, pdv, bp, own);
       if (bar) __syncthreads();
     }
@@ -752,9 +789,9 @@ static int32_t lds_cap_doubles() {
 }
 
 // LDS bytes of a chunk: results and imports (rounded to an even count of
-// doubles), then 16 bytes of meta per level.
-static size_t chunk_lds_bytes(int64_t rows, int64_t nimp, int64_t nlev) {
-  return sizeof(double) * (size_t)(((rows + nimp + 1) & ~1) + 2 * nlev);
+// doubles), 16 bytes of meta per step, then the grouped rows' product scratch.
+static size_t chunk_lds_bytes(int64_t rows, int64_t nimp, int64_t nsteps, int64_t scratch) {
+  return sizeof(double) * (size_t)(((rows + nimp + 1) & ~1) + 2 * nsteps + scratch);
 }
 
 // Builds (once per pattern and form) and refreshes (whenever the values
@@ -784,31 +821,123 @@ static void ensure_pack(mtjg_prec M, Levels &L, int form, const std::vector<int3
         for (int32_t j = d + 1; j < r1; ++j) out.push_back(j);
       }
     };
-    std::vector<int32_t> prow((size_t)n), pnlo((size_t)n), pdmap((size_t)n), tmp;
+    std::vector<int32_t> prow((size_t)n), pnlo((size_t)n), pdmap((size_t)n), tmp, pos((size_t)n);
     std::vector<int4> desc((size_t)n);
-    // slab levels: rows of at most one load batch.  Measured on one box: C3's
-    // BiCGstab + ILU(0) 0.521 s with every level a slab against 0.547 s
-    // row-major; on AMG's coarse levels (20-100 entries per row) slabs cost
-    // 27.0 ms per V-cycle on 64^3 against 23.5 ms row-major.
-    std::vector<uint8_t> slab((size_t)std::max(L.nlev, 1), 0);
-    P.eoff.assign((size_t)L.nlev + 1, 0);
-    P.width.assign((size_t)std::max(L.nlev, 1), 0);
+    for (int32_t t = 0; t < n; ++t) pos[(size_t)L.h_rows[(size_t)t]] = t;
+    auto is_dep = [&](int32_t k, int32_t nlo) {
+      return form == PACK_SSOR ? k < nlo : form == PACK_SSOR_BWD ? k >= nlo : true;
+    };
+    const size_t nl = (size_t)std::max(L.nlev, 1);
+    auto cnt_of = [&](int32_t l) { return L.ptr[l + 1] - L.ptr[l]; };
+    // per level: widest row, entries, and lanes per row in an LDS chunk (the
+    // fewest, a power of two, that leave each lane at most ROWDOT_K slots)
+    std::vector<int32_t> wrow(nl, 0), grp(nl, 1);
+    std::vector<int64_t> tot(nl, 0);
     for (int32_t l = 0; l < L.nlev; ++l) {
-      int32_t w = 0, nlo = 0;
-      int64_t tot = 0;
+      int32_t nlo = 0;
       for (int32_t t = L.ptr[l]; t < L.ptr[l + 1]; ++t) {
         entries(L.h_rows[(size_t)t], tmp, nlo);
-        w = std::max(w, (int32_t)tmp.size());
-        tot += (int64_t)tmp.size();
+        wrow[(size_t)l] = std::max(wrow[(size_t)l], (int32_t)tmp.size());
+        tot[(size_t)l] += (int64_t)tmp.size();
       }
-      slab[(size_t)l] = w <= ROWDOT_K;
-      P.width[(size_t)l] = slab[(size_t)l] ? w : -1;
-      P.eoff[(size_t)l + 1] = P.eoff[(size_t)l] + (slab[(size_t)l] ? (int64_t)w * (L.ptr[l + 1] - L.ptr[l]) : tot);
+      while (wrow[(size_t)l] > ROWDOT_K * grp[(size_t)l] && grp[(size_t)l] < SWEEP_GROUP_MAX) grp[(size_t)l] *= 2;
+    }
+    auto lanes_of = [&](int32_t l) { return (int64_t)cnt_of(l) * grp[(size_t)l]; };
+    auto lane_width = [&](int32_t l) { return (wrow[(size_t)l] + grp[(size_t)l] - 1) / grp[(size_t)l]; };
+    auto nsteps_of = [](int64_t lanes) { return (int32_t)((lanes + FUSE_BS - 1) / FUSE_BS); };
+    const int32_t cap = lds_cap_doubles();
+    const size_t cap_bytes = (size_t)cap * sizeof(double);
+    constexpr int64_t SCRATCH_MAX = (int64_t)FUSE_BS * ROWDOT_K;  // product scratch of grouped rows, doubles
+    std::vector<int32_t> mark((size_t)n, -1), impidx((size_t)n, -1);
+    int32_t stamp = 0;
+    // distinct operands of level l at positions before t0 (marked with stamp)
+    auto new_operands = [&](int32_t l, int32_t t0, std::vector<int32_t> &out) {
+      int32_t nlo = 0;
+      for (int32_t t = L.ptr[l]; t < L.ptr[l + 1]; ++t) {
+        entries(L.h_rows[(size_t)t], tmp, nlo);
+        for (int32_t k = 0; k < (int32_t)tmp.size(); ++k) {
+          if (!is_dep(k, nlo)) continue;
+          const int32_t col = ci[(size_t)tmp[(size_t)k]];
+          if (pos[(size_t)col] < t0 && mark[(size_t)col] != stamp) {
+            mark[(size_t)col] = stamp;
+            out.push_back(col);
+          }
+        }
+      }
+    };
+    // Each level is launched on its own, joins a fused row-major run (rows
+    // of more than ROWDOT_K entries, at most FUSE_MAX rows) or an LDS chunk:
+    // rows of at most ROWDOT_K entries (a slab) on up to LDS_LEVEL_MAX lanes,
+    // or FUSE_MAX lanes when its rows hold more than four entries; longer
+    // rows (grouped) on up to LDS_GROUP_LANES lanes; in both cases only if
+    // its rows, operands and steps fit one chunk by themselves.  Stretches of
+    // at least two fused or LDS levels run together.
+    enum { CAT_LAUNCH, CAT_FUSED, CAT_LDS };
+    std::vector<uint8_t> cat(nl, CAT_LAUNCH);
+    std::vector<int32_t> tmpimp;
+    for (int32_t l = 0; l < L.nlev; ++l) {
+      const int32_t w = wrow[(size_t)l], g = grp[(size_t)l];
+      const int64_t lanes = lanes_of(l);
+      const bool fits_lanes = w <= ROWDOT_K * g &&
+                              lanes <= (g > 1 ? LDS_GROUP_LANES : w <= 4 ? LDS_LEVEL_MAX : FUSE_MAX);
+      if (fits_lanes) {
+        tmpimp.clear();
+        ++stamp;
+        new_operands(l, L.ptr[l], tmpimp);
+        if (chunk_lds_bytes(cnt_of(l), (int64_t)tmpimp.size(), nsteps_of(lanes) + LDS_DEPTH - 1,
+                            g > 1 ? SCRATCH_MAX : 0) <= cap_bytes)
+          cat[(size_t)l] = CAT_LDS;
+      }
+      if (cat[(size_t)l] != CAT_LDS && w > ROWDOT_K && cnt_of(l) <= FUSE_MAX) cat[(size_t)l] = CAT_FUSED;
+    }
+    // final layout: a slab (rows of at most ROWDOT_K entries), grouped slab
+    // (longer rows in an LDS chunk: lane (row i, g) of V = cnt * G lanes holds
+    // slot kk * V + i * G + g for the row's entries g + kk * G), or row-major
+    std::vector<uint8_t> in_lds(nl, 0);
+    for (int32_t l = 0; l < L.nlev;) {
+      int32_t e = l + 1;
+      while (e < L.nlev && cat[(size_t)e] == cat[(size_t)l]) ++e;
+      if (cat[(size_t)l] == CAT_LDS && e - l >= 2)
+        for (int32_t lv = l; lv < e; ++lv) in_lds[(size_t)lv] = 1;
+      l = e;
+    }
+    enum { LAY_SLAB, LAY_GROUP, LAY_ROWS };
+    std::vector<uint8_t> lay(nl, LAY_ROWS), slab(nl, 0);
+    P.eoff.assign((size_t)L.nlev + 1, 0);
+    P.width.assign(nl, 0);
+    for (int32_t l = 0; l < L.nlev; ++l) {
+      const int32_t w = wrow[(size_t)l], cnt = cnt_of(l);
+      int64_t region;
+      if (w <= ROWDOT_K) {
+        lay[(size_t)l] = LAY_SLAB;
+        slab[(size_t)l] = 1;
+        P.width[(size_t)l] = w;
+        region = (int64_t)w * cnt;
+      } else if (in_lds[(size_t)l]) {
+        lay[(size_t)l] = LAY_GROUP;
+        P.width[(size_t)l] = lane_width(l);
+        region = (int64_t)lane_width(l) * lanes_of(l);
+      } else {
+        P.width[(size_t)l] = -1;
+        region = tot[(size_t)l];
+      }
+      P.eoff[(size_t)l + 1] = P.eoff[(size_t)l] + region;
     }
     P.slots = P.eoff[(size_t)L.nlev];
+    if (P.slots > INT32_MAX - LDS_PAD) fail(MTJG_E_UNSUPPORTED, "triangular sweep with more than 2^31 packed entries");
+    // slot of entry k of the row at position t (row-major levels: `next` is
+    // the row's first slot)
+    auto slot = [&](int32_t l, int32_t k, int32_t t, int64_t next) -> int64_t {
+      const int32_t i = t - L.ptr[l];
+      if (lay[(size_t)l] == LAY_SLAB) return P.eoff[(size_t)l] + (int64_t)k * cnt_of(l) + i;
+      if (lay[(size_t)l] == LAY_GROUP) {
+        const int32_t g = grp[(size_t)l];
+        return P.eoff[(size_t)l] + (int64_t)(k / g) * lanes_of(l) + (int64_t)i * g + (k % g);
+      }
+      return next + k;
+    };
     std::vector<int32_t> eci((size_t)P.slots, -1), emap((size_t)P.slots, -1);
     for (int32_t l = 0; l < L.nlev; ++l) {
-      const int32_t cnt = L.ptr[l + 1] - L.ptr[l];
       int64_t next = P.eoff[(size_t)l];
       for (int32_t t = L.ptr[l]; t < L.ptr[l + 1]; ++t) {
         const int32_t i = L.h_rows[(size_t)t];
@@ -818,136 +947,103 @@ static void ensure_pack(mtjg_prec M, Levels &L, int form, const std::vector<int3
         pnlo[(size_t)t] = nlo;
         pdmap[(size_t)t] = diag[(size_t)i];
         desc[(size_t)t] = make_int4(i, (int32_t)next, (int32_t)tmp.size(), nlo);
-        for (size_t k = 0; k < tmp.size(); ++k) {
-          const int64_t e = slab[(size_t)l] ? P.eoff[(size_t)l] + (int64_t)k * cnt + (t - L.ptr[l]) : next + (int64_t)k;
-          eci[(size_t)e] = ci[(size_t)tmp[k]];
-          emap[(size_t)e] = tmp[k];
+        for (int32_t k = 0; k < (int32_t)tmp.size(); ++k) {
+          const int64_t e = slot(l, k, t, next);
+          eci[(size_t)e] = ci[(size_t)tmp[(size_t)k]];
+          emap[(size_t)e] = tmp[(size_t)k];
         }
-        if (!slab[(size_t)l]) next += (int64_t)tmp.size();
+        if (lay[(size_t)l] == LAY_ROWS) next += (int64_t)tmp.size();
       }
     }
-    if (P.slots > INT32_MAX) fail(MTJG_E_UNSUPPORTED, "triangular sweep with more than 2^31 packed entries");
     P.slab.assign(slab.begin(), slab.end());
-    // Each level is launched on its own, joins a fused row-major run (at
-    // most FUSE_MAX rows, row-major) or an LDS chunk (a slab of at most
-    // LDS_LEVEL_MAX rows, or FUSE_MAX rows when its rows hold more than four
-    // slots, whose rows, operands and steps fit one chunk by themselves);
-    // stretches of at least two fused or LDS levels run together.
     std::vector<int32_t> lci((size_t)P.slots + LDS_PAD, -1), imps;
     std::vector<int64_t> fixslot;
     std::vector<int4> steps;
-    {
-      const int32_t cap = lds_cap_doubles();
-      const size_t cap_bytes = (size_t)cap * sizeof(double);
-      std::vector<int32_t> pos((size_t)n), mark((size_t)n, -1), impidx((size_t)n, -1);
-      for (int32_t t = 0; t < n; ++t) pos[(size_t)L.h_rows[(size_t)t]] = t;
-      auto is_dep = [&](int32_t k, int32_t nlo) {
-        return form == PACK_SSOR ? k < nlo : form == PACK_SSOR_BWD ? k >= nlo : true;
-      };
-      auto slot = [&](int32_t l, int32_t k, int32_t t) {
-        return P.eoff[(size_t)l] + (int64_t)k * (L.ptr[l + 1] - L.ptr[l]) + (t - L.ptr[l]);
-      };
-      auto nsteps_of = [](int32_t cnt) { return (cnt + FUSE_BS - 1) / FUSE_BS; };
-      int32_t stamp = 0;
-      // distinct operands of level l outside position t0, marked with stamp
-      auto new_operands = [&](int32_t l, int32_t t0, std::vector<int32_t> &out) {
-        for (int32_t t = L.ptr[l]; t < L.ptr[l + 1]; ++t)
-          for (int32_t k = 0; k < desc[(size_t)t].z; ++k) {
-            if (!is_dep(k, pnlo[(size_t)t])) continue;
-            const int32_t col = eci[(size_t)slot(l, k, t)];
-            if (pos[(size_t)col] < t0 && mark[(size_t)col] != stamp) {
-              mark[(size_t)col] = stamp;
-              out.push_back(col);
-            }
-          }
-      };
-      enum { CAT_LAUNCH, CAT_FUSED, CAT_LDS };
-      std::vector<uint8_t> cat((size_t)std::max(L.nlev, 1), CAT_LAUNCH);
-      std::vector<int32_t> tmpimp;
-      for (int32_t l = 0; l < L.nlev; ++l) {
-        const int32_t cnt = L.ptr[l + 1] - L.ptr[l];
-        if (slab[(size_t)l] && cnt <= (P.width[(size_t)l] <= 4 ? LDS_LEVEL_MAX : FUSE_MAX)) {
-          tmpimp.clear();
-          ++stamp;
-          new_operands(l, L.ptr[l], tmpimp);
-          if (chunk_lds_bytes(cnt, (int64_t)tmpimp.size(), nsteps_of(cnt) + LDS_DEPTH - 1) <= cap_bytes)
-            cat[(size_t)l] = CAT_LDS;
-        } else if (!slab[(size_t)l] && cnt <= FUSE_MAX) {
-          cat[(size_t)l] = CAT_FUSED;
-        }
+    P.runs.clear();
+    for (int32_t l = 0; l < L.nlev;) {
+      int32_t e = l + 1;
+      while (e < L.nlev && cat[(size_t)e] == cat[(size_t)l]) ++e;
+      if (cat[(size_t)l] == CAT_FUSED && e - l >= 2) {
+        P.runs.push_back({l, e, true, false});
+        l = e;
+        continue;
       }
-      P.runs.clear();
-      for (int32_t l = 0; l < L.nlev;) {
-        int32_t e = l + 1;
-        while (e < L.nlev && cat[(size_t)e] == cat[(size_t)l]) ++e;
-        if (cat[(size_t)l] == CAT_FUSED && e - l >= 2) {
-          P.runs.push_back({l, e, true, false});
-          l = e;
-          continue;
-        }
-        if (cat[(size_t)l] != CAT_LDS || e - l < 2) {
-          for (int32_t lv = l; lv < e; ++lv) P.runs.push_back({lv, lv + 1, false, slab[(size_t)lv] != 0});
-          l = e;
-          continue;
-        }
-        // LDS stretch [l, e): chunks of whole levels within the LDS budget
+      if (!in_lds[(size_t)l]) {
+        for (int32_t lv = l; lv < e; ++lv) P.runs.push_back({lv, lv + 1, false, slab[(size_t)lv] != 0});
+        l = e;
+        continue;
+      }
+      // LDS stretch [l, e): chunks of whole levels within the LDS budget
+      while (l < e) {
+        const int32_t c0 = l, t0 = L.ptr[l];
+        const size_t i0 = imps.size();
+        int32_t rows = 0, nst = 0;
+        bool grouped = false;
+        ++stamp;
         while (l < e) {
-          const int32_t c0 = l, t0 = L.ptr[l];
-          const size_t i0 = imps.size();
-          int32_t rows = 0, nst = 0;
-          ++stamp;
-          while (l < e) {
-            const int32_t cnt = L.ptr[l + 1] - L.ptr[l];
-            const size_t before = imps.size();
-            new_operands(l, t0, imps);
-            if (l > c0 && chunk_lds_bytes((int64_t)rows + cnt, (int64_t)(imps.size() - i0),
-                                          nst + nsteps_of(cnt) + LDS_DEPTH - 1) > cap_bytes) {
-              imps.resize(before);  // the level starts the next chunk
-              break;
-            }
-            rows += cnt;
-            nst += nsteps_of(cnt);
-            ++l;
+          const int32_t cnt = cnt_of(l);
+          const size_t before = imps.size();
+          new_operands(l, t0, imps);
+          const bool gr = grouped || grp[(size_t)l] > 1;
+          if (l > c0 && chunk_lds_bytes((int64_t)rows + cnt, (int64_t)(imps.size() - i0),
+                                        nst + nsteps_of(lanes_of(l)) + LDS_DEPTH - 1,
+                                        gr ? SCRATCH_MAX : 0) > cap_bytes) {
+            imps.resize(before);  // the level starts the next chunk
+            break;
           }
-          LevelRun c{c0, l, true, true};
-          c.lds = true;
-          c.t0 = t0;
-          c.t1 = L.ptr[l];
-          c.imp0 = (int64_t)i0;
-          c.nimp = (int32_t)(imps.size() - i0);
-          c.s0 = (int64_t)steps.size();
-          for (int32_t lv = c0; lv < l; ++lv) {
-            const int32_t cnt = L.ptr[lv + 1] - L.ptr[lv], w = P.width[(size_t)lv];
-            c.wmax = std::max(c.wmax, w);
-            for (int32_t q = 0; q < nsteps_of(cnt); ++q) {
-              const int32_t rq = std::min(FUSE_BS, cnt - q * FUSE_BS), bar = (q == nsteps_of(cnt) - 1) ? 1 : 0;
-              steps.push_back(make_int4(L.ptr[lv] + q * FUSE_BS - t0, rq | (w << 16) | (bar << 24),
-                                        (int32_t)(P.eoff[(size_t)lv] + (int64_t)q * FUSE_BS), cnt));
-            }
-          }
-          // padding steps: the last step's loads again, flagged to compute nothing
-          while (((int64_t)steps.size() - c.s0) % LDS_DEPTH) {
-            int4 pad = steps.back();
-            pad.y = (pad.y & 0xFFFFFF) | (1 << 25);
-            steps.push_back(pad);
-          }
-          c.nsteps = (int32_t)((int64_t)steps.size() - c.s0);
-          for (size_t q = i0; q < imps.size(); ++q) impidx[(size_t)imps[q]] = rows + (int32_t)(q - i0);
-          for (int32_t lv = c0; lv < l; ++lv)
-            for (int32_t t = L.ptr[lv]; t < L.ptr[lv + 1]; ++t)
-              for (int32_t k = 0; k < desc[(size_t)t].z; ++k) {
-                const int64_t se = slot(lv, k, t);
-                if (!is_dep(k, pnlo[(size_t)t])) {
-                  lci[(size_t)se] = -2;
-                  fixslot.push_back(se);
-                  continue;
-                }
-                const int32_t col = eci[(size_t)se], pc = pos[(size_t)col];
-                lci[(size_t)se] = pc >= t0 ? pc - t0 : impidx[(size_t)col];
-              }
-          for (size_t q = i0; q < imps.size(); ++q) impidx[(size_t)imps[q]] = -1;
-          P.runs.push_back(c);
+          rows += cnt;
+          nst += nsteps_of(lanes_of(l));
+          grouped = gr;
+          ++l;
         }
+        LevelRun c{c0, l, true, true};
+        c.lds = true;
+        c.t0 = t0;
+        c.t1 = L.ptr[l];
+        c.imp0 = (int64_t)i0;
+        c.nimp = (int32_t)(imps.size() - i0);
+        c.s0 = (int64_t)steps.size();
+        c.grouped = grouped;
+        // steps of FUSE_BS lanes also for narrow levels: 256-lane blocks made
+        // the 2-D ILU apply 0.61 ms against 0.57 ms
+        for (int32_t lv = c0; lv < l; ++lv) c.wmax = std::max(c.wmax, P.width[(size_t)lv]);
+        for (int32_t lv = c0; lv < l; ++lv) {
+          const int32_t g = grp[(size_t)lv], lg = __builtin_ctz((unsigned)g);
+          const int64_t lanes = lanes_of(lv);
+          const int32_t nq = nsteps_of(lanes);
+          for (int32_t q = 0; q < nq; ++q) {
+            const int32_t vq = (int32_t)std::min<int64_t>(FUSE_BS, lanes - (int64_t)q * FUSE_BS);
+            const int32_t bar = (q == nq - 1) ? 1 : 0;
+            steps.push_back(make_int4(L.ptr[lv] - t0 + q * (FUSE_BS / g),
+                                      vq | (P.width[(size_t)lv] << 16) | (bar << 24) | (lg << 26),
+                                      (int32_t)(P.eoff[(size_t)lv] + (int64_t)q * FUSE_BS), (int32_t)lanes));
+          }
+        }
+        // padding steps: the last step's loads again, flagged to compute nothing
+        while (((int64_t)steps.size() - c.s0) % LDS_DEPTH) {
+          int4 pad = steps.back();
+          pad.y = (pad.y & ~(1 << 24)) | (1 << 25);
+          steps.push_back(pad);
+        }
+        c.nsteps = (int32_t)((int64_t)steps.size() - c.s0);
+        for (size_t q = i0; q < imps.size(); ++q) impidx[(size_t)imps[q]] = rows + (int32_t)(q - i0);
+        for (int32_t lv = c0; lv < l; ++lv)
+          for (int32_t t = L.ptr[lv]; t < L.ptr[lv + 1]; ++t) {
+            int32_t nlo = 0;
+            entries(L.h_rows[(size_t)t], tmp, nlo);
+            for (int32_t k = 0; k < (int32_t)tmp.size(); ++k) {
+              const int64_t se = slot(lv, k, t, 0);
+              if (!is_dep(k, nlo)) {
+                lci[(size_t)se] = -2;
+                fixslot.push_back(se);
+                continue;
+              }
+              const int32_t col = ci[(size_t)tmp[(size_t)k]], pc = pos[(size_t)col];
+              lci[(size_t)se] = pc >= t0 ? pc - t0 : impidx[(size_t)col];
+            }
+          }
+        for (size_t q = i0; q < imps.size(); ++q) impidx[(size_t)imps[q]] = -1;
+        P.runs.push_back(c);
       }
     }
     P.any_lds = false;
@@ -988,26 +1084,26 @@ static void ensure_pack(mtjg_prec M, Levels &L, int form, const std::vector<int3
 }
 
 template <int KIND, int W>
-static void launch_sweep_lds_w(const LevelRun &r, const Levels &L, const Packed &P, const double *vals,
-                               double *xdep, double omega, hipStream_t s) {
+static void launch_sweep_lds_w(const LevelRun &r, const Packed &P, const double *vals, double *xdep, double omega,
+                               hipStream_t s) {
   static bool attr_set = false;  // per instantiation: allow LDS beyond the 64 KiB default
   if (!attr_set) {
     HIPCHK(hipFuncSetAttribute((const void *)k_sweep_lds<KIND, W>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                lds_cap_doubles() * (int)sizeof(double)));
     attr_set = true;
   }
-  (void)L;
-  k_sweep_lds<KIND, W><<<1, FUSE_BS, chunk_lds_bytes(r.t1 - r.t0, r.nimp, r.nsteps), s>>>(
-      r.nsteps, r.t0, r.t1, P.steps + r.s0, P.lci, vals, P.pdv, P.bp, P.own, P.imp + r.imp0, r.nimp, P.prow, xdep,
-      omega);
+  const size_t bytes = chunk_lds_bytes(r.t1 - r.t0, r.nimp, r.nsteps, r.grouped ? (int64_t)FUSE_BS * W : 0);
+  k_sweep_lds<KIND, W><<<1, FUSE_BS, bytes, s>>>(r.nsteps, r.t0, r.t1, P.steps + r.s0, P.lci, vals, P.pdv, P.bp,
+                                                 P.own, P.imp + r.imp0, r.nimp, P.prow, xdep, omega);
 }
 
+// Launches one LDS chunk with W slot registers for its widest lane.
 template <int KIND>
-static void launch_sweep_lds(const LevelRun &r, const Levels &L, const Packed &P, const double *vals, double *xdep,
-                             double omega, hipStream_t s) {
-  if (r.wmax <= 2) launch_sweep_lds_w<KIND, 2>(r, L, P, vals, xdep, omega, s);
-  else if (r.wmax <= 4) launch_sweep_lds_w<KIND, 4>(r, L, P, vals, xdep, omega, s);
-  else launch_sweep_lds_w<KIND, ROWDOT_K>(r, L, P, vals, xdep, omega, s);
+static void launch_sweep_lds(const LevelRun &r, const Packed &P, const double *vals, double *xdep, double omega,
+                             hipStream_t s) {
+  if (r.wmax <= 2) launch_sweep_lds_w<KIND, 2>(r, P, vals, xdep, omega, s);
+  else if (r.wmax <= 4) launch_sweep_lds_w<KIND, 4>(r, P, vals, xdep, omega, s);
+  else launch_sweep_lds_w<KIND, ROWDOT_K>(r, P, vals, xdep, omega, s);
 }
 
 template <int FORM>
@@ -1021,7 +1117,7 @@ static void run_levels(mtjg_prec M, Levels &L, hipStream_t s, const std::vector<
                                                nullptr);
   for (const LevelRun &r : P.runs) {
     if (r.lds) {
-      launch_sweep_lds<FORM>(r, L, P, P.eval, x, 1.0, s);
+      launch_sweep_lds<FORM>(r, P, P.eval, x, 1.0, s);
     } else if (r.fused) {
       k_trsv_fused<FORM><<<1, FUSE_BS, 0, s>>>(r.l0, r.l1, L.dptr, P.desc, P.pdv, P.eci, P.eval, b, x);
     } else {
@@ -1071,7 +1167,7 @@ static void ssor_sweep(mtjg_prec M, Levels &L, hipStream_t s, const double *b, d
                                                                          P.nfix, P.fixslot, P.eci, P.eval, P.fval);
   for (const LevelRun &r : P.runs) {
     if (r.lds) {
-      launch_sweep_lds<KIND>(r, L, P, P.fval, xdep, omega, s);
+      launch_sweep_lds<KIND>(r, P, P.fval, xdep, omega, s);
     } else if (r.fused) {
       k_ssor_fused<FORWARD><<<1, FUSE_BS, 0, s>>>(r.l0, r.l1, L.dptr, P.desc, P.pdv, P.eci, P.eval, b, x, M->xx,
                                                   omega);

@@ -153,6 +153,21 @@ def _wide_level_systems():
     yield "poisson2d_200", P.poisson2d(200)
     # 2-D variable-coefficient Poisson 190 x 170: the same with unequal sides
     yield "varcoef2d_190x170", P.varcoef_poisson(2, 190, 170, seed=6)
+    # 27-point stencil on 24^3, SPD with random weights: rows of 13 strictly
+    # lower and 13 strictly upper entries, levels of up to 144 rows; in LDS
+    # chunks rows spread over 2 (triangles) or 4 lanes (SSOR), SSOR's widest
+    # levels over two 512-lane steps
+    m = 24
+    T = sp.diags([np.ones(m - 1), np.ones(m), np.ones(m - 1)], [-1, 0, 1])
+    pat = sp.kron(sp.kron(T, T), T).tocsr()
+    rng = np.random.default_rng(13)
+    off = sp.triu(pat, 1).tocsr()
+    off.data = -rng.random(off.nnz) - 0.1
+    off = off + off.T
+    d = sp.csr_matrix(off + sp.diags(np.asarray(abs(off).sum(axis=1)).ravel() + 1.0))
+    d.sort_indices()
+    n = d.shape[0]
+    yield "stencil27_24", (n, n, d.indptr.astype(np.int32), d.indices.astype(np.int32), d.data)
 
 
 @pytest.mark.parametrize("name,sysm", list(_wide_level_systems()))
