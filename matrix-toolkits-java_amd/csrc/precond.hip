@@ -548,6 +548,8 @@ __device__ __forceinline__ void sweep_stage_row(const SweepStage<W> &S, double *
     pr[g + kk * G] = p;
   }
   if (g != 0) return;
+  // (reading the products ROWDOT_K at a time ahead of their adds measured
+  // slower: AMG V-cycle 5.07 ms against 4.74 ms on 2-D)
   double s = SUB ? S.bp : 0.0;
   for (int32_t k = 0; k < S.w * G; ++k) {
     const double p = pr[k];
