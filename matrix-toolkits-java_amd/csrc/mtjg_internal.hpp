@@ -196,6 +196,13 @@ struct SpmvArgs {
 int64_t spmv_dot_red_cap(mtjg_csr A);
 
 void spmv_f64(mtjg_csr A, const double *x, double *y, const SpmvArgs &a, hipStream_t s);
+// CG's p' = (first ? z : p * beta + z) into pn, q = A p', and the fused
+// product's sums and FIN_CG_ALPHA epilogue, in one pass over the SELL-64
+// copy; false (nothing launched) when A has none or its grid exceeds the
+// persistent fused product's.
+bool spmv_cg_fused(mtjg_csr A, const double *p, const double *z, double *pn, double *q, const SpmvArgs &a,
+                   hipStream_t s);
+bool spmv_cg_fusable(mtjg_csr A);  // whether spmv_cg_fused launches for A
 void spmv_f32(mtjg_csr A, const float *x, float *y, const SpmvArgs &a, hipStream_t s);
 mtjg_csr_s *csr_transpose(mtjg_csr A);
 void csr_ensure_fp32(mtjg_csr A);

@@ -680,10 +680,21 @@ void solve_cg(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   qa.st = st;
   qa.fin = FIN_CG_ALPHA;
   qa.red = S->red;
+  // For SELL-64 matrices whose fused product runs one persistent grid the
+  // update p = z + beta p rides in the product (spmv_cg_fused), writing p'
+  // to the other of two buffers each iteration.
+  const bool fused = spmv_cg_fusable(A);
+  double *pbuf[2] = {p, fused ? scratch(S, 4) : p};
+  int cur = 0;
   run_chunks(S, st, pk == PK_EXTERNAL ? 1 : chunk_for(A), [&]() {
-    if (nt) k_cg_p<double, true><<<g, BS, 0, s>>>(n, p, z, st);
-    else k_cg_p<double><<<g, BS, 0, s>>>(n, p, z, st);
-    spmv_f64(A, p, q, qa, s);
+    if (fused && spmv_cg_fused(A, pbuf[cur], z, pbuf[cur ^ 1], q, qa, s)) {
+      cur ^= 1;
+      p = pbuf[cur];
+    } else {
+      if (nt) k_cg_p<double, true><<<g, BS, 0, s>>>(n, p, z, st);
+      else k_cg_p<double><<<g, BS, 0, s>>>(n, p, z, st);
+      spmv_f64(A, p, q, qa, s);
+    }
     resid(true);
     HIPCHK(hipGetLastError());
   });

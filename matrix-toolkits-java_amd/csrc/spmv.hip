@@ -196,6 +196,73 @@ __global__ __launch_bounds__(BS) void k_sell_fin(int np, const double *part, KSt
   if (grid_sum_dd<2>(v, red, tot) && threadIdx.x == 0) spmv_finish(st, fin, tot[0], tot[1]);
 }
 
+// CG's direction update folded into its fused product (S/CG.java:71-79):
+// p' = z on the first iteration and p' = (p * beta) + z after it, formed at
+// every gather from p and z (the operations of the separate update, so p' is
+// bitwise the same), q = A p' summed as sell_row sums it, p' of the row
+// stored to pn (a buffer other than p, which other rows still gather), and
+// the sums p'.q and q.q accumulated as the fused product's epilogue does.
+template <typename VT>
+__device__ __forceinline__ void sell_row_cgp(int slice, int lane, int n, const int32_t *__restrict__ sptr,
+                                             const int32_t *__restrict__ scol, const VT *__restrict__ sval,
+                                             const double *__restrict__ p, const double *__restrict__ z,
+                                             double *pn, double *q, bool first, double beta, dd &acc0,
+                                             dd &acc1) {
+  const int row = slice * SELL_C + lane;
+  const int off = sptr[slice];
+  const int width = (sptr[slice + 1] - off) / SELL_C;
+  double s = 0;
+  for (int k0 = 0; k0 < width; k0 += SELL_UNROLL) {
+    int c[SELL_UNROLL];
+    VT v[SELL_UNROLL];
+    double xv[SELL_UNROLL];
+#pragma unroll
+    for (int u = 0; u < SELL_UNROLL; ++u) {
+      c[u] = -1;
+      if (k0 + u < width) {
+        const int e = off + (k0 + u) * SELL_C + lane;
+        c[u] = __builtin_nontemporal_load(&scol[e]);
+        v[u] = __builtin_nontemporal_load(&sval[e]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SELL_UNROLL; ++u)
+      if (c[u] >= 0) {
+        const double zc = z[c[u]];
+        xv[u] = first ? zc : (p[c[u]] * beta) + zc;
+      }
+#pragma unroll
+    for (int u = 0; u < SELL_UNROLL; ++u)
+      if (c[u] >= 0) s += (double)v[u] * xv[u];
+  }
+  if (row < n) {
+    const double pr = first ? z[row] : (p[row] * beta) + z[row];
+    pn[row] = pr;
+    const double vq = (s != 0.0) ? s : 0.0;
+    q[row] = vq;
+    dd_fma(acc0, pr, vq);
+    dd_fma(acc1, vq, vq);
+  }
+}
+
+// k_sell_dot with the folded update.
+template <typename VT>
+__global__ __launch_bounds__(BS) void k_sell_dot_cgp(int nslice, int n, const int32_t *__restrict__ sptr,
+                                                     const int32_t *__restrict__ scol, const VT *__restrict__ sval,
+                                                     const double *p, const double *z, double *pn, double *q,
+                                                     KState *st, GridRed red) {
+  if (st->status != ST_RUNNING) return;
+  const bool first = st->iter == 0;
+  const double beta = st->beta;
+  dd acc0{0.0, 0.0}, acc1{0.0, 0.0};
+  constexpr int WPB = BS / SELL_C;
+  for (int slice = blockIdx.x * WPB + (int)threadIdx.x / SELL_C; slice < nslice; slice += gridDim.x * WPB)
+    sell_row_cgp<VT>(slice, threadIdx.x % SELL_C, n, sptr, scol, sval, p, z, pn, q, first, beta, acc0, acc1);
+  dd v[2] = {acc0, acc1};
+  double tot[2];
+  if (grid_sum_dd<2>(v, red, tot) && threadIdx.x == 0) spmv_finish(st, FIN_CG_ALPHA, tot[0], tot[1]);
+}
+
 // Copies each row's values (and, with scol, its columns) from the CSR
 // arrays into its lane of the SELL-64 copy (one thread per row).
 __global__ void k_sell_fill(int32_t nr, const int32_t *rp, const int32_t *ci, const int32_t *sptr,
@@ -326,6 +393,26 @@ void spmv_f64(mtjg_csr A, const double *x, double *y, const SpmvArgs &a, hipStre
     return;
   }
   spmv_launch<double, double>(A, A->val, x, y, a, s);
+}
+
+// Larger grids keep the separate update: gathering p and z at every entry
+// made the 256^3 Poisson CG 0.313 s against 0.289 s, while the one-kernel
+// form saves a launch per iteration on latency-bound systems (2-D Poisson
+// 256^2 CG 0.0112 -> 0.0104 s).
+bool spmv_cg_fusable(mtjg_csr A) {
+  return A->sell && A->nr > 0 && (A->nslice + BS / SELL_C - 1) / (BS / SELL_C) <= SPMV_DOT_GRID;
+}
+
+bool spmv_cg_fused(mtjg_csr A, const double *p, const double *z, double *pn, double *q, const SpmvArgs &a,
+                   hipStream_t s) {
+  if (!spmv_cg_fusable(A)) return false;
+  const int grid = (A->nslice + BS / SELL_C - 1) / (BS / SELL_C);
+  sell_ready(A, false);
+  if (grid > a.red.cap) fail(MTJG_E_INVALID_ARGUMENT, "reduction workspace smaller than the grid");
+  k_sell_dot_cgp<double><<<grid, BS, 0, s>>>(A->nslice, A->nr, A->sptr, A->scol, A->sval, p, z, pn, q, a.st,
+                                             a.red);
+  HIPCHK(hipGetLastError());
+  return true;
 }
 
 void spmv_f32(mtjg_csr A, const float *x, float *y, const SpmvArgs &a, hipStream_t s) {
