@@ -115,6 +115,19 @@ def test_cg_plain_and_ilu(ctx):
     _compare_to_oracle(ctx, P.varcoef_poisson(3, 16, seed=3), "cg", "ilu", 1e-8)
 
 
+@pytest.mark.parametrize("method,gen", [
+    ("cg", lambda: P.poisson3d(82)),
+    ("bicgstab", lambda: P.convdiff3d(82, peclet=10.0, seed=4)),
+])
+def test_split_fused_product(ctx, method, gen):
+    """551,368 rows: the fused product's grid (2,154 workgroups) exceeds one
+    persistent grid, so it runs as the full-grid product with unticketed
+    partial sums plus the reduction kernel, and CG keeps its separate
+    direction update; iterations and solution as the oracle's."""
+    it_g, it_o, xg, xo = _compare_to_oracle(ctx, gen(), method, "diag", 1e-8)
+    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-6
+
+
 @pytest.mark.parametrize("prec", ["none", "diag", "ilu"])
 def test_bicgstab_convdiff(ctx, prec):
     _compare_to_oracle(ctx, P.convdiff3d(24, seed=7), "bicgstab", prec, 1e-8)
