@@ -64,23 +64,25 @@ struct LevelRun {
 // eoff[l] + k * cnt_l + (t - ptr[l]), a row's entries in its first slots in
 // the order it consumes them and column -1 in the rest; the thread of a row
 // then loads its row index, diagonal and entries in one coalesced stage.
-// Other levels (width[l] = -1) keep each row's entries contiguous (row-major)
-// and find them through a 16-byte descriptor, which measured faster for long
-// rows.  Per position: the row index, diagonal value and (SSOR) number of
-// strictly-lower entries.  Values are gathered from the factor through
-// emap / pdmap and gathered again whenever it changes.
+// Levels of longer rows inside LDS chunks are grouped slabs: a row spreads
+// over G lanes, lane (i, g) of V = cnt_l * G holding slot kk at
+// eoff[l] + kk * V + i * G + g for the row's entry g + kk * G (width[l] slots
+// per lane).  Other levels (width[l] = -1) keep each row's entries
+// contiguous (row-major) and find them through a 16-byte descriptor, which
+// measured faster for long rows.  Per position: the row index, diagonal
+// value and (SSOR) number of strictly-lower entries.  Values are gathered
+// from the factor through emap / pdmap and gathered again whenever it
+// changes.
 struct Packed {
   int form = -1;             // TRSV_* form or PACK_SSOR; -1: not built
   uint64_t version = ~0ull;  // factor values gathered at this version
   int64_t slots = 0;
-  std::vector<int64_t> eoff;   // slab offset of each level (host)
-  std::vector<int32_t> width;  // entry slots per row of each level (host)
-  int64_t *deoff = nullptr;    // device copies for the fused runs
-  int32_t *dwidth = nullptr;
+  std::vector<int64_t> eoff;   // entry region of each level (host)
+  std::vector<int32_t> width;  // entry slots per row (per lane when grouped), -1 row-major (host)
   int32_t *prow = nullptr, *pnlo = nullptr, *pdmap = nullptr;
   double *pdv = nullptr;
   int4 *desc = nullptr;  // row-major levels: {row, first entry, count, lower count}
-  std::vector<LevelRun> runs;  // the level set's runs split where the layout changes
+  std::vector<LevelRun> runs;  // launches of one sweep, in order
   std::vector<uint8_t> slab;   // per level: stored as a slab
   int32_t *eci = nullptr, *emap = nullptr;
   double *eval = nullptr;
@@ -90,7 +92,9 @@ struct Packed {
   // per-apply level-order copies of b and of SSOR's own operand
   bool any_lds = false;
   int32_t *lci = nullptr, *imp = nullptr;
-  int4 *steps = nullptr;  // per step {first position - t0, rows | width << 16 | barrier << 24, slot base, stride}
+  // per step {first position - t0, lanes | width << 16 | barrier << 24 |
+  // padding << 25 | log2(lanes per row) << 26, slot base, stride}
+  int4 *steps = nullptr;
   int64_t *fixslot = nullptr;
   int64_t nfix = 0;
   double *fval = nullptr, *bp = nullptr, *own = nullptr;
@@ -427,16 +431,17 @@ __global__ __launch_bounds__(FUSE_BS) void k_ssor_fused(int32_t l0, int32_t l1, 
 
 // ---- fused sweeps through LDS -----------------------------------------------
 //
-// A chunk of consecutive slab levels runs on one workgroup whose LDS holds the chunk's results (LDS index t - t0 for
-// position t) followed by the values it reads from outside the chunk (its
-// imports, loaded once at the start).  Every operand a row waits for is then
-// an LDS read, and each level's own loads (the row's slots, its diagonal, b
-// and, for SSOR, its own incoming value and the precomputed products of the
-// fixed operands) are issued one level ahead, before the barrier that ends
-// the previous level, so a level costs an LDS round trip and a barrier rather
-// than the L2 store-to-load round trip.  The results leave LDS at the end of
-// the chunk.  Every row still sums its slots in MTJ's order with the same
-// operations, so the output is bitwise the launched kernels'.
+// A chunk of consecutive levels runs on one workgroup whose LDS holds the
+// chunk's results (LDS index t - t0 for position t) followed by the values
+// it reads from outside the chunk (its imports, loaded once at the start).
+// Every operand a row waits for is then an LDS read, and each step's own
+// loads (the row's slots, its diagonal, b and, for SSOR, its own incoming
+// value and the precomputed products of the fixed operands) are issued
+// LDS_DEPTH steps ahead, so a level costs an LDS round trip and a barrier
+// rather than the L2 store-to-load round trip.  The results leave LDS at the
+// end of the chunk.  Every row still sums its slots in MTJ's order with the
+// same operations, so the output is bitwise the launched kernels'.
+//
 // Widest level of at most four slots an LDS chunk takes, in steps of FUSE_BS
 // rows (wider levels of up to eight slots: FUSE_MAX).  One compute unit walks
 // a chunk, so a wide level is better launched over the device: measured on
@@ -768,7 +773,7 @@ __global__ void k_gather_pad(int64_t n, const int32_t *map, const double *src, d
 }
 
 static void free_pack(Packed &P) {
-  for (void *q : {(void *)P.deoff, (void *)P.dwidth, (void *)P.prow, (void *)P.pnlo, (void *)P.pdmap, (void *)P.pdv,
+  for (void *q : {(void *)P.prow, (void *)P.pnlo, (void *)P.pdmap, (void *)P.pdv,
                   (void *)P.desc, (void *)P.eci, (void *)P.emap, (void *)P.eval, (void *)P.lci, (void *)P.imp,
                   (void *)P.fixslot, (void *)P.fval, (void *)P.bp, (void *)P.own, (void *)P.steps})
     (void)hipFree(q);
@@ -1069,8 +1074,6 @@ static void ensure_pack(mtjg_prec M, Levels &L, int form, const std::vector<int3
       }
     }
     P.desc = to_device(desc, s);
-    P.deoff = to_device(P.eoff, s);
-    P.dwidth = to_device(P.width, s);
     P.prow = to_device(prow, s);
     P.pnlo = to_device(pnlo, s);
     P.pdmap = to_device(pdmap, s);
