@@ -99,7 +99,9 @@ def cpu_baseline(sysm, seconds):
 
 def copy_calibration(steps):
     """Device-to-device copies of 1 GiB: the known-size reference whose HBM
-    counters (FETCH_SIZE, WRITE_SIZE) scale the SpMV's counter readings."""
+    counters (FETCH_SIZE, WRITE_SIZE) scale the SpMV's counter readings.
+    Also times the library's dot product over two 1 GiB vectors, a
+    read-only stream, as the achievable-bandwidth reference for the SpMV."""
     import mtjgpu as M
     ctx = M.Context(0)
     n = 1 << 27  # 2^27 doubles = 1 GiB
@@ -112,8 +114,16 @@ def copy_calibration(steps):
     for _ in range(steps):
         assert hip.hipMemcpyAsync(b.device_ptr(), a.device_ptr(), n * 8, 3, timer.stream) == 0
     ms = timer.stop_ms() / steps
+    for _ in range(3):
+        a.dot(b)
+    timer.start()
+    for _ in range(steps):
+        a.dot(b)
+    ms_dot = timer.stop_ms() / steps
     print(json.dumps({"mode": "copy", "bytes_read": n * 8, "bytes_written": n * 8,
-                      "ms_per_copy": round(ms, 5), "GBps_rw": round(2 * n * 8 / (ms * 1e-3) / 1e9, 1)}))
+                      "ms_per_copy": round(ms, 5), "GBps_rw": round(2 * n * 8 / (ms * 1e-3) / 1e9, 1),
+                      "ms_per_dot": round(ms_dot, 5),
+                      "dot_read_GBps": round(2 * n * 8 / (ms_dot * 1e-3) / 1e9, 1)}))
     return 0
 
 
