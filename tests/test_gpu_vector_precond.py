@@ -168,6 +168,19 @@ def _wide_level_systems():
     d.sort_indices()
     n = d.shape[0]
     yield "stencil27_24", (n, n, d.indptr.astype(np.int32), d.indices.astype(np.int32), d.data)
+    # tridiagonal with a dense last row and column, SPD: a chain of
+    # one-row levels in LDS chunks, then a level whose row holds 1,499
+    # entries (too long to group), launched on its own after the chunks
+    n = 1500
+    rng = np.random.default_rng(21)
+    d = sp.diags([-np.ones(n - 1), np.full(n, 4.0), -np.ones(n - 1)], [-1, 0, 1]).tolil()
+    tail = -rng.random(n - 1) * 0.01
+    d[n - 1, : n - 1] = tail
+    d[: n - 1, n - 1] = tail.reshape(-1, 1)
+    d[n - 1, n - 1] = 4.0 + np.abs(tail).sum()
+    d = sp.csr_matrix(d)
+    d.sort_indices()
+    yield "dense_last_row_1500", (n, n, d.indptr.astype(np.int32), d.indices.astype(np.int32), d.data)
 
 
 @pytest.mark.parametrize("name,sysm", list(_wide_level_systems()))
