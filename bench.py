@@ -333,7 +333,7 @@ def precond_mode(prec, wl, steps, with_cpu=True):
     return 0
 
 
-def serve_mode(count, lam, seed):
+def serve_mode(count, lam, seed, tune=True):
     """Served stream: `count` requests of the 2D/3D variable-kappa Poisson
     stream (Poisson arrivals at `lam`/s, n log-uniform 1k-262k), solved to
     relative residual 1e-8 (a) by the batched mixed-precision engine with
@@ -345,9 +345,14 @@ def serve_mode(count, lam, seed):
     t0 = time.perf_counter()
     systems = [S.make_system(r) for r in reqs]
     gen_s = time.perf_counter() - t0
+    print("serve: %d systems generated in %.1f s" % (count, gen_s), file=sys.stderr, flush=True)
     ctx = M.Context(0)
-    st_b, res_b = S.serve_batched(ctx, reqs, systems)
+    best, tinfo = tune_batch(ctx, lam, seed, tune)
+    print("serve: launch shape %s" % best, file=sys.stderr, flush=True)
+    st_b, res_b = S.serve_batched(ctx, reqs, systems, **best)
+    print("serve: batched %.1f solves/s" % st_b["solves_per_s"], file=sys.stderr, flush=True)
     st_p, res_p = S.serve_plain(ctx, reqs, systems)
+    print("serve: plain %.1f solves/s" % st_p["solves_per_s"], file=sys.stderr, flush=True)
     worst_b = max(S.true_residual(systems[k], res_b[k]["x"]) for k in range(count))
     worst_p = max(S.true_residual(systems[k], res_p[k]["x"]) for k in range(count))
     out = {"mode": "serve", "requests": count, "lambda_per_s": lam,
@@ -356,7 +361,8 @@ def serve_mode(count, lam, seed):
            "plain_fp64_cg": {k: round(v, 4) for k, v in st_p.items()},
            "speedup_solves_per_s": round(st_b["solves_per_s"] / st_p["solves_per_s"], 3),
            "worst_true_rel_residual": {"batched": worst_b, "plain": worst_p},
-           "all_converged": all(res_b[k]["code"] == 0 and res_p[k]["code"] == 0 for k in range(count))}
+           "all_converged": all(res_b[k]["code"] == 0 and res_p[k]["code"] == 0 for k in range(count)),
+           "autotune": tinfo}
     print(json.dumps(out))
     return 0
 
@@ -430,6 +436,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=256,
                     help="requests of the served-vs-plain comparison attached to the SpMV line (0: none)")
     ap.add_argument("--lam", type=float, default=2000.0)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="serve with the batch engine's default launch shape instead of the autotuner's")
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--solve", default="cg_poisson2d_256", choices=sorted(SOLVE_WORKLOADS))
     ap.add_argument("--repeats", type=int, default=3)
@@ -443,7 +451,7 @@ def main():
     if args.mode == "solve":
         return solve_mode(args.solve, args.repeats)
     if args.mode == "serve":
-        return serve_mode(args.requests, args.lam, args.seed)
+        return serve_mode(args.requests, args.lam, args.seed, not args.no_tune)
     if args.mode == "precond":
         return precond_mode(args.prec, args.prec_workload, min(args.steps, 50), not args.no_cpu)
     if args.mode == "mmload":
@@ -504,18 +512,41 @@ def main():
     if args.serve_requests > 0:
         for o in (A, x, y):
             o.close()
-        out["serving"] = serve_summary(ctx, args.serve_requests, args.lam, args.seed)
+        out["serving"] = serve_summary(ctx, args.serve_requests, args.lam, args.seed, not args.no_tune)
     print(json.dumps(out))
     return 0
 
 
-def serve_summary(ctx, count, lam, seed):
-    """The metric's second half on a bounded stream: `count` requests served
-    by the batch engine and by plain per-request FP64 Jacobi-CG."""
+TUNE_SAMPLE = 48  # systems of the autotuner's sample (a separate stream seed)
+
+
+def tune_batch(ctx, lam, seed, enabled=True):
+    """Launch shape of the batch engine from the autotuner, run on a sample of
+    the same request distribution (seed + 1) before any timed serving.  With
+    tuning off the sample still runs once, with the engine's default shape,
+    so both cases start the timed stream equally warm."""
     from mtjgpu import serving as S
+    sample = [S.make_system(r) for r in S.request_stream(TUNE_SAMPLE, lam=lam, seed=seed + 1)]
+    t0 = time.perf_counter()
+    if enabled:
+        best, table = S.autotune(ctx, sample)
+    else:
+        best, table = S.autotune(ctx, sample, rows_per_workgroup=(S.DEFAULT_ROWS_PER_WORKGROUP,),
+                                 iters_per_step=(S.DEFAULT_ITERS_PER_STEP,))
+    info = {"sample_systems": TUNE_SAMPLE, "key": [float(v) for v in S.shape_key(sample)], "chosen": best,
+            "table": [[r, i, round(v, 1)] for r, i, v in table], "tune_s": round(time.perf_counter() - t0, 2)}
+    return best, info
+
+
+def serve_summary(ctx, count, lam, seed, tune=True):
+    """The metric's second half on a bounded stream: `count` requests served
+    by the batch engine (launch shape from the autotuner) and by plain
+    per-request FP64 Jacobi-CG."""
+    from mtjgpu import serving as S
+    best, tinfo = tune_batch(ctx, lam, seed, tune)
     reqs = S.request_stream(count, lam=lam, seed=seed)
     systems = [S.make_system(r) for r in reqs]
-    st_b, res_b = S.serve_batched(ctx, reqs, systems)
+    st_b, res_b = S.serve_batched(ctx, reqs, systems, **best)
     st_p, res_p = S.serve_plain(ctx, reqs, systems)
     worst = max(S.true_residual(systems[k], res_b[k]["x"]) for k in range(count))
     return {"requests": count, "lambda_per_s": lam, "rtol": 1e-8,
@@ -525,7 +556,8 @@ def serve_summary(ctx, count, lam, seed):
             "plain_p99_ms": round(st_p["p99_ms"], 1),
             "speedup": round(st_b["solves_per_s"] / st_p["solves_per_s"], 3),
             "batched_worst_true_rel_residual": worst,
-            "all_converged": all(res_b[k]["code"] == 0 and res_p[k]["code"] == 0 for k in range(count))}
+            "all_converged": all(res_b[k]["code"] == 0 and res_p[k]["code"] == 0 for k in range(count)),
+            "autotune": tinfo}
 
 
 if __name__ == "__main__":

@@ -20,11 +20,15 @@ from . import (CG, CompRowMatrix, DefaultIterationMonitor, DenseVector, Diagonal
                IterativeSolverNotConvergedException, _ck)
 
 
+DEFAULT_ROWS_PER_WORKGROUP = 1024
+DEFAULT_ITERS_PER_STEP = 16
+
+
 class BatchServer:
     """Handle on one mtjg_batch (see include/mtjgpu.h)."""
 
     def __init__(self, ctx, max_slots=2048, max_workgroups=16384, arena_bytes=32 << 30,
-                 rows_per_workgroup=1024, iters_per_step=16):
+                 rows_per_workgroup=DEFAULT_ROWS_PER_WORKGROUP, iters_per_step=DEFAULT_ITERS_PER_STEP):
         self.ctx = ctx
         self._lib = ctx._lib
         h = C.c_void_p()
@@ -150,6 +154,74 @@ def serve_batched(ctx, reqs, systems, rtol=1e-8, **kw):
         return _stats(lat, 0.0, t_last - t0, len(reqs)), results
     finally:
         B.close()
+
+
+# Launch shapes the autotuner tries: rows per workgroup (the tile each
+# workgroup of a batch launch takes; multiples of 256 rows, since a
+# workgroup's four wavefronts each take 64-row SELL slices, one row per lane)
+# and batch iterations per replayed graph step.
+TUNE_ROWS_PER_WORKGROUP = (256, 512, 1024, 2048)
+TUNE_ITERS_PER_STEP = (8, 16, 32)
+TUNE_MARGIN = 0.03
+_tuned = {}
+
+
+def shape_key(systems):
+    """Autotuner key of a set of systems: the histogram of row lengths (rows
+    of 1-3, 4-5, 6-7, 8-15 and 16+ stored entries, as fractions rounded to
+    tenths) and the rounded log2 of the median system size."""
+    edges = (1, 4, 6, 8, 16, 1 << 30)
+    counts = np.zeros(len(edges) - 1)
+    sizes = []
+    for n, rp, _ci, _va, _b in systems:
+        rl = np.diff(np.asarray(rp))
+        counts += np.histogram(rl, bins=edges)[0]
+        sizes.append(n)
+    frac = tuple(round(float(c) / max(float(counts.sum()), 1.0), 1) for c in counts)
+    return frac + (int(round(math.log2(max(np.median(sizes), 1.0)))),)
+
+
+def autotune(ctx, systems, rtol=1e-8, rows_per_workgroup=TUNE_ROWS_PER_WORKGROUP,
+             iters_per_step=TUNE_ITERS_PER_STEP, use_cache=True, **kw):
+    """Picks the batch engine's launch shape for systems like `systems`: for
+    every (rows per workgroup, iterations per step) candidate, submits all of
+    them at once to a fresh batch, steps it to completion and times it; the
+    candidate with the most solves per second wins, unless the engine's
+    default shape is among the candidates and within TUNE_MARGIN of it (the
+    sample's timings vary by a few percent from run to run, so a smaller lead
+    is not a reason to leave the default).  Results are cached per
+    shape_key, so a workload with the same row-length histogram and scale is
+    tuned once per process.  Returns (best kwargs for BatchServer, table of
+    (rows_per_workgroup, iters_per_step, solves_per_s))."""
+    key = (shape_key(systems), tuple(rows_per_workgroup), tuple(iters_per_step))
+    if use_cache and key in _tuned:
+        return _tuned[key]
+    table = []
+    for rpw in rows_per_workgroup:
+        for ips in iters_per_step:
+            B = BatchServer(ctx, rows_per_workgroup=rpw, iters_per_step=ips, **kw)
+            try:
+                ids = {}
+                t0 = time.perf_counter()
+                for n, rp, ci, va, b in systems:
+                    ids[B.submit(n, rp, ci, va, b, rtol=rtol)] = n
+                while ids:
+                    B.step()
+                    for rid in B.completed():
+                        code = B.result(rid, ids.pop(rid))[1]
+                        if code != 0:
+                            raise RuntimeError("autotune: a sample system ended with status %d" % code)
+                dt = time.perf_counter() - t0
+            finally:
+                B.close()
+            table.append((rpw, ips, len(systems) / dt))
+    best = max(table, key=lambda r: r[2])
+    default = [r for r in table if (r[0], r[1]) == (DEFAULT_ROWS_PER_WORKGROUP, DEFAULT_ITERS_PER_STEP)]
+    if default and best[2] < default[0][2] * (1.0 + TUNE_MARGIN):
+        best = default[0]
+    out = ({"rows_per_workgroup": best[0], "iters_per_step": best[1]}, table)
+    _tuned[key] = out
+    return out
 
 
 def serve_plain(ctx, reqs, systems, rtol=1e-8):

@@ -120,3 +120,31 @@ def test_serving_loops_small_stream(ctx):
         assert res_b[k]["code"] == 0 and res_p[k]["code"] == 0
         assert S.true_residual(systems[k], res_b[k]["x"]) <= 1e-8
         assert S.true_residual(systems[k], res_p[k]["x"]) <= 1e-8
+
+
+def test_autotune_picks_a_measured_shape(ctx):
+    """The autotuner times every candidate launch shape on a sample, returns
+    the fastest as BatchServer arguments (the default shape unless beaten by
+    more than TUNE_MARGIN), caches it per row-length histogram, and the
+    chosen shape serves the stream to 1e-8."""
+    reqs = S.request_stream(6, n_max=20000, seed=31)
+    systems = [S.make_system(r) for r in reqs]
+    best, table = S.autotune(ctx, systems, rows_per_workgroup=(256, 1024), iters_per_step=(8, 16),
+                             max_slots=16, max_workgroups=1024, arena_bytes=1 << 30, use_cache=False)
+    assert sorted((r, i) for r, i, _ in table) == [(256, 8), (256, 16), (1024, 8), (1024, 16)]
+    assert all(v > 0 for _, _, v in table)
+    top = max(table, key=lambda t: t[2])
+    default = [t for t in table if (t[0], t[1]) == (S.DEFAULT_ROWS_PER_WORKGROUP, S.DEFAULT_ITERS_PER_STEP)][0]
+    if top[2] < default[2] * (1 + S.TUNE_MARGIN):
+        assert best == {"rows_per_workgroup": default[0], "iters_per_step": default[1]}
+    else:
+        assert best == {"rows_per_workgroup": top[0], "iters_per_step": top[1]}
+    again = S.autotune(ctx, systems, rows_per_workgroup=(256, 1024), iters_per_step=(8, 16),
+                       max_slots=16, max_workgroups=1024, arena_bytes=1 << 30)
+    assert S.autotune(ctx, systems, rows_per_workgroup=(256, 1024), iters_per_step=(8, 16)) is again
+    st, res = S.serve_batched(ctx, reqs, systems, max_slots=16, max_workgroups=1024, arena_bytes=1 << 30,
+                              **again[0])
+    assert st["solves"] == len(reqs)
+    for k in range(len(reqs)):
+        assert res[k]["code"] == 0
+        assert S.true_residual(systems[k], res[k]["x"]) < 1e-8
