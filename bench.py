@@ -237,6 +237,18 @@ def solve_mode(name, repeats):
            "rel_error_vs_x_true": float(np.linalg.norm(xs - x_true) / np.linalg.norm(x_true))}
     if method == "mixed":
         out["inner_iterations"] = S.inner_iterations()
+        # the FP64 reference: Jacobi-CG on the device, same system and tolerance
+        x64 = M.DenseVector(n, ctx)
+        C64 = M.CG(x64)
+        C64.setPreconditioner(M.DiagonalPreconditioner(n, ctx))
+        C64.getPreconditioner().setMatrix(A)
+        C64.setIterationMonitor(M.DefaultIterationMonitor(100000, rtol))
+        C64.solve(A, b, x64)
+        xr = x64.getData()
+        out["fp64_cg"] = {"iterations": C64.getIterationMonitor().iterations(),
+                          "true_rel_residual": float(np.linalg.norm(bh - Ah @ xr) / np.linalg.norm(bh)),
+                          "rel_error_vs_x_true": float(np.linalg.norm(xr - x_true) / np.linalg.norm(x_true)),
+                          "rel_diff_mixed_vs_fp64_cg": float(np.linalg.norm(xs - xr) / np.linalg.norm(xr))}
     if with_oracle:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
