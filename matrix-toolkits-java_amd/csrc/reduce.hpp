@@ -240,6 +240,19 @@ __device__ bool grid_sum_dd(dd (&v)[K], const GridRed &r, double (&out)[K]) {
     }
   }
   block_sum_dd<K>(acc);
+  if (ngroups == 1) {
+    // A single group needs no second level: its last workgroup finishes, with
+    // the group sum folded once into an exact zero, which is what the second
+    // level's first step does (it only turns a negative zero positive), so
+    // the result is bitwise the two-level one, one atomic round trip sooner.
+    if (threadIdx.x == 0) __hip_atomic_store(&r.gticket[g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const dd z = dd_add(dd{0.0, 0.0}, acc[k]);
+      out[k] = z.hi + z.lo;
+    }
+    return true;
+  }
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
