@@ -449,11 +449,11 @@ __global__ __launch_bounds__(FUSE_BS) void k_ssor_fused(int32_t l0, int32_t l1, 
 // 8192-row levels in chunks, SSOR's (six slots) 1.33 / 1.40 / 2.12 ms.
 constexpr int LDS_LEVEL_MAX = 2048;
 constexpr int LDS_PAD = LDS_LEVEL_MAX;  // slots a stage may read past the last level
-constexpr int LDS_DEPTH = 4;           // step lists are padded to a multiple of this
-// Steps whose loads are in flight.  Eight measured no faster than four on
-// 2-D ILU (0.573 ms either way): one compute unit's L1 misses, not the
-// latency of a step's loads, set the pace.
-constexpr int sweep_depth(int) { return LDS_DEPTH; }
+// Steps whose loads are in flight (step lists are padded to a multiple of
+// it).  Eight measured no faster than four on 2-D ILU (0.573 ms either way):
+// one compute unit's L1 misses, not the latency of a step's loads, set the
+// pace.
+constexpr int LDS_DEPTH = 4;
 constexpr int SWEEP_GROUP_MAX = 64;    // lanes per row at most (one wavefront)
 constexpr int LDS_GROUP_LANES = 1024;  // lanes of a level of grouped rows an LDS chunk takes
 
@@ -573,11 +573,11 @@ __device__ __forceinline__ void sweep_stage_row(const SweepStage<W> &S, double *
 // consecutive steps of whole rows, the last of which ends with the barrier
 // (rows of one level never read each other).  LDS: the chunk's results, its
 // imports, the steps' meta (at an even double offset), then FUSE_BS * W doubles of
-// product scratch for grouped rows.  D = sweep_depth(W) stage buffers are used in
+// product scratch for grouped rows.  D = LDS_DEPTH stage buffers are used in
 // rotation: stage j always holds a step congruent to j, so no register
 // copies, which would wait for the loads in flight; a step's buffer is
 // refilled with the step D ahead right after the step is computed.
-// The step list is padded to a multiple of LDS_DEPTH, so the unrolled body
+// The step list is padded to a multiple of D, so the unrolled body
 // has no exit: an exit inside it would give the loop head a path on which
 // some stages were not refilled, and the compiler would then wait for every
 // load at the head.
@@ -594,8 +594,7 @@ __global__ __launch_bounds__(FUSE_BS) void k_sweep_lds(int32_t nsteps, int32_t t
   for (int32_t k = threadIdx.x; k < nimp; k += FUSE_BS) lx[nrows + k] = xdep[imp[k]];
   for (int32_t k = threadIdx.x; k < nsteps; k += FUSE_BS) lm[k] = steps[k];
   __syncthreads();
-  constexpr int D = sweep_depth(W);
-  static_assert(LDS_DEPTH % D == 0, "step lists are padded to a multiple of LDS_DEPTH");
+  constexpr int D = LDS_DEPTH;
   SweepStage<W> st[D];
 #pragma unroll
   for (int j = 0; j < D; ++j)
