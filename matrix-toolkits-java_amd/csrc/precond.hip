@@ -501,10 +501,6 @@ __device__ __forceinline__ void sweep_stage_load(SweepStage<W> &S, const int4 *l
   }
 }
 
-// Marks an empty slot among a grouped row's products: a signalling NaN,
-// which no arithmetic result is.
-constexpr uint64_t SWEEP_NO_SLOT = 0x7FF4DEAD5EA1ED01ULL;
-
 // The row's result from its sum s, in the form's operation order.
 template <int KIND>
 __device__ __forceinline__ double sweep_finish(double s, double bp, double d, double own, double omega) {
@@ -544,14 +540,16 @@ __device__ __forceinline__ void sweep_stage_row(const SweepStage<W> &S, double *
   // grouped row: every lane stores its products (slot g + kk * G at index
   // g + kk * G of the row's scratch), then the row's first lane adds them in
   // slot order.  The row's lanes share a wavefront, whose LDS accesses are
-  // performed in order, so no barrier is needed between the two.
+  // performed in order, so no barrier is needed between the two.  An empty
+  // slot stores the zero that leaves every sum bitwise unchanged: s + (-0.0)
+  // and s - (+0.0) are s for every s, signed zeros included.
   const int32_t G = 1 << S.lg, g = (int32_t)threadIdx.x & (G - 1);
   double *pr = scr + ((int32_t)threadIdx.x - g) * W;
 #pragma unroll
   for (int kk = 0; kk < W; ++kk) {
     if (kk >= S.w) break;
     const int32_t c = S.c[kk];
-    double p = __longlong_as_double((long long)SWEEP_NO_SLOT);
+    double p = SUB ? 0.0 : -0.0;
     if (c >= 0) p = S.v[kk] * lx[c];
     else if (c == -2) p = S.v[kk];
     pr[g + kk * G] = p;
@@ -565,13 +563,12 @@ __device__ __forceinline__ void sweep_stage_row(const SweepStage<W> &S, double *
 #pragma unroll 4
   for (int32_t k = 0; k < np; k += 2) {
     const double2 p2 = *reinterpret_cast<const double2 *>(pr + k);
-    if ((uint64_t)__double_as_longlong(p2.x) != SWEEP_NO_SLOT) {
-      if (SUB) s -= p2.x;
-      else s += p2.x;
-    }
-    if ((uint64_t)__double_as_longlong(p2.y) != SWEEP_NO_SLOT) {
-      if (SUB) s -= p2.y;
-      else s += p2.y;
+    if (SUB) {
+      s -= p2.x;
+      s -= p2.y;
+    } else {
+      s += p2.x;
+      s += p2.y;
     }
   }
   lx[S.tb + ((int32_t)threadIdx.x >> S.lg)] = sweep_finish<KIND>(s, S.bp, S.d, S.own, omega);
