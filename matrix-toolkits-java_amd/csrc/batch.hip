@@ -1,8 +1,9 @@
 // Batched, continuously admitted mixed-precision CG for many independent SPD
 // systems (the serving path).  No MTJ counterpart: each system is solved the
 // way solve_mixed solves one (FP64 residual r = b - A x with FP64 values, FP32
-// inner Jacobi-CG on r / ||r|| to a relative tolerance, FP64 update
-// x += ||r|| d), but every active system advances in the same launches.
+// inner Jacobi-CG on r / ||r|| to a relative tolerance, the larger of the
+// request's and half the reduction the outer residual still lacks, FP64
+// update x += ||r|| d), but every active system advances in the same launches.
 //
 // Work layout: system s is cut into nb_s row tiles of rows_per_wg rows (a
 // multiple of 256, so of the 64-row slices of the system's SELL-64 copy,
@@ -212,7 +213,11 @@ __global__ __launch_bounds__(BS) void k_batch_b2(BDev d) {
     } else {
       S.inner++;
       S.inner_total++;
-      if (rn < S.inner_rtol * S.inner_init || S.inner >= S.max_inner || rn != rn) {
+      // the inner solve stops at the larger of the request's inner tolerance
+      // and half the reduction the outer residual still lacks
+      const double itol =
+          java_max(S.inner_rtol, fmin(0.5, 0.5 * java_max(S.rtol * S.init_r, S.atol) / S.rnorm));
+      if (rn < itol * S.inner_init || S.inner >= S.max_inner || rn != rn) {
         S.action3 = A3_PROMOTE;
         S.phase = PH_OUTER;
       } else {

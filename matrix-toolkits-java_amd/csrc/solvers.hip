@@ -857,7 +857,9 @@ void solve_ir(mtjg_solver S, mtjg_csr A, double *b, double *x) {
 // Mixed precision: FP64 refinement x += M32^-1 (b - A x) where M32^-1 is an
 // FP32 Jacobi-CG solve on the FP32 copy of A's values, run on the residual
 // scaled to unit norm.  The FP64 residual uses the FP64 values, so the outer
-// tolerance is met for A itself.
+// tolerance is met for A itself.  Each inner solve stops at the larger of the
+// set inner tolerance and half the reduction the outer residual still lacks,
+// so the last correction is not solved further than the outer test needs.
 
 void solve_mixed(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   hipStream_t s = S->ctx->stream;
@@ -894,7 +896,10 @@ void solve_mixed(mtjg_solver S, mtjg_csr A, double *b, double *x) {
     // inner FP32 Jacobi-CG on A32 d = r/||r||, from d = 0
     k_demote_scaled<<<g, BS, 0, s>>>(n, r, r32, st);
     k_copy_f32_zero<<<g, BS, 0, s>>>(n, r32, rin, d32);
-    state_init(S, si, S->inner_max, S->inner_rtol, 0.0, 1e300);
+    const double target = std::max(k.rtol * k.init_r, k.atol);
+    double itol = S->inner_rtol;
+    if (k.residual > 0 && std::isfinite(k.residual)) itol = std::max(itol, std::min(0.5, 0.5 * target / k.residual));
+    state_init(S, si, S->inner_max, itol, 0.0, 1e300);
     k_cg_resid<float, PK_JACOBI, false><<<g, BS, 0, s>>>(n, d32, p32, rin, q32, inv32, z32, si, S->red);
     HIPCHK(hipGetLastError());
     run_chunks(S, si, chunk, [&]() {
