@@ -283,9 +283,11 @@ int mtjg_solver_set_monitor(mtjg_solver S, const mtjg_monitor *m);
 int mtjg_solver_set_reporter(mtjg_solver S, mtjg_reporter cb, void *user);
 /* GMRES.setRestart(restart) (S/GMRES.java:112-125). */
 int mtjg_gmres_set_restart(mtjg_solver S, int32_t restart);
-/* Inner solve of MTJG_MIXED_CG: FP32 Jacobi-CG to inner_rtol relative to its
- * own initial residual, at most inner_max_iter iterations per refinement
- * step.  No MTJ counterpart. */
+/* Inner solve of MTJG_MIXED_CG: FP32 Jacobi-CG relative to its own initial
+ * residual, to the larger of inner_rtol and half the reduction the outer
+ * residual still lacks (so the last refinement is not over-solved), at most
+ * inner_max_iter iterations per refinement step.  The batch engine's
+ * inner_rtol has the same meaning.  No MTJ counterpart. */
 int mtjg_mixed_set_inner(mtjg_solver S, double inner_rtol,
                          int32_t inner_max_iter);
 /* solve(A, b, x): x holds the initial guess and receives the solution
