@@ -239,6 +239,61 @@ void mtjp_powerlaw_fill(int32_t n, const int32_t *len, uint64_t seed,
     }
 }
 
+/* Symmetric positive definite companion of a CSR matrix B with a stored
+ * diagonal: S = B_off + B_off^T (B_off = B without its diagonal), then every
+ * diagonal entry set to 1 + sum_j |s_ij| summed in column order, so S is
+ * strictly diagonally dominant with a positive diagonal.  Rows of S hold the
+ * union of row i and column i of B_off, sorted, plus the diagonal.  With
+ * sci == NULL only srp is filled; returns nnz(S), or -1 if out of memory. */
+int64_t mtjp_symmetrize(int32_t n, const int32_t *rp, const int32_t *ci, const double *val,
+                        int32_t *srp, int32_t *sci, double *sval) {
+    int64_t nnz = rp[n];
+    int32_t *tp = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
+    int32_t *tr = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1));
+    double *tv = (double *)malloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
+    if (!tp || !tr || !tv) { free(tp); free(tr); free(tv); return -1; }
+    for (int32_t i = 0; i < n; ++i)
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k)
+            if (ci[k] != i) tp[ci[k] + 1]++;
+    for (int32_t j = 0; j < n; ++j) tp[j + 1] += tp[j];
+    int32_t *fill = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    if (!fill) { free(tp); free(tr); free(tv); return -1; }
+    memcpy(fill, tp, sizeof(int32_t) * (size_t)n);
+    /* rows visited in order, so each column of the transpose is sorted */
+    for (int32_t i = 0; i < n; ++i)
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k)
+            if (ci[k] != i) { int32_t p = fill[ci[k]]++; tr[p] = i; tv[p] = val[k]; }
+    free(fill);
+    int64_t pos = 0;
+    srp[0] = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        int32_t a = rp[i], ae = rp[i + 1], b = tp[i], be = tp[i + 1];
+        int64_t diag = -1;
+        double sum = 0;
+        for (;;) {
+            while (a < ae && ci[a] == i) a++;
+            int32_t ca = a < ae ? ci[a] : INT32_MAX, cb = b < be ? tr[b] : INT32_MAX;
+            int32_t c = ca < cb ? ca : cb;
+            if (diag < 0 && i < c) {
+                diag = pos;
+                if (sci) sci[pos] = i;
+                pos++;
+            }
+            if (c == INT32_MAX) break;
+            double v = 0;
+            if (ca == c && cb == c) v = val[a++] + tv[b++];
+            else if (ca == c) v = val[a++];
+            else v = tv[b++];
+            if (sci) { sci[pos] = c; sval[pos] = v; sum += fabs(v); }
+            pos++;
+        }
+        if (sci) sval[diag] = 1.0 + sum;
+        srp[i + 1] = (int32_t)pos;
+    }
+    free(tp); free(tr); free(tv);
+    return pos;
+}
+
 /* Writes a CSR matrix as a Matrix Market coordinate file in the layout of
  * MTJ's MatrixVectorWriter (M/io/MatrixVectorWriter.java:89-116, :249-263):
  * MatrixInfo header, "%10d %10d %19d" size line, "%10d %10d % .12e" entries,

@@ -35,6 +35,8 @@ def _load():
         lib.mtjp_powerlaw_lengths.restype = C.c_int64
         lib.mtjp_powerlaw_lengths.argtypes = [C.c_int32, C.c_double, C.c_int32, C.c_uint64, P]
         lib.mtjp_powerlaw_fill.argtypes = [C.c_int32, P, C.c_uint64, P, P, P]
+        lib.mtjp_symmetrize.restype = C.c_int64
+        lib.mtjp_symmetrize.argtypes = [C.c_int32, P, P, P, P, P, P]
         lib.mtjp_write_mm.argtypes = [C.c_char_p, C.c_int32, C.c_int32, P, P, P, C.c_int]
         _lib = lib
     return _lib
@@ -110,6 +112,24 @@ def powerlaw(n, alpha=2.1, kmax=2000, seed=0):
     va = np.empty(nnz, np.float64)
     lib.mtjp_powerlaw_fill(n, lens.ctypes.data, seed, rp.ctypes.data, ci.ctypes.data, va.ctypes.data)
     return n, n, rp, ci, va
+
+
+def powerlaw_spd(n, alpha=2.1, kmax=2000, seed=0):
+    """SPD companion of powerlaw(): S = B_off + B_off^T with every diagonal
+    entry 1 + sum_j |s_ij| (strictly diagonally dominant, positive diagonal),
+    where B_off is powerlaw(n, alpha, kmax, seed) without its diagonal.  Row i
+    of S holds the union of row i and column i of B_off plus the diagonal."""
+    n, _, rp, ci, va = powerlaw(n, alpha, kmax, seed)
+    lib = _load()
+    srp = np.empty(n + 1, np.int32)
+    nnz = int(lib.mtjp_symmetrize(n, rp.ctypes.data, ci.ctypes.data, va.ctypes.data, srp.ctypes.data, None, None))
+    if nnz < 0:
+        raise MemoryError("symmetrize: out of host memory")
+    sci = np.empty(nnz, np.int32)
+    sva = np.empty(nnz, np.float64)
+    lib.mtjp_symmetrize(n, rp.ctypes.data, ci.ctypes.data, va.ctypes.data, srp.ctypes.data, sci.ctypes.data,
+                        sva.ctypes.data)
+    return n, n, srp, sci, sva
 
 
 def spmv_bytes(n_rows, nnz, value_bytes=8, index_bytes=4):
