@@ -228,6 +228,24 @@ int mtjg_prec_amg_create(mtjg_ctx ctx, double omega_pre_f, double omega_pre_r,
                          double omega_post_f, double omega_post_r, int32_t nu1,
                          int32_t nu2, int32_t gamma, int32_t min, double omega,
                          int32_t reverse, mtjg_prec *out);
+/* A Preconditioner (S/Preconditioner.java:42, :63) whose apply(b, x) is an
+ * FP32 Jacobi-CG solve: x = ||b|| d, where d solves A32 d = (float)(b/||b||)
+ * from d = 0 by CG (the loop of S/CG.java:58-87 on FP32 vectors, with
+ * DiagonalPreconditioner's multiply by the reciprocal diagonal, in FP32) on
+ * the FP32 copy of A's values, until ||r|| < inner_rtol ||r_0|| or
+ * inner_max_iter iterations; sums are accumulated in double-double.
+ * x's incoming values are ignored; b = 0 gives x = 0 and a non-finite b
+ * gives NaN.  Passed to IR (S/IR.java:57-70: r = b - A x, M.apply(r, z),
+ * x += z) this is FP32-inner / FP64 iterative refinement through MTJ's own
+ * plug-in points.  setMatrix(A) builds the FP32 values and inverse diagonal
+ * (MTJG_E_ZERO_DIAGONAL with *bad_row) and keeps a reference to A, which must
+ * outlive the preconditioner's use.  transApply is apply (A symmetric).
+ * apply synchronises the context's stream.  No MTJ counterpart as a class. */
+int mtjg_prec_cg32_create(mtjg_ctx ctx, int32_t n, double inner_rtol,
+                          int32_t inner_max_iter, mtjg_prec *out);
+/* Inner CG iterations of an MTJG_PREC_CG32 preconditioner over all its
+ * applies. */
+int mtjg_prec_inner_iterations(mtjg_prec M, int64_t *total);
 /* The hierarchy built by the last setMatrix: *levels, and the first
  * max_sizes level sizes (finest first) into sizes (may be NULL). */
 int mtjg_prec_amg_levels(mtjg_prec M, int32_t *levels, int32_t *sizes, int32_t max_sizes);
@@ -308,9 +326,13 @@ int mtjg_solver_solve(mtjg_solver S, mtjg_csr A, mtjg_vec b, mtjg_vec x,
  * max_outer refinement steps). */
 typedef struct mtjg_batch_s *mtjg_batch;
 
-/* arena_bytes of device memory hold every admitted system (CSR in FP64 and
- * FP32, 3 FP64 and 6 FP32 vectors); each step runs iters_per_step batch
- * iterations from one captured hipGraph. */
+/* Admitted systems (CSR in FP64 and FP32, 3 FP64 and 6 FP32 vectors) live in
+ * a device arena.  arena_bytes > 0 fixes it at that size; arena_bytes == 0
+ * admits by free HBM: the arena grows in chunks while hipMemGetInfo shows the
+ * chunk plus a reserve (4 GiB or 1/16 of HBM, the larger) free, and keeps
+ * its chunks until the batch is destroyed.  Every host<->device copy of the
+ * engine goes through pinned staging owned by the batch.  Each step runs
+ * iters_per_step batch iterations from one captured hipGraph. */
 int mtjg_batch_create(mtjg_ctx ctx, int32_t max_slots, int32_t max_workgroups,
                       int64_t arena_bytes, int32_t rows_per_workgroup,
                       int32_t iters_per_step, mtjg_batch *out);

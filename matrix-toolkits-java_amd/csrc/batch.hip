@@ -275,49 +275,159 @@ __global__ void k_zero_f64(int64_t n, double *x) {
 
 // ---- host side ---------------------------------------------------------
 
-// Simple first-fit allocator over one device arena.
+inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Device memory of the admitted systems: chunks of device memory, each with a
+// first-fit free list in 256-byte granules.  With a fixed size, one chunk of
+// that size is allocated at creation and never grows.  Without one, a chunk is
+// added when no chunk has room, provided hipMemGetInfo still shows the new
+// chunk plus a reserve (kept for the rest of the process) free, so admission
+// is bounded by free HBM.  Chunks are freed only when the batch is destroyed
+// (hipFree would synchronise the device in the middle of a stream).
 struct Arena {
-  char *base = nullptr;
+  struct Chunk {
+    char *base;
+    size_t size;
+    std::map<size_t, size_t> free_;  // offset -> length
+  };
+  std::vector<Chunk> chunks;
+  bool fixed = false;
   size_t size = 0, used = 0;
-  std::map<size_t, size_t> free_;  // offset -> length
-  void init(size_t bytes) {
-    HIPCHK(hipMalloc(&base, bytes));
-    size = bytes;
-    free_.clear();
-    free_[0] = bytes;
+  static constexpr size_t CHUNK = (size_t)4 << 30;
+
+  void add_chunk(char *base, size_t bytes) {
+    Chunk c{base, bytes, {}};
+    c.free_[0] = bytes;
+    chunks.push_back(std::move(c));
+    size += bytes;
   }
-  // returns offset or SIZE_MAX
-  size_t alloc(size_t bytes) {
-    bytes = (bytes + 255) & ~(size_t)255;
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-      if (it->second >= bytes) {
-        size_t off = it->first, len = it->second;
-        free_.erase(it);
-        if (len > bytes) free_[off + bytes] = len - bytes;
+  void init_fixed(size_t bytes) {
+    char *base = nullptr;
+    HIPCHK(hipMalloc(&base, bytes));
+    add_chunk(base, bytes);
+    fixed = true;
+  }
+  // Free HBM the growing arena leaves to the rest of the process.
+  static size_t reserve(size_t total) { return std::max((size_t)4 << 30, total / 16); }
+  // Largest system footprint the arena can ever hold.
+  size_t capacity() const {
+    if (fixed) return size;
+    size_t fr = 0, total = 0;
+    HIPCHK(hipMemGetInfo(&fr, &total));
+    const size_t res = reserve(total);
+    return total > res ? total - res : 0;
+  }
+  // Device address of `bytes` free bytes (its chunk in *chunk), or nullptr.
+  char *alloc(size_t bytes, int *chunk) {
+    bytes = al256(bytes);
+    for (size_t c = 0; c < chunks.size(); ++c) {
+      auto &fl = chunks[c].free_;
+      for (auto it = fl.begin(); it != fl.end(); ++it) {
+        if (it->second < bytes) continue;
+        const size_t off = it->first, len = it->second;
+        fl.erase(it);
+        if (len > bytes) fl[off + bytes] = len - bytes;
         used += bytes;
-        return off;
+        *chunk = (int)c;
+        return chunks[c].base + off;
       }
     }
-    return SIZE_MAX;
-  }
-  void release(size_t off, size_t bytes) {
-    bytes = (bytes + 255) & ~(size_t)255;
-    used -= bytes;
-    auto it = free_.emplace(off, bytes).first;
-    auto nx = std::next(it);
-    if (nx != free_.end() && it->first + it->second == nx->first) {
-      it->second += nx->second;
-      free_.erase(nx);
+    if (fixed) return nullptr;
+    size_t fr = 0, total = 0;
+    HIPCHK(hipMemGetInfo(&fr, &total));
+    const size_t want = std::max(CHUNK, (bytes + ((size_t)256 << 20) - 1) & ~(((size_t)256 << 20) - 1));
+    if (fr < want + reserve(total)) return nullptr;
+    char *base = nullptr;
+    if (hipMalloc(&base, want) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
     }
-    if (it != free_.begin()) {
+    add_chunk(base, want);
+    return alloc(bytes, chunk);
+  }
+  void release(int chunk, char *p, size_t bytes) {
+    bytes = al256(bytes);
+    Chunk &C = chunks.at((size_t)chunk);
+    const size_t off = (size_t)(p - C.base);
+    if (p < C.base || off + bytes > C.size) fail(MTJG_E_DEVICE, "batch arena: release outside its chunk");
+    used -= bytes;
+    auto &fl = C.free_;
+    auto it = fl.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != fl.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      fl.erase(nx);
+    }
+    if (it != fl.begin()) {
       auto pv = std::prev(it);
       if (pv->first + pv->second == it->first) {
         pv->second += it->second;
-        free_.erase(it);
+        fl.erase(it);
       }
     }
   }
+  void free_all() {
+    for (auto &c : chunks) (void)hipFree(c.base);
+    chunks.clear();
+    size = used = 0;
+  }
 };
+
+// Pinned host staging owned by the batch.  Every copy between a system's
+// host arrays and the device goes through it, so the runtime never stages a
+// pageable buffer and every transfer's host range has a known owner and size.
+// Ranges are handed out in 256-byte steps; the owner resets (or grows) it
+// only after the stream that used its ranges has synchronised.
+struct Pinned {
+  char *base = nullptr;
+  size_t cap = 0, used = 0;
+  void grow(size_t bytes) {  // only with no copy in flight
+    if (bytes <= cap) return;
+    const size_t nc = std::max(bytes, 2 * cap);
+    if (base) HIPCHK(hipHostFree(base));
+    base = nullptr;
+    cap = used = 0;
+    HIPCHK(hipHostMalloc((void **)&base, nc, hipHostMallocDefault));
+    cap = nc;
+  }
+  char *take(size_t bytes) {
+    const size_t b = al256(bytes);
+    if (used + b > cap) return nullptr;
+    char *p = base + used;
+    used += b;
+    return p;
+  }
+  bool holds(const void *p, size_t bytes) const {
+    const char *q = (const char *)p;
+    return q >= base && bytes <= cap && (size_t)(q - base) <= cap - bytes;
+  }
+  void release() {
+    if (base) (void)hipHostFree(base);
+    base = nullptr;
+    cap = used = 0;
+  }
+};
+
+// Device range [lo, lo + len) that a copy may touch.
+struct DevRange {
+  const char *lo;
+  size_t len;
+  bool holds(const void *p, size_t bytes) const {
+    const char *q = (const char *)p;
+    return q >= lo && bytes <= len && (size_t)(q - lo) <= len - bytes;
+  }
+};
+
+// hipMemcpyAsync between a pinned staging range and a device range, both
+// checked against their owners first.
+void copy_checked(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, const Pinned &host,
+                  const DevRange &dev, hipStream_t s) {
+  const void *h = kind == hipMemcpyHostToDevice ? src : dst;
+  const void *d = kind == hipMemcpyHostToDevice ? dst : src;
+  if (!host.holds(h, bytes) || !dev.holds(d, bytes))
+    fail(MTJG_E_DEVICE, "batch: a copy falls outside its staging buffer or device range");
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, kind, s));
+}
 
 struct Request {
   int64_t id;
@@ -339,12 +449,12 @@ struct Done {
 
 struct HostSlot {
   int64_t id = -1;  // request id, -1 when free
-  size_t off = 0, bytes = 0;
+  char *dev = nullptr;  // the system's arena range
+  size_t bytes = 0;
+  int chunk = -1;
   int32_t n = 0, nb = 0;
   double *x = nullptr;
 };
-
-inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 size_t req_bytes(const Request &R) {
   const size_t n = (size_t)R.n, nnz = R.ci.size(), sell = (size_t)R.sptr.back();
@@ -380,6 +490,7 @@ struct mtjg_batch_s {
   int max_slots = 0, max_work = 0, iters = 32;
   int rows_per_wg = 1024;
   Arena arena;
+  Pinned up, down;              // staging of system uploads and solution downloads
   BSlot *d_slots = nullptr;     // device slot table
   BWork *d_work = nullptr;
   int32_t *d_nwork = nullptr;
@@ -443,16 +554,32 @@ hipGraphExec_t graph_for(mtjg_batch B, int grid) {
   return ex;
 }
 
-// Retires the systems the pending snapshot shows finished: solution to the
-// host (copy stream, after the snapshot), slot and arena range freed.
+// Retires the systems the pending snapshot shows finished: solution copied
+// (copy stream, after the snapshot) into the batch's pinned download staging,
+// then into the request's Done record once the copy stream has synchronised;
+// slot and arena range freed.
 int retire(mtjg_batch B) {
   if (B->snap_pending < 0) return 0;
   const int i = B->snap_pending;
   B->snap_pending = -1;
   HIPCHK(hipEventSynchronize(B->ev_snap[i]));
   const BSlot *snap = B->h_snap[i];
+  Pinned &dn = B->down;
+  struct Pending {
+    Done *D;
+    const char *h;
+  };
+  std::vector<Pending> pend;
+  auto drain = [&]() {
+    if (pend.empty()) return;
+    HIPCHK(hipStreamSynchronize(B->copy));
+    for (const Pending &q : pend) memcpy(q.D->x.data(), q.h, sizeof(double) * q.D->x.size());
+    pend.clear();
+    dn.used = 0;
+  };
   int n = 0;
   bool waited = false;
+  if (B->snap_hi > B->max_slots) fail(MTJG_E_DEVICE, "batch: snapshot larger than the slot table");
   for (int s = 0; s < B->snap_hi; ++s) {
     HostSlot &H = B->slots[s];
     const BSlot &S = snap[s];
@@ -461,33 +588,42 @@ int retire(mtjg_batch B) {
       HIPCHK(hipStreamWaitEvent(B->copy, B->ev_snap[i], 0));
       waited = true;
     }
-    Done D;
+    const size_t bytes = 8 * (size_t)H.n;
+    char *h = dn.take(bytes);
+    if (!h) {
+      drain();
+      dn.grow(al256(bytes));
+      h = dn.take(bytes);
+    }
+    Done &D = B->done[H.id];
     D.status = S.status;
     D.outer = S.outer;
     D.inner = S.inner_total;
     D.residual = S.rnorm;
     D.x.resize((size_t)H.n);
-    HIPCHK(hipMemcpyAsync(D.x.data(), H.x, 8 * (size_t)H.n, hipMemcpyDeviceToHost, B->copy));
-    B->done[H.id] = std::move(D);
+    copy_checked(h, H.x, bytes, hipMemcpyDeviceToHost, dn, DevRange{H.dev, H.bytes}, B->copy);
+    pend.push_back(Pending{&D, h});
     B->finished.push_back(H.id);
-    B->arena.release(H.off, H.bytes);
+    B->arena.release(H.chunk, H.dev, H.bytes);
     H = HostSlot{};
     ++n;
   }
-  if (waited) HIPCHK(hipStreamSynchronize(B->copy));
+  drain();
   return n;
 }
 
 // Admits queued requests while a slot, work-table room and arena memory are
 // free (first come, first served; a request that does not fit blocks the
-// queue, so large requests are not starved).  Uploads go on the copy
-// stream; the new slot records are staged in h_new[stage].
+// queue, so large requests are not starved).  Each system's arrays are
+// packed into the batch's pinned upload staging and copied from there on the
+// copy stream; the new slot records are staged in h_new[stage].
 int admit(mtjg_batch B, std::vector<int> &new_slots) {
   hipStream_t cs = B->copy;
+  Pinned &up = B->up;
+  up.used = 0;  // the previous admission synchronised the copy stream
   int admitted = 0;
+  bool in_flight = false;
   int used = used_work(B);
-  // admitted requests keep their host arrays until the uploads are done
-  std::vector<Request> sent;
   while (!B->queue.empty()) {
     Request &R = B->queue.front();
     int slot = -1;
@@ -499,38 +635,57 @@ int admit(mtjg_batch B, std::vector<int> &new_slots) {
     if (slot < 0) break;
     const int nb = R.nb;
     if (used + nb > B->max_work) break;
-    const int64_t nnz = (int64_t)R.ci.size();
+    const size_t n = (size_t)R.n, nnz = R.ci.size(), nsp = R.sptr.size(), sell = (size_t)R.sptr.back();
+    if (R.rp.size() != n + 1 || R.val.size() != nnz || R.b.size() != n)
+      fail(MTJG_E_DEVICE, "batch: request arrays do not match its sizes");
     const size_t bytes = req_bytes(R);
-    const size_t off = B->arena.alloc(bytes);
-    if (off == SIZE_MAX) break;
-    char *p = B->arena.base + off;
+    int chunk = -1;
+    char *base = B->arena.alloc(bytes, &chunk);
+    if (!base) break;
+    const DevRange dev{base, bytes};
+    char *p = base;
     auto take = [&](size_t b) {
       char *q = p;
       p += al256(b);
       return q;
     };
-    int32_t *rp = (int32_t *)take(4 * (size_t)(R.n + 1));
-    int32_t *ci = (int32_t *)take(4 * (size_t)nnz);
-    double *v64 = (double *)take(8 * (size_t)nnz);
-    float *inv = (float *)take(4 * (size_t)R.n);
-    double *b = (double *)take(8 * (size_t)R.n);
-    double *x = (double *)take(8 * (size_t)R.n);
-    double *r = (double *)take(8 * (size_t)R.n);
-    float *rr = (float *)take(4 * (size_t)R.n);
-    float *dv = (float *)take(4 * (size_t)R.n);
-    float *pv = (float *)take(4 * (size_t)R.n);
-    float *qv = (float *)take(4 * (size_t)R.n);
-    float *zv = (float *)take(4 * (size_t)R.n);
-    const size_t sell = (size_t)R.sptr.back();
-    int32_t *sptr = (int32_t *)take(4 * R.sptr.size());
+    int32_t *rp = (int32_t *)take(4 * (n + 1));
+    int32_t *ci = (int32_t *)take(4 * nnz);
+    double *v64 = (double *)take(8 * nnz);
+    float *inv = (float *)take(4 * n);
+    double *b = (double *)take(8 * n);
+    double *x = (double *)take(8 * n);
+    double *r = (double *)take(8 * n);
+    float *rr = (float *)take(4 * n);
+    float *dv = (float *)take(4 * n);
+    float *pv = (float *)take(4 * n);
+    float *qv = (float *)take(4 * n);
+    float *zv = (float *)take(4 * n);
+    int32_t *sptr = (int32_t *)take(4 * nsp);
     int32_t *scol = (int32_t *)take(4 * sell);
     double *sv64 = (double *)take(8 * sell);
     float *sv32 = (float *)take(4 * sell);
-    HIPCHK(hipMemcpyAsync(rp, R.rp.data(), 4 * (size_t)(R.n + 1), hipMemcpyHostToDevice, cs));
-    HIPCHK(hipMemcpyAsync(ci, R.ci.data(), 4 * (size_t)nnz, hipMemcpyHostToDevice, cs));
-    HIPCHK(hipMemcpyAsync(v64, R.val.data(), 8 * (size_t)nnz, hipMemcpyHostToDevice, cs));
-    HIPCHK(hipMemcpyAsync(b, R.b.data(), 8 * (size_t)R.n, hipMemcpyHostToDevice, cs));
-    HIPCHK(hipMemcpyAsync(sptr, R.sptr.data(), 4 * R.sptr.size(), hipMemcpyHostToDevice, cs));
+    if ((size_t)(p - base) != al256(bytes)) fail(MTJG_E_DEVICE, "batch: system layout does not match its size");
+    // host side: the five uploaded arrays, packed into the pinned staging
+    const size_t hb = al256(4 * (n + 1)) + al256(4 * nnz) + al256(8 * nnz) + al256(8 * n) + al256(4 * nsp);
+    if (up.used + hb > up.cap) {
+      if (in_flight) HIPCHK(hipStreamSynchronize(cs));
+      in_flight = false;
+      up.used = 0;
+      up.grow(hb);
+    }
+    auto stage = [&](void *dst, const void *src, size_t bytes_) {
+      char *h = up.take(bytes_);
+      if (!h) fail(MTJG_E_DEVICE, "batch: upload staging overflow");
+      if (bytes_) memcpy(h, src, bytes_);
+      copy_checked(dst, h, bytes_, hipMemcpyHostToDevice, up, dev, cs);
+    };
+    stage(rp, R.rp.data(), 4 * (n + 1));
+    stage(ci, R.ci.data(), 4 * nnz);
+    stage(v64, R.val.data(), 8 * nnz);
+    stage(b, R.b.data(), 8 * n);
+    stage(sptr, R.sptr.data(), 4 * nsp);
+    in_flight = true;
     if (sell) {
       HIPCHK(hipMemsetAsync(scol, 0xFF, 4 * sell, cs));  // padding column -1
       HIPCHK(hipMemsetAsync(sv64, 0, 8 * sell, cs));
@@ -574,14 +729,14 @@ int admit(mtjg_batch B, std::vector<int> &new_slots) {
     new_slots.push_back(slot);
     HostSlot &H = B->slots[slot];
     H.id = R.id;
-    H.off = off;
+    H.dev = base;
     H.bytes = bytes;
+    H.chunk = chunk;
     H.n = R.n;
     H.nb = nb;
     H.x = x;
     used += nb;
-    sent.push_back(std::move(R));
-    B->queue.pop_front();
+    B->queue.pop_front();  // the host arrays are staged: R may go
     ++admitted;
   }
   if (admitted) HIPCHK(hipStreamSynchronize(cs));
@@ -597,8 +752,10 @@ void publish_tables(mtjg_batch B, const std::vector<int> &new_slots, bool upload
     HIPCHK(hipEventRecord(B->ev_copy, B->copy));
     HIPCHK(hipStreamWaitEvent(st, B->ev_copy, 0));
   }
-  for (int s : new_slots)
+  for (int s : new_slots) {
+    if (s < 0 || s >= B->max_slots) fail(MTJG_E_DEVICE, "batch: new slot outside the slot table");
     HIPCHK(hipMemcpyAsync(B->d_slots + s, &B->h_new[i][s], sizeof(BSlot), hipMemcpyHostToDevice, st));
+  }
   BWork *w = B->h_work[i];
   int nw = 0;
   int64_t rows = 0;
@@ -606,6 +763,7 @@ void publish_tables(mtjg_batch B, const std::vector<int> &new_slots, bool upload
   for (int s = 0; s < B->max_slots; ++s) {
     const HostSlot &H = B->slots[s];
     if (H.id < 0) continue;
+    if (nw + H.nb > B->max_work) fail(MTJG_E_DEVICE, "batch: work table overflow");
     for (int t = 0; t < H.nb; ++t) w[nw++] = BWork{s, t, off, 0};
     off += H.nb;
     rows += H.n;
@@ -633,7 +791,7 @@ extern "C" int mtjg_batch_create(mtjg_ctx ctx, int32_t max_slots, int32_t max_wo
                                  mtjg_batch *out) {
   MTJG_API_BEGIN
   if (!ctx || !out) fail(MTJG_E_NULL, "null argument");
-  if (max_slots <= 0 || max_workgroups <= 0 || arena_bytes <= 0 || iters_per_step <= 0)
+  if (max_slots <= 0 || max_workgroups <= 0 || arena_bytes < 0 || iters_per_step <= 0)
     fail(MTJG_E_INVALID_ARGUMENT, "batch sizes must be positive");
   if (rows_per_workgroup < BS || rows_per_workgroup % BS)
     fail(MTJG_E_INVALID_ARGUMENT, "rows per workgroup must be a positive multiple of 256");
@@ -643,7 +801,9 @@ extern "C" int mtjg_batch_create(mtjg_ctx ctx, int32_t max_slots, int32_t max_wo
   B->max_work = max_workgroups;
   B->iters = iters_per_step;
   B->rows_per_wg = rows_per_workgroup;
-  B->arena.init((size_t)arena_bytes);
+  if (arena_bytes > 0) B->arena.init_fixed((size_t)arena_bytes);
+  B->up.grow((size_t)64 << 20);
+  B->down.grow((size_t)32 << 20);
   HIPCHK(hipStreamCreateWithFlags(&B->copy, hipStreamNonBlocking));
   HIPCHK(hipMalloc(&B->d_slots, sizeof(BSlot) * max_slots));
   HIPCHK(hipMemsetAsync(B->d_slots, 0, sizeof(BSlot) * max_slots, ctx->stream));
@@ -678,7 +838,9 @@ extern "C" int mtjg_batch_destroy(mtjg_batch B) {
   HIPCHK(hipStreamSynchronize(B->ctx->stream));
   HIPCHK(hipStreamSynchronize(B->copy));
   for (auto &kv : B->graphs) (void)hipGraphExecDestroy(kv.second);
-  (void)hipFree(B->arena.base);
+  B->arena.free_all();
+  B->up.release();
+  B->down.release();
   (void)hipFree(B->d_slots);
   (void)hipFree(B->d_work);
   (void)hipFree(B->d_nwork);
@@ -723,7 +885,7 @@ extern "C" int mtjg_batch_submit(mtjg_batch B, int32_t n, int64_t nnz, const int
   build_sell(R, B->rows_per_wg);
   const int nb = R.nb;
   R.ci.assign(column_index, column_index + nnz);
-  if (nb > B->max_work || req_bytes(R) > B->arena.size)
+  if (nb > B->max_work || req_bytes(R) > B->arena.capacity())
     fail(MTJG_E_OUT_OF_MEMORY, "system larger than the batch can ever hold");
   R.id = B->next_id++;
   R.val.assign(data, data + nnz);
@@ -752,6 +914,7 @@ extern "C" int mtjg_batch_step(mtjg_batch B, int32_t *n_active, int32_t *n_compl
     for (int s = 0; s < B->max_slots; ++s)
       if (B->slots[s].id >= 0) hi = s + 1;
     const int i = B->steps & 1;
+    if (hi > B->max_slots) fail(MTJG_E_DEVICE, "batch: snapshot larger than the slot table");
     HIPCHK(hipMemcpyAsync(B->h_snap[i], B->d_slots, sizeof(BSlot) * hi, hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(B->ev_snap[i], st));
     // 2. while it runs: retire what the previous snapshot shows finished

@@ -27,7 +27,7 @@ DEFAULT_ITERS_PER_STEP = 16
 class BatchServer:
     """Handle on one mtjg_batch (see include/mtjgpu.h)."""
 
-    def __init__(self, ctx, max_slots=2048, max_workgroups=16384, arena_bytes=32 << 30,
+    def __init__(self, ctx, max_slots=2048, max_workgroups=16384, arena_bytes=0,
                  rows_per_workgroup=DEFAULT_ROWS_PER_WORKGROUP, iters_per_step=DEFAULT_ITERS_PER_STEP):
         self.ctx = ctx
         self._lib = ctx._lib

@@ -111,9 +111,37 @@ def test_nan_system_is_isolated(ctx):
         assert out[k][1] == 0 and S.true_residual(systems[k], out[k][0]) <= 1e-8
 
 
+def test_hbm_admitted_arena(ctx):
+    """arena_bytes = 0: the arena starts empty and grows in chunks of device
+    memory as systems are admitted, bounded by the free HBM hipMemGetInfo
+    reports; the chunks are kept until the batch is destroyed."""
+    reqs, systems = _systems(6, seed=51, n_max=30000)
+    B = S.BatchServer(ctx, max_slots=16, max_workgroups=1024, arena_bytes=0)
+    try:
+        assert B.info()["arena_size"] == 0
+        ids = {B.submit(*sysm[:4], sysm[4], rtol=1e-8): k for k, sysm in enumerate(systems)}
+        B.step()
+        info = B.info()
+        assert info["arena_size"] >= 4 << 30 and info["arena_used"] >= sum(S_bytes(s) for s in systems)
+        out = {}
+        for _ in range(100000):
+            B.step()
+            for rid in B.completed():
+                k = ids[rid]
+                out[k] = B.result(rid, systems[k][0])
+            if len(out) == len(systems):
+                break
+        info = B.info()
+        assert info["arena_used"] == 0 and info["arena_size"] >= 4 << 30
+        for k, (x, code, *_r) in out.items():
+            assert code == 0 and S.true_residual(systems[k], x) <= 1e-8
+    finally:
+        B.close()
+
+
 def test_serving_loops_small_stream(ctx):
     reqs, systems = _systems(16, seed=41, n_max=20000)
-    st_b, res_b = S.serve_batched(ctx, reqs, systems, max_slots=64, max_workgroups=4096, arena_bytes=1 << 30)
+    st_b, res_b = S.serve_batched(ctx, reqs, systems, max_slots=64, max_workgroups=4096)
     st_p, res_p = S.serve_plain(ctx, reqs, systems)
     assert st_b["solves"] == st_p["solves"] == 16
     for k in range(16):
