@@ -209,7 +209,7 @@ void csr_ensure_fp32(mtjg_csr A);
 
 // ---- preconditioners (precond.hip) ----
 enum PrecKind : int { PREC_IDENTITY = 0, PREC_DIAGONAL = 1, PREC_ILU = 2, PREC_SSOR = 3, PREC_ICC = 4,
-                      PREC_AMG = 5 };
+                      PREC_AMG = 5, PREC_CG32 = 6 };
 // Applies M (identity when M is null) to n entries; kernels skip work once st's
 // status leaves running (st may be null).
 void prec_apply_dev(mtjg_prec M, int64_t n, const double *b, double *x, hipStream_t s, KState *st);
@@ -218,6 +218,19 @@ void prec_trans_apply_dev(mtjg_prec M, const double *b, double *x, hipStream_t s
 // (synchronises the stream).
 void prec_check(mtjg_prec M);
 int prec_kind(mtjg_prec M);
+
+// ---- FP32 inner-CG preconditioner (solvers.hip) ----
+// Workspace of MTJG_PREC_CG32: an internal solver object over n rows.
+mtjg_solver cg32_create(mtjg_ctx ctx, int32_t n, double rtol, int32_t max_iter);
+// FP32 values of A and the FP32 inverse diagonal (zero diagonal: fails with
+// MTJG_E_ZERO_DIAGONAL and the row).
+void cg32_set_matrix(mtjg_solver S, mtjg_csr A, int32_t *bad_row);
+// x = ||b|| d, d from FP32 Jacobi-CG on A32 d = b / ||b|| (d0 = 0); does
+// nothing once outer's status has left running (outer may be null).
+// Synchronises the stream (the inner loop reads its status word).
+void cg32_apply(mtjg_solver S, mtjg_csr A, const double *b, double *x, KState *outer);
+int64_t cg32_iterations(mtjg_solver S);
+void cg32_destroy(mtjg_solver S);
 
 // ---- algebraic multigrid (amg.hip) ----
 namespace amg {

@@ -142,6 +142,10 @@ struct mtjg_prec_s {
   // AMG: parameters and the device hierarchy built by setMatrix
   mtjg::amg::Params amg_params;
   mtjg::AmgDevice *amg = nullptr;
+  // FP32 inner CG: its workspace and the matrix set by setMatrix (referenced,
+  // not copied, as SSOR references F)
+  mtjg_solver inner = nullptr;
+  mtjg_csr A = nullptr;
 };
 
 namespace mtjg {
@@ -1446,6 +1450,9 @@ void prec_apply_dev(mtjg_prec M, int64_t n, const double *b, double *x, hipStrea
   } else if (M->kind == PREC_AMG) {
     if (!M->amg) fail(MTJG_E_INVALID_ARGUMENT, "AMG.setMatrix must precede apply");
     amg_apply_device(M->amg, b, x, false, s);
+  } else if (M->kind == PREC_CG32) {
+    if (!M->A) fail(MTJG_E_INVALID_ARGUMENT, "setMatrix must precede apply");
+    cg32_apply(M->inner, M->A, b, x, st);
   } else if (M->kind == PREC_ICC) {
     if (!M->factored) fail(MTJG_E_INVALID_ARGUMENT, "ICC.setMatrix must precede apply");
     // Rt.transSolve(b, y); Rt.solve(y, x) (S/ICC.java:65-71)
@@ -1528,6 +1535,35 @@ extern "C" int mtjg_prec_diagonal_create(mtjg_ctx ctx, int32_t n, mtjg_prec *out
   HIPCHK(hipMemsetAsync(M->invdiag, 0, sizeof(double) * (size_t)std::max(n, 1), ctx->stream));
   HIPCHK(hipMalloc(&M->bad, sizeof(int32_t)));
   *out = M;
+  MTJG_API_END
+}
+
+extern "C" int mtjg_prec_cg32_create(mtjg_ctx ctx, int32_t n, double inner_rtol, int32_t inner_max_iter,
+                                     mtjg_prec *out) {
+  MTJG_API_BEGIN
+  if (!ctx || !out) fail(MTJG_E_NULL, "null argument");
+  if (n < 1) fail(MTJG_E_INVALID_ARGUMENT, "size must be positive");
+  if (!(inner_rtol > 0 && inner_rtol < 1) || inner_max_iter < 1)
+    fail(MTJG_E_INVALID_ARGUMENT, "inner tolerance must be in (0, 1) and the iteration limit positive");
+  auto *M = new mtjg_prec_s();
+  M->ctx = ctx;
+  M->kind = PREC_CG32;
+  M->n = n;
+  try {
+    M->inner = cg32_create(ctx, n, inner_rtol, inner_max_iter);
+  } catch (...) {
+    delete M;
+    throw;
+  }
+  *out = M;
+  MTJG_API_END
+}
+
+extern "C" int mtjg_prec_inner_iterations(mtjg_prec M, int64_t *total) {
+  MTJG_API_BEGIN
+  if (!M || !total) fail(MTJG_E_NULL, "null argument");
+  if (M->kind != PREC_CG32) fail(MTJG_E_INVALID_ARGUMENT, "not an FP32 inner-CG preconditioner");
+  *total = cg32_iterations(M->inner);
   MTJG_API_END
 }
 
@@ -1667,6 +1703,11 @@ extern "C" int mtjg_prec_set_matrix(mtjg_prec M, mtjg_csr A, int32_t *bad_row) {
   } else if (M->kind == PREC_ICC) {
     if (A->nr != M->n || A->nc != M->n) fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "matrix size differs from ICC size");
     icc_set_matrix(M, A, bad_row);
+  } else if (M->kind == PREC_CG32) {
+    if (A->nr != M->n || A->nc != M->n) fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "matrix size differs from preconditioner size");
+    M->A = nullptr;
+    cg32_set_matrix(M->inner, A, bad_row);
+    M->A = A;
   } else if (M->kind == PREC_SSOR) {
     if (A->nr != M->n || A->nc != M->n) fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "matrix size differs from SSOR size");
     M->factored = false;
@@ -1679,6 +1720,7 @@ extern "C" int mtjg_prec_set_matrix(mtjg_prec M, mtjg_csr A, int32_t *bad_row) {
 static void check_apply(mtjg_prec M, mtjg_vec b, mtjg_vec x) {
   if (!M || !b || !x) fail(MTJG_E_NULL, "null argument");
   if (M->kind == PREC_AMG && !M->amg) fail(MTJG_E_INVALID_ARGUMENT, "AMG.setMatrix must precede apply");
+  if (M->kind == PREC_CG32 && !M->A) fail(MTJG_E_INVALID_ARGUMENT, "setMatrix must precede apply");
   if (b->dtype != MTJG_F64 || x->dtype != MTJG_F64)
     fail(MTJG_E_INVALID_ARGUMENT, "Vector must be DenseVectors");
   if (b->n != M->n || x->n != M->n) fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "vector size differs from preconditioner size");
@@ -1713,6 +1755,7 @@ extern "C" int mtjg_prec_destroy(mtjg_prec M) {
   (void)hipFree(M->sf_err);
   (void)hipFree(M->xx);
   amg_destroy_device(M->amg);
+  if (M->inner) cg32_destroy(M->inner);
   ilu_free_storage(M);
   delete M;
   MTJG_API_END

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 
 #include "mtjg_internal.hpp"
 #include "reduce.hpp"
@@ -918,6 +919,128 @@ void solve_mixed(mtjg_solver S, mtjg_csr A, double *b, double *x) {
     HIPCHK(hipGetLastError());
   }
 }
+
+// ---------------------------------------------------------------------------
+// The FP32 inner-CG preconditioner (MTJG_PREC_CG32): apply(b, x) is the inner
+// solve of solve_mixed as a Preconditioner (S/Preconditioner.java:42), so
+// that MTJ's own IR (S/IR.java:57-70: r = b - A x, M.apply(r, z), x += z)
+// performs the FP64 refinement.  x = ||b|| d, where d solves A32 d =
+// (float)(b / ||b||) by Jacobi-CG from d = 0 on the FP32 copy of A's values
+// with an FP32 inverse diagonal, to the fixed relative tolerance rtol (at
+// most max_iter iterations).  The workspace is an internal solver object.
+
+// x = ||b|| * (double)d32 (the inner solution scaled back, FP64).
+__global__ __launch_bounds__(BS) void k_promote_set(int64_t n, double *x, const float *d32, const KState *st) {
+  const double sc = st->normr;
+  VLOOP(i)
+    x[i] = sc * (double)d32[i];
+}
+
+// FP32 inverse diagonal; the first zero or structurally missing diagonal is
+// recorded in *bad (S/DiagonalPreconditioner.java:69-75).
+__global__ __launch_bounds__(BS) void k_invdiag32_check(int64_t n, const int32_t *rp, const int32_t *ci,
+                                                        const double *val, float *inv, int32_t *bad) {
+  VLOOP(i) {
+    double d = 0;
+    for (int32_t j = rp[i]; j < rp[i + 1]; ++j)
+      if (ci[j] == (int32_t)i) d = val[j];
+    if (d == 0) atomicMin(bad, (int32_t)i);
+    inv[i] = (float)(1 / d);
+  }
+}
+
+}  // namespace
+
+namespace mtjg {
+
+mtjg_solver cg32_create(mtjg_ctx ctx, int32_t n, double rtol, int32_t max_iter) {
+  mtjg_vec_s t;
+  t.ctx = ctx;
+  t.n = n;
+  mtjg_solver S = nullptr;
+  const int rc = mtjg_solver_create(ctx, MTJG_MIXED_CG, &t, &S);
+  if (rc != MTJG_OK) fail(rc, mtjg_last_error());
+  S->inner_rtol = rtol;
+  S->inner_max = max_iter;
+  return S;
+}
+
+void cg32_set_matrix(mtjg_solver S, mtjg_csr A, int32_t *bad_row) {
+  hipStream_t s = S->ctx->stream;
+  const int64_t n = S->n;
+  csr_ensure_fp32(A);
+  ensure_red(S, A);
+  int32_t *bad = (int32_t *)scratch(S, 1);  // n >= 1 doubles of room
+  const int32_t none = INT32_MAX;
+  HIPCHK(hipMemcpyAsync(bad, &none, sizeof(int32_t), hipMemcpyHostToDevice, s));
+  k_invdiag32_check<<<vgrid(n), BS, 0, s>>>(n, A->rp, A->ci, A->val, scratch32(S, 6), bad);
+  HIPCHK(hipGetLastError());
+  int32_t b = none;
+  HIPCHK(hipMemcpyAsync(&b, bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (b != none) {
+    if (bad_row) *bad_row = b;
+    fail(MTJG_E_ZERO_DIAGONAL, "Zero diagonal on row " + std::to_string(b + 1));
+  }
+}
+
+void cg32_apply(mtjg_solver S, mtjg_csr A, const double *b, double *x, KState *outer) {
+  hipStream_t s = S->ctx->stream;
+  const int64_t n = S->n;
+  if (outer) {
+    // the outer solve's kernels skip work once it has finished; so does this
+    KState *h = &S->h_st[1];
+    HIPCHK(hipMemcpyAsync(&h->status, &outer->status, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (h->status != ST_RUNNING) return;
+  }
+  float *r32 = scratch32(S, 0), *d32 = scratch32(S, 1), *p32 = scratch32(S, 2), *q32 = scratch32(S, 3),
+        *z32 = scratch32(S, 4), *rin = scratch32(S, 5), *inv32 = scratch32(S, 6);
+  const int g = vgrid(n);
+  KState *st = S->st, *si = S->st_in;
+  // ||b|| (st->normr), with a monitor that cannot stop: rtol = atol = 0
+  state_init(S, st, 1, 0.0, 0.0, 1e300);
+  k_norm_check<<<g, BS, 0, s>>>(n, b, st, 0, S->red);
+  HIPCHK(hipGetLastError());
+  KState k;
+  state_read(S, st, k);
+  if (!(k.normr > 0) || !std::isfinite(k.normr)) {
+    // b = 0 gives x = 0; a non-finite b gives NaN, which the outer monitor
+    // reports as divergence
+    launch_fill(s, n, x, k.normr == 0 ? 0.0 : std::numeric_limits<double>::quiet_NaN());
+    return;
+  }
+  SpmvArgs qa;
+  qa.mode = SPMV_DOT;
+  qa.w = p32;
+  qa.st = si;
+  qa.fin = FIN_CG_ALPHA;
+  qa.red = S->red;
+  k_demote_scaled<<<g, BS, 0, s>>>(n, b, r32, st);
+  k_copy_f32_zero<<<g, BS, 0, s>>>(n, r32, rin, d32);
+  state_init(S, si, S->inner_max, S->inner_rtol, 0.0, 1e300);
+  k_cg_resid<float, PK_JACOBI, false><<<g, BS, 0, s>>>(n, d32, p32, rin, q32, inv32, z32, si, S->red);
+  HIPCHK(hipGetLastError());
+  run_chunks(S, si, chunk_for(A), [&]() {
+    k_cg_p<float><<<g, BS, 0, s>>>(n, p32, z32, si);
+    spmv_f32(A, p32, q32, qa, s);
+    k_cg_resid<float, PK_JACOBI, true><<<g, BS, 0, s>>>(n, d32, p32, rin, q32, inv32, z32, si, S->red);
+    HIPCHK(hipGetLastError());
+  });
+  KState ki;
+  state_read(S, si, ki);
+  S->inner_total += ki.iter;
+  k_promote_set<<<g, BS, 0, s>>>(n, x, d32, st);
+  HIPCHK(hipGetLastError());
+}
+
+int64_t cg32_iterations(mtjg_solver S) { return S ? S->inner_total : 0; }
+
+void cg32_destroy(mtjg_solver S) { (void)mtjg_solver_destroy(S); }
+
+}  // namespace mtjg
+
+namespace {
 
 // ---------------------------------------------------------------------------
 // Transpose-using solvers: BiCG (S/BiCG.java:63-102), CGS (S/CGS.java:65-105)

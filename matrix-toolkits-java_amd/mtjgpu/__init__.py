@@ -455,6 +455,27 @@ class DiagonalPreconditioner(_Preconditioner):
         self.handle = h
 
 
+class FP32CGPreconditioner(_Preconditioner):
+    """A Preconditioner whose apply(b, x) is an FP32 Jacobi-CG solve of
+    A x = b on the FP32 copy of A (x = ||b|| d, d from CG on b / ||b|| to the
+    relative tolerance inner_rtol); with IR it is FP32-inner / FP64 iterative
+    refinement through MTJ's own plug-in points (S/IR.java:57-70,
+    S/Preconditioner.java:42)."""
+
+    def __init__(self, n, inner_rtol=1e-4, inner_max_iter=1000, ctx=None):
+        self.ctx = ctx or default_context()
+        self._lib = self.ctx._lib
+        h = C.c_void_p()
+        _ck(self._lib.mtjg_prec_cg32_create(self.ctx.handle, int(n), float(inner_rtol), int(inner_max_iter),
+                                            C.byref(h)))
+        self.handle = h
+
+    def inner_iterations(self):
+        t = C.c_int64()
+        _ck(self._lib.mtjg_prec_inner_iterations(self.handle, C.byref(t)))
+        return t.value
+
+
 class ILU(_Preconditioner):
     """S/ILU.java: ILU(0) in the pattern of the matrix given to setMatrix."""
 

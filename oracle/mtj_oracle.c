@@ -440,6 +440,130 @@ static void ssor_apply(const oc_prec *p, const double *bd, double *xd) {
     }
 }
 
+/* ---- FP32 inner-CG preconditioner ------------------------------------------
+ * Not an MTJ class: the library's MTJG_PREC_CG32, a Preconditioner
+ * (S/Preconditioner.java:42) whose apply is an FP32 solve, restated here so
+ * that IR (S/IR.java:57-70) with it can be checked.  setMatrix keeps A's
+ * pattern, rounds its values to binary32 and forms the binary32 reciprocal
+ * diagonal 1/a_ii (rounded from double), as DiagonalPreconditioner does
+ * (S/DiagonalPreconditioner.java:64-76).  apply(b, x):
+ *   nb = ||b|| (exactly rounded sum of squares), rin = (float)(b * (1/nb)),
+ *   then MTJ's CG loop (S/CG.java:58-87) from d = 0 on binary32 vectors:
+ *   z = r * inv32 (S/DiagonalPreconditioner.java:54-55), rho = r.z,
+ *   p = z | p = p*beta + z (two roundings), q = A32 p (each row summed in
+ *   storage order in binary32, a zero sum stored as +0), alpha = rho / p.q,
+ *   d += alpha p and r += (-alpha) q skipped when alpha == 0 (DenseVector's
+ *   add(0, y) shortcut), with alpha and beta rounded to binary32 where they
+ *   scale a vector; the monitor (atol 0, dtol 1e300) checks ||r|| against
+ *   rtol ||rin|| at the top of every iteration; finally x = nb * (double)d.
+ *   Dots and norms are accumulated in double-double (TwoSum / TwoProduct) in
+ *   index order, so each is the exact sum rounded once (up to a tie), as the
+ *   device's compensated reductions are.  b = 0 gives x = 0, a non-finite
+ *   ||b|| gives NaN. */
+typedef struct { double hi, lo; } oc_dd;
+
+static void dd_acc(oc_dd *a, double x, double y) {
+    double p = x * y;
+    double e = fma(x, y, -p);
+    double s = a->hi + p;
+    double bb = s - a->hi;
+    double t = (a->hi - (s - bb)) + (p - bb);
+    a->hi = s;
+    a->lo += t + e;
+}
+
+static double dd_val(oc_dd a) { return a.hi + a.lo; }
+
+int oc_prec_cg32(oc_prec *p, int n, const int32_t *rp, const int32_t *ci,
+                 const double *a, double rtol, int max_iter, int *bad_row) {
+    memset(p, 0, sizeof(*p));
+    p->kind = OC_PREC_CG32;
+    p->n = n;
+    p->rtol = rtol;
+    p->max_iter = max_iter;
+    int nnz = rp[n];
+    p->rp = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
+    p->ci = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1));
+    p->a32 = (float *)malloc(sizeof(float) * (size_t)(nnz > 0 ? nnz : 1));
+    p->inv32 = (float *)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
+    memcpy(p->rp, rp, sizeof(int32_t) * (size_t)(n + 1));
+    if (nnz) memcpy(p->ci, ci, sizeof(int32_t) * (size_t)nnz);
+    for (int k = 0; k < nnz; ++k) p->a32[k] = (float)a[k];
+    for (int i = 0; i < n; ++i) {
+        double d = 0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k)
+            if (ci[k] == i) d = a[k];
+        if (d == 0) {
+            if (bad_row) *bad_row = i;
+            return OC_ZERO_DIAGONAL;
+        }
+        p->inv32[i] = (float)(1 / d);
+    }
+    return OC_OK;
+}
+
+static void cg32_apply(oc_prec *p, const double *b, double *x) {
+    int n = p->n;
+    oc_dd bb = {0, 0};
+    for (int i = 0; i < n; ++i) dd_acc(&bb, b[i], b[i]);
+    double nb = sqrt(dd_val(bb));
+    if (!(nb > 0) || !isfinite(nb)) {
+        for (int i = 0; i < n; ++i) x[i] = nb == 0 ? 0.0 : NAN;
+        return;
+    }
+    size_t sz = sizeof(float) * (size_t)(n > 0 ? n : 1);
+    float *r = (float *)malloc(sz), *z = (float *)malloc(sz), *pp = (float *)malloc(sz),
+          *q = (float *)malloc(sz), *d = (float *)malloc(sz);
+    double sc = 1.0 / nb;
+    for (int i = 0; i < n; ++i) {
+        r[i] = (float)(b[i] * sc);
+        d[i] = 0.0f;
+    }
+    oc_monitor m;
+    oc_monitor_init(&m, p->max_iter, p->rtol, 0.0, 1e300);
+    double rho = 0, rho_1 = 0, beta = 0, alpha = 0;
+    for (m.iter = 0;; m.iter++) {
+        if (m.iter > 0) {
+            float fa = (float)alpha;
+            if (fa != 0.0f)
+                for (int i = 0; i < n; ++i) {
+                    d[i] = d[i] + fa * pp[i];
+                    r[i] = r[i] + (-fa) * q[i];
+                }
+        }
+        oc_dd rz = {0, 0}, rr = {0, 0};
+        for (int i = 0; i < n; ++i) {
+            z[i] = r[i] * p->inv32[i];
+            dd_acc(&rz, (double)r[i], (double)z[i]);
+            dd_acc(&rr, (double)r[i], (double)r[i]);
+        }
+        if (oc_monitor_converged(&m, sqrt(dd_val(rr))) != 0) break;
+        rho = dd_val(rz);
+        if (m.iter == 0) {
+            for (int i = 0; i < n; ++i) pp[i] = z[i];
+        } else {
+            beta = rho / rho_1;
+            float fb = (float)beta;
+            for (int i = 0; i < n; ++i) {
+                float t = pp[i] * fb;
+                pp[i] = t + z[i];
+            }
+        }
+        oc_dd pq = {0, 0};
+        for (int i = 0; i < n; ++i) {
+            float s = 0.0f;
+            for (int k = p->rp[i]; k < p->rp[i + 1]; ++k) s += p->a32[k] * pp[p->ci[k]];
+            q[i] = s != 0.0f ? s : 0.0f;
+            dd_acc(&pq, (double)pp[i], (double)q[i]);
+        }
+        alpha = rho / dd_val(pq);
+        rho_1 = rho;
+    }
+    p->inner_total += m.iter;
+    for (int i = 0; i < n; ++i) x[i] = nb * (double)d[i];
+    free(r); free(z); free(pp); free(q); free(d);
+}
+
 void oc_prec_apply(const oc_prec *p, const double *b, double *x) {
     int n = p->n;
     if (p->kind == OC_PREC_AMG) {
@@ -448,6 +572,8 @@ void oc_prec_apply(const oc_prec *p, const double *b, double *x) {
         ssor_apply(p, b, x);
     } else if (p->kind == OC_PREC_IDENTITY) {
         oc_dv_set(n, b, x);
+    } else if (p->kind == OC_PREC_CG32) {
+        cg32_apply((oc_prec *)p, b, x);
     } else if (p->kind == OC_PREC_DIAGONAL) {
         for (int i = 0; i < n; ++i) x[i] = b[i] * p->invdiag[i];
     } else if (p->kind == OC_PREC_ICC) {
@@ -483,6 +609,8 @@ void oc_prec_free(oc_prec *p) {
     free(p->diagind);
     free(p->y);
     free(p->xx);
+    free(p->a32);
+    free(p->inv32);
     memset(p, 0, sizeof(*p));
 }
 

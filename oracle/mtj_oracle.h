@@ -89,7 +89,7 @@ int oc_monitor_converged(oc_monitor *m, double r);
 
 /* ---- Preconditioners ------------------------------------------------------ */
 enum { OC_PREC_IDENTITY = 0, OC_PREC_DIAGONAL = 1, OC_PREC_ILU = 2, OC_PREC_SSOR = 3, OC_PREC_ICC = 4,
-       OC_PREC_AMG = 5 };
+       OC_PREC_AMG = 5, OC_PREC_CG32 = 6 };
 
 struct oc_amg_s;
 
@@ -105,6 +105,11 @@ typedef struct {
     int reverse;         /* SSOR: run the reverse sweep */
     double *xx;          /* SSOR: half iterate */
     struct oc_amg_s *amg; /* AMG: the hierarchy (owned by the caller) */
+    float *a32;          /* CG32: binary32 copy of A's values (pattern in rp, ci) */
+    float *inv32;        /* CG32: binary32 reciprocal diagonal */
+    double rtol;         /* CG32: inner relative tolerance */
+    int max_iter;        /* CG32: inner iteration limit */
+    long inner_total;    /* CG32: inner iterations over all applies */
 } oc_prec;
 
 void oc_prec_identity(oc_prec *p, int n);
@@ -127,6 +132,11 @@ int oc_prec_ssor(oc_prec *p, int n, const int32_t *rp, const int32_t *ci,
  * *bad_row. */
 int oc_prec_icc(oc_prec *p, int n, const int32_t *rp, const int32_t *ci,
                 const double *a, int *bad_row);
+/* FP32 inner-CG preconditioner + setMatrix(A): OC_ZERO_DIAGONAL (row in
+ * *bad_row).  apply(b, x): x = ||b|| d, d from CG in binary32 on
+ * (float)(b / ||b||); see mtj_oracle.c. */
+int oc_prec_cg32(oc_prec *p, int n, const int32_t *rp, const int32_t *ci,
+                 const double *a, double rtol, int max_iter, int *bad_row);
 void oc_prec_apply(const oc_prec *p, const double *b, double *x);
 void oc_prec_trans_apply(const oc_prec *p, const double *b, double *x);
 void oc_prec_free(oc_prec *p);

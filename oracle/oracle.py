@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "build", "libmtjoracle.so")
 
 OK, ITERATIONS, DIVERGENCE, BREAKDOWN = 0, 1, 2, 3
 ZERO_DIAGONAL, MISSING_DIAGONAL, ZERO_PIVOT, SINGULAR, BAD_ARG, UNSUPPORTED = 10, 11, 12, 13, 20, 22
-PREC_IDENTITY, PREC_DIAGONAL, PREC_ILU, PREC_SSOR, PREC_ICC, PREC_AMG = 0, 1, 2, 3, 4, 5
+PREC_IDENTITY, PREC_DIAGONAL, PREC_ILU, PREC_SSOR, PREC_ICC, PREC_AMG, PREC_CG32 = 0, 1, 2, 3, 4, 5, 6
 
 
 class Monitor(C.Structure):
@@ -27,7 +27,8 @@ class Prec(C.Structure):
     _fields_ = [("kind", C.c_int), ("n", C.c_int), ("invdiag", C.c_void_p), ("rp", C.c_void_p),
                 ("ci", C.c_void_p), ("lu", C.c_void_p), ("diagind", C.c_void_p),
                 ("y", C.c_void_p), ("omega_f", C.c_double), ("omega_r", C.c_double),
-                ("reverse", C.c_int), ("xx", C.c_void_p), ("amg", C.c_void_p)]
+                ("reverse", C.c_int), ("xx", C.c_void_p), ("amg", C.c_void_p), ("a32", C.c_void_p),
+                ("inv32", C.c_void_p), ("rtol", C.c_double), ("max_iter", C.c_int), ("inner_total", C.c_long)]
 
 
 _PI = C.POINTER(C.c_int32)
@@ -93,6 +94,7 @@ def load():
         lib.oc_java_hashset_order.argtypes = [P, C.c_int, P]
         lib.oc_dgetf2.argtypes = [C.c_int, P, P]
         lib.oc_dgetrs.argtypes = [C.c_int, C.c_int, P, P, P]
+        lib.oc_prec_cg32.argtypes = [C.POINTER(Prec), C.c_int, P, P, P, C.c_double, C.c_int, C.POINTER(C.c_int)]
         lib.oc_prec_ssor.argtypes = [C.POINTER(Prec), C.c_int, P, P, P, C.c_int, C.c_double, C.c_double,
                                      C.POINTER(C.c_int)]
         lib.oc_prec_apply.argtypes = [C.POINTER(Prec), P, P]
@@ -159,7 +161,8 @@ class Csr:
 class Precond:
     """Identity / DiagonalPreconditioner / ILU / SSOR / ICC restatement."""
 
-    def __init__(self, kind, A=None, n=None, reverse=True, omega_f=1.0, omega_r=1.0):
+    def __init__(self, kind, A=None, n=None, reverse=True, omega_f=1.0, omega_r=1.0, inner_rtol=1e-4,
+                 inner_max_iter=1000):
         self.s = Prec()
         self.bad = C.c_int(-1)
         lib = load()
@@ -172,6 +175,9 @@ class Precond:
         elif kind == PREC_SSOR:
             self.code = lib.oc_prec_ssor(C.byref(self.s), A.nr, _p(A.rp), _p(A.ci), _p(A.va), int(reverse),
                                          float(omega_f), float(omega_r), C.byref(self.bad))
+        elif kind == PREC_CG32:
+            self.code = lib.oc_prec_cg32(C.byref(self.s), A.nr, _p(A.rp), _p(A.ci), _p(A.va), float(inner_rtol),
+                                         int(inner_max_iter), C.byref(self.bad))
         elif kind == PREC_ICC:
             self.code = lib.oc_prec_icc(C.byref(self.s), A.nr, _p(A.rp), _p(A.ci), _p(A.va),
                                         C.byref(self.bad))

@@ -205,3 +205,60 @@ def test_reporter_and_determinism(ctx):
     x2 = M.DenseVector(n, ctx)
     S.solve(A, b, x2)
     np.testing.assert_array_equal(x1.getData(), x2.getData())
+
+
+def test_ir_with_fp32_cg_preconditioner(ctx):
+    """C4's scheme through MTJ's plug-in points: IR (S/IR.java:57-70) with a
+    Preconditioner whose apply is the FP32 inner Jacobi-CG, against the
+    oracle's IR with the same preconditioner restated: the same refinement
+    count, inner iterations within 5%, the FP64 residual at 1e-10, and the
+    oracle's solution to FP64 accuracy."""
+    sysm = P.varcoef_poisson(3, 32, seed=5)
+    n = sysm[0]
+    O = oracle.Csr(*sysm)
+    x_true = P.uniform(n, 1)
+    bh = O.mult(x_true)
+    op = oracle.Precond(oracle.PREC_CG32, O, inner_rtol=1e-4)
+    code, xo, it_o, _ = oracle.solve("ir", O, bh, np.zeros(n), op, oracle.monitor(100, 1e-10))
+    assert code == oracle.OK
+    A = M.CompRowMatrix(*sysm, ctx=ctx)
+    b = M.DenseVector(bh, ctx)
+    x = M.DenseVector(n, ctx)
+    Mp = M.FP32CGPreconditioner(n, inner_rtol=1e-4, ctx=ctx)
+    Mp.setMatrix(A)
+    # one apply on its own: the FP32 solve of the device against the oracle's
+    z = M.DenseVector(n, ctx)
+    zg = Mp.apply(b, z).getData()
+    zo = oracle.Precond(oracle.PREC_CG32, O, inner_rtol=1e-4).apply(bh)
+    assert np.linalg.norm(zg - zo) <= 1e-6 * np.linalg.norm(zo)
+    S = M.IR(x)
+    S.setPreconditioner(Mp)
+    S.setIterationMonitor(M.DefaultIterationMonitor(100, 1e-10))
+    before = Mp.inner_iterations()
+    S.solve(A, b, x)
+    inner = Mp.inner_iterations() - before
+    xg = x.getData()
+    assert S.getIterationMonitor().iterations() == it_o
+    assert abs(inner - op.s.inner_total) <= max(2, 0.05 * op.s.inner_total), (inner, op.s.inner_total)
+    m = sp.csr_matrix((sysm[4], sysm[3], sysm[2]), shape=(n, n))
+    assert np.linalg.norm(bh - m @ xg) / np.linalg.norm(bh) <= 1e-10
+    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-9
+
+
+def test_fp32_cg_preconditioner_errors(ctx):
+    A = M.CompRowMatrix(3, 3, [0, 1, 2, 3], [0, 1, 2], [1.0, 0.0, 2.0], ctx=ctx)
+    Mp = M.FP32CGPreconditioner(3, ctx=ctx)
+    x = M.DenseVector(3, ctx)
+    with pytest.raises(M.IllegalArgumentException):
+        Mp.apply(M.DenseVector(np.ones(3), ctx), x)
+    with pytest.raises(M.ZeroDiagonalException) as e:
+        Mp.setMatrix(A)
+    assert e.value.row == 1
+    A = M.CompRowMatrix(3, 3, [0, 1, 2, 3], [0, 1, 2], [1.0, 4.0, 2.0], ctx=ctx)
+    Mp.setMatrix(A)
+    x.upload(np.ones(3))
+    np.testing.assert_array_equal(Mp.apply(M.DenseVector(np.zeros(3), ctx), x).getData(), np.zeros(3))
+    np.testing.assert_allclose(Mp.apply(M.DenseVector(np.array([1.0, 2.0, 4.0]), ctx), x).getData(),
+                               [1.0, 0.5, 2.0], rtol=1e-7)
+    with pytest.raises(M.IllegalArgumentException):
+        M.FP32CGPreconditioner(3, inner_rtol=0.0, ctx=ctx)
