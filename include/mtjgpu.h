@@ -313,6 +313,26 @@ int mtjg_mixed_set_inner(mtjg_solver S, double inner_rtol,
 int mtjg_solver_solve(mtjg_solver S, mtjg_csr A, mtjg_vec b, mtjg_vec x,
                       mtjg_result *result);
 
+/* ---- Allocation-free CG solve ------------------------------------------------ */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the solve of an MTJG_CG solver with no preconditioner or
+ * a DiagonalPreconditioner (S/CG.java:58-87, same kernels, same x, iteration
+ * count and residual bit for bit as mtjg_solver_solve), with every scratch
+ * vector, the solver state and the reduction workspace taken from a fixed
+ * device array of MTJG_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends.  Makes no hipMalloc or hipFree.  Systems of at most
+ * MTJG_NOALLOC_MAX_ROWS rows (else MTJG_E_INVALID_ARGUMENT); other
+ * preconditioners give MTJG_E_UNSUPPORTED; monitor may be NULL
+ * (DefaultIterationMonitor's defaults); no reporter.  Returns like
+ * mtjg_solver_solve.  Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (from several contexts or threads) race. */
+#define MTJG_NOALLOC_MAX_ROWS (1 << 20)
+#define MTJG_NOALLOC_SCRATCH_BYTES (41 << 20)
+int mtjg_cg_solve_noalloc(mtjg_ctx ctx, mtjg_csr A, mtjg_prec M, mtjg_vec b,
+                          mtjg_vec x, const mtjg_monitor *monitor,
+                          mtjg_result *result);
+
 /* ---- Batched serving of many systems (no MTJ counterpart) ------------------ */
 /* A batch solves many independent SPD systems with mixed-precision CG (FP64
  * residual r = b - A x with the FP64 values, FP32 inner Jacobi-CG on r/||r||,

@@ -14,9 +14,11 @@
 // compiled with -ffp-contract=off) and therefore equal MTJ bitwise given the
 // same scalars; dot products and norms differ only in summation order.
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
 
 #include "mtjg_internal.hpp"
 #include "reduce.hpp"
@@ -571,7 +573,7 @@ int status_to_code(int st) {
   }
 }
 
-double *scratch(mtjg_solver S, size_t k) {
+double *solver_vec(mtjg_solver S, size_t k) {
   while (S->v64.size() <= k) {
     double *p = nullptr;
     HIPCHK(hipMalloc(&p, sizeof(double) * (size_t)std::max<int64_t>(S->n, 1)));
@@ -636,14 +638,25 @@ int prec_k(mtjg_prec M) {
 // ---------------------------------------------------------------------------
 // CG (S/CG.java:58-87)
 
-void solve_cg(mtjg_solver S, mtjg_csr A, double *b, double *x) {
-  hipStream_t s = S->ctx->stream;
-  const int64_t n = S->n;
-  double *p = scratch(S, 0), *z = scratch(S, 1), *q = scratch(S, 2), *r = scratch(S, 3);
-  const int pk = prec_k(S->M);
-  const double *invd = prec_invdiag(S->M);
+// Workspace of one CG solve: the scratch vectors (S/CG.java:51-56), the
+// device solver state and the reduction workspace.  p2 is the second
+// direction buffer of the fused product (equal to p when it is not used).
+struct CgWork {
+  double *p, *z, *q, *r, *p2;
+  KState *st;
+  GridRed red;
+};
+
+// S/CG.java:64-86 on the device, from an initialised state: r = b - A x,
+// the first residual step, then iterations handed to `run(chunk, iteration)`,
+// which queues them until the status word leaves running.
+template <typename Run>
+void cg_iterate(hipStream_t s, int64_t n, mtjg_csr A, mtjg_prec M, double *b, double *x, const CgWork &w,
+                Run &&run) {
+  double *p = w.p, *z = w.z, *q = w.q, *r = w.r;
+  const int pk = prec_k(M);
+  const double *invd = prec_invdiag(M);
   if (pk == PK_IDENTITY) z = r;  // identity: z = r bitwise, no copy needed
-  state_init(S, S->st, S->mon.max_iter, S->mon.rtol, S->mon.atol, S->mon.dtol);
   // r = b - A x (S/CG.java:64)
   SpmvArgs ra;
   ra.mode = SPMV_MULTADD;
@@ -652,25 +665,25 @@ void solve_cg(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   spmv_f64(A, x, r, ra, s);
   const int g = vgrid(n);
   const bool nt = n * (int64_t)sizeof(double) >= NT_VECTOR_BYTES;
-  ensure_red(S, A);
-  KState *st = S->st;
+  KState *st = w.st;
+  const GridRed &red = w.red;
   auto resid = [&](bool update) {
     if (pk == PK_IDENTITY) {
-      if (update) k_cg_resid<double, PK_IDENTITY, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->red);
-      else k_cg_resid<double, PK_IDENTITY, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->red);
+      if (update) k_cg_resid<double, PK_IDENTITY, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, red);
+      else k_cg_resid<double, PK_IDENTITY, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, red);
     } else if (pk == PK_JACOBI) {
       if (nt) {
-        if (update) k_cg_resid<double, PK_JACOBI, true, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->red);
-        else k_cg_resid<double, PK_JACOBI, false, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->red);
+        if (update) k_cg_resid<double, PK_JACOBI, true, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, red);
+        else k_cg_resid<double, PK_JACOBI, false, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, red);
       } else {
-        if (update) k_cg_resid<double, PK_JACOBI, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->red);
-        else k_cg_resid<double, PK_JACOBI, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->red);
+        if (update) k_cg_resid<double, PK_JACOBI, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, red);
+        else k_cg_resid<double, PK_JACOBI, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, red);
       }
     } else {
-      if (update) k_cg_resid<double, PK_EXTERNAL, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->red);
-      else k_cg_resid<double, PK_EXTERNAL, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, S->red);
-      prec_apply_dev(S->M, n, r, z, s, st);
-      k_cg_rho<double><<<g, BS, 0, s>>>(n, r, z, st, S->red);
+      if (update) k_cg_resid<double, PK_EXTERNAL, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, red);
+      else k_cg_resid<double, PK_EXTERNAL, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, red);
+      prec_apply_dev(M, n, r, z, s, st);
+      k_cg_rho<double><<<g, BS, 0, s>>>(n, r, z, st, red);
     }
   };
   resid(false);
@@ -680,14 +693,14 @@ void solve_cg(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   qa.w = p;
   qa.st = st;
   qa.fin = FIN_CG_ALPHA;
-  qa.red = S->red;
+  qa.red = red;
   // For SELL-64 matrices whose fused product runs one persistent grid the
   // update p = z + beta p rides in the product (spmv_cg_fused), writing p'
   // to the other of two buffers each iteration.
-  const bool fused = spmv_cg_fusable(A);
-  double *pbuf[2] = {p, fused ? scratch(S, 4) : p};
+  const bool fused = spmv_cg_fusable(A) && w.p2 != w.p;
+  double *pbuf[2] = {p, fused ? w.p2 : p};
   int cur = 0;
-  run_chunks(S, st, pk == PK_EXTERNAL ? 1 : chunk_for(A), [&]() {
+  run(pk == PK_EXTERNAL ? 1 : chunk_for(A), [&]() {
     if (fused && spmv_cg_fused(A, pbuf[cur], z, pbuf[cur ^ 1], q, qa, s)) {
       cur ^= 1;
       p = pbuf[cur];
@@ -701,14 +714,29 @@ void solve_cg(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   });
 }
 
+void solve_cg(mtjg_solver S, mtjg_csr A, double *b, double *x) {
+  CgWork w;
+  w.p = solver_vec(S, 0);
+  w.z = solver_vec(S, 1);
+  w.q = solver_vec(S, 2);
+  w.r = solver_vec(S, 3);
+  w.p2 = spmv_cg_fusable(A) ? solver_vec(S, 4) : w.p;
+  state_init(S, S->st, S->mon.max_iter, S->mon.rtol, S->mon.atol, S->mon.dtol);
+  ensure_red(S, A);
+  w.st = S->st;
+  w.red = S->red;
+  cg_iterate(S->ctx->stream, S->n, A, S->M, b, x, w,
+             [&](int chunk, auto &&iteration) { run_chunks(S, S->st, chunk, iteration); });
+}
+
 // ---------------------------------------------------------------------------
 // BiCGstab (S/BiCGstab.java:65-116)
 
 void solve_bicgstab(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   hipStream_t s = S->ctx->stream;
   const int64_t n = S->n;
-  double *p = scratch(S, 0), *sv = scratch(S, 1), *phat = scratch(S, 2), *shat = scratch(S, 3),
-         *t = scratch(S, 4), *v = scratch(S, 5), *r = scratch(S, 6), *rtilde = scratch(S, 7);
+  double *p = solver_vec(S, 0), *sv = solver_vec(S, 1), *phat = solver_vec(S, 2), *shat = solver_vec(S, 3),
+         *t = solver_vec(S, 4), *v = solver_vec(S, 5), *r = solver_vec(S, 6), *rtilde = solver_vec(S, 7);
   const int pk = prec_k(S->M);
   if (pk == PK_IDENTITY) {
     phat = p;  // M.apply = x.set(b): the copy is exact
@@ -781,7 +809,7 @@ void solve_gmres(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   hipStream_t s = S->ctx->stream;
   const int64_t n = S->n;
   gmres_alloc(S);
-  double *w = scratch(S, 0), *u = scratch(S, 1), *r = scratch(S, 2);
+  double *w = solver_vec(S, 0), *u = solver_vec(S, 1), *r = solver_vec(S, 2);
   KState *st = S->st;
   state_init(S, st, S->mon.max_iter, S->mon.rtol, S->mon.atol, S->mon.dtol);
   GmresDev gd{S->H, S->gs, S->restart};
@@ -833,7 +861,7 @@ void solve_gmres(mtjg_solver S, mtjg_csr A, double *b, double *x) {
 void solve_ir(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   hipStream_t s = S->ctx->stream;
   const int64_t n = S->n;
-  double *z = scratch(S, 0), *r = scratch(S, 1);
+  double *z = solver_vec(S, 0), *r = solver_vec(S, 1);
   KState *st = S->st;
   state_init(S, st, S->mon.max_iter, S->mon.rtol, S->mon.atol, S->mon.dtol);
   SpmvArgs ra;
@@ -865,7 +893,7 @@ void solve_ir(mtjg_solver S, mtjg_csr A, double *b, double *x) {
 void solve_mixed(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   hipStream_t s = S->ctx->stream;
   const int64_t n = S->n;
-  double *r = scratch(S, 0);
+  double *r = solver_vec(S, 0);
   float *r32 = scratch32(S, 0), *d32 = scratch32(S, 1), *p32 = scratch32(S, 2), *q32 = scratch32(S, 3),
         *z32 = scratch32(S, 4), *rin = scratch32(S, 5), *inv32 = scratch32(S, 6);
   csr_ensure_fp32(A);
@@ -970,7 +998,7 @@ void cg32_set_matrix(mtjg_solver S, mtjg_csr A, int32_t *bad_row) {
   const int64_t n = S->n;
   csr_ensure_fp32(A);
   ensure_red(S, A);
-  int32_t *bad = (int32_t *)scratch(S, 1);  // n >= 1 doubles of room
+  int32_t *bad = (int32_t *)solver_vec(S, 1);  // n >= 1 doubles of room
   const int32_t none = INT32_MAX;
   HIPCHK(hipMemcpyAsync(bad, &none, sizeof(int32_t), hipMemcpyHostToDevice, s));
   k_invdiag32_check<<<vgrid(n), BS, 0, s>>>(n, A->rp, A->ci, A->val, scratch32(S, 6), bad);
@@ -1122,8 +1150,8 @@ struct HostMonitor {
 };
 
 void solve_bicg(mtjg_solver S, mtjg_csr A, mtjg_vec b, mtjg_vec x) {
-  DV z(S, scratch(S, 0)), p(S, scratch(S, 1)), q(S, scratch(S, 2)), r(S, scratch(S, 3)),
-      zt(S, scratch(S, 4)), pt(S, scratch(S, 5)), qt(S, scratch(S, 6)), rt(S, scratch(S, 7));
+  DV z(S, solver_vec(S, 0)), p(S, solver_vec(S, 1)), q(S, solver_vec(S, 2)), r(S, solver_vec(S, 3)),
+      zt(S, solver_vec(S, 4)), pt(S, solver_vec(S, 5)), qt(S, solver_vec(S, 6)), rt(S, solver_vec(S, 7));
   HostMonitor mon(S);
   double rho_1 = 1, rho_2 = 1, alpha = 1, beta = 1;
   ck(mtjg_vec_set(r.h(), b));
@@ -1160,9 +1188,9 @@ void solve_bicg(mtjg_solver S, mtjg_csr A, mtjg_vec b, mtjg_vec x) {
 }
 
 void solve_cgs(mtjg_solver S, mtjg_csr A, mtjg_vec b, mtjg_vec x) {
-  DV p(S, scratch(S, 0)), q(S, scratch(S, 1)), u(S, scratch(S, 2)), phat(S, scratch(S, 3)),
-      qhat(S, scratch(S, 4)), vhat(S, scratch(S, 5)), uhat(S, scratch(S, 6)), sum(S, scratch(S, 7)),
-      r(S, scratch(S, 8)), rt(S, scratch(S, 9));
+  DV p(S, solver_vec(S, 0)), q(S, solver_vec(S, 1)), u(S, solver_vec(S, 2)), phat(S, solver_vec(S, 3)),
+      qhat(S, solver_vec(S, 4)), vhat(S, solver_vec(S, 5)), uhat(S, solver_vec(S, 6)), sum(S, solver_vec(S, 7)),
+      r(S, solver_vec(S, 8)), rt(S, solver_vec(S, 9));
   HostMonitor mon(S);
   double rho_1 = 0, rho_2 = 0, alpha = 0, beta = 0;
   ck(mtjg_vec_set(r.h(), b));
@@ -1203,10 +1231,10 @@ void solve_cgs(mtjg_solver S, mtjg_csr A, mtjg_vec b, mtjg_vec x) {
 }
 
 void solve_qmr(mtjg_solver S, mtjg_csr A, mtjg_vec b, mtjg_vec x) {
-  DV r(S, scratch(S, 0)), y(S, scratch(S, 1)), z(S, scratch(S, 2)), v(S, scratch(S, 3)), w(S, scratch(S, 4)),
-      vt(S, scratch(S, 5)), wt(S, scratch(S, 6)), yt(S, scratch(S, 7)), zt(S, scratch(S, 8)),
-      p(S, scratch(S, 9)), q(S, scratch(S, 10)), pt(S, scratch(S, 11)), d(S, scratch(S, 12)),
-      s(S, scratch(S, 13));
+  DV r(S, solver_vec(S, 0)), y(S, solver_vec(S, 1)), z(S, solver_vec(S, 2)), v(S, solver_vec(S, 3)), w(S, solver_vec(S, 4)),
+      vt(S, solver_vec(S, 5)), wt(S, solver_vec(S, 6)), yt(S, solver_vec(S, 7)), zt(S, solver_vec(S, 8)),
+      p(S, solver_vec(S, 9)), q(S, solver_vec(S, 10)), pt(S, solver_vec(S, 11)), d(S, solver_vec(S, 12)),
+      s(S, solver_vec(S, 13));
   HostMonitor mon(S);
   double rho = 0, rho_1 = 0, xi = 0, gamma = 1., gamma_1 = 0, theta = 0, theta_1 = 0, eta = -1., delta = 0, ep = 0,
          beta = 0;
@@ -1292,6 +1320,84 @@ void solve_qmr(mtjg_solver S, mtjg_csr A, mtjg_vec b, mtjg_vec x) {
 }  // namespace
 
 // ===========================================================================
+// The allocation-free CG solve (mtjg_cg_solve_noalloc).
+//
+// Every device buffer of the solve comes from `scratch`, one fixed array per
+// GPU that the code object carries, so the call makes no hipMalloc or
+// hipFree: the scratch vectors p z q r and the fused product's second
+// direction buffer (MTJG_NOALLOC_MAX_ROWS doubles each), the solver state and
+// the two-level reduction workspace.  In host code the array's name is not a
+// device address: its address on each device is looked up once, with that
+// device current, by hipGetSymbolAddress and kept per device; only that
+// pointer is handed to kernels.  What must start at zero (the state, the
+// reduction tickets) is cleared by a kernel at every call.  Every call on a
+// GPU shares that GPU's array, so calls that overlap on one GPU race.
+
+namespace {
+
+constexpr int64_t NOALLOC_VEC = MTJG_NOALLOC_MAX_ROWS;  // doubles per vector
+constexpr int NOALLOC_RED_CAP = 16384;                   // reduction workspace columns
+constexpr int NOALLOC_RED_GCAP = (NOALLOC_RED_CAP + RED_GROUP - 1) / RED_GROUP;
+constexpr size_t NOALLOC_OFF_STATE = 5 * sizeof(double) * (size_t)NOALLOC_VEC;
+constexpr size_t NOALLOC_OFF_PART = NOALLOC_OFF_STATE + 256;
+constexpr size_t NOALLOC_OFF_GPART = NOALLOC_OFF_PART + sizeof(double) * RED_MAX_K * (size_t)NOALLOC_RED_CAP;
+constexpr size_t NOALLOC_OFF_TICKET = NOALLOC_OFF_GPART + sizeof(double) * RED_MAX_K * (size_t)NOALLOC_RED_GCAP;
+constexpr size_t NOALLOC_BYTES = NOALLOC_OFF_TICKET + sizeof(uint32_t) * (NOALLOC_RED_GCAP + 1);
+static_assert(sizeof(KState) <= 256, "solver state exceeds its slot");
+static_assert(NOALLOC_BYTES <= (size_t)MTJG_NOALLOC_SCRATCH_BYTES, "scratch layout exceeds the array");
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[MTJG_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+unsigned char *g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on `device`, looked up on the first call
+// for that device (with it current) and kept.
+unsigned char *noalloc_scratch(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) fail(MTJG_E_UNSUPPORTED, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[device]) {
+    int cur = -1;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != device) HIPCHK(hipSetDevice(device));
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != device) HIPCHK(hipSetDevice(cur));
+    if (e != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      fail(MTJG_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    g_noalloc_ptr[device] = (unsigned char *)ptr;
+  }
+  return g_noalloc_ptr[device];
+}
+
+// The solver state of a new solve (DefaultIterationMonitor fields, status
+// running, every other field zero) and zero reduction tickets; one
+// workgroup.
+__global__ void k_noalloc_init(KState *st, int32_t max_iter, double rtol, double atol, double dtol,
+                               uint32_t *tickets, int nticket) {
+  uint32_t *w = reinterpret_cast<uint32_t *>(st);
+  for (int i = threadIdx.x; i < (int)(sizeof(KState) / sizeof(uint32_t)); i += blockDim.x) w[i] = 0u;
+  for (int i = threadIdx.x; i < nticket; i += blockDim.x) tickets[i] = 0u;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    st->max_iter = max_iter;
+    st->rtol = rtol;
+    st->atol = atol;
+    st->dtol = dtol;
+    st->status = ST_RUNNING;
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
 // C ABI
 
 extern "C" int mtjg_solver_create(mtjg_ctx ctx, int kind, mtjg_vec template_vec, mtjg_solver *out) {
@@ -1319,7 +1425,7 @@ extern "C" int mtjg_solver_create(mtjg_ctx ctx, int kind, mtjg_vec template_vec,
              : kind == MTJG_CGS      ? 10
              : kind == MTJG_QMR      ? 14
                                      : 1;
-  for (int k = 0; k < nvec; ++k) scratch(S, (size_t)k);
+  for (int k = 0; k < nvec; ++k) solver_vec(S, (size_t)k);
   HIPCHK(hipStreamSynchronize(ctx->stream));
   *out = S;
   MTJG_API_END
@@ -1430,3 +1536,77 @@ extern "C" int mtjg_solver_solve(mtjg_solver S, mtjg_csr A, mtjg_vec b, mtjg_vec
 }
 
 extern "C" int mtjg_solver_inner_iterations(mtjg_solver S) { return S ? S->inner_total : -1; }
+
+extern "C" int mtjg_cg_solve_noalloc(mtjg_ctx ctx, mtjg_csr A, mtjg_prec M, mtjg_vec b, mtjg_vec x,
+                                     const mtjg_monitor *monitor, mtjg_result *result) {
+  MTJG_API_BEGIN
+  if (!ctx || !A || !b || !x) fail(MTJG_E_NULL, "null argument");
+  // checkSizes (S/AbstractIterativeSolver.java:70-77)
+  if (A->nr != A->nc) fail(MTJG_E_INVALID_ARGUMENT, "!A.isSquare()");
+  if (b->n != A->nr) fail(MTJG_E_INVALID_ARGUMENT, "b.size() != A.numRows()");
+  if (b->n != x->n) fail(MTJG_E_INVALID_ARGUMENT, "b.size() != x.size()");
+  if (b->dtype != MTJG_F64 || x->dtype != MTJG_F64) fail(MTJG_E_UNSUPPORTED, "solvers take FP64 vectors");
+  if (b->ptr == x->ptr && b->n > 0) fail(MTJG_E_INVALID_ARGUMENT, "b and x must not alias");
+  if (A->nr == 0) fail(MTJG_E_INVALID_ARGUMENT, "empty system");
+  const int pk = prec_kind(M);
+  if (pk != PREC_IDENTITY && pk != PREC_DIAGONAL)
+    fail(MTJG_E_UNSUPPORTED, "the allocation-free solve takes no preconditioner or a DiagonalPreconditioner");
+  if (M && prec_n(M) != A->nr) fail(MTJG_E_INDEX_OUT_OF_BOUNDS, "preconditioner size differs");
+  const int64_t n = A->nr;
+  if (n > MTJG_NOALLOC_MAX_ROWS)
+    fail(MTJG_E_INVALID_ARGUMENT, "system of " + std::to_string(n) + " rows exceeds the allocation-free solve's " +
+                                      std::to_string((long long)MTJG_NOALLOC_MAX_ROWS) + "-row scratch");
+  const int64_t need = std::max<int64_t>(VGRID_MAX, spmv_dot_red_cap(A));
+  if (need > NOALLOC_RED_CAP) fail(MTJG_E_INVALID_ARGUMENT, "reduction grid exceeds the allocation-free scratch");
+  const mtjg_monitor mon = monitor ? *monitor : mtjg_monitor{100000, 1e-5, 1e-50, 1e5};
+  HIPCHK(hipSetDevice(ctx->device));
+  unsigned char *base = noalloc_scratch(ctx->device);
+  hipStream_t s = ctx->stream;
+  CgWork w;
+  double *v = reinterpret_cast<double *>(base);
+  w.p = v;
+  w.z = v + NOALLOC_VEC;
+  w.q = v + 2 * NOALLOC_VEC;
+  w.r = v + 3 * NOALLOC_VEC;
+  w.p2 = v + 4 * NOALLOC_VEC;
+  w.st = reinterpret_cast<KState *>(base + NOALLOC_OFF_STATE);
+  w.red.partials = reinterpret_cast<double *>(base + NOALLOC_OFF_PART);
+  w.red.gpart = reinterpret_cast<double *>(base + NOALLOC_OFF_GPART);
+  w.red.gticket = reinterpret_cast<uint32_t *>(base + NOALLOC_OFF_TICKET);
+  w.red.fticket = w.red.gticket + NOALLOC_RED_GCAP;
+  w.red.cap = NOALLOC_RED_CAP;
+  w.red.gcap = NOALLOC_RED_GCAP;
+  k_noalloc_init<<<1, BS, 0, s>>>(w.st, mon.max_iter, mon.rtol, mon.atol, mon.dtol, w.red.gticket,
+                                  NOALLOC_RED_GCAP + 1);
+  HIPCHK(hipGetLastError());
+  // The status word (with the residual and iteration count beside it) lands
+  // in the context's pinned scalars, allocated when the context was made.
+  const size_t lo = offsetof(KState, residual), hi = offsetof(KState, flags) + sizeof(int32_t);
+  static_assert(offsetof(KState, flags) + sizeof(int32_t) - offsetof(KState, residual) <= 8 * sizeof(double),
+                "status tail exceeds the context's pinned scalars");
+  KState tail;
+  auto read_tail = [&]() {
+    HIPCHK(hipMemcpyAsync(ctx->h_out, reinterpret_cast<char *>(w.st) + lo, hi - lo, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    memcpy(reinterpret_cast<char *>(&tail) + lo, ctx->h_out, hi - lo);
+  };
+  cg_iterate(s, n, A, M, (double *)b->ptr, (double *)x->ptr, w, [&](int chunk, auto &&iteration) {
+    for (;;) {
+      for (int c = 0; c < chunk; ++c) iteration();
+      read_tail();
+      if (tail.status != ST_RUNNING) break;
+    }
+  });
+  read_tail();
+  const int code = status_to_code(tail.status);
+  if (result) {
+    result->iterations = tail.iter;
+    result->residual = tail.residual;
+    result->reason = code;
+  }
+  if (code != MTJG_OK)
+    set_last_error(std::string("Solver did not converge. Residual=") + std::to_string(tail.residual) +
+                   ", iterations=" + std::to_string(tail.iter));
+  return code;
+  MTJG_API_END
+}

@@ -112,6 +112,7 @@ PROTOTYPES = [
     ("mtjg_gmres_set_restart", C.c_int, [P, I32]),
     ("mtjg_mixed_set_inner", C.c_int, [P, D, I32]),
     ("mtjg_solver_solve", C.c_int, [P, P, P, P, C.POINTER(Result)]),
+    ("mtjg_cg_solve_noalloc", C.c_int, [P, P, P, P, P, C.POINTER(Monitor), C.POINTER(Result)]),
     ("mtjg_batch_create", C.c_int, [P, I32, I32, I64, I32, I32, PP]),
     ("mtjg_batch_destroy", C.c_int, [P]),
     ("mtjg_batch_submit", C.c_int, [P, I32, I64, P, P, P, P, D, I32, D, I32, PI64]),
