@@ -57,6 +57,15 @@ struct LevelRun {
   int64_t imp0 = 0, s0 = 0;
 };
 
+// One tile of a tiled sweep: its LDS chunk (steps [s0, s0 + nsteps),
+// positions [t0, t1), imports [imp0, imp0 + nimp)) and the tiles it imports
+// from, deps[dep0, dep0 + ndep).
+struct TileMeta {
+  int32_t nsteps, t0, t1, nimp;
+  int64_t s0, imp0;
+  int32_t dep0, ndep;
+};
+
 // The entries one sweep over a level set reads, copied in level order.  Level
 // l owns the entry region [eoff[l], eoff[l+1]).  A level whose rows hold at
 // most one load batch (ROWDOT_K entries) is stored as a column-major slab
@@ -98,9 +107,25 @@ struct Packed {
   int64_t *fixslot = nullptr;
   int64_t nfix = 0;
   double *fval = nullptr, *bp = nullptr, *own = nullptr;
+  // tiled sweeps (every run an LDS chunk, run k = tile k): tile table, the
+  // tiles each imports from, completion flags stamped with the sweep's epoch,
+  // ticket and exit counters, the largest LDS need without grouped-row
+  // scratch, whether any tile has grouped rows, widest lane
+  int32_t ntiles = 0;
+  TileMeta *tiles = nullptr;
+  int32_t *tdeps = nullptr;
+  uint32_t *tflags = nullptr, *tctr = nullptr;
+  uint32_t epoch = 0;
+  size_t tile_lds = 0;
+  bool tile_grouped = false;
+  int32_t tile_wmax = 0;
 };
 
 struct Levels {
+  bool forward = true;  // dependencies have smaller row indices
+  int tiling = 0;       // tiled sweeps: 0 not yet decided, 1 in use, -1 not used
+  Levels *tiled = nullptr;  // the tiled order of this level set (tile_levels)
+  std::vector<int32_t> tile_lev;  // tiled order: first level of each tile [ntiles + 1]
   int32_t nlev = 0;
   std::vector<int32_t> ptr;     // host copy of level offsets [nlev + 1]
   int32_t *dptr = nullptr;      // device copy of the level offsets
@@ -471,6 +496,7 @@ struct SweepStage {
   double bp, d, own;
   int32_t c[W];
   double v[W];
+  int32_t row;  // the row of the lane's position (tiled sweeps only)
 };
 
 // Step li of the chunk: its meta from LDS, then the lane's operands.  Lanes
@@ -478,10 +504,11 @@ struct SweepStage {
 // width load the last slot (or, for a level without slots, a slot no further
 // than LDS_PAD past the end), so every load is in bounds and the loads of a
 // stage are the same on every path.
-template <int KIND, int W>
+template <int KIND, int W, bool ROW = false>
 __device__ __forceinline__ void sweep_stage_load(SweepStage<W> &S, const int4 *lm, int32_t li, int32_t t0,
                                                  const int32_t *lci, const double *vals, const double *pdv,
-                                                 const double *bp, const double *own) {
+                                                 const double *bp, const double *own,
+                                                 const int32_t *prow = nullptr) {
   const int4 m = lm[li];
   S.tb = __builtin_amdgcn_readfirstlane(m.x);
   const int32_t y = __builtin_amdgcn_readfirstlane(m.y);
@@ -493,7 +520,14 @@ __device__ __forceinline__ void sweep_stage_load(SweepStage<W> &S, const int4 *l
   const int32_t base = __builtin_amdgcn_readfirstlane(m.z), stride = __builtin_amdgcn_readfirstlane(m.w);
   const int32_t tl = min((int32_t)threadIdx.x, S.cnt - 1);
   const int32_t t = t0 + S.tb + (tl >> S.lg);
-  S.bp = bp[t];
+  // with ROW the lane's row is loaded and bp is the right-hand side in row
+  // order (no level-order copy)
+  if (ROW) {
+    S.row = prow[t];
+    S.bp = bp[S.row];
+  } else {
+    S.bp = bp[t];
+  }
   S.d = (KIND == TRSV_ROW_UNIT || KIND == TRSV_COL_UNIT_REV) ? 1.0 : pdv[t];
   S.own = (KIND >= PACK_SSOR) ? own[t] : 0.0;
   const int32_t klast = max(S.w - 1, 0);
@@ -518,8 +552,11 @@ __device__ __forceinline__ double sweep_finish(double s, double bp, double d, do
   return s;
 }
 
-template <int KIND, int W>
-__device__ __forceinline__ void sweep_stage_row(const SweepStage<W> &S, double *lx, double *scr, double omega) {
+// With GSTORE the result also goes straight to x (agent scope) as soon as it
+// is computed (tiled sweeps).
+template <int KIND, int W, bool GSTORE = false>
+__device__ __forceinline__ void sweep_stage_row(const SweepStage<W> &S, double *lx, double *scr, double omega,
+                                                double *xdep = nullptr) {
   constexpr bool SUB = (KIND == TRSV_COL_DIV || KIND == TRSV_COL_UNIT_REV);  // s = b, s -= a x
   if ((int32_t)threadIdx.x >= S.act) return;
   if (S.lg == 0) {
@@ -538,7 +575,9 @@ __device__ __forceinline__ void sweep_stage_row(const SweepStage<W> &S, double *
         s += S.v[k];
       }
     }
-    lx[S.tb + (int32_t)threadIdx.x] = sweep_finish<KIND>(s, S.bp, S.d, S.own, omega);
+    const double r = sweep_finish<KIND>(s, S.bp, S.d, S.own, omega);
+    lx[S.tb + (int32_t)threadIdx.x] = r;
+    if (GSTORE) store_agent(&xdep[S.row], r);
     return;
   }
   // grouped row: every lane stores its products (slot g + kk * G at index
@@ -575,7 +614,9 @@ __device__ __forceinline__ void sweep_stage_row(const SweepStage<W> &S, double *
       s += p2.y;
     }
   }
-  lx[S.tb + ((int32_t)threadIdx.x >> S.lg)] = sweep_finish<KIND>(s, S.bp, S.d, S.own, omega);
+  const double r = sweep_finish<KIND>(s, S.bp, S.d, S.own, omega);
+  lx[S.tb + ((int32_t)threadIdx.x >> S.lg)] = r;
+  if (GSTORE) store_agent(&xdep[S.row], r);
 }
 
 // A chunk is a list of steps of at most FUSE_BS lanes: a level is cut into
@@ -590,17 +631,20 @@ __device__ __forceinline__ void sweep_stage_row(const SweepStage<W> &S, double *
 // has no exit: an exit inside it would give the loop head a path on which
 // some stages were not refilled, and the compiler would then wait for every
 // load at the head.
-template <int KIND, int W>
-__global__ __launch_bounds__(FUSE_BS) void k_sweep_lds(int32_t nsteps, int32_t t0, int32_t t1, const int4 *steps,
-                                                   const int32_t *lci, const double *vals, const double *pdv,
-                                                   const double *bp, const double *own, const int32_t *imp,
-                                                   int32_t nimp, const int32_t *prow, double *xdep, double omega) {
-  extern __shared__ double lx[];
+// The body of one chunk.  With AGENT the imports are read and the results
+// written with agent-scope atomics (sc1: served from and written through to
+// the device-wide coherence point), for chunks whose imports other
+// workgroups of the same launch produced (k_sweep_tiles).
+template <int KIND, int W, bool AGENT>
+__device__ __forceinline__ void sweep_chunk(double *lx, int32_t nsteps, int32_t t0, int32_t t1, const int4 *steps,
+                                            const int32_t *lci, const double *vals, const double *pdv,
+                                            const double *bp, const double *own, const int32_t *imp, int32_t nimp,
+                                            const int32_t *prow, double *xdep, double omega) {
   const int32_t nrows = t1 - t0;
   const int32_t moff = (nrows + nimp + 1) & ~1;
   int4 *lm = (int4 *)(lx + moff);
   double *scr = lx + moff + 2 * nsteps;
-  for (int32_t k = threadIdx.x; k < nimp; k += FUSE_BS) lx[nrows + k] = xdep[imp[k]];
+  for (int32_t k = threadIdx.x; k < nimp; k += FUSE_BS) lx[nrows + k] = AGENT ? load_agent(&xdep[imp[k]]) : xdep[imp[k]];
   for (int32_t k = threadIdx.x; k < nsteps; k += FUSE_BS) lm[k] = steps[k];
   __syncthreads();
   constexpr int D = LDS_DEPTH;
@@ -622,7 +666,162 @@ __global__ __launch_bounds__(FUSE_BS) void k_sweep_lds(int32_t nsteps, int32_t t
     }
   }
   __syncthreads();
-  for (int32_t t = t0 + threadIdx.x; t < t1; t += FUSE_BS) xdep[prow[t]] = lx[t - t0];
+  for (int32_t t = t0 + threadIdx.x; t < t1; t += FUSE_BS) {
+    if (AGENT) store_agent(&xdep[prow[t]], lx[t - t0]);
+    else xdep[prow[t]] = lx[t - t0];
+  }
+}
+
+template <int KIND, int W>
+__global__ __launch_bounds__(FUSE_BS) void k_sweep_lds(int32_t nsteps, int32_t t0, int32_t t1, const int4 *steps,
+                                                   const int32_t *lci, const double *vals, const double *pdv,
+                                                   const double *bp, const double *own, const int32_t *imp,
+                                                   int32_t nimp, const int32_t *prow, double *xdep, double omega) {
+  extern __shared__ double lx[];
+  sweep_chunk<KIND, W, false>(lx, nsteps, t0, t1, steps, lci, vals, pdv, bp, own, imp, nimp, prow, xdep, omega);
+}
+
+// ---- sentinel of the single-launch and tiled solves -----------------------------
+
+constexpr uint64_t TRSV_PENDING = 0x7FF4DEAD5EA1ED00ULL;
+constexpr long long TRSV_POLL_BUDGET = 1LL << 20;
+
+__device__ __forceinline__ uint64_t load_bits(const double *p) {
+  return __hip_atomic_load((const uint64_t *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A finished value never equals the sentinel: arithmetic produces only quiet
+// NaNs, and a value copied unchanged from the input (b in the transposed
+// unit-lower solve) that carries the sentinel's payload is stored as the
+// default quiet NaN instead, which every later operation treats the same way.
+__device__ __forceinline__ uint64_t finished_bits(double v) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  return b == TRSV_PENDING ? 0x7FF8000000000000ULL : b;
+}
+
+__global__ __launch_bounds__(BS) void k_fill_pending(int64_t n, double *x) {
+  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+    *(uint64_t *)&x[i] = TRSV_PENDING;
+}
+
+// ---- tiled sweeps ------------------------------------------------------------
+//
+// A whole sweep in one launch over tiles (see tile_levels): each tile is an
+// LDS chunk, and tiles are numbered so that every tile a tile imports from
+// has a smaller number.  The launch has one workgroup per tile; each takes
+// its tile number from a ticket counter when it starts, stages everything
+// that does not depend on other tiles (the step meta in LDS, the first
+// stages' operands in flight), waits until the tiles it imports from are
+// complete (one lane of wavefront 0 per source tile), runs the chunk, and
+// stamps the tile's flag with the sweep's epoch (so flags never need
+// clearing).  A waiting workgroup only ever waits on smaller numbers, taken
+// by workgroups that started earlier and are running or done, so the lowest
+// unfinished tile can always run and the launch cannot deadlock, whatever
+// the hardware's dispatch order.  Results and imports go through the
+// device-wide coherence point (agent-scope stores and loads), and a tile's
+// flag is written only after its result stores have completed (s_waitcnt
+// vmcnt(0) in every thread, then the barrier): the publication protocol of
+// the grid reductions in reduce.hpp.  (Polling the imported values instead,
+// with the output prefilled with the sentinel, measured slower: 1.6 against
+// 1.1 ms per 128^3 sweep, every lane of a waiting tile polling.)  A bounded
+// poll budget turns a stall into an error flag: the remaining tiles are
+// skipped (still stamped) and the host reports MTJG_E_DEVICE.  Every lane
+// loads its row index with its operands, reads b (passed as bp, in row
+// order) through it and writes its result to x as soon as it is computed,
+// so only the last step's stores are waited for before the flag.  On the
+// 128^3 convection-diffusion sweeps (per-tile timestamps): hand-off from a
+// source tile's flag to the waiting tile ~0.6 us, imports ~1.8 us, eight
+// steps ~3.5 us, final store acknowledgement ~0.5 us; a sweep 0.68 ms.
+constexpr long long TILE_POLL_BUDGET = 1LL << 22;  // polls of one dependency (about 0.5 s)
+constexpr size_t TILE_STATIC_LDS = 64;             // bytes of LDS the tile kernel declares itself
+constexpr int TILE_DEPTH = 8;                      // steps in flight in a tile (slots of at most four)
+
+template <int KIND, int W>
+__global__ __launch_bounds__(FUSE_BS) void k_sweep_tiles(int32_t ntiles, const TileMeta *tiles, const int32_t *deps,
+                                                     uint32_t *flags, uint32_t epoch, uint32_t *ctr, int32_t *err,
+                                                     const int4 *steps, const int32_t *lci, const double *vals,
+                                                     const double *pdv, const double *bp, const double *own,
+                                                     const int32_t *imp, const int32_t *prow, double *xdep,
+                                                     double omega) {
+  extern __shared__ double lx[];
+  __shared__ int32_t s_tile, s_skip;
+  // One tile per workgroup, no loop: a persistent loop whose body starts and
+  // ends with lane-0 work (ticket, flag) was restructured by the compiler
+  // into a loop that lane 0 leaves after every tile while the other lanes
+  // keep circling the barriers, which never ends.  Values read from LDS go
+  // through readfirstlane so every branch on them is uniform.
+  if (threadIdx.x == 0) s_tile = (int32_t)take_ticket(&ctr[0]);
+  __syncthreads();
+  const int32_t tile = __builtin_amdgcn_readfirstlane(s_tile);
+  if (tile < ntiles) {
+    const TileMeta tm = tiles[tile];
+    const int32_t nrows = tm.t1 - tm.t0, nsteps = tm.nsteps;
+    const int32_t moff = (nrows + tm.nimp + 1) & ~1;
+    int4 *lm = (int4 *)(lx + moff);
+    double *scr = lx + moff + 2 * nsteps;
+    for (int32_t k = threadIdx.x; k < nsteps; k += FUSE_BS) lm[k] = steps[tm.s0 + k];
+    __syncthreads();
+    // deeper than the chunks' pipeline: a tile of up to D steps has all its
+    // operands in flight before it waits (step lists are padded to a
+    // multiple of TILE_DEPTH)
+    constexpr int D = W <= 4 ? TILE_DEPTH : LDS_DEPTH;
+    SweepStage<W> st[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+      sweep_stage_load<KIND, W, true>(st[j], lm, min(j, nsteps - 1), tm.t0, lci, vals, pdv, bp, own, prow);
+    // wait for the source tiles: wavefront 0, one lane per source tile
+    if (threadIdx.x < 64) {
+      const int32_t lane = (int32_t)threadIdx.x;
+      int32_t skip = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (int32_t base = 0; base < tm.ndep && !skip; base += 64) {
+        const bool mine = base + lane < tm.ndep;
+        const int32_t dt = mine ? deps[tm.dep0 + base + lane] : 0;
+        long long budget = TILE_POLL_BUDGET;
+        for (;;) {
+          const bool wait = mine && __hip_atomic_load(&flags[dt], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch;
+          if (__builtin_amdgcn_ballot_w64(wait) == 0ULL) break;
+          if (--budget < 0) {
+            if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            skip = 1;
+            break;
+          }
+          if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            skip = 1;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+      if (lane == 0) s_skip = skip;
+    }
+    __syncthreads();
+    if (!__builtin_amdgcn_readfirstlane(s_skip)) {
+      for (int32_t k = threadIdx.x; k < tm.nimp; k += FUSE_BS)
+        lx[nrows + k] = load_agent(&xdep[imp[tm.imp0 + k]]);
+      __syncthreads();
+      // every result is written out as it is computed, so that only the
+      // last step's stores are in flight at the end; no refill is issued
+      // past the last step (its loads would be waited for at the end)
+      for (int32_t li = 0; li < nsteps; li += D) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          const int32_t bar = st[j].bar;
+          sweep_stage_row<KIND, W, true>(st[j], lx, scr, omega, xdep);
+          if (li + j + D < nsteps)
+            sweep_stage_load<KIND, W, true>(st[j], lm, li + j + D, tm.t0, lci, vals, pdv, bp, own, prow);
+          if (bar) __syncthreads();
+        }
+      }
+    }
+    stores_complete();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(&flags[tile], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // the last workgroup out leaves both counters at zero for the next launch
+  if (threadIdx.x == 0 && take_ticket(&ctr[1]) == gridDim.x - 1) {
+    __hip_atomic_store(&ctr[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&ctr[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // Per-sweep preparation of the LDS chunks: b and (SSOR) the row's own fixed
@@ -656,18 +855,6 @@ __global__ void k_sweep_prep(int32_t n, const int32_t *prow, const double *b, do
 // in storage order (pausing while an input is pending), which keeps MTJ's
 // operation order.  A bounded poll budget turns any violation of these
 // assumptions into an error flag instead of a hang.
-constexpr uint64_t TRSV_PENDING = 0x7FF4DEAD5EA1ED00ULL;
-constexpr long long TRSV_POLL_BUDGET = 1LL << 20;
-
-__device__ __forceinline__ uint64_t load_bits(const double *p) {
-  return __hip_atomic_load((const uint64_t *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__global__ __launch_bounds__(BS) void k_fill_pending(int64_t n, double *x) {
-  for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
-    *(uint64_t *)&x[i] = TRSV_PENDING;
-}
-
 template <int FORM>
 __global__ __launch_bounds__(BS) void k_trsv_syncfree(int32_t n, const int32_t *__restrict__ rows,
                                                       const int32_t *__restrict__ rp,
@@ -715,8 +902,7 @@ __global__ __launch_bounds__(BS) void k_trsv_syncfree(int32_t n, const int32_t *
           else if (FORM == TRSV_ROW_DIV) v = (b[i] - s) / a[diag[i]];
           else if (FORM == TRSV_COL_DIV) v = s / a[diag[i]];
           else v = s;
-          __hip_atomic_store((uint64_t *)&x[i], (uint64_t)__double_as_longlong(v), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store((uint64_t *)&x[i], finished_bits(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           done = true;
         } else if (--budget < 0) {
           atomicExch(err, 1);
@@ -754,6 +940,7 @@ static void build_levels(mtjg_ctx ctx, int32_t n, const std::vector<int32_t> &rp
     maxlev = std::max(maxlev, l);
   }
   (void)rp;
+  L.forward = forward;
   L.nlev = n > 0 ? maxlev + 1 : 0;
   L.ptr.assign((size_t)L.nlev + 1, 0);
   for (int32_t i = 0; i < n; ++i) L.ptr[(size_t)lev[(size_t)i] + 1]++;
@@ -783,7 +970,8 @@ __global__ void k_gather_pad(int64_t n, const int32_t *map, const double *src, d
 static void free_pack(Packed &P) {
   for (void *q : {(void *)P.prow, (void *)P.pnlo, (void *)P.pdmap, (void *)P.pdv,
                   (void *)P.desc, (void *)P.eci, (void *)P.emap, (void *)P.eval, (void *)P.lci, (void *)P.imp,
-                  (void *)P.fixslot, (void *)P.fval, (void *)P.bp, (void *)P.own, (void *)P.steps})
+                  (void *)P.fixslot, (void *)P.fval, (void *)P.bp, (void *)P.own, (void *)P.steps,
+                  (void *)P.tiles, (void *)P.tdeps, (void *)P.tflags, (void *)P.tctr})
     (void)hipFree(q);
   P = Packed();
 }
@@ -817,10 +1005,13 @@ static size_t chunk_lds_bytes(int64_t rows, int64_t nimp, int64_t nsteps, int64_
 
 // Builds (once per pattern and form) and refreshes (whenever the values
 // changed) the packed copy of level set L over the structure (rp, ci, diag),
-// whose values are `a`.
-static void ensure_pack(mtjg_prec M, Levels &L, int form, const std::vector<int32_t> &rp,
+// whose values are `a`.  With tile_lev (a tiled level set, see tile_levels)
+// every level goes to an LDS chunk and tile k's levels [tile_lev[k],
+// tile_lev[k + 1]) form chunk k, with the tiles it imports from; returns
+// false, building nothing, when a tile does not fit one chunk.
+static bool ensure_pack(mtjg_prec M, Levels &L, int form, const std::vector<int32_t> &rp,
                         const std::vector<int32_t> &ci, const std::vector<int32_t> &diag, const double *a,
-                        hipStream_t s) {
+                        hipStream_t s, const std::vector<int32_t> *tile_lev = nullptr) {
   Packed &P = L.pk;
   const int32_t n = (int32_t)L.h_rows.size();
   if (P.form != form) {
@@ -901,6 +1092,11 @@ static void ensure_pack(mtjg_prec M, Levels &L, int form, const std::vector<int3
       const int64_t lanes = lanes_of(l);
       const bool fits_lanes = w <= ROWDOT_K * g &&
                               lanes <= (g > 1 ? LDS_GROUP_LANES : w <= 4 ? LDS_LEVEL_MAX : FUSE_MAX);
+      if (tile_lev) {
+        if (!fits_lanes) return false;
+        cat[(size_t)l] = CAT_LDS;
+        continue;
+      }
       if (fits_lanes) {
         tmpimp.clear();
         ++stamp;
@@ -918,7 +1114,7 @@ static void ensure_pack(mtjg_prec M, Levels &L, int form, const std::vector<int3
     for (int32_t l = 0; l < L.nlev;) {
       int32_t e = l + 1;
       while (e < L.nlev && cat[(size_t)e] == cat[(size_t)l]) ++e;
-      if (cat[(size_t)l] == CAT_LDS && e - l >= 2)
+      if (cat[(size_t)l] == CAT_LDS && (e - l >= 2 || tile_lev))
         for (int32_t lv = l; lv < e; ++lv) in_lds[(size_t)lv] = 1;
       l = e;
     }
@@ -981,6 +1177,19 @@ static void ensure_pack(mtjg_prec M, Levels &L, int form, const std::vector<int3
     std::vector<int64_t> fixslot;
     std::vector<int4> steps;
     P.runs.clear();
+    // tiled: the tile of every position, and per tile its import sources
+    size_t tk = 0;
+    std::vector<int32_t> pos_tile, tmark, tdeps;
+    std::vector<TileMeta> tiles;
+    size_t tile_lds = 0;
+    if (tile_lev) {
+      pos_tile.assign((size_t)n, -1);
+      const int32_t nt = (int32_t)tile_lev->size() - 1;
+      for (int32_t k = 0; k < nt; ++k)
+        for (int32_t t = L.ptr[(size_t)(*tile_lev)[k]]; t < L.ptr[(size_t)(*tile_lev)[k + 1]]; ++t)
+          pos_tile[(size_t)t] = k;
+      tmark.assign((size_t)std::max(nt, 1), -1);
+    }
     for (int32_t l = 0; l < L.nlev;) {
       int32_t e = l + 1;
       while (e < L.nlev && cat[(size_t)e] == cat[(size_t)l]) ++e;
@@ -1001,14 +1210,15 @@ static void ensure_pack(mtjg_prec M, Levels &L, int form, const std::vector<int3
         int32_t rows = 0, nst = 0;
         bool grouped = false;
         ++stamp;
-        while (l < e) {
+        const int32_t lend = tile_lev ? (*tile_lev)[tk + 1] : e;
+        while (l < lend) {
           const int32_t cnt = cnt_of(l);
           const size_t before = imps.size();
           new_operands(l, t0, imps);
           const bool gr = grouped || grp[(size_t)l] > 1;
-          if (l > c0 && chunk_lds_bytes((int64_t)rows + cnt, (int64_t)(imps.size() - i0),
-                                        nst + nsteps_of(lanes_of(l)) + LDS_DEPTH - 1,
-                                        gr ? SCRATCH_MAX : 0) > cap_bytes) {
+          if (!tile_lev && l > c0 &&
+              chunk_lds_bytes((int64_t)rows + cnt, (int64_t)(imps.size() - i0),
+                              nst + nsteps_of(lanes_of(l)) + LDS_DEPTH - 1, gr ? SCRATCH_MAX : 0) > cap_bytes) {
             imps.resize(before);  // the level starts the next chunk
             break;
           }
@@ -1016,6 +1226,28 @@ static void ensure_pack(mtjg_prec M, Levels &L, int form, const std::vector<int3
           nst += nsteps_of(lanes_of(l));
           grouped = gr;
           ++l;
+        }
+        if (tile_lev) {
+          const size_t need = chunk_lds_bytes(rows, (int64_t)(imps.size() - i0), nst + TILE_DEPTH - 1,
+                                              grouped ? SCRATCH_MAX : 0);
+          if (need + TILE_STATIC_LDS > cap_bytes) return false;
+          tile_lds = std::max(tile_lds, chunk_lds_bytes(rows, (int64_t)(imps.size() - i0), nst + TILE_DEPTH - 1, 0));
+          P.tile_grouped |= grouped;
+          // the tiles this one imports from, each once; every one must be an
+          // earlier tile (what keeps the launch free of deadlock)
+          TileMeta tm{};
+          tm.dep0 = (int32_t)tdeps.size();
+          for (size_t q = i0; q < imps.size(); ++q) {
+            const int32_t src = pos_tile[(size_t)pos[(size_t)imps[q]]];
+            if (src < 0 || src >= (int32_t)tk) fail(MTJG_E_DEVICE, "tiled sweep: an import outside the earlier tiles");
+            if (tmark[(size_t)src] != (int32_t)tk) {
+              tmark[(size_t)src] = (int32_t)tk;
+              tdeps.push_back(src);
+            }
+          }
+          tm.ndep = (int32_t)tdeps.size() - tm.dep0;
+          tiles.push_back(tm);
+          ++tk;
         }
         LevelRun c{c0, l, true, true};
         c.lds = true;
@@ -1041,7 +1273,7 @@ static void ensure_pack(mtjg_prec M, Levels &L, int form, const std::vector<int3
           }
         }
         // padding steps: the last step's loads again, flagged to compute nothing
-        while (((int64_t)steps.size() - c.s0) % LDS_DEPTH) {
+        while (((int64_t)steps.size() - c.s0) % (tile_lev ? TILE_DEPTH : LDS_DEPTH)) {
           int4 pad = steps.back();
           pad.y = (pad.y & ~(1 << 24)) | (1 << 25);
           steps.push_back(pad);
@@ -1064,8 +1296,29 @@ static void ensure_pack(mtjg_prec M, Levels &L, int form, const std::vector<int3
             }
           }
         for (size_t q = i0; q < imps.size(); ++q) impidx[(size_t)imps[q]] = -1;
+        if (tile_lev) {
+          TileMeta &tm = tiles.back();
+          tm.nsteps = c.nsteps;
+          tm.t0 = c.t0;
+          tm.t1 = c.t1;
+          tm.nimp = c.nimp;
+          tm.s0 = c.s0;
+          tm.imp0 = c.imp0;
+          P.tile_wmax = std::max(P.tile_wmax, c.wmax);
+        }
         P.runs.push_back(c);
       }
+    }
+    if (tile_lev) {
+      P.ntiles = (int32_t)tiles.size();
+      P.tile_lds = tile_lds;
+      P.tiles = to_device(tiles, s);
+      P.tdeps = to_device(tdeps, s);
+      HIPCHK(hipMalloc(&P.tflags, sizeof(uint32_t) * (size_t)std::max(P.ntiles, 1)));
+      HIPCHK(hipMemsetAsync(P.tflags, 0, sizeof(uint32_t) * (size_t)std::max(P.ntiles, 1), s));
+      HIPCHK(hipMalloc(&P.tctr, sizeof(uint32_t) * 2));
+      HIPCHK(hipMemsetAsync(P.tctr, 0, sizeof(uint32_t) * 2, s));
+      P.epoch = 0;
     }
     P.any_lds = false;
     for (const LevelRun &r : P.runs) P.any_lds |= r.lds;
@@ -1100,6 +1353,7 @@ static void ensure_pack(mtjg_prec M, Levels &L, int form, const std::vector<int3
     HIPCHK(hipGetLastError());
     P.version = M->vals_version;
   }
+  return true;
 }
 
 template <int KIND, int W>
@@ -1125,10 +1379,130 @@ static void launch_sweep_lds(const LevelRun &r, const Packed &P, const double *v
   else launch_sweep_lds_w<KIND, ROWDOT_K>(r, P, vals, xdep, omega, s);
 }
 
+// Cuts level set L into tiles: rows by their sweep index (r in a forward
+// set, n - 1 - r in a backward one) into chunks of R rows, and by level into
+// windows of W levels; tile (w, c) holds window w's rows of chunk c.  Every
+// row a row reads has a smaller level and a smaller sweep index, so it lies
+// in the same tile at an earlier level or in a tile (w', c') with w' <= w and
+// c' <= c.  Tiles are ordered by w + c, then c, which puts every tile after
+// the tiles it reads.  T receives the rows in that order as levels of its own
+// (one tile's rows of one level each, rows ascending, as in L), and
+// T.tile_lev the first of each tile's levels.
+static void tile_levels(const Levels &L, int32_t R, int32_t W, Levels &T) {
+  const int32_t n = (int32_t)L.h_rows.size();
+  const int32_t nc = (n + R - 1) / R, nw = (L.nlev + W - 1) / W;
+  T.forward = L.forward;
+  T.nlev = 0;
+  T.ptr.assign(1, 0);
+  T.h_rows.clear();
+  T.h_rows.reserve((size_t)n);
+  T.tile_lev.clear();
+  for (int32_t d = 0; d <= nw + nc - 2; ++d) {
+    for (int32_t c = std::max(0, d - (nw - 1)); c <= std::min(d, nc - 1); ++c) {
+      const int32_t w = d - c;
+      const int32_t first = T.nlev;
+      // rows of chunk c: [lo, hi) in row index
+      const int32_t lo = L.forward ? c * R : std::max(0, n - (c + 1) * R);
+      const int32_t hi = L.forward ? std::min(n, (c + 1) * R) : n - c * R;
+      for (int32_t l = w * W; l < std::min((w + 1) * W, L.nlev); ++l) {
+        const int32_t *b = L.h_rows.data() + L.ptr[(size_t)l], *e = L.h_rows.data() + L.ptr[(size_t)l + 1];
+        const int32_t *p0 = std::lower_bound(b, e, lo), *p1 = std::lower_bound(p0, e, hi);
+        if (p0 == p1) continue;
+        T.h_rows.insert(T.h_rows.end(), p0, p1);
+        T.ptr.push_back((int32_t)T.h_rows.size());
+        ++T.nlev;
+      }
+      if (T.nlev > first) T.tile_lev.push_back(first);
+    }
+  }
+  T.tile_lev.push_back(T.nlev);
+  if ((int32_t)T.h_rows.size() != n) fail(MTJG_E_DEVICE, "tiled sweep: tiles do not cover the level set");
+}
+
+static int32_t env_int(const char *name, int32_t dflt) {
+  const char *e = getenv(name);
+  return e && *e ? (int32_t)atoi(e) : dflt;
+}
+
+static void free_levels(Levels &L);
+
+// Whether sweeps over L run tiled (k_sweep_tiles).  By default they do when
+// some level is wider than an LDS chunk takes, so that the untiled plan would
+// launch levels one by one; MTJG_SWEEP_TILES=0 or 1 forces the choice.
+// Tiles are R rows by W levels: R = n / ceil(widest level / 384), so that a
+// tile's share of the widest level is about 384 rows (under one 512-lane
+// step), and W = 8 (MTJG_TILE_ROWS and MTJG_TILE_LEVELS override them).  On
+// the 128^3 convection-diffusion matrix this gives 4 z-planes by 8 levels.
+// Falls back to the untiled plan when a tile does not fit one LDS chunk.
+static void decide_tiling(mtjg_prec M, Levels &L, int form, const std::vector<int32_t> &rp,
+                          const std::vector<int32_t> &ci, const std::vector<int32_t> &diag, const double *a,
+                          hipStream_t s) {
+  L.tiling = -1;
+  const int32_t n = (int32_t)L.h_rows.size();
+  int32_t maxw = 0;
+  for (int32_t l = 0; l < L.nlev; ++l) maxw = std::max(maxw, L.ptr[(size_t)l + 1] - L.ptr[(size_t)l]);
+  const int32_t mode = env_int("MTJG_SWEEP_TILES", -1);
+  if (n == 0 || mode == 0 || (mode < 0 && maxw <= LDS_LEVEL_MAX)) return;
+  const int32_t nc = std::max(1, (maxw + 383) / 384);
+  const int32_t R = std::max(1, env_int("MTJG_TILE_ROWS", (int32_t)(((int64_t)n + nc - 1) / nc)));
+  const int32_t W = std::max(1, env_int("MTJG_TILE_LEVELS", 8));
+  auto *T = new Levels();
+  tile_levels(L, R, W, *T);
+  const size_t cap = (size_t)lds_cap_doubles() * sizeof(double) - TILE_STATIC_LDS;
+  if (!ensure_pack(M, *T, form, rp, ci, diag, a, s, &T->tile_lev) ||
+      T->pk.tile_lds + (T->pk.tile_grouped ? sizeof(double) * FUSE_BS * ROWDOT_K : 0) > cap) {
+    free_levels(*T);
+    delete T;
+    return;
+  }
+  L.tiled = T;
+  L.tiling = 1;
+}
+
+template <int KIND, int W>
+static void launch_sweep_tiles_w(Packed &P, const double *vals, const double *b, double *xdep, double omega,
+                                 int32_t *err, hipStream_t s) {
+  static bool attr_set = false;  // per instantiation: allow LDS beyond the 64 KiB default
+  if (!attr_set) {
+    HIPCHK(hipFuncSetAttribute((const void *)k_sweep_tiles<KIND, W>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               lds_cap_doubles() * (int)sizeof(double) - (int)TILE_STATIC_LDS));
+    attr_set = true;
+  }
+  const size_t bytes = P.tile_lds + (P.tile_grouped ? sizeof(double) * FUSE_BS * W : 0);
+  const int grid = P.ntiles;
+  if (++P.epoch == 0) {  // flags hold epochs; after 2^32 sweeps start again from clear flags
+    HIPCHK(hipMemsetAsync(P.tflags, 0, sizeof(uint32_t) * (size_t)std::max(P.ntiles, 1), s));
+    P.epoch = 1;
+  }
+  k_sweep_tiles<KIND, W><<<grid, FUSE_BS, bytes, s>>>(P.ntiles, P.tiles, P.tdeps, P.tflags, P.epoch, P.tctr, err,
+                                                      P.steps, P.lci, vals, P.pdv, b, P.own, P.imp, P.prow, xdep,
+                                                      omega);
+}
+
+// One tiled sweep, W slot registers for the widest lane of any tile.
+// b: the right-hand side in row order.
+template <int KIND>
+static void launch_sweep_tiles(Packed &P, const double *vals, const double *b, double *xdep, double omega,
+                               int32_t *err, hipStream_t s) {
+  if (P.ntiles == 0) return;
+  if (P.tile_wmax <= 2) launch_sweep_tiles_w<KIND, 2>(P, vals, b, xdep, omega, err, s);
+  else if (P.tile_wmax <= 4) launch_sweep_tiles_w<KIND, 4>(P, vals, b, xdep, omega, err, s);
+  else launch_sweep_tiles_w<KIND, ROWDOT_K>(P, vals, b, xdep, omega, err, s);
+}
+
 template <int FORM>
 static void run_levels(mtjg_prec M, Levels &L, hipStream_t s, const std::vector<int32_t> &rp,
                        const std::vector<int32_t> &ci, const std::vector<int32_t> &diag, const double *a,
                        const double *b, double *x) {
+  if (L.tiling == 0) decide_tiling(M, L, FORM, rp, ci, diag, a, s);
+  if (L.tiling > 0) {
+    Levels &T = *L.tiled;
+    ensure_pack(M, T, FORM, rp, ci, diag, a, s, &T.tile_lev);
+    Packed &P = T.pk;
+    launch_sweep_tiles<FORM>(P, P.eval, b, x, 1.0, M->sf_err, s);
+    HIPCHK(hipGetLastError());
+    return;
+  }
   ensure_pack(M, L, FORM, rp, ci, diag, a, s);
   const Packed &P = L.pk;
   if (P.any_lds)
@@ -1217,6 +1591,12 @@ static void run_ssor(mtjg_prec M, hipStream_t s, const double *b, double *x) {
 }
 
 static void free_levels(Levels &L) {
+  if (L.tiled) {
+    free_levels(*L.tiled);
+    delete L.tiled;
+    L.tiled = nullptr;
+  }
+  L.tiling = 0;
   if (L.rows) (void)hipFree(L.rows);
   if (L.dptr) (void)hipFree(L.dptr);
   L.rows = nullptr;
