@@ -597,6 +597,13 @@ __device__ __forceinline__ void sweep_stage_row(const SweepStage<W> &S, double *
     else if (c == -2) p = S.v[kk];
     pr[g + kk * G] = p;
   }
+  // the row's first lane reads the other lanes' products: a wavefront-scope
+  // release/acquire around a wave barrier orders those LDS stores before its
+  // reads (it generates no instruction for LDS within one wavefront, but
+  // keeps the compiler from sinking the stores into the lanes' exit path)
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   if (g != 0) return;
   // products read two at a time (16-byte LDS reads: the row's scratch and
   // every even slot are 16-byte aligned, and S.w * G is even since G >= 2):
