@@ -73,6 +73,25 @@ def measured_traffic(workload):
     return (int(m.group(1)), os.path.relpath(path, ROOT)) if m else None
 
 
+def cpu_info():
+    """The host the CPU baselines ran on: logical CPUs visible to this
+    process, the machine's count, and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        visible = len(os.sched_getaffinity(0))
+    except AttributeError:
+        visible = os.cpu_count()
+    return {"nproc": visible, "machine_cpus": os.cpu_count(), "cpu_model": model}
+
+
 def cpu_baseline(sysm, seconds):
     """Oracle restatement of MTJ's loop (one core) on the same matrix."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -91,10 +110,12 @@ def cpu_baseline(sysm, seconds):
         if el >= seconds or calls >= 1000:
             break
     per = el / calls
-    return {"value": round(P.spmv_bytes(n, len(va)) / per / 1e9, 3), "unit": "GB/s", "cores": 1,
-            "kind": "port",
-            "sample": "%d oracle CompRowMatrix.mult calls on the same matrix (%.1f s, %.3f s/call)"
-                      % (calls, el, per)}
+    out = {"value": round(P.spmv_bytes(n, len(va)) / per / 1e9, 3), "unit": "GB/s", "cores": 1,
+           "kind": "port",
+           "sample": "%d oracle CompRowMatrix.mult calls on the same matrix (%.1f s, %.3f s/call)"
+                     % (calls, el, per)}
+    out.update(cpu_info())
+    return out
 
 
 def copy_calibration(steps):
@@ -137,6 +158,10 @@ SOLVE_WORKLOADS = {
                                "cd", "gmres", "ilu", 1e-8, False),
     "mixed_varcoef_192": ("3D variable-kappa Poisson 192^3, FP32 inner Jacobi-CG (1e-4) under FP64 refinement to 1e-10",
                           "vk", "mixed", "diag", 1e-10, False),
+    "ir_cg32_varcoef_192": ("3D variable-kappa Poisson 192^3, IR with the FP32 inner Jacobi-CG preconditioner (1e-4) "
+                            "to 1e-10", "vk", "ir", "cg32", 1e-10, False),
+    "cg_powerlaw_spd_4m": ("random SPD CSR n=4M, S = B + B^T + diagonal dominance over the power-law B "
+                           "(alpha 2.1, 1-2000), Jacobi-CG to rtol 1e-8", "pls", "cg", "diag", 1e-8, False),
     "cg_varcoef_192": ("3D variable-kappa Poisson 192^3, FP64 Jacobi-CG to 1e-10 (oracle of the mixed solve)",
                        "vk", "cg", "diag", 1e-10, False),
     "bicg_ilu_convdiff_64": ("3D upwind convection-diffusion 64^3 (Pe 10), BiCG + ILU(0) to rtol 1e-8",
@@ -156,7 +181,70 @@ def _solve_system(gen):
     return {"p2d": lambda: P.poisson2d(256), "p3d": lambda: P.poisson3d(256),
             "cd": lambda: P.convdiff3d(128, peclet=10.0, seed=1),
             "cd64": lambda: P.convdiff3d(64, peclet=10.0, seed=1),
-            "vk": lambda: P.varcoef_poisson(3, 192, seed=1)}[gen]()
+            "vk": lambda: P.varcoef_poisson(3, 192, seed=1),
+            "pls": lambda: P.powerlaw_spd(4_000_000, seed=0)}[gen]()
+
+
+# Full-size oracle solves recorded in tests/golden by make_fullsize_fixtures.py
+# (the reference solution of each workload; the mixed-precision ones compare
+# with the FP64 Jacobi-CG solve of the same system).
+FIXTURE_OF = {"cg_poisson3d_256": "cg_poisson3d_256", "cg_powerlaw_spd_4m": "cg_powerlaw_spd_4m",
+              "bicgstab_ilu_convdiff_128": "bicgstab_ilu_convdiff_128",
+              "gmres_ilu_convdiff_128": "gmres_ilu_convdiff_128", "cg_varcoef_192": "cg_varcoef_192",
+              "mixed_varcoef_192": "cg_varcoef_192", "ir_cg32_varcoef_192": "cg_varcoef_192"}
+FIXTURE_FIELDS = ("iterations", "code", "residual", "b_norm", "x_norm", "err_norm", "true_rel_residual", "n",
+                  "nnz", "cpu_s", "rtol", "restart")
+
+
+def load_fixture(name):
+    path = os.path.join(ROOT, "tests", "golden", "fullsize_%s.npz" % name)
+    if not os.path.exists(path):
+        return None
+    with np.load(path, allow_pickle=False) as f:
+        out = dict(zip(FIXTURE_FIELDS, f["meta"].tolist()))
+        out["idx"], out["x_samples"] = f["idx"], f["x_samples"]
+    return out
+
+
+def oracle_solve_sample(name, sysm, bh, iterations_full, budget_s):
+    """The oracle's one-core solve of the same system: in full when its
+    recorded time fits the budget, else the first k iterations (k sized from
+    the recorded time per iteration), extrapolated to the full count."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    desc, gen, method, prec, rtol, _ = SOLVE_WORKLOADS[name]
+    fx = load_fixture(FIXTURE_OF.get(name, name))
+    O = oracle.Csr(*sysm)
+    n = sysm[0]
+    if method in ("mixed", "ir"):
+        # the mixed-precision schemes are measured against the FP64 Jacobi-CG
+        # solve of the same system
+        method, prec = "cg", "diag"
+    kind = {"diag": oracle.PREC_DIAGONAL, "ilu": oracle.PREC_ILU}[prec]
+    t0 = time.perf_counter()
+    op = oracle.Precond(kind, O)
+    setup = time.perf_counter() - t0
+    per_it = fx["cpu_s"] / max(fx["iterations"], 1) if fx else None
+    k = iterations_full
+    if per_it is not None and per_it * iterations_full > budget_s:
+        k = max(1, int(budget_s / per_it))
+        if method == "gmres":
+            k = max(31, k // 31 * 31)  # whole restart cycles (30 Arnoldi steps + the restart)
+    t0 = time.perf_counter()
+    code, xo, it_o, _ = oracle.solve(method, O, bh, np.zeros(n), op, oracle.monitor(k, rtol))
+    dt = time.perf_counter() - t0
+    full = it_o < k or code == oracle.OK
+    out = {"kind": "port", "cores": 1, "method": method + "+" + prec, "setup_s": round(setup, 3),
+           "iterations_timed": int(it_o), "seconds": round(dt, 3)}
+    if full:
+        out.update({"solve_s": round(dt, 3), "iterations": int(it_o), "sample": "the whole solve"})
+    else:
+        out.update({"s_per_iteration": round(dt / max(it_o, 1), 5),
+                    "solve_s_est": round(dt / max(it_o, 1) * iterations_full, 2),
+                    "iterations": int(iterations_full),
+                    "sample": "the first %d of %d iterations, extrapolated" % (it_o, iterations_full)})
+    out.update(cpu_info())
+    return out
 
 
 def _mg_gpu(M, Mp, A, b, x, bh, rtol, max_cycles=200):
@@ -182,7 +270,7 @@ def _mg_oracle(oracle, O, g, bh, rtol, max_cycles=200):
     return max_cycles, x
 
 
-def solve_mode(name, repeats):
+def solve_mode(name, repeats, cpu_budget=0.0):
     """Times one solve of a named configuration: x_true ~ U[-1,1) (seed 1),
     b = A x_true, x0 = 0.  Reports iterations, device time, true relative
     residual and error, and (where it runs in seconds) the oracle's solve."""
@@ -201,11 +289,11 @@ def solve_mode(name, repeats):
     x = M.DenseVector(n, ctx)
     S = None
     if method != "mg":
-        cls = {"cg": M.CG, "bicgstab": M.BiCGstab, "gmres": M.GMRES, "mixed": M.MixedCG, "bicg": M.BiCG,
+        cls = {"cg": M.CG, "bicgstab": M.BiCGstab, "gmres": M.GMRES, "mixed": M.MixedCG, "ir": M.IR, "bicg": M.BiCG,
                "cgs": M.CGS, "qmr": M.QMR}[method]
         S = cls(x)
     Mp = {"diag": lambda: M.DiagonalPreconditioner(n, ctx), "ilu": lambda: M.ILU(A), "icc": lambda: M.ICC(A),
-          "amg": lambda: M.AMG(ctx=ctx)}[prec]()
+          "amg": lambda: M.AMG(ctx=ctx), "cg32": lambda: M.FP32CGPreconditioner(n, inner_rtol=1e-4, ctx=ctx)}[prec]()
     t0 = time.perf_counter()
     Mp.setMatrix(A)
     ctx.synchronize()
@@ -213,7 +301,7 @@ def solve_mode(name, repeats):
     if S is not None:
         if method != "mixed":
             S.setPreconditioner(Mp)
-        S.setIterationMonitor(M.DefaultIterationMonitor(100000 if method != "mixed" else 100, rtol))
+        S.setIterationMonitor(M.DefaultIterationMonitor(100 if method == "mixed" or prec == "cg32" else 100000, rtol))
     times = []
     for _ in range(repeats):
         x.zero()
@@ -235,6 +323,19 @@ def solve_mode(name, repeats):
            "setup_s": round(setup, 4),
            "true_rel_residual": float(np.linalg.norm(bh - Ah @ xs) / np.linalg.norm(bh)),
            "rel_error_vs_x_true": float(np.linalg.norm(xs - x_true) / np.linalg.norm(x_true))}
+    if prec == "cg32":
+        out["inner_iterations"] = Mp.inner_iterations() // max(repeats, 1)
+    fx = load_fixture(FIXTURE_OF.get(name, ""))
+    if fx is not None:
+        xt = float(np.linalg.norm(x_true))
+        out["oracle_fixture"] = {
+            "workload": FIXTURE_OF[name], "iterations": int(fx["iterations"]),
+            "true_rel_residual": fx["true_rel_residual"], "rel_err_vs_x_true": fx["err_norm"] / xt,
+            "rel_diff_x_sampled": float(np.linalg.norm(xs[fx["idx"]] - fx["x_samples"]) /
+                                        np.linalg.norm(fx["x_samples"])),
+            "cpu_s_build_container": round(fx["cpu_s"], 2)}
+        if cpu_budget > 0:
+            out["cpu_baseline"] = oracle_solve_sample(name, sysm, bh, int(fx["iterations"]), cpu_budget)
     if method == "mixed":
         out["inner_iterations"] = S.inner_iterations()
         # the FP64 reference: Jacobi-CG on the device, same system and tolerance
@@ -445,7 +546,7 @@ def main():
     ap.add_argument("--prec-workload", default="p2d_256", choices=sorted(PRECOND_WORKLOADS))
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--requests", type=int, default=512)
-    ap.add_argument("--serve-requests", type=int, default=256,
+    ap.add_argument("--serve-requests", type=int, default=8192,
                     help="requests of the served-vs-plain comparison attached to the SpMV line (0: none)")
     ap.add_argument("--lam", type=float, default=2000.0)
     ap.add_argument("--no-tune", action="store_true",
@@ -461,7 +562,7 @@ def main():
     if args.mode == "copy":
         return copy_calibration(args.steps)
     if args.mode == "solve":
-        return solve_mode(args.solve, args.repeats)
+        return solve_mode(args.solve, args.repeats, 0.0 if args.no_cpu else args.cpu_seconds * 2)
     if args.mode == "serve":
         return serve_mode(args.requests, args.lam, args.seed, not args.no_tune)
     if args.mode == "precond":
@@ -557,13 +658,20 @@ def serve_summary(ctx, count, lam, seed, tune=True):
     from mtjgpu import serving as S
     best, tinfo = tune_batch(ctx, lam, seed, tune)
     reqs = S.request_stream(count, lam=lam, seed=seed)
+    t0 = time.perf_counter()
     systems = [S.make_system(r) for r in reqs]
+    print("serve: %d systems generated in %.1f s" % (count, time.perf_counter() - t0), file=sys.stderr, flush=True)
     st_b, res_b = S.serve_batched(ctx, reqs, systems, **best)
+    print("serve: batched %.1f solves/s" % st_b["solves_per_s"], file=sys.stderr, flush=True)
     st_p, res_p = S.serve_plain(ctx, reqs, systems)
+    print("serve: plain %.1f solves/s" % st_p["solves_per_s"], file=sys.stderr, flush=True)
     worst = max(S.true_residual(systems[k], res_b[k]["x"]) for k in range(count))
-    return {"requests": count, "lambda_per_s": lam, "rtol": 1e-8,
+    return {"requests": count, "lambda_per_s": lam, "rtol": 1e-8, "sum_n": int(sum(s[0] for s in systems)),
             "batched_solves_per_s": round(st_b["solves_per_s"], 2), "batched_p50_ms": round(st_b["p50_ms"], 1),
             "batched_p99_ms": round(st_b["p99_ms"], 1),
+            "batched_gpu_busy_frac": round(st_b["busy_frac"], 3),
+            "batched_stream_GBps_span": round(st_b["stream_GBps_span"], 1),
+            "batched_stream_GBps_busy": round(st_b["stream_GBps_busy"], 1),
             "plain_solves_per_s": round(st_p["solves_per_s"], 2), "plain_p50_ms": round(st_p["p50_ms"], 1),
             "plain_p99_ms": round(st_p["p99_ms"], 1),
             "speedup": round(st_b["solves_per_s"] / st_p["solves_per_s"], 3),

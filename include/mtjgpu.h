@@ -357,13 +357,24 @@ int mtjg_batch_create(mtjg_ctx ctx, int32_t max_slots, int32_t max_workgroups,
                       int64_t arena_bytes, int32_t rows_per_workgroup,
                       int32_t iters_per_step, mtjg_batch *out);
 int mtjg_batch_destroy(mtjg_batch B);
-/* Queues A x = b (CSR copied on the host; x0 = 0); *id names the request. */
+/* Queues A x = b (CSR copied on the host; x0 = 0); *id names the request.
+ * The CSR structure is checked (row pointers 0..nnz non-decreasing, columns
+ * in range, a stored nonzero diagonal in every row), in parallel on the
+ * batch's host threads for large systems; the error names the lowest failing
+ * row, as a serial check would. */
 int mtjg_batch_submit(mtjg_batch B, int32_t n, int64_t nnz,
                       const int32_t *row_pointer, const int32_t *column_index,
                       const double *data, const double *b, double rtol,
                       int32_t max_outer, double inner_rtol, int32_t max_inner,
                       int64_t *id);
-/* Admits what fits, runs one step, retires finished systems. */
+/* Queues A x = b (CSR copied on the host; x0 = 0); *id names the request.
+ * The CSR structure is checked as the single-system entry points check it
+ * (row pointers 0..nnz non-decreasing, columns in range, a stored nonzero
+ * diagonal in every row), in parallel on the batch's host threads for large
+ * systems; the error names the lowest failing row. *//* Admits what fits, runs one step, retires finished systems.  A system whose
+ * step finished it is reported completed by the next call (its solution is
+ * downloaded while that step's successor runs); *n_active counts admitted
+ * requests not yet reported completed. */
 int mtjg_batch_step(mtjg_batch B, int32_t *n_active, int32_t *n_completed);
 /* Collects a finished request (x may be NULL); returns its status
  * (MTJG_OK or MTJG_NOT_CONVERGED_*); iterations = refinement steps. */
@@ -375,6 +386,14 @@ int mtjg_batch_completed(mtjg_batch B, int64_t *ids, int32_t max_ids,
                          int32_t *count);
 int mtjg_batch_info(mtjg_batch B, int32_t *queued, int32_t *active,
                     int64_t *arena_used, int64_t *arena_size);
+/* Device time of the replayed graphs so far (HIP events around each step's
+ * graph, counted once its snapshot has been consumed), the steps run, and
+ * (host_ms[8], may be NULL) the host time (ms) spent in: submit, admission,
+ * waiting for step snapshots, retiring finished systems, retiring's wait for
+ * solution downloads, admission's wait for uploads, publishing work tables,
+ * launching steps. */
+int mtjg_batch_stats(mtjg_batch B, double *busy_ms, int64_t *steps,
+                     double *host_ms);
 
 /* ------------------------------------------------------------------------
  * Matrix Market input (host side).  A parsed file is held in an mtjg_mm

@@ -33,10 +33,17 @@
 // slot records and the new work table are copied on the main stream after
 // the running graph, before the next one.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "mtjg_internal.hpp"
@@ -293,6 +300,7 @@ struct Arena {
   std::vector<Chunk> chunks;
   bool fixed = false;
   size_t size = 0, used = 0;
+  size_t total = 0;  // the device's memory (hipMemGetInfo at creation)
   static constexpr size_t CHUNK = (size_t)4 << 30;
 
   void add_chunk(char *base, size_t bytes) {
@@ -312,8 +320,6 @@ struct Arena {
   // Largest system footprint the arena can ever hold.
   size_t capacity() const {
     if (fixed) return size;
-    size_t fr = 0, total = 0;
-    HIPCHK(hipMemGetInfo(&fr, &total));
     const size_t res = reserve(total);
     return total > res ? total - res : 0;
   }
@@ -333,10 +339,10 @@ struct Arena {
       }
     }
     if (fixed) return nullptr;
-    size_t fr = 0, total = 0;
-    HIPCHK(hipMemGetInfo(&fr, &total));
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
     const size_t want = std::max(CHUNK, (bytes + ((size_t)256 << 20) - 1) & ~(((size_t)256 << 20) - 1));
-    if (fr < want + reserve(total)) return nullptr;
+    if (fr < want + reserve(tot)) return nullptr;
     char *base = nullptr;
     if (hipMalloc(&base, want) != hipSuccess) {
       (void)hipGetLastError();
@@ -429,11 +435,179 @@ void copy_checked(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, 
   HIPCHK(hipMemcpyAsync(dst, src, bytes, kind, s));
 }
 
+// Host memory of queued requests and finished solutions: blocks recycled
+// through a pool, so that a stream of systems keeps reusing memory it has
+// already touched instead of taking fresh pages (a page fault per 4 KiB) for
+// every request.  Blocks are taken and returned by the API calls only (not by
+// the workers).
+class HostPool;
+struct HostBlock {
+  HostPool *pool = nullptr;
+  char *p = nullptr;
+  size_t cap = 0;
+  HostBlock() = default;
+  HostBlock(HostPool *pl, char *q, size_t c) : pool(pl), p(q), cap(c) {}
+  HostBlock(HostBlock &&o) noexcept : pool(o.pool), p(o.p), cap(o.cap) { o.p = nullptr; }
+  HostBlock &operator=(HostBlock &&o) noexcept {
+    if (this != &o) {
+      reset();
+      pool = o.pool;
+      p = o.p;
+      cap = o.cap;
+      o.p = nullptr;
+    }
+    return *this;
+  }
+  HostBlock(const HostBlock &) = delete;
+  HostBlock &operator=(const HostBlock &) = delete;
+  ~HostBlock() { reset(); }
+  inline void reset();
+};
+
+class HostPool {
+ public:
+  // spare blocks kept for reuse, at most this many bytes
+  static constexpr size_t KEEP = (size_t)4 << 30;
+  ~HostPool() {
+    for (auto &kv : free_) std::free(kv.second);
+  }
+  // A block of at least `bytes` bytes: a spare one at most twice that size,
+  // else a new one (sizes rounded to a quarter of their power of two).
+  HostBlock get(size_t bytes) {
+    bytes = std::max<size_t>(bytes, 4096);
+    auto it = free_.lower_bound(bytes);
+    if (it != free_.end() && it->first <= 2 * bytes) {
+      HostBlock b(this, it->second, it->first);
+      spare_ -= it->first;
+      free_.erase(it);
+      return b;
+    }
+    size_t q = (size_t)1 << (63 - __builtin_clzll(bytes));
+    q = std::max<size_t>(q / 4, 4096);
+    const size_t cap = (bytes + q - 1) / q * q;
+    char *p = (char *)std::malloc(cap);
+    if (!p) fail(MTJG_E_OUT_OF_MEMORY, "batch: host memory for a request");
+    return HostBlock(this, p, cap);
+  }
+  void put(char *p, size_t cap) {
+    if (spare_ + cap > KEEP) {
+      std::free(p);
+      return;
+    }
+    free_.emplace(cap, p);
+    spare_ += cap;
+  }
+
+ private:
+  std::multimap<size_t, char *> free_;
+  size_t spare_ = 0;
+};
+
+inline void HostBlock::reset() {
+  if (p) pool->put(p, cap);
+  p = nullptr;
+}
+
+// A few worker threads for the host side of large requests: checking and
+// copying a system's arrays at submit, packing them into pinned staging at
+// admission, copying solutions out at retirement.  run(parts, f) calls
+// f(0 .. parts - 1) on the workers and the calling thread and returns when
+// all have finished; f must not throw.
+class Workers {
+ public:
+  explicit Workers(int n) {
+    for (int i = 0; i < n; ++i) th_.emplace_back([this] { loop(); });
+  }
+  ~Workers() {
+    {
+      std::lock_guard<std::mutex> l(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : th_) t.join();
+  }
+  int size() const { return (int)th_.size() + 1; }
+  void run(int parts, const std::function<void(int)> &f) {
+    if (parts <= 1 || th_.empty()) {
+      for (int p = 0; p < parts; ++p) f(p);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> l(mu_);
+      job_ = &f;
+      parts_ = parts;
+      next_ = 0;
+      left_ = parts;
+      ++gen_;
+    }
+    cv_.notify_all();
+    work();
+    std::unique_lock<std::mutex> l(mu_);
+    done_.wait(l, [this] { return left_ == 0; });
+    job_ = nullptr;
+  }
+
+ private:
+  void work() {
+    for (;;) {
+      int p;
+      const std::function<void(int)> *f;
+      {
+        std::lock_guard<std::mutex> l(mu_);
+        if (!job_ || next_ >= parts_) return;
+        p = next_++;
+        f = job_;
+      }
+      (*f)(p);
+      {
+        std::lock_guard<std::mutex> l(mu_);
+        if (--left_ == 0) done_.notify_all();
+      }
+    }
+  }
+  void loop() {
+    uint64_t seen = 0;
+    std::unique_lock<std::mutex> l(mu_);
+    for (;;) {
+      cv_.wait(l, [&] { return stop_ || gen_ != seen; });
+      if (stop_) return;
+      seen = gen_;
+      l.unlock();
+      work();
+      l.lock();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(int)> *job_ = nullptr;
+  int parts_ = 0, next_ = 0, left_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
+
+// memcpy split over the workers when large (parts of at least 2 MiB).
+void par_copy(Workers &W, void *dst, const void *src, size_t bytes) {
+  constexpr size_t PART = (size_t)2 << 20;
+  const int parts = (int)std::min<size_t>((size_t)W.size(), bytes / PART);
+  if (parts <= 1) {
+    if (bytes) memcpy(dst, src, bytes);
+    return;
+  }
+  const size_t step = ((bytes + parts - 1) / parts + 63) & ~(size_t)63;
+  W.run(parts, [&](int p) {
+    const size_t lo = std::min(bytes, (size_t)p * step), hi = std::min(bytes, lo + step);
+    if (hi > lo) memcpy((char *)dst + lo, (const char *)src + lo, hi - lo);
+  });
+}
+
 struct Request {
   int64_t id;
   int32_t n;
-  std::vector<int32_t> rp, ci;
-  std::vector<double> val, b;
+  int64_t nnz;
+  HostBlock mem;                // rp | ci | val | b, each 64-byte aligned
+  int32_t *rp, *ci;
+  double *val, *b;
   std::vector<int32_t> sptr;    // SELL-64 slice offsets
   int32_t nb = 0;               // tiles
   double rtol;
@@ -444,7 +618,20 @@ struct Request {
 struct Done {
   int32_t status, outer, inner;
   double residual;
-  std::vector<double> x;
+  int32_t n = 0;
+  bool ready = false;           // x holds the solution (its download drained)
+  HostBlock x;                  // n doubles
+};
+
+// A solution download in flight: pinned staging source, and the arena range
+// it reads (freed once drained).
+struct Download {
+  int64_t id;
+  const char *h;
+  int32_t n;
+  int chunk;
+  char *dev;
+  size_t bytes;
 };
 
 struct HostSlot {
@@ -456,23 +643,122 @@ struct HostSlot {
   double *x = nullptr;
 };
 
+inline size_t al64(size_t b) { return (b + 63) & ~(size_t)63; }
+
+// Host bytes of a request's arrays, laid out rp | ci | val | b.
+size_t req_host_bytes(size_t n, size_t nnz) { return al64(4 * (n + 1)) + al64(4 * nnz) + al64(8 * nnz) + 8 * n; }
+
+void place_arrays(Request &R) {
+  const size_t n = (size_t)R.n, nnz = (size_t)R.nnz;
+  char *p = R.mem.p;
+  R.rp = (int32_t *)p;
+  p += al64(4 * (n + 1));
+  R.ci = (int32_t *)p;
+  p += al64(4 * nnz);
+  R.val = (double *)p;
+  p += al64(8 * nnz);
+  R.b = (double *)p;
+}
+
+// Threads for the host work of large requests: the calling thread plus up to
+// seven workers (fewer on a host with fewer hardware threads).
+int host_workers() {
+  const unsigned hw = std::thread::hardware_concurrency();
+  return (int)std::max(0u, std::min(hw, 8u) - (hw > 0 ? 1u : 0u));
+}
+
+// Checks a system's CSR structure and copies it into R (whose block is
+// sized for it), in parallel parts of rows on the workers.  Each row is
+// checked in the order a serial pass would: its pointers (0 <= rp[i] <=
+// rp[i+1] <= nnz), each entry's column, the diagonal's value, the diagonal's
+// presence.  A part stops at its first failing row and copies nothing; the
+// error reported is the lowest failing row's, as a serial pass would report.
+void check_and_copy(Workers &W, Request &R, const int32_t *rp, const int32_t *ci, const double *val,
+                    const double *b) {
+  const int32_t n = R.n;
+  const int64_t nnz = R.nnz;
+  enum { OK_ = 0, BAD_PTR, BAD_COL, ZERO_DIAG, NO_DIAG };
+  struct Fail {
+    int32_t row = INT32_MAX;
+    int kind = OK_;
+  };
+  const int parts = nnz < (1 << 16) ? 1 : std::min(4 * W.size(), std::max(1, n / 256));
+  const int32_t per = (int32_t)(((int64_t)n + parts - 1) / parts);
+  std::vector<Fail> fails((size_t)parts);
+  W.run(parts, [&](int p) {
+    const int32_t r0 = (int32_t)std::min<int64_t>(n, (int64_t)p * per);
+    const int32_t r1 = (int32_t)std::min<int64_t>(n, (int64_t)r0 + per);
+    for (int32_t i = r0; i < r1; ++i) {
+      const int32_t a = rp[i], e = rp[i + 1];
+      if (a < 0 || e < a || e > nnz) {
+        fails[p] = Fail{i, BAD_PTR};
+        return;
+      }
+      bool diag = false;
+      for (int32_t k = a; k < e; ++k) {
+        const int32_t c = ci[k];
+        if (c < 0 || c >= n) {
+          fails[p] = Fail{i, BAD_COL};
+          return;
+        }
+        if (c == i) {
+          if (val[k] == 0.0) {
+            fails[p] = Fail{i, ZERO_DIAG};
+            return;
+          }
+          diag = true;
+        }
+      }
+      if (!diag) {
+        fails[p] = Fail{i, NO_DIAG};
+        return;
+      }
+    }
+    // the part's rows hold the entries rp[r0] .. rp[r1] (checked above)
+    if (r1 > r0) {
+      const size_t k0 = (size_t)rp[r0], k1 = (size_t)rp[r1];
+      memcpy(R.ci + k0, ci + k0, 4 * (k1 - k0));
+      memcpy(R.val + k0, val + k0, 8 * (k1 - k0));
+    }
+    memcpy(R.rp + r0, rp + r0, 4 * (size_t)(r1 - r0));
+    memcpy(R.b + r0, b + r0, 8 * (size_t)(r1 - r0));
+  });
+  for (const Fail &f : fails) {
+    if (f.kind == OK_) continue;
+    const std::string row = std::to_string(f.row);
+    if (f.kind == BAD_PTR) fail(MTJG_E_INVALID_ARGUMENT, "row pointers must be non-decreasing");
+    if (f.kind == BAD_COL) fail(MTJG_E_INVALID_ARGUMENT, "column index out of range in row " + row);
+    if (f.kind == ZERO_DIAG) fail(MTJG_E_ZERO_DIAGONAL, "zero diagonal in row " + row);
+    fail(MTJG_E_MISSING_DIAGONAL, "missing diagonal in row " + row);
+  }
+  R.rp[n] = rp[n];
+}
+
 size_t req_bytes(const Request &R) {
-  const size_t n = (size_t)R.n, nnz = R.ci.size(), sell = (size_t)R.sptr.back();
+  const size_t n = (size_t)R.n, nnz = (size_t)R.nnz, sell = (size_t)R.sptr.back();
   return al256(4 * (n + 1)) + al256(4 * nnz) + al256(8 * nnz) + al256(4 * n) + 3 * al256(8 * n) +
          5 * al256(4 * n) + al256(4 * R.sptr.size()) + al256(4 * sell) + al256(8 * sell) + al256(4 * sell);
 }
 
 // Slice offsets of the system's SELL-64 copy (each slice as wide as its
 // longest row) and its tile count for tiles of `rows` rows.
-void build_sell(Request &R, int rows) {
+void build_sell(Workers &W, Request &R, int rows) {
   const int32_t n = R.n;
   const int32_t nslice = (n + SELL_C - 1) / SELL_C;
   R.sptr.assign((size_t)nslice + 1, 0);
+  // slice widths in parallel parts, then the prefix sum
+  const int parts = std::max(1, std::min(W.size(), nslice / 4096));
+  const int32_t per = (nslice + parts - 1) / parts;
+  W.run(parts, [&](int p) {
+    for (int32_t sl = p * per; sl < std::min(nslice, (p + 1) * per); ++sl) {
+      int32_t w = 0;
+      for (int32_t r = sl * SELL_C; r < std::min(n, (sl + 1) * SELL_C); ++r) w = std::max(w, R.rp[r + 1] - R.rp[r]);
+      R.sptr[(size_t)sl + 1] = w;
+    }
+  });
   int64_t total = 0;
   for (int32_t sl = 0; sl < nslice; ++sl) {
-    int32_t w = 0;
-    for (int32_t r = sl * SELL_C; r < std::min(n, (sl + 1) * SELL_C); ++r) w = std::max(w, R.rp[r + 1] - R.rp[r]);
-    total += (int64_t)w * SELL_C;
+    total += (int64_t)R.sptr[(size_t)sl + 1] * SELL_C;
     if (total > INT32_MAX) fail(MTJG_E_UNSUPPORTED, "system too large for the batch engine");
     R.sptr[(size_t)sl + 1] = (int32_t)total;
   }
@@ -486,11 +772,15 @@ using namespace mtjg;
 
 struct mtjg_batch_s {
   mtjg_ctx ctx = nullptr;
-  hipStream_t copy = nullptr;  // uploads of admitted systems, solution downloads
+  hipStream_t copy = nullptr;  // uploads of admitted systems
+  hipStream_t dl = nullptr;    // solution downloads
+  bool uploads_in_flight = false;
   int max_slots = 0, max_work = 0, iters = 32;
   int rows_per_wg = 1024;
   Arena arena;
   Pinned up, down;              // staging of system uploads and solution downloads
+  std::unique_ptr<Workers> workers;
+  HostPool pool;                // host blocks of queued requests and solutions (outlives both)
   BSlot *d_slots = nullptr;     // device slot table
   BWork *d_work = nullptr;
   int32_t *d_nwork = nullptr;
@@ -502,6 +792,12 @@ struct mtjg_batch_s {
   BSlot *h_snap[2] = {nullptr, nullptr};
   hipEvent_t ev_stage[2] = {nullptr, nullptr};  // staging buffer i consumed
   hipEvent_t ev_snap[2] = {nullptr, nullptr};   // snapshot i written
+  hipEvent_t ev_g0[2] = {nullptr, nullptr}, ev_g1[2] = {nullptr, nullptr};  // around step i's graph
+  double busy_ms = 0;                           // device time of the graphs whose snapshots were consumed
+  // host time (ms) in: submit, admission, waiting for snapshots, retiring,
+  // retiring's wait for solution downloads, admission's wait for uploads,
+  // publishing tables, launching steps
+  double host_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   hipEvent_t ev_copy = nullptr;                 // uploads on the copy stream done
   int stage = 0;
   int snap_pending = -1, snap_hi = 0;          // buffer of the unprocessed snapshot
@@ -510,6 +806,7 @@ struct mtjg_batch_s {
   std::vector<HostSlot> slots;
   std::deque<Request> queue;
   std::map<int64_t, Done> done;
+  std::vector<Download> downloads;  // started by the last retirement, not yet drained
   std::vector<int64_t> finished;  // ids retired since the last mtjg_batch_completed
   int64_t next_id = 1;
   std::map<int, hipGraphExec_t> graphs;  // per grid size
@@ -518,12 +815,18 @@ struct mtjg_batch_s {
 
 namespace {
 
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 BDev bdev(mtjg_batch B) { return BDev{B->d_slots, B->d_work, B->d_nwork, B->d_partials, B->max_work}; }
 
+// Admitted requests not yet reported finished: those in a slot and those
+// whose solution download is still to be drained.
 int active_slots(mtjg_batch B) {
   int a = 0;
   for (const auto &h : B->slots) a += h.id >= 0;
-  return a;
+  return a + (int)B->downloads.size();
 }
 
 int used_work(mtjg_batch B) {
@@ -554,44 +857,74 @@ hipGraphExec_t graph_for(mtjg_batch B, int grid) {
   return ex;
 }
 
-// Retires the systems the pending snapshot shows finished: solution copied
-// (copy stream, after the snapshot) into the batch's pinned download staging,
-// then into the request's Done record once the copy stream has synchronised;
-// slot and arena range freed.
+// Finishes the solution downloads the previous retirement started: waits for
+// the download stream (by now the copies are normally complete), moves each
+// solution into its Done record, reports the request finished and frees its
+// arena range (kept until here because the copy read it).  Returns the count.
+int drain_downloads(mtjg_batch B) {
+  auto &dls = B->downloads;
+  if (dls.empty()) return 0;
+  const double t = now_ms();
+  HIPCHK(hipStreamSynchronize(B->dl));
+  B->host_ms[4] += now_ms() - t;
+  size_t total = 0;
+  for (const Download &d : dls) total += 8 * (size_t)d.n;
+  auto out = [&](const Download &d) {
+    auto it = B->done.find(d.id);
+    if (it == B->done.end()) return;  // cannot happen: Done records outlive their downloads
+    memcpy(it->second.x.p, d.h, 8 * (size_t)d.n);
+  };
+  if (total < ((size_t)4 << 20)) {
+    for (const Download &d : dls) out(d);
+  } else {
+    B->workers->run((int)dls.size(), [&](int k) { out(dls[(size_t)k]); });
+  }
+  for (const Download &d : dls) {
+    B->done[d.id].ready = true;
+    B->finished.push_back(d.id);
+    B->arena.release(d.chunk, d.dev, d.bytes);
+  }
+  const int c = (int)dls.size();
+  dls.clear();
+  B->down.used = 0;
+  return c;
+}
+
+// Retires: finishes the previous retirement's downloads, then, for each
+// system the pending snapshot shows finished, records its outcome, starts the
+// download of its solution into the pinned download staging (download
+// stream) and frees its slot.  The host has waited for the snapshot's event
+// before it reads the snapshot, so the kernels that finished these systems
+// are complete and the copies, enqueued after that wait, read their final
+// solutions; no later kernel touches a finished system's memory (every batch
+// kernel returns at once for a slot that is not running), and its arena range
+// is freed only when the download has been drained.  The downloads run
+// while the step launched last runs and are drained by the next retirement,
+// so the host never waits for them in the step that starts them.  Returns
+// the number of requests reported finished.
 int retire(mtjg_batch B) {
-  if (B->snap_pending < 0) return 0;
+  int n = drain_downloads(B);
+  if (B->snap_pending < 0) return n;
   const int i = B->snap_pending;
   B->snap_pending = -1;
+  const double tw = now_ms();
   HIPCHK(hipEventSynchronize(B->ev_snap[i]));
+  const double tr = now_ms();
+  B->host_ms[2] += tr - tw;
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, B->ev_g0[i], B->ev_g1[i]));
+  B->busy_ms += ms;
   const BSlot *snap = B->h_snap[i];
   Pinned &dn = B->down;
-  struct Pending {
-    Done *D;
-    const char *h;
-  };
-  std::vector<Pending> pend;
-  auto drain = [&]() {
-    if (pend.empty()) return;
-    HIPCHK(hipStreamSynchronize(B->copy));
-    for (const Pending &q : pend) memcpy(q.D->x.data(), q.h, sizeof(double) * q.D->x.size());
-    pend.clear();
-    dn.used = 0;
-  };
-  int n = 0;
-  bool waited = false;
   if (B->snap_hi > B->max_slots) fail(MTJG_E_DEVICE, "batch: snapshot larger than the slot table");
   for (int s = 0; s < B->snap_hi; ++s) {
     HostSlot &H = B->slots[s];
     const BSlot &S = snap[s];
     if (H.id < 0 || S.rid != H.id || S.status == ST_RUNNING) continue;
-    if (!waited) {
-      HIPCHK(hipStreamWaitEvent(B->copy, B->ev_snap[i], 0));
-      waited = true;
-    }
     const size_t bytes = 8 * (size_t)H.n;
     char *h = dn.take(bytes);
     if (!h) {
-      drain();
+      n += drain_downloads(B);  // frees the staging (waits for the copies just started)
       dn.grow(al256(bytes));
       h = dn.take(bytes);
     }
@@ -600,15 +933,14 @@ int retire(mtjg_batch B) {
     D.outer = S.outer;
     D.inner = S.inner_total;
     D.residual = S.rnorm;
-    D.x.resize((size_t)H.n);
-    copy_checked(h, H.x, bytes, hipMemcpyDeviceToHost, dn, DevRange{H.dev, H.bytes}, B->copy);
-    pend.push_back(Pending{&D, h});
-    B->finished.push_back(H.id);
-    B->arena.release(H.chunk, H.dev, H.bytes);
+    D.n = H.n;
+    D.ready = false;
+    D.x = B->pool.get(bytes);
+    copy_checked(h, H.x, bytes, hipMemcpyDeviceToHost, dn, DevRange{H.dev, H.bytes}, B->dl);
+    B->downloads.push_back(Download{H.id, h, H.n, H.chunk, H.dev, H.bytes});
     H = HostSlot{};
-    ++n;
   }
-  drain();
+  B->host_ms[3] += now_ms() - tr;
   return n;
 }
 
@@ -620,7 +952,14 @@ int retire(mtjg_batch B) {
 int admit(mtjg_batch B, std::vector<int> &new_slots) {
   hipStream_t cs = B->copy;
   Pinned &up = B->up;
-  up.used = 0;  // the previous admission synchronised the copy stream
+  if (B->uploads_in_flight) {
+    // the previous admission's uploads (normally complete by now) free the staging
+    const double t = now_ms();
+    HIPCHK(hipStreamSynchronize(cs));
+    B->host_ms[5] += now_ms() - t;
+    B->uploads_in_flight = false;
+  }
+  up.used = 0;
   int admitted = 0;
   bool in_flight = false;
   int used = used_work(B);
@@ -635,8 +974,8 @@ int admit(mtjg_batch B, std::vector<int> &new_slots) {
     if (slot < 0) break;
     const int nb = R.nb;
     if (used + nb > B->max_work) break;
-    const size_t n = (size_t)R.n, nnz = R.ci.size(), nsp = R.sptr.size(), sell = (size_t)R.sptr.back();
-    if (R.rp.size() != n + 1 || R.val.size() != nnz || R.b.size() != n)
+    const size_t n = (size_t)R.n, nnz = (size_t)R.nnz, nsp = R.sptr.size(), sell = (size_t)R.sptr.back();
+    if (!R.mem.p || R.mem.cap < req_host_bytes(n, nnz))
       fail(MTJG_E_DEVICE, "batch: request arrays do not match its sizes");
     const size_t bytes = req_bytes(R);
     int chunk = -1;
@@ -677,13 +1016,13 @@ int admit(mtjg_batch B, std::vector<int> &new_slots) {
     auto stage = [&](void *dst, const void *src, size_t bytes_) {
       char *h = up.take(bytes_);
       if (!h) fail(MTJG_E_DEVICE, "batch: upload staging overflow");
-      if (bytes_) memcpy(h, src, bytes_);
+      par_copy(*B->workers, h, src, bytes_);
       copy_checked(dst, h, bytes_, hipMemcpyHostToDevice, up, dev, cs);
     };
-    stage(rp, R.rp.data(), 4 * (n + 1));
-    stage(ci, R.ci.data(), 4 * nnz);
-    stage(v64, R.val.data(), 8 * nnz);
-    stage(b, R.b.data(), 8 * n);
+    stage(rp, R.rp, 4 * (n + 1));
+    stage(ci, R.ci, 4 * nnz);
+    stage(v64, R.val, 8 * nnz);
+    stage(b, R.b, 8 * n);
     stage(sptr, R.sptr.data(), 4 * nsp);
     in_flight = true;
     if (sell) {
@@ -739,7 +1078,10 @@ int admit(mtjg_batch B, std::vector<int> &new_slots) {
     B->queue.pop_front();  // the host arrays are staged: R may go
     ++admitted;
   }
-  if (admitted) HIPCHK(hipStreamSynchronize(cs));
+  // the uploads run on; the next admission waits for them before it reuses
+  // the staging, and the main stream waits for them (publish_tables) before
+  // the next step
+  B->uploads_in_flight = in_flight;
   return admitted;
 }
 
@@ -801,10 +1143,17 @@ extern "C" int mtjg_batch_create(mtjg_ctx ctx, int32_t max_slots, int32_t max_wo
   B->max_work = max_workgroups;
   B->iters = iters_per_step;
   B->rows_per_wg = rows_per_workgroup;
-  if (arena_bytes > 0) B->arena.init_fixed((size_t)arena_bytes);
+  if (arena_bytes > 0) {
+    B->arena.init_fixed((size_t)arena_bytes);
+  } else {
+    size_t fr = 0;
+    HIPCHK(hipMemGetInfo(&fr, &B->arena.total));
+  }
+  B->workers.reset(new Workers(host_workers()));
   B->up.grow((size_t)64 << 20);
   B->down.grow((size_t)32 << 20);
   HIPCHK(hipStreamCreateWithFlags(&B->copy, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&B->dl, hipStreamNonBlocking));
   HIPCHK(hipMalloc(&B->d_slots, sizeof(BSlot) * max_slots));
   HIPCHK(hipMemsetAsync(B->d_slots, 0, sizeof(BSlot) * max_slots, ctx->stream));
   HIPCHK(hipMalloc(&B->d_work, sizeof(BWork) * max_workgroups));
@@ -817,6 +1166,8 @@ extern "C" int mtjg_batch_create(mtjg_ctx ctx, int32_t max_slots, int32_t max_wo
     HIPCHK(hipHostMalloc(&B->h_snap[i], sizeof(BSlot) * max_slots, hipHostMallocDefault));
     HIPCHK(hipEventCreateWithFlags(&B->ev_stage[i], hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&B->ev_snap[i], hipEventDisableTiming));
+    HIPCHK(hipEventCreate(&B->ev_g0[i]));
+    HIPCHK(hipEventCreate(&B->ev_g1[i]));
   }
   HIPCHK(hipEventCreateWithFlags(&B->ev_copy, hipEventDisableTiming));
   B->slots.assign(max_slots, HostSlot{});
@@ -837,6 +1188,7 @@ extern "C" int mtjg_batch_destroy(mtjg_batch B) {
   if (!B) return MTJG_OK;
   HIPCHK(hipStreamSynchronize(B->ctx->stream));
   HIPCHK(hipStreamSynchronize(B->copy));
+  HIPCHK(hipStreamSynchronize(B->dl));
   for (auto &kv : B->graphs) (void)hipGraphExecDestroy(kv.second);
   B->arena.free_all();
   B->up.release();
@@ -851,9 +1203,12 @@ extern "C" int mtjg_batch_destroy(mtjg_batch B) {
     (void)hipHostFree(B->h_snap[i]);
     (void)hipEventDestroy(B->ev_stage[i]);
     (void)hipEventDestroy(B->ev_snap[i]);
+    (void)hipEventDestroy(B->ev_g0[i]);
+    (void)hipEventDestroy(B->ev_g1[i]);
   }
   (void)hipEventDestroy(B->ev_copy);
   (void)hipStreamDestroy(B->copy);
+  (void)hipStreamDestroy(B->dl);
   delete B;
   MTJG_API_END
 }
@@ -864,38 +1219,26 @@ extern "C" int mtjg_batch_submit(mtjg_batch B, int32_t n, int64_t nnz, const int
                                  int64_t *id) {
   MTJG_API_BEGIN
   if (!B || !row_pointer || !b || !id || (nnz > 0 && (!column_index || !data))) fail(MTJG_E_NULL, "null argument");
+  const double t0 = now_ms();
   if (n <= 0 || nnz < n) fail(MTJG_E_INVALID_ARGUMENT, "system must be square with a stored diagonal");
   if (row_pointer[0] != 0 || row_pointer[n] != nnz) fail(MTJG_E_INVALID_ARGUMENT, "row pointers must run from 0 to nnz");
-  for (int32_t i = 0; i < n; ++i) {
-    if (row_pointer[i + 1] < row_pointer[i]) fail(MTJG_E_INVALID_ARGUMENT, "row pointers must be non-decreasing");
-    bool diag = false;
-    for (int32_t k = row_pointer[i]; k < row_pointer[i + 1]; ++k) {
-      if (column_index[k] < 0 || column_index[k] >= n)
-        fail(MTJG_E_INVALID_ARGUMENT, "column index out of range in row " + std::to_string(i));
-      if (column_index[k] == i) {
-        if (data[k] == 0.0) fail(MTJG_E_ZERO_DIAGONAL, "zero diagonal in row " + std::to_string(i));
-        diag = true;
-      }
-    }
-    if (!diag) fail(MTJG_E_MISSING_DIAGONAL, "missing diagonal in row " + std::to_string(i));
-  }
   Request R;
   R.n = n;
-  R.rp.assign(row_pointer, row_pointer + n + 1);
-  build_sell(R, B->rows_per_wg);
-  const int nb = R.nb;
-  R.ci.assign(column_index, column_index + nnz);
-  if (nb > B->max_work || req_bytes(R) > B->arena.capacity())
+  R.nnz = nnz;
+  R.mem = B->pool.get(req_host_bytes((size_t)n, (size_t)nnz));
+  place_arrays(R);
+  check_and_copy(*B->workers, R, row_pointer, column_index, data, b);
+  build_sell(*B->workers, R, B->rows_per_wg);
+  if (R.nb > B->max_work || req_bytes(R) > B->arena.capacity())
     fail(MTJG_E_OUT_OF_MEMORY, "system larger than the batch can ever hold");
   R.id = B->next_id++;
-  R.val.assign(data, data + nnz);
-  R.b.assign(b, b + n);
   R.rtol = rtol;
   R.max_outer = max_outer;
   R.inner_rtol = inner_rtol;
   R.max_inner = max_inner;
   *id = R.id;
   B->queue.push_back(std::move(R));
+  B->host_ms[0] += now_ms() - t0;
   MTJG_API_END
 }
 
@@ -909,14 +1252,19 @@ extern "C" int mtjg_batch_step(mtjg_batch B, int32_t *n_active, int32_t *n_compl
     int grid = 64;
     while (grid < B->table_work) grid *= 2;
     grid = std::min(grid, B->max_work);
-    HIPCHK(hipGraphLaunch(graph_for(B, grid), st));
+    const int i = B->steps & 1;
+    const double tl = now_ms();
+    hipGraphExec_t g = graph_for(B, grid);
+    HIPCHK(hipEventRecord(B->ev_g0[i], st));
+    HIPCHK(hipGraphLaunch(g, st));
+    HIPCHK(hipEventRecord(B->ev_g1[i], st));
     int hi = 0;
     for (int s = 0; s < B->max_slots; ++s)
       if (B->slots[s].id >= 0) hi = s + 1;
-    const int i = B->steps & 1;
     if (hi > B->max_slots) fail(MTJG_E_DEVICE, "batch: snapshot larger than the slot table");
     HIPCHK(hipMemcpyAsync(B->h_snap[i], B->d_slots, sizeof(BSlot) * hi, hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(B->ev_snap[i], st));
+    B->host_ms[7] += now_ms() - tl;
     // 2. while it runs: retire what the previous snapshot shows finished
     const int completed = retire(B);
     B->snap_pending = i;
@@ -930,13 +1278,19 @@ extern "C" int mtjg_batch_step(mtjg_batch B, int32_t *n_active, int32_t *n_compl
   }
   // 3. admit, then publish the next table behind the running graph
   std::vector<int> new_slots;
+  const double ta = now_ms();
   const int admitted = admit(B, new_slots);
+  B->host_ms[1] += now_ms() - ta;
   bool changed = admitted > 0;
   if (!changed) {
     // a retirement also changes the table
     changed = used_work(B) != B->table_work;
   }
-  if (changed) publish_tables(B, new_slots, admitted > 0);
+  if (changed) {
+    const double tp = now_ms();
+    publish_tables(B, new_slots, admitted > 0);
+    B->host_ms[6] += now_ms() - tp;
+  }
   if (n_active) *n_active = active_slots(B);
   MTJG_API_END
 }
@@ -945,9 +1299,9 @@ extern "C" int mtjg_batch_result(mtjg_batch B, int64_t id, double *x, mtjg_resul
   MTJG_API_BEGIN
   if (!B) fail(MTJG_E_NULL, "null batch");
   auto it = B->done.find(id);
-  if (it == B->done.end()) fail(MTJG_E_INVALID_ARGUMENT, "request not finished (or unknown)");
+  if (it == B->done.end() || !it->second.ready) fail(MTJG_E_INVALID_ARGUMENT, "request not finished (or unknown)");
   const Done &D = it->second;
-  if (x) memcpy(x, D.x.data(), sizeof(double) * D.x.size());
+  if (x) memcpy(x, D.x.p, sizeof(double) * (size_t)D.n);
   int code = D.status == ST_CONVERGED ? MTJG_OK
              : D.status == ST_ITERATIONS ? MTJG_NOT_CONVERGED_ITERATIONS
                                          : MTJG_NOT_CONVERGED_DIVERGENCE;
@@ -980,5 +1334,15 @@ extern "C" int mtjg_batch_info(mtjg_batch B, int32_t *queued, int32_t *active, i
   if (active) *active = active_slots(B);
   if (arena_used) *arena_used = (int64_t)B->arena.used;
   if (arena_size) *arena_size = (int64_t)B->arena.size;
+  MTJG_API_END
+}
+
+extern "C" int mtjg_batch_stats(mtjg_batch B, double *busy_ms, int64_t *steps, double *host_ms) {
+  MTJG_API_BEGIN
+  if (!B) fail(MTJG_E_NULL, "null batch");
+  if (busy_ms) *busy_ms = B->busy_ms;
+  if (steps) *steps = B->steps;
+  if (host_ms)
+    for (int k = 0; k < 8; ++k) host_ms[k] = B->host_ms[k];
   MTJG_API_END
 }

@@ -120,6 +120,7 @@ PROTOTYPES = [
     ("mtjg_batch_result", C.c_int, [P, I64, P, C.POINTER(Result), PI32]),
     ("mtjg_batch_completed", C.c_int, [P, PI64, I32, PI32]),
     ("mtjg_batch_info", C.c_int, [P, PI32, PI32, PI64, PI64]),
+    ("mtjg_batch_stats", C.c_int, [P, C.POINTER(C.c_double), PI64, C.POINTER(C.c_double)]),
     ("mtjg_mm_read_matrix", C.c_int, [C.c_char_p, PP]),
     ("mtjg_mm_parse_matrix", C.c_int, [C.c_char_p, I64, PP]),
     ("mtjg_mm_read_vector", C.c_int, [C.c_char_p, PP]),

@@ -72,6 +72,16 @@ class BatchServer:
         _ck(self._lib.mtjg_batch_info(self.handle, C.byref(q), C.byref(a), C.byref(u), C.byref(s)))
         return {"queued": q.value, "active": a.value, "arena_used": u.value, "arena_size": s.value}
 
+    def stats(self):
+        """Device time of the replayed graphs (ms) and steps run so far."""
+        ms, st = C.c_double(), C.c_int64()
+        host = (C.c_double * 8)()
+        _ck(self._lib.mtjg_batch_stats(self.handle, C.byref(ms), C.byref(st), host))
+        names = ("submit", "admit", "wait", "retire", "retire_copy_wait", "admit_upload_wait", "publish", "launch")
+        out = {"busy_ms": ms.value, "steps": st.value}
+        out.update({"host_%s_ms" % k: host[j] for j, k in enumerate(names)})
+        return out
+
     def close(self):
         if getattr(self, "handle", None):
             self._lib.mtjg_batch_destroy(self.handle)
@@ -119,6 +129,20 @@ def _stats(lat, t_first, t_last, count):
             "mean_ms": float(lat.mean() * 1e3)}
 
 
+def batch_bytes(system, outer, inner):
+    """Algorithmic HBM bytes the batch engine moves for one solved system:
+    per inner FP32 CG iteration 8 nnz + 52 n (q = A32 p over the FP32 values
+    and columns, p gathered once, q written; d p rr q invdiag read and d rr z
+    written; p z read and p written), per refinement 12 nnz + 76 n (the FP64
+    residual r = b - A x: values, columns, x and b read, r written; the
+    restart rr = r/||r||, d = 0, z = D^-1 rr; p = z; x += ||r|| d), plus the
+    final residual check."""
+    n, rp, ci, va, b = system
+    nnz = len(ci)
+    resid = 12 * nnz + 24 * n
+    return inner * (8 * nnz + 52 * n) + outer * (resid + 52 * n) + resid
+
+
 def serve_batched(ctx, reqs, systems, rtol=1e-8, **kw):
     """Real-time loop: submit each request when its arrival time has passed,
     step the batch, collect completions.  Returns (stats, results by index)."""
@@ -129,6 +153,7 @@ def serve_batched(ctx, reqs, systems, rtol=1e-8, **kw):
         results = {}
         lat = []
         nxt = 0
+        t_step = t_collect = 0.0
         t0 = time.perf_counter()
         t_last = t0
         while len(results) < len(reqs):
@@ -143,15 +168,27 @@ def serve_batched(ctx, reqs, systems, rtol=1e-8, **kw):
                 if nxt < len(order):
                     time.sleep(max(0.0, reqs[order[nxt]]["arrival"] - (time.perf_counter() - t0)))
                 continue
+            ts = time.perf_counter()
             B.step()
             tnow = time.perf_counter()
+            t_step += tnow - ts
             for rid in B.completed():
                 k = id2k.pop(rid)
                 x, code, outer, inner, res = B.result(rid, systems[k][0])
                 results[k] = {"x": x, "code": code, "outer": outer, "inner": inner}
                 lat.append(tnow - t0 - reqs[k]["arrival"])
                 t_last = tnow
-        return _stats(lat, 0.0, t_last - t0, len(reqs)), results
+            t_collect += time.perf_counter() - tnow
+        st = _stats(lat, 0.0, t_last - t0, len(reqs))
+        st.update(B.stats())
+        st["host_step_ms"] = t_step * 1e3
+        st["host_collect_ms"] = t_collect * 1e3
+        nbytes = sum(batch_bytes(systems[k], r["outer"], r["inner"]) for k, r in results.items())
+        st["stream_bytes"] = float(nbytes)
+        st["stream_GBps_span"] = nbytes / st["span_s"] / 1e9
+        st["stream_GBps_busy"] = nbytes / max(st["busy_ms"] * 1e-3, 1e-12) / 1e9
+        st["busy_frac"] = st["busy_ms"] * 1e-3 / st["span_s"]
+        return st, results
     finally:
         B.close()
 

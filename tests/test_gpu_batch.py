@@ -111,6 +111,57 @@ def test_nan_system_is_isolated(ctx):
         assert out[k][1] == 0 and S.true_residual(systems[k], out[k][0]) <= 1e-8
 
 
+def _plant(sysm, defects):
+    n, _, rp, ci, va = sysm
+    rp, ci, va = rp.copy(), ci.copy(), va.copy()
+    for kind, row in defects:
+        lo, hi = rp[row], rp[row + 1]
+        d = lo + int(np.flatnonzero(ci[lo:hi] == row)[0])
+        if kind == "column":
+            ci[hi - 1] = n + 5
+        elif kind == "zero":
+            va[d] = 0.0
+        elif kind == "missing":
+            ci[d] = (row + 2) % n
+        elif kind == "pointer":
+            rp[row + 1] = rp[row] - 1
+    return n, rp, ci, va
+
+
+@pytest.mark.parametrize("grid", [20, 300])
+def test_submit_checks_structure(ctx, grid):
+    """Submit rejects a broken CSR structure with the error of its lowest
+    failing row (checked serially for the small system, in parallel parts of
+    rows for the 90,000-row one), and the batch then serves a valid system."""
+    sysm = P.poisson2d(grid)
+    n = sysm[0]
+    lo, mid, hi = n // 7, n // 3, n - 3
+    cases = [([("column", hi), ("zero", mid)], M.ZeroDiagonalException, mid),
+             ([("missing", lo), ("column", hi)], M.MissingDiagonalException, lo),
+             ([("column", mid), ("missing", hi)], M.IllegalArgumentException, mid),
+             ([("zero", hi), ("pointer", lo)], M.IllegalArgumentException, None)]
+    b = P.uniform(n, 2)
+    B = S.BatchServer(ctx, max_slots=8, max_workgroups=1024)
+    try:
+        for defects, exc, row in cases:
+            with pytest.raises(exc) as e:
+                B.submit(*_plant(sysm, defects), b, rtol=1e-8)
+            if row is not None:
+                assert str(e.value).endswith("row %d" % row), str(e.value)
+        assert B.info()["queued"] == 0
+        rid = B.submit(n, sysm[2], sysm[3], sysm[4], b, rtol=1e-8)
+        out = None
+        for _ in range(100000):
+            B.step()
+            if rid in B.completed():
+                out = B.result(rid, n)
+                break
+        assert out is not None and out[1] == 0
+        assert S.true_residual((n, sysm[2], sysm[3], sysm[4], b), out[0]) <= 1e-8
+    finally:
+        B.close()
+
+
 def test_hbm_admitted_arena(ctx):
     """arena_bytes = 0: the arena starts empty and grows in chunks of device
     memory as systems are admitted, bounded by the free HBM hipMemGetInfo
