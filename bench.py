@@ -446,7 +446,7 @@ def precond_mode(prec, wl, steps, with_cpu=True):
     return 0
 
 
-def serve_mode(count, lam, seed, tune=True):
+def serve_mode(count, lam, seed, tune=True, plain=True):
     """Served stream: `count` requests of the 2D/3D variable-kappa Poisson
     stream (Poisson arrivals at `lam`/s, n log-uniform 1k-262k), solved to
     relative residual 1e-8 (a) by the batched mixed-precision engine with
@@ -464,6 +464,14 @@ def serve_mode(count, lam, seed, tune=True):
     print("serve: launch shape %s" % best, file=sys.stderr, flush=True)
     st_b, res_b = S.serve_batched(ctx, reqs, systems, **best)
     print("serve: batched %.1f solves/s" % st_b["solves_per_s"], file=sys.stderr, flush=True)
+    if not plain:
+        worst_b = max(S.true_residual(systems[k], res_b[k]["x"]) for k in range(count))
+        print(json.dumps({"mode": "serve", "requests": count, "lambda_per_s": lam,
+                          "sum_n": int(sum(s[0] for s in systems)),
+                          "batched_mixed": {k: round(v, 4) for k, v in st_b.items()},
+                          "worst_true_rel_residual": {"batched": worst_b},
+                          "all_converged": all(res_b[k]["code"] == 0 for k in range(count)), "autotune": tinfo}))
+        return 0
     st_p, res_p = S.serve_plain(ctx, reqs, systems)
     print("serve: plain %.1f solves/s" % st_p["solves_per_s"], file=sys.stderr, flush=True)
     worst_b = max(S.true_residual(systems[k], res_b[k]["x"]) for k in range(count))
@@ -552,6 +560,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="serve with the batch engine's default launch shape instead of the autotuner's")
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--no-plain", action="store_true",
+                    help="serve mode: run only the batch engine (for profiling its kernels)")
     ap.add_argument("--solve", default="cg_poisson2d_256", choices=sorted(SOLVE_WORKLOADS))
     ap.add_argument("--repeats", type=int, default=3)
     args = ap.parse_args()
@@ -564,7 +574,7 @@ def main():
     if args.mode == "solve":
         return solve_mode(args.solve, args.repeats, 0.0 if args.no_cpu else args.cpu_seconds * 2)
     if args.mode == "serve":
-        return serve_mode(args.requests, args.lam, args.seed, not args.no_tune)
+        return serve_mode(args.requests, args.lam, args.seed, not args.no_tune, not args.no_plain)
     if args.mode == "precond":
         return precond_mode(args.prec, args.prec_workload, min(args.steps, 50), not args.no_cpu)
     if args.mode == "mmload":
