@@ -32,6 +32,10 @@ void set_last_error(const std::string &msg);
 
 // Workgroup size of every kernel in this library: four 64-wide wavefronts.
 constexpr int BS = 256;
+// Vectors of at least this many bytes are streamed with the non-temporal
+// hint (CG's vector kernels, the result vector of the plain products): too
+// large for the 256 MB Infinity Cache to pass them from one kernel to the next.
+constexpr int64_t NT_VECTOR_BYTES = 96ll << 20;
 // Upper bound on the grid of any kernel that reduces across the grid; the
 // per-block partial sums of such a kernel fit in RED_MAX_BLOCKS slots.
 constexpr int RED_MAX_BLOCKS = 4096;
@@ -127,7 +131,8 @@ struct mtjg_csr_s {
   float *val32 = nullptr;  // FP32 copy of val (mixed precision), lazily built
   // SpMV row-block schedule: row-block b covers rows [rb[b].x, rb[b+1].x)
   // and stored entries [rb[b].y, rb[b+1].y); `slabs` orders the blocks over
-  // the XCDs.
+  // the XCDs (> 0: that many contiguous slabs; -R: runs of R row-blocks per
+  // XCD, the plain products only).
   int32_t nrb = 0;
   int32_t slabs = 1;
   int2 *rb = nullptr;

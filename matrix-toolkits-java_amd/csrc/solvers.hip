@@ -178,9 +178,6 @@ __global__ __launch_bounds__(BS) void k_cg_p(int64_t n, T *p, const T *z, const 
   }
 }
 
-// Vectors of at least this many bytes use the non-temporal streaming kernels.
-constexpr int64_t NT_VECTOR_BYTES = 96ll << 20;
-
 // ---------------------------------------------------------------------------
 // BiCGstab kernels (S/BiCGstab.java:74-113)
 

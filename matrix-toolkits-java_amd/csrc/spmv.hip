@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 
 #include "mtjg_internal.hpp"
 #include "reduce.hpp"
@@ -22,16 +23,28 @@ namespace mtjg {
 
 constexpr int SPMV_DOT_GRID = 2048;  // workgroups of the fused product
 
-// Number of XCD slabs used to order row-blocks (see ticket_to_block).  Plain
-// row order measured fastest on the 256^3 Poisson product (5.25 TB/s against
-// 4.99 TB/s with 16 slabs); MTJG_SPMV_SLABS overrides it for experiments.
+// Order of the row-blocks over the XCDs.  The plain products take runs of 16
+// consecutive row-blocks per XCD (xcd_run_block; -16 here): on the 256^3
+// Poisson product 5.52 TB/s against 5.21 in plain row order, 5.35 / 5.42 /
+// 5.44 with runs of 8 / 32 / 64, and 4.99 with 16 contiguous slabs.  The
+// fused product walks its tickets grid-stride in plain row order.
+// MTJG_SPMV_SLABS overrides the choice for experiments (1-32: slabs, -R:
+// runs of R).
 static int choose_slabs(int32_t nrb) {
   (void)nrb;
   if (const char *e = getenv("MTJG_SPMV_SLABS")) {
     int v = atoi(e);
     if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16 || v == 32) return v;
+    if (v == -4 || v == -8 || v == -16 || v == -32 || v == -64 || v == -256) return v;
   }
-  return 1;
+  return -16;
+}
+
+// Whether the plain products store y with the non-temporal hint: when y is
+// at least NT_VECTOR_BYTES (MTJG_SPMV_NT=0|1 forces it for experiments).
+static bool spmv_nt_store(int64_t rows, size_t elem) {
+  if (const char *e = getenv("MTJG_SPMV_NT")) return atoi(e) != 0;
+  return rows * (int64_t)elem >= NT_VECTOR_BYTES;
 }
 
 // Scalar epilogues of fused solver kernels: run by the last workgroup.
@@ -59,20 +72,42 @@ __device__ __forceinline__ int ticket_to_block(int t, int nrb, int slabs, int pe
   return (l < per && b < nrb) ? b : -1;
 }
 
-// Plain products: one workgroup per row-block ticket.
-template <typename VT, typename XT, int MODE>
+// Maps workgroup `blk` of a grid dealt round-robin over the eight XCDs so
+// that each XCD takes runs of `run` consecutive row-blocks, the runs rotating
+// over the XCDs (the tail past the last whole round keeps plain order): the
+// x entries a row gathers from neighbouring rows were mostly fetched into the
+// same XCD's L2 by the rows just before it.
+__device__ __forceinline__ int xcd_run_block(int blk, int grid, int run) {
+  const int full = (grid / (8 * run)) * (8 * run);
+  if (blk < full) {
+    const int xcd = blk % 8, j = blk / 8;
+    blk = ((j / run) * 8 + xcd) * run + j % run;
+  }
+  return blk;
+}
+
+// Plain products: one workgroup per row-block, under the XCD run mapping
+// (slabs = -run) or per slab-ordered ticket (slabs > 0).  NTY: y is stored
+// with the non-temporal hint.
+template <typename VT, typename XT, int MODE, bool NTY>
 __global__ __launch_bounds__(BS) void k_spmv(int nrb, int slabs, const int2 *__restrict__ info,
                                              const int32_t *__restrict__ rp,
                                              const int32_t *__restrict__ ci,
                                              const VT *__restrict__ val, const XT *__restrict__ x,
                                              XT *y, const XT *y0, double alpha, double inv_alpha) {
   __shared__ XT prod[SPMV_CAP];
-  const int per = (nrb + slabs - 1) / slabs;
-  const int b = ticket_to_block(blockIdx.x, nrb, slabs, per);
-  if (b < 0) return;
+  int b;
+  if (slabs < 0) {
+    b = xcd_run_block(blockIdx.x, gridDim.x, -slabs);
+    if (b >= nrb) return;
+  } else {
+    const int per = (nrb + slabs - 1) / slabs;
+    b = ticket_to_block(blockIdx.x, nrb, slabs, per);
+    if (b < 0) return;
+  }
   dd acc0{0.0, 0.0}, acc1{0.0, 0.0};
-  spmv_rowblock<VT, XT, MODE>(b, prod, info, rp, ci, val, x, y, y0, alpha, inv_alpha, nullptr, acc0,
-                              acc1);
+  spmv_rowblock<VT, XT, MODE, NTY>(b, prod, info, rp, ci, val, x, y, y0, alpha, inv_alpha, nullptr, acc0,
+                                   acc1);
 }
 
 // Fused solver product q = A p with the grid sums w.q and q.q: at most
@@ -105,7 +140,7 @@ __global__ __launch_bounds__(BS) void k_spmv_dot(int nrb, int slabs, const int2 
 // Products from the SELL-64 copy: four slices (wavefronts) per workgroup.
 constexpr int SELL_XCD_RUN = 64;
 
-template <typename VT, typename XT, int MODE>
+template <typename VT, typename XT, int MODE, bool NTY>
 __global__ __launch_bounds__(BS) void k_sell(int nslice, int n, const int32_t *__restrict__ sptr,
                                              const int32_t *__restrict__ scol, const VT *__restrict__ sval,
                                              const XT *__restrict__ x, XT *y, const XT *y0, double alpha,
@@ -117,17 +152,12 @@ __global__ __launch_bounds__(BS) void k_sell(int nslice, int n, const int32_t *_
   // Poisson product: 5.93 TB/s against 5.82 TB/s round-robin (runs of 2:
   // 5.91, of 4: 5.84, of 16-128: 5.87-5.88; one contiguous range per XCD:
   // 5.29).
-  int blk = blockIdx.x;
-  const int full = (int)(gridDim.x / (8 * SELL_XCD_RUN)) * (8 * SELL_XCD_RUN);
-  if (blk < full) {
-    const int xcd = blk % 8, j = blk / 8;
-    blk = ((j / SELL_XCD_RUN) * 8 + xcd) * SELL_XCD_RUN + j % SELL_XCD_RUN;
-  }
+  const int blk = xcd_run_block(blockIdx.x, gridDim.x, SELL_XCD_RUN);
   const int slice = blk * (BS / SELL_C) + (int)threadIdx.x / SELL_C;
   if (slice >= nslice) return;
   dd acc0{0.0, 0.0}, acc1{0.0, 0.0};
-  sell_row<VT, XT, MODE>(slice, threadIdx.x % SELL_C, n, sptr, scol, sval, x, y, y0, alpha, inv_alpha, nullptr,
-                         acc0, acc1);
+  sell_row<VT, XT, MODE, NTY>(slice, threadIdx.x % SELL_C, n, sptr, scol, sval, x, y, y0, alpha, inv_alpha,
+                              nullptr, acc0, acc1);
 }
 
 // Fused solver product from the SELL-64 copy (see k_spmv_dot) for grids of
@@ -162,12 +192,7 @@ __global__ __launch_bounds__(BS) void k_sell_part(int nslice, int n, const int32
                                                   const XT *__restrict__ x, XT *y, const XT *w, const KState *st,
                                                   double *part, int cap) {
   if (st->status != ST_RUNNING) return;
-  int blk = blockIdx.x;
-  const int full = (int)(gridDim.x / (8 * SELL_XCD_RUN)) * (8 * SELL_XCD_RUN);
-  if (blk < full) {
-    const int xcd = blk % 8, j = blk / 8;
-    blk = ((j / SELL_XCD_RUN) * 8 + xcd) * SELL_XCD_RUN + j % SELL_XCD_RUN;
-  }
+  const int blk = xcd_run_block(blockIdx.x, gridDim.x, SELL_XCD_RUN);
   const int slice = blk * (BS / SELL_C) + (int)threadIdx.x / SELL_C;
   dd v[2] = {{0.0, 0.0}, {0.0, 0.0}};
   if (slice < nslice)
@@ -313,18 +338,23 @@ template <typename VT, typename XT>
 static void sell_launch(mtjg_csr A, const VT *sval, const XT *x, XT *y, const SpmvArgs &a, hipStream_t s) {
   const int grid = (A->nslice + BS / SELL_C - 1) / (BS / SELL_C);
   const XT *y0 = (const XT *)a.y0;
+  const bool nt = spmv_nt_store(A->nr, sizeof(XT));
+  auto plain = [&](auto mode, auto nty) {
+    k_sell<VT, XT, decltype(mode)::value, decltype(nty)::value><<<grid, BS, 0, s>>>(
+        A->nslice, A->nr, A->sptr, A->scol, sval, x, y, y0, a.alpha, a.inv_alpha);
+  };
   switch (a.mode) {
     case SPMV_MULT:
-      k_sell<VT, XT, SPMV_MULT><<<grid, BS, 0, s>>>(A->nslice, A->nr, A->sptr, A->scol, sval, x, y, y0, a.alpha,
-                                                   a.inv_alpha);
+      if (nt) plain(std::integral_constant<int, SPMV_MULT>{}, std::true_type{});
+      else plain(std::integral_constant<int, SPMV_MULT>{}, std::false_type{});
       break;
     case SPMV_MULTADD:
-      k_sell<VT, XT, SPMV_MULTADD><<<grid, BS, 0, s>>>(A->nslice, A->nr, A->sptr, A->scol, sval, x, y, y0,
-                                                      a.alpha, a.inv_alpha);
+      if (nt) plain(std::integral_constant<int, SPMV_MULTADD>{}, std::true_type{});
+      else plain(std::integral_constant<int, SPMV_MULTADD>{}, std::false_type{});
       break;
     case SPMV_TMULTADD:
-      k_sell<VT, XT, SPMV_TMULTADD><<<grid, BS, 0, s>>>(A->nslice, A->nr, A->sptr, A->scol, sval, x, y, y0,
-                                                       a.alpha, a.inv_alpha);
+      if (nt) plain(std::integral_constant<int, SPMV_TMULTADD>{}, std::true_type{});
+      else plain(std::integral_constant<int, SPMV_TMULTADD>{}, std::false_type{});
       break;
     case SPMV_DOT: {
       if (grid > SPMV_DOT_GRID) {  // 256^3 Poisson CG: 0.310 -> 0.288 s
@@ -356,24 +386,31 @@ static void spmv_launch(mtjg_csr A, const VT *val, const XT *x, XT *y, const Spm
     if (a.mode == SPMV_DOT) fail(MTJG_E_INVALID_ARGUMENT, "empty matrix in fused product");
     return;
   }
-  // one workgroup per row-block ticket
-  const int per = (A->nrb + A->slabs - 1) / A->slabs;
-  const int grid = (a.mode == SPMV_DOT) ? std::min(per * A->slabs, SPMV_DOT_GRID) : per * A->slabs;
+  // plain products: one workgroup per row-block (XCD run mapping) or per
+  // slab-ordered ticket; the fused product walks plain-order tickets
+  // grid-stride on at most SPMV_DOT_GRID workgroups
+  const int sl = a.mode == SPMV_DOT ? std::max(A->slabs, 1) : A->slabs;
+  const int per = sl > 0 ? (A->nrb + sl - 1) / sl : A->nrb;
+  const int grid = (a.mode == SPMV_DOT) ? std::min(per * sl, SPMV_DOT_GRID) : sl > 0 ? per * sl : A->nrb;
   const XT *y0 = (const XT *)a.y0;
   const XT *w = (const XT *)a.w;
-  const int sl = A->slabs;
+  const bool nt = spmv_nt_store(A->nr, sizeof(XT));
+  auto plain = [&](auto mode, auto nty) {
+    k_spmv<VT, XT, decltype(mode)::value, decltype(nty)::value><<<grid, BS, 0, s>>>(
+        A->nrb, sl, A->rb, A->rp, A->ci, val, x, y, y0, a.alpha, a.inv_alpha);
+  };
   switch (a.mode) {
     case SPMV_MULT:
-      k_spmv<VT, XT, SPMV_MULT><<<grid, BS, 0, s>>>(A->nrb, sl, A->rb, A->rp, A->ci, val, x, y, y0,
-                                                   a.alpha, a.inv_alpha);
+      if (nt) plain(std::integral_constant<int, SPMV_MULT>{}, std::true_type{});
+      else plain(std::integral_constant<int, SPMV_MULT>{}, std::false_type{});
       break;
     case SPMV_MULTADD:
-      k_spmv<VT, XT, SPMV_MULTADD><<<grid, BS, 0, s>>>(A->nrb, sl, A->rb, A->rp, A->ci, val, x, y, y0,
-                                                      a.alpha, a.inv_alpha);
+      if (nt) plain(std::integral_constant<int, SPMV_MULTADD>{}, std::true_type{});
+      else plain(std::integral_constant<int, SPMV_MULTADD>{}, std::false_type{});
       break;
     case SPMV_TMULTADD:
-      k_spmv<VT, XT, SPMV_TMULTADD><<<grid, BS, 0, s>>>(A->nrb, sl, A->rb, A->rp, A->ci, val, x, y,
-                                                       y0, a.alpha, a.inv_alpha);
+      if (nt) plain(std::integral_constant<int, SPMV_TMULTADD>{}, std::true_type{});
+      else plain(std::integral_constant<int, SPMV_TMULTADD>{}, std::false_type{});
       break;
     case SPMV_DOT:
       if (grid > a.red.cap) fail(MTJG_E_INVALID_ARGUMENT, "reduction workspace smaller than the grid");
@@ -515,7 +552,9 @@ static mtjg_csr_s *csr_upload(mtjg_ctx ctx, int32_t nr, int32_t nc, std::vector<
       HIPCHK(hipMemcpyAsync(A->val, val, sizeof(double) * (size_t)A->nnz, hipMemcpyHostToDevice,
                             ctx->stream));
     }
-    std::vector<int2> rb = build_row_blocks(A->h_rp, nr);
+    int cut = SPMV_CUT_ROWS;  // MTJG_SPMV_MAX_ROWS overrides it for experiments
+    if (const char *e = getenv("MTJG_SPMV_MAX_ROWS")) cut = std::min(SPMV_MAX_ROWS, std::max(129, atoi(e)));
+    std::vector<int2> rb = build_row_blocks(A->h_rp, nr, cut);
     A->nrb = (int32_t)rb.size() - 1;
     A->slabs = choose_slabs(A->nrb);
     HIPCHK(hipMalloc(&A->rb, sizeof(int2) * rb.size()));

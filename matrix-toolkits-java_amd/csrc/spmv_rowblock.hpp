@@ -14,7 +14,11 @@
 //      lanes that add strided partials and combine them with an xor
 //      butterfly.
 // A row longer than SPMV_CAP gets a row-block of its own, reduced by the
-// whole workgroup straight from global memory.
+// whole workgroup straight from global memory.  Blocks are cut at 256 rows
+// (SPMV_CUT_ROWS) so that on short-row matrices every thread sums one row in
+// a single pass: on the 3D Poisson 256^3 product 335.2 -> 315.6 us against
+// blocks of up to 512 rows, both with the XCD run mapping of spmv.hip (either
+// change alone: 333-335 us, plain order and 512 rows: 331-334 us).
 #pragma once
 
 #include <vector>
@@ -26,12 +30,15 @@ namespace mtjg {
 
 constexpr int SPMV_CAP = 2048;           // products staged per row-block
 constexpr int SPMV_PER = SPMV_CAP / BS;  // entries per thread per row-block
-constexpr int SPMV_MAX_ROWS = 512;       // rows per row-block (2 per thread)
+constexpr int SPMV_MAX_ROWS = 512;       // rows a row-block may hold (2 per thread)
+constexpr int SPMV_CUT_ROWS = 256;       // rows the cutter puts in a row-block
 
-// Row-block schedule: greedy cut so every block holds <= SPMV_CAP entries,
-// except single rows longer than that.  Entry b is (first row, first stored
-// entry) of row-block b; entry nrb closes the last block.
-inline std::vector<int2> build_row_blocks(const std::vector<int32_t> &rp, int32_t nr) {
+// Row-block schedule: greedy cut so every block holds <= SPMV_CAP entries
+// and <= max_rows (<= SPMV_MAX_ROWS) rows, except single rows longer than
+// SPMV_CAP.  Entry b is (first row, first stored entry) of row-block b; entry
+// nrb closes the last block.
+inline std::vector<int2> build_row_blocks(const std::vector<int32_t> &rp, int32_t nr,
+                                          int max_rows = SPMV_CUT_ROWS) {
   std::vector<int2> rb;
   rb.reserve((size_t)nr / 64 + 2);
   rb.push_back(make_int2(0, 0));
@@ -44,7 +51,7 @@ inline std::vector<int2> build_row_blocks(const std::vector<int32_t> &rp, int32_
       continue;
     }
     int32_t acc = 0, rows = 0;
-    while (r < nr && rows < SPMV_MAX_ROWS) {
+    while (r < nr && rows < max_rows) {
       int32_t l = rp[r + 1] - rp[r];
       if (acc + l > SPMV_CAP) break;
       acc += l;
@@ -58,26 +65,34 @@ inline std::vector<int2> build_row_blocks(const std::vector<int32_t> &rp, int32_
 
 __device__ __forceinline__ int pow2_floor(int v) { return 1 << (31 - __clz(v)); }
 
+// Store of a result entry; NTY: with the non-temporal hint (y is written
+// once and is too large to stay cached for the next kernel).
+template <bool NTY, typename XT>
+__device__ __forceinline__ void y_store(XT *p, XT v) {
+  if (NTY) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 // Writes row `row`'s result and accumulates the epilogue dots.
-template <typename XT, int MODE>
+template <typename XT, int MODE, bool NTY = false>
 __device__ __forceinline__ void spmv_store(int row, XT s, XT *__restrict__ y, const XT *y0,
                                            double alpha, double inv_alpha, const XT *w,
                                            dd &acc0, dd &acc1) {
   if (MODE == SPMV_MULT || MODE == SPMV_DOT) {
     XT v = (s != XT(0)) ? s : XT(0);
-    y[row] = v;
+    y_store<NTY>(&y[row], v);
     if (MODE == SPMV_DOT) {
       dd_fma(acc0, (double)w[row], (double)v);
       dd_fma(acc1, (double)v, (double)v);
     }
   } else if (MODE == SPMV_MULTADD) {
-    y[row] = y0[row] + (XT)alpha * s;
+    y_store<NTY>(&y[row], y0[row] + (XT)alpha * s);
   } else if (MODE == SPMV_RESID) {
     const XT r = y0[row] + (XT)alpha * s;
     y[row] = r;
     dd_fma(acc1, (double)r, (double)r);
   } else {  // SPMV_TMULTADD: the initial value was folded into s
-    y[row] = s * (XT)alpha;
+    y_store<NTY>(&y[row], s * (XT)alpha);
   }
 }
 
@@ -88,7 +103,7 @@ __device__ __forceinline__ XT spmv_init(int row, const XT *y0, double inv_alpha)
 }
 
 // One row-block of y = A x (and the mode's epilogue) by one workgroup.
-template <typename VT, typename XT, int MODE>
+template <typename VT, typename XT, int MODE, bool NTY = false>
 __device__ __forceinline__ void spmv_rowblock(int b, XT *prod, const int2 *__restrict__ info,
                                               const int32_t *__restrict__ rp,
                                               const int32_t *__restrict__ ci,
@@ -112,7 +127,7 @@ __device__ __forceinline__ void spmv_rowblock(int b, XT *prod, const int2 *__res
       if (threadIdx.x == 0) {
         XT tot = (XT)v[0];
         if (MODE == SPMV_TMULTADD) tot = spmv_init<XT, MODE>(r0, y0, inv_alpha) + tot;
-        spmv_store<XT, MODE>(r0, tot, y, y0, alpha, inv_alpha, w, acc0, acc1);
+        spmv_store<XT, MODE, NTY>(r0, tot, y, y0, alpha, inv_alpha, w, acc0, acc1);
       }
       return;
     }
@@ -162,7 +177,7 @@ __device__ __forceinline__ void spmv_rowblock(int b, XT *prod, const int2 *__res
           const int row = r0 + g;
           XT s = spmv_init<XT, MODE>(row, y0, inv_alpha);
           for (int k = rs[u]; k < re[u]; ++k) s += prod[k];
-          spmv_store<XT, MODE>(row, s, y, y0, alpha, inv_alpha, w, acc0, acc1);
+          spmv_store<XT, MODE, NTY>(row, s, y, y0, alpha, inv_alpha, w, acc0, acc1);
         }
       }
     } else {
@@ -175,7 +190,7 @@ __device__ __forceinline__ void spmv_rowblock(int b, XT *prod, const int2 *__res
         XT s = (lane == 0) ? spmv_init<XT, MODE>(row, y0, inv_alpha) : XT(0);
         for (int k = beg + lane; k < end; k += tpr) s += prod[k];
         for (int off = tpr >> 1; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        if (lane == 0) spmv_store<XT, MODE>(row, s, y, y0, alpha, inv_alpha, w, acc0, acc1);
+        if (lane == 0) spmv_store<XT, MODE, NTY>(row, s, y, y0, alpha, inv_alpha, w, acc0, acc1);
       }
     }
   }
@@ -193,7 +208,7 @@ constexpr int SELL_UNROLL = 8;   // entries of a row in flight per lane
 // The matrix entries are read once per product, so they are loaded with the
 // non-temporal hint: they stream past L2 instead of evicting the lines of x
 // that neighbouring rows gather again (256^3 Poisson: 5.94 -> 6.43 TB/s).
-template <typename VT, typename XT, int MODE>
+template <typename VT, typename XT, int MODE, bool NTY = false>
 __device__ __forceinline__ void sell_row(int slice, int lane, int n, const int32_t *__restrict__ sptr,
                                          const int32_t *__restrict__ scol, const VT *__restrict__ sval,
                                          const XT *__restrict__ x, XT *y, const XT *y0, double alpha,
@@ -222,7 +237,7 @@ __device__ __forceinline__ void sell_row(int slice, int lane, int n, const int32
     for (int u = 0; u < SELL_UNROLL; ++u)
       if (c[u] >= 0) s += (XT)v[u] * xv[u];
   }
-  if (row < n) spmv_store<XT, MODE>(row, s, y, y0, alpha, inv_alpha, w, acc0, acc1);
+  if (row < n) spmv_store<XT, MODE, NTY>(row, s, y, y0, alpha, inv_alpha, w, acc0, acc1);
 }
 
 }  // namespace mtjg
