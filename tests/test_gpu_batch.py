@@ -162,6 +162,24 @@ def test_submit_checks_structure(ctx, grid):
         B.close()
 
 
+def test_persistent_step_matches_graph(ctx, monkeypatch):
+    """MTJG_BATCH_PERSISTENT=1 runs each step as one cooperative launch with
+    grid barriers instead of the captured graph of three launches per
+    iteration: the same systems end with bitwise the same solutions and
+    counts (every per-system sum adds its tiles' partials in tile order,
+    whichever workgroup produced them)."""
+    reqs, systems = _systems(10, seed=23, n_max=60000)
+    outs = []
+    for mode in ("0", "1"):
+        monkeypatch.setenv("MTJG_BATCH_PERSISTENT", mode)
+        outs.append(_run(ctx, systems))
+    for k in range(len(systems)):
+        g, p = outs[0][k], outs[1][k]
+        assert g[1] == 0 and p[1] == 0
+        assert g[2:4] == p[2:4], (k, g[2:4], p[2:4])
+        np.testing.assert_array_equal(g[0], p[0])
+
+
 def test_hbm_admitted_arena(ctx):
     """arena_bytes = 0: the arena starts empty and grows in chunks of device
     memory as systems are admitted, bounded by the free HBM hipMemGetInfo
