@@ -14,6 +14,7 @@
 // compiled with -ffp-contract=off) and therefore equal MTJ bitwise given the
 // same scalars; dot products and norms differ only in summation order.
 #include <algorithm>
+#include <initializer_list>
 #include <cstddef>
 #include <cmath>
 #include <cstring>
@@ -98,13 +99,129 @@ __device__ __forceinline__ void vst(T *p, T v) {
   else *p = v;
 }
 
-template <typename T, int PK, bool UPDATE, bool NT = false>
+// 16-byte accesses of the vector kernels' VEC form: a tile's elements are
+// taken in rounds of BS * VW (VW elements of 16 bytes per thread), so every
+// access instruction of a wavefront covers 1 KB contiguously; with 4-byte
+// elements the scalar form moves only 256 B per instruction (FP32 CG
+// residual step 4.0 TB/s).  Groups that run past n are done element by
+// element.
+template <typename T>
+struct V16;
+template <>
+struct V16<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct V16<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+template <bool NT, typename T>
+__device__ __forceinline__ void vld16(const T *p, T *o) {
+  using VT = typename V16<T>::type;
+  constexpr int VW = 16 / sizeof(T);
+  const VT v = NT ? __builtin_nontemporal_load((const VT *)p) : *(const VT *)p;
+#pragma unroll
+  for (int k = 0; k < VW; ++k) o[k] = v[k];
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void vst16(T *p, const T *o) {
+  using VT = typename V16<T>::type;
+  constexpr int VW = 16 / sizeof(T);
+  VT v;
+#pragma unroll
+  for (int k = 0; k < VW; ++k) v[k] = o[k];
+  if (NT) __builtin_nontemporal_store(v, (VT *)p);
+  else *(VT *)p = v;
+}
+// Element index of slot u (0 .. VPT-1) of this thread in the tile at `tile`.
+template <typename T>
+__device__ __forceinline__ int64_t vec_index(int64_t tile, int u) {
+  constexpr int VW = 16 / sizeof(T);
+  return tile + (int64_t)(u / VW) * (BS * VW) + (int64_t)threadIdx.x * VW + u % VW;
+}
+// Whether every pointer is 16-byte aligned (the VEC form's precondition).
+inline bool aligned16(std::initializer_list<const void *> ps) {
+  for (const void *p : ps)
+    if (p && ((uintptr_t)p & 15)) return false;
+  return true;
+}
+
+template <typename T, int PK, bool UPDATE, bool NT = false, bool VEC = false>
 __global__ __launch_bounds__(BS) void k_cg_resid(int64_t n, T *x, const T *p, T *r, const T *q,
                                                  const T *invdiag, T *z, KState *st, GridRed red) {
   if (st->status != ST_RUNNING) return;
   const T alpha = (T)st->alpha;
   dd v[2] = {{0.0, 0.0}, {0.0, 0.0}};
-  TILES(base) {
+  if (VEC) {
+    constexpr int VW = 16 / sizeof(T);
+    static_assert(VPT % VW == 0, "a thread's slots are whole 16-byte groups");
+    for (int64_t tile = (int64_t)blockIdx.x * (BS * VPT); tile < n; tile += (int64_t)gridDim.x * (BS * VPT)) {
+      T ri[VPT], xi[VPT], pi[VPT], qi[VPT], di[VPT];
+#pragma unroll
+      for (int g = 0; g < VPT; g += VW) {
+        const int64_t i = vec_index<T>(tile, g);
+        if (i + VW <= n) {
+          vld16<NT>(&r[i], &ri[g]);
+          if (UPDATE) {
+            vld16<NT>(&x[i], &xi[g]);
+            vld16<NT>(&p[i], &pi[g]);
+            vld16<NT>(&q[i], &qi[g]);
+          }
+          if (PK == PK_JACOBI) vld16<NT>(&invdiag[i], &di[g]);
+        } else {
+#pragma unroll
+          for (int k = 0; k < VW; ++k)
+            if (i + k < n) {
+              ri[g + k] = r[i + k];
+              if (UPDATE) {
+                xi[g + k] = x[i + k];
+                pi[g + k] = p[i + k];
+                qi[g + k] = q[i + k];
+              }
+              if (PK == PK_JACOBI) di[g + k] = invdiag[i + k];
+            }
+        }
+      }
+      const bool upd = UPDATE && alpha != T(0);
+#pragma unroll
+      for (int u = 0; u < VPT; ++u) {
+        if (upd) {
+          xi[u] = xi[u] + alpha * pi[u];
+          ri[u] = ri[u] + (-alpha) * qi[u];
+        }
+        if (PK == PK_JACOBI) di[u] = ri[u] * di[u];  // z
+      }
+#pragma unroll
+      for (int g = 0; g < VPT; g += VW) {
+        const int64_t i = vec_index<T>(tile, g);
+        if (i + VW <= n) {
+          if (upd) {
+            vst16<NT>(&x[i], &xi[g]);
+            vst16<NT>(&r[i], &ri[g]);
+          }
+          if (PK == PK_JACOBI) vst16<NT>(&z[i], &di[g]);
+        } else {
+#pragma unroll
+          for (int k = 0; k < VW; ++k)
+            if (i + k < n) {
+              if (upd) {
+                x[i + k] = xi[g + k];
+                r[i + k] = ri[g + k];
+              }
+              if (PK == PK_JACOBI) z[i + k] = di[g + k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < VW; ++k)
+          if (i + k < n) {
+            const int u = g + k;
+            if (PK == PK_JACOBI) dd_fma(v[0], (double)ri[u], (double)di[u]);
+            else if (PK == PK_IDENTITY) dd_fma(v[0], (double)ri[u], (double)ri[u]);
+            dd_fma(v[1], (double)ri[u], (double)ri[u]);
+          }
+      }
+    }
+  } else TILES(base) {
     T ri[VPT];
 #pragma unroll
     for (int u = 0; u < VPT; ++u) {
@@ -163,11 +280,32 @@ __global__ __launch_bounds__(BS) void k_cg_rho(int64_t n, const T *r, const T *z
 
 // p = z on the first iteration, else p.scale(beta).add(z) (S/CG.java:70-75):
 // two roundings, p*beta then + z.
-template <typename T, bool NT = false>
+template <typename T, bool NT = false, bool VEC = false>
 __global__ __launch_bounds__(BS) void k_cg_p(int64_t n, T *p, const T *z, const KState *st) {
   if (st->status != ST_RUNNING) return;
   const bool first = st->iter == 0;
   const T beta = (T)st->beta;
+  if (VEC) {
+    constexpr int VW = 16 / sizeof(T);
+    for (int64_t tile = (int64_t)blockIdx.x * (BS * VPT); tile < n; tile += (int64_t)gridDim.x * (BS * VPT)) {
+#pragma unroll
+      for (int g = 0; g < VPT; g += VW) {
+        const int64_t i = vec_index<T>(tile, g);
+        T pv[VW], zv[VW];
+        if (i + VW <= n) {
+          vld16<NT>(&z[i], zv);
+          if (!first) vld16<NT>(&p[i], pv);
+#pragma unroll
+          for (int k = 0; k < VW; ++k) pv[k] = first ? zv[k] : pv[k] * beta + zv[k];
+          vst16<NT>(&p[i], pv);
+        } else {
+          for (int k = 0; k < VW; ++k)
+            if (i + k < n) p[i + k] = first ? z[i + k] : p[i + k] * beta + z[i + k];
+        }
+      }
+    }
+    return;
+  }
   VLOOP(i) {
     if (first) {
       vst<NT>(&p[i], vld<NT>(&z[i]));
@@ -175,6 +313,31 @@ __global__ __launch_bounds__(BS) void k_cg_p(int64_t n, T *p, const T *z, const 
       T pb = vld<NT>(&p[i]) * beta;
       vst<NT>(&p[i], pb + vld<NT>(&z[i]));
     }
+  }
+}
+
+// Launches of the CG residual step and direction update, in the VEC form
+// when every operand is 16-byte aligned (always for the library's own
+// vectors; a caller's vector view may not be).
+template <typename T, int PK, bool UPDATE>
+void launch_cg_resid(bool nt, bool vec, int g, hipStream_t s, int64_t n, T *x, const T *p, T *r, const T *q,
+                     const T *invd, T *z, KState *st, const GridRed &red) {
+  if (nt) {
+    if (vec) k_cg_resid<T, PK, UPDATE, true, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, red);
+    else k_cg_resid<T, PK, UPDATE, true, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, red);
+  } else {
+    if (vec) k_cg_resid<T, PK, UPDATE, false, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, red);
+    else k_cg_resid<T, PK, UPDATE, false, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, red);
+  }
+}
+template <typename T>
+void launch_cg_p(bool nt, bool vec, int g, hipStream_t s, int64_t n, T *p, const T *z, const KState *st) {
+  if (nt) {
+    if (vec) k_cg_p<T, true, true><<<g, BS, 0, s>>>(n, p, z, st);
+    else k_cg_p<T, true, false><<<g, BS, 0, s>>>(n, p, z, st);
+  } else {
+    if (vec) k_cg_p<T, false, true><<<g, BS, 0, s>>>(n, p, z, st);
+    else k_cg_p<T, false, false><<<g, BS, 0, s>>>(n, p, z, st);
   }
 }
 
@@ -662,23 +825,19 @@ void cg_iterate(hipStream_t s, int64_t n, mtjg_csr A, mtjg_prec M, double *b, do
   spmv_f64(A, x, r, ra, s);
   const int g = vgrid(n);
   const bool nt = n * (int64_t)sizeof(double) >= NT_VECTOR_BYTES;
+  const bool vec = aligned16({x, w.p, w.p2, r, q, invd, z});
   KState *st = w.st;
   const GridRed &red = w.red;
   auto resid = [&](bool update) {
     if (pk == PK_IDENTITY) {
-      if (update) k_cg_resid<double, PK_IDENTITY, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, red);
-      else k_cg_resid<double, PK_IDENTITY, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, red);
+      if (update) launch_cg_resid<double, PK_IDENTITY, true>(false, vec, g, s, n, x, p, r, q, invd, z, st, red);
+      else launch_cg_resid<double, PK_IDENTITY, false>(false, vec, g, s, n, x, p, r, q, invd, z, st, red);
     } else if (pk == PK_JACOBI) {
-      if (nt) {
-        if (update) k_cg_resid<double, PK_JACOBI, true, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, red);
-        else k_cg_resid<double, PK_JACOBI, false, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, red);
-      } else {
-        if (update) k_cg_resid<double, PK_JACOBI, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, red);
-        else k_cg_resid<double, PK_JACOBI, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, red);
-      }
+      if (update) launch_cg_resid<double, PK_JACOBI, true>(nt, vec, g, s, n, x, p, r, q, invd, z, st, red);
+      else launch_cg_resid<double, PK_JACOBI, false>(nt, vec, g, s, n, x, p, r, q, invd, z, st, red);
     } else {
-      if (update) k_cg_resid<double, PK_EXTERNAL, true><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, red);
-      else k_cg_resid<double, PK_EXTERNAL, false><<<g, BS, 0, s>>>(n, x, p, r, q, invd, z, st, red);
+      if (update) launch_cg_resid<double, PK_EXTERNAL, true>(false, vec, g, s, n, x, p, r, q, invd, z, st, red);
+      else launch_cg_resid<double, PK_EXTERNAL, false>(false, vec, g, s, n, x, p, r, q, invd, z, st, red);
       prec_apply_dev(M, n, r, z, s, st);
       k_cg_rho<double><<<g, BS, 0, s>>>(n, r, z, st, red);
     }
@@ -702,8 +861,7 @@ void cg_iterate(hipStream_t s, int64_t n, mtjg_csr A, mtjg_prec M, double *b, do
       cur ^= 1;
       p = pbuf[cur];
     } else {
-      if (nt) k_cg_p<double, true><<<g, BS, 0, s>>>(n, p, z, st);
-      else k_cg_p<double><<<g, BS, 0, s>>>(n, p, z, st);
+      launch_cg_p<double>(nt, vec, g, s, n, p, z, st);
       spmv_f64(A, p, q, qa, s);
     }
     resid(true);
@@ -895,6 +1053,7 @@ void solve_mixed(mtjg_solver S, mtjg_csr A, double *b, double *x) {
         *z32 = scratch32(S, 4), *rin = scratch32(S, 5), *inv32 = scratch32(S, 6);
   csr_ensure_fp32(A);
   const int g = vgrid(n);
+  const bool vec32 = aligned16({d32, p32, rin, q32, inv32, z32});
   ensure_red(S, A);
   k_invdiag32<<<g, BS, 0, s>>>(n, A->rp, A->ci, A->val, inv32);
   KState *st = S->st, *si = S->st_in;
@@ -926,12 +1085,12 @@ void solve_mixed(mtjg_solver S, mtjg_csr A, double *b, double *x) {
     double itol = S->inner_rtol;
     if (k.residual > 0 && std::isfinite(k.residual)) itol = std::max(itol, std::min(0.5, 0.5 * target / k.residual));
     state_init(S, si, S->inner_max, itol, 0.0, 1e300);
-    k_cg_resid<float, PK_JACOBI, false><<<g, BS, 0, s>>>(n, d32, p32, rin, q32, inv32, z32, si, S->red);
+    launch_cg_resid<float, PK_JACOBI, false>(false, vec32, g, s, n, d32, p32, rin, q32, inv32, z32, si, S->red);
     HIPCHK(hipGetLastError());
     run_chunks(S, si, chunk, [&]() {
-      k_cg_p<float><<<g, BS, 0, s>>>(n, p32, z32, si);
+      launch_cg_p<float>(false, vec32, g, s, n, p32, z32, si);
       spmv_f32(A, p32, q32, qa, s);
-      k_cg_resid<float, PK_JACOBI, true><<<g, BS, 0, s>>>(n, d32, p32, rin, q32, inv32, z32, si, S->red);
+      launch_cg_resid<float, PK_JACOBI, true>(false, vec32, g, s, n, d32, p32, rin, q32, inv32, z32, si, S->red);
       HIPCHK(hipGetLastError());
     });
     KState ki;
@@ -1022,6 +1181,7 @@ void cg32_apply(mtjg_solver S, mtjg_csr A, const double *b, double *x, KState *o
   float *r32 = scratch32(S, 0), *d32 = scratch32(S, 1), *p32 = scratch32(S, 2), *q32 = scratch32(S, 3),
         *z32 = scratch32(S, 4), *rin = scratch32(S, 5), *inv32 = scratch32(S, 6);
   const int g = vgrid(n);
+  const bool vec32 = aligned16({d32, p32, rin, q32, inv32, z32});
   KState *st = S->st, *si = S->st_in;
   // ||b|| (st->normr), with a monitor that cannot stop: rtol = atol = 0
   state_init(S, st, 1, 0.0, 0.0, 1e300);
@@ -1044,12 +1204,12 @@ void cg32_apply(mtjg_solver S, mtjg_csr A, const double *b, double *x, KState *o
   k_demote_scaled<<<g, BS, 0, s>>>(n, b, r32, st);
   k_copy_f32_zero<<<g, BS, 0, s>>>(n, r32, rin, d32);
   state_init(S, si, S->inner_max, S->inner_rtol, 0.0, 1e300);
-  k_cg_resid<float, PK_JACOBI, false><<<g, BS, 0, s>>>(n, d32, p32, rin, q32, inv32, z32, si, S->red);
+  launch_cg_resid<float, PK_JACOBI, false>(false, vec32, g, s, n, d32, p32, rin, q32, inv32, z32, si, S->red);
   HIPCHK(hipGetLastError());
   run_chunks(S, si, chunk_for(A), [&]() {
-    k_cg_p<float><<<g, BS, 0, s>>>(n, p32, z32, si);
+    launch_cg_p<float>(false, vec32, g, s, n, p32, z32, si);
     spmv_f32(A, p32, q32, qa, s);
-    k_cg_resid<float, PK_JACOBI, true><<<g, BS, 0, s>>>(n, d32, p32, rin, q32, inv32, z32, si, S->red);
+    launch_cg_resid<float, PK_JACOBI, true>(false, vec32, g, s, n, d32, p32, rin, q32, inv32, z32, si, S->red);
     HIPCHK(hipGetLastError());
   });
   KState ki;
