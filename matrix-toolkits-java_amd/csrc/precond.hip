@@ -742,6 +742,7 @@ __global__ __launch_bounds__(BS) void k_fill_pending(int64_t n, double *x) {
 constexpr long long TILE_POLL_BUDGET = 1LL << 22;  // polls of one dependency (about 0.5 s)
 constexpr size_t TILE_STATIC_LDS = 64;             // bytes of LDS the tile kernel declares itself
 constexpr int TILE_DEPTH = 8;                      // steps in flight in a tile (slots of at most four)
+constexpr int TILE_IMP_PER = 4;                    // import indices a lane loads before the wait
 
 template <int KIND, int W>
 __global__ __launch_bounds__(FUSE_BS) void k_sweep_tiles(int32_t ntiles, const TileMeta *tiles, const int32_t *deps,
@@ -776,6 +777,15 @@ __global__ __launch_bounds__(FUSE_BS) void k_sweep_tiles(int32_t ntiles, const T
 #pragma unroll
     for (int j = 0; j < D; ++j)
       sweep_stage_load<KIND, W, true>(st[j], lm, min(j, nsteps - 1), tm.t0, lci, vals, pdv, bp, own, prow);
+    // the import indices do not depend on other tiles either: load them now,
+    // so that after the wait the imported values take one round trip
+    // (128^3 convection-diffusion: imports 1.44 -> 1.04 us per tile)
+    int32_t ii[TILE_IMP_PER];
+#pragma unroll
+    for (int u = 0; u < TILE_IMP_PER; ++u) {
+      const int32_t k = (int32_t)threadIdx.x + u * FUSE_BS;
+      ii[u] = k < tm.nimp ? imp[tm.imp0 + k] : 0;
+    }
     // wait for the source tiles: wavefront 0, one lane per source tile
     if (threadIdx.x < 64) {
       const int32_t lane = (int32_t)threadIdx.x;
@@ -803,7 +813,16 @@ __global__ __launch_bounds__(FUSE_BS) void k_sweep_tiles(int32_t ntiles, const T
     }
     __syncthreads();
     if (!__builtin_amdgcn_readfirstlane(s_skip)) {
-      for (int32_t k = threadIdx.x; k < tm.nimp; k += FUSE_BS)
+      double iv[TILE_IMP_PER];
+#pragma unroll
+      for (int u = 0; u < TILE_IMP_PER; ++u)
+        if ((int32_t)threadIdx.x + u * FUSE_BS < tm.nimp) iv[u] = load_agent(&xdep[ii[u]]);
+#pragma unroll
+      for (int u = 0; u < TILE_IMP_PER; ++u) {
+        const int32_t k = (int32_t)threadIdx.x + u * FUSE_BS;
+        if (k < tm.nimp) lx[nrows + k] = iv[u];
+      }
+      for (int32_t k = threadIdx.x + TILE_IMP_PER * FUSE_BS; k < tm.nimp; k += FUSE_BS)
         lx[nrows + k] = load_agent(&xdep[imp[tm.imp0 + k]]);
       __syncthreads();
       // every result is written out as it is computed, so that only the
