@@ -134,6 +134,35 @@ __device__ bool slot_sum(dd (&v)[K], BSlot &S, const BWork &w, const BDev &d, do
                i##_end = min((S).n, ((tile) + 1) * (S).rows_per_wg);               \
        i < i##_end; i += BS)
 
+// Rows of tile `tile` in groups of four: thread t owns rows g .. g+3 with
+// g = r0 + 4t + 4k*BS, so each FP32 access of a wavefront is one 1 KB
+// contiguous 16-byte-per-lane load or store (4-byte accesses moved 256 B per
+// instruction).  Every vector of a slot starts 256-byte aligned in the arena
+// and tiles are multiples of 256 rows, so a whole group is 16-byte aligned;
+// the group that runs past the system's last row is done row by row.
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+#define TILE_GROUPS(g, full, S, tile)                                                              \
+  for (int32_t g = (tile) * (S).rows_per_wg + 4 * (int32_t)threadIdx.x,                           \
+               g##_end = min((S).n, ((tile) + 1) * (S).rows_per_wg);                              \
+       g < g##_end; g += 4 * BS)                                                                   \
+    for (bool full = g + 4 <= g##_end, g##_once = true; g##_once; g##_once = false)
+__device__ __forceinline__ f4v ld4f(const float *p, int32_t g, int32_t end, bool full) {
+  if (full) return *(const f4v *)(p + g);
+  f4v v = {0.f, 0.f, 0.f, 0.f};
+  for (int k = 0; k < 4; ++k)
+    if (g + k < end) v[k] = p[g + k];
+  return v;
+}
+__device__ __forceinline__ void st4f(float *p, int32_t g, int32_t end, bool full, f4v v) {
+  if (full) {
+    *(f4v *)(p + g) = v;
+    return;
+  }
+  for (int k = 0; k < 4; ++k)
+    if (g + k < end) p[g + k] = v[k];
+}
+
 // The three phases of a batch iteration, each for one work-table entry `wi`
 // (a tile of one system), run by a whole workgroup.
 __device__ __forceinline__ void b1_work(const BDev &d, int wi) {
@@ -187,25 +216,50 @@ __device__ __forceinline__ void b2_work(const BDev &d, int wi) {
   dd v[2] = {{0.0, 0.0}, {0.0, 0.0}};
   if (phase == PH_START) {
     const double sc = 1.0 / S.rnorm;
-    TILE_ROWS(i, S, w.tile) {
-      const float ri = (float)(S.r[i] * sc);
-      S.rr[i] = ri;
-      S.d[i] = 0.0f;
-      const float zi = ri * S.inv[i];
-      S.z[i] = zi;
-      dd_fma(v[0], (double)ri, (double)zi);
-      dd_fma(v[1], (double)ri, (double)ri);
+    TILE_GROUPS(g, full, S, w.tile) {
+      const int32_t end = g_end;
+      double r4[4];
+      if (full) {
+        const d2v a0 = *(const d2v *)(S.r + g), a1 = *(const d2v *)(S.r + g + 2);
+        r4[0] = a0[0], r4[1] = a0[1], r4[2] = a1[0], r4[3] = a1[1];
+      } else {
+        for (int k = 0; k < 4; ++k) r4[k] = g + k < end ? S.r[g + k] : 0.0;
+      }
+      const f4v inv = ld4f(S.inv, g, end, full);
+      f4v rr, z;
+      for (int k = 0; k < 4; ++k) {
+        rr[k] = (float)(r4[k] * sc);
+        z[k] = rr[k] * inv[k];
+      }
+      st4f(S.rr, g, end, full, rr);
+      st4f(S.d, g, end, full, f4v{0.f, 0.f, 0.f, 0.f});
+      st4f(S.z, g, end, full, z);
+      for (int k = 0; k < 4; ++k)
+        if (full || g + k < end) {
+          dd_fma(v[0], (double)rr[k], (double)z[k]);
+          dd_fma(v[1], (double)rr[k], (double)rr[k]);
+        }
     }
   } else {
     const float a = (float)S.alpha;
-    TILE_ROWS(i, S, w.tile) {
-      S.d[i] = S.d[i] + a * S.p[i];
-      const float ri = S.rr[i] + (-a) * S.q[i];
-      S.rr[i] = ri;
-      const float zi = ri * S.inv[i];
-      S.z[i] = zi;
-      dd_fma(v[0], (double)ri, (double)zi);
-      dd_fma(v[1], (double)ri, (double)ri);
+    TILE_GROUPS(g, full, S, w.tile) {
+      const int32_t end = g_end;
+      f4v dv = ld4f(S.d, g, end, full), rr = ld4f(S.rr, g, end, full);
+      const f4v pv = ld4f(S.p, g, end, full), qv = ld4f(S.q, g, end, full), inv = ld4f(S.inv, g, end, full);
+      f4v z;
+      for (int k = 0; k < 4; ++k) {
+        dv[k] = dv[k] + a * pv[k];
+        rr[k] = rr[k] + (-a) * qv[k];
+        z[k] = rr[k] * inv[k];
+      }
+      st4f(S.d, g, end, full, dv);
+      st4f(S.rr, g, end, full, rr);
+      st4f(S.z, g, end, full, z);
+      for (int k = 0; k < 4; ++k)
+        if (full || g + k < end) {
+          dd_fma(v[0], (double)rr[k], (double)z[k]);
+          dd_fma(v[1], (double)rr[k], (double)rr[k]);
+        }
     }
   }
   double tot[2];
@@ -242,12 +296,17 @@ __device__ __forceinline__ void b3_work(const BDev &d, int wi) {
   if (!slot_live(S)) return;
   const int act = S.action3;
   if (act == A3_PINIT) {
-    TILE_ROWS(i, S, w.tile) S.p[i] = S.z[i];
+    TILE_GROUPS(g, full, S, w.tile) st4f(S.p, g, g_end, full, ld4f(S.z, g, g_end, full));
   } else if (act == A3_PUPDATE) {
     const float b = (float)S.beta;
-    TILE_ROWS(i, S, w.tile) {
-      const float pb = S.p[i] * b;
-      S.p[i] = pb + S.z[i];
+    TILE_GROUPS(g, full, S, w.tile) {
+      f4v pv = ld4f(S.p, g, g_end, full);
+      const f4v zv = ld4f(S.z, g, g_end, full);
+      for (int k = 0; k < 4; ++k) {
+        const float pb = pv[k] * b;
+        pv[k] = pb + zv[k];
+      }
+      st4f(S.p, g, g_end, full, pv);
     }
   } else if (act == A3_PROMOTE) {
     const double sc = S.rnorm;
