@@ -113,7 +113,7 @@ __device__ bool grid_sum_h(double (&v)[K], const GridRed &r) {
 #pragma unroll
     for (int k = 0; k < K; ++k) store_agent(&r.partials[k * r.cap + b], v[k]);
     stores_complete();
-    uint32_t t = take_ticket(&r.gticket[g]);
+    uint32_t t = take_ticket(&r.gticket[g * TICKET_STRIDE]);
     last = (t == (uint32_t)gsize - 1);
   }
   __syncthreads();
@@ -126,7 +126,7 @@ __device__ bool grid_sum_h(double (&v)[K], const GridRed &r) {
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; ++k) store_agent(&r.gpart[k * r.gcap + g], acc[k]);
-    __hip_atomic_store(&r.gticket[g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&r.gticket[g * TICKET_STRIDE], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     stores_complete();
     uint32_t t = take_ticket(r.fticket);
     last = (t == (uint32_t)ngroups - 1);
@@ -225,7 +225,7 @@ __device__ bool grid_sum_dd(dd (&v)[K], const GridRed &r, double (&out)[K]) {
       store_agent(&r.partials[(2 * k + 1) * r.cap + b], v[k].lo);
     }
     stores_complete();
-    uint32_t t = take_ticket(&r.gticket[g]);
+    uint32_t t = take_ticket(&r.gticket[g * TICKET_STRIDE]);
     last = (t == (uint32_t)gsize - 1);
   }
   __syncthreads();
@@ -245,7 +245,8 @@ __device__ bool grid_sum_dd(dd (&v)[K], const GridRed &r, double (&out)[K]) {
     // the group sum folded once into an exact zero, which is what the second
     // level's first step does (it only turns a negative zero positive), so
     // the result is bitwise the two-level one, one atomic round trip sooner.
-    if (threadIdx.x == 0) __hip_atomic_store(&r.gticket[g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0)
+      __hip_atomic_store(&r.gticket[g * TICKET_STRIDE], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const dd z = dd_add(dd{0.0, 0.0}, acc[k]);
@@ -259,7 +260,7 @@ __device__ bool grid_sum_dd(dd (&v)[K], const GridRed &r, double (&out)[K]) {
       store_agent(&r.gpart[(2 * k) * r.gcap + g], acc[k].hi);
       store_agent(&r.gpart[(2 * k + 1) * r.gcap + g], acc[k].lo);
     }
-    __hip_atomic_store(&r.gticket[g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&r.gticket[g * TICKET_STRIDE], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     stores_complete();
     uint32_t t = take_ticket(r.fticket);
     last = (t == (uint32_t)ngroups - 1);

@@ -139,6 +139,53 @@ __device__ __forceinline__ int64_t vec_index(int64_t tile, int u) {
   constexpr int VW = 16 / sizeof(T);
   return tile + (int64_t)(u / VW) * (BS * VW) + (int64_t)threadIdx.x * VW + u % VW;
 }
+// This thread's VPT elements of the tile at `tile` for the VEC form: loads
+// of any number of vectors, all issued before any value is used.  A tile
+// that lies wholly below n (the test is uniform over the workgroup) takes
+// straight-line 16-byte loads with no branch between them; the last tile is
+// read element by element.  Each operand is a (source, registers) pair.
+struct VIo {
+  const double *p;
+  double *v;
+};
+__device__ __forceinline__ bool vtile_full(int64_t n, int64_t tile) { return tile + BS * VPT <= n; }
+template <bool NT = false, typename... Io>
+__device__ __forceinline__ void vload_tile(int64_t n, int64_t tile, Io... io) {
+  constexpr int VW = 2;
+  if (vtile_full(n, tile)) {
+#pragma unroll
+    for (int g = 0; g < VPT; g += VW) {
+      const int64_t i = vec_index<double>(tile, g);
+      (vld16<NT>(io.p + i, io.v + g), ...);
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < VPT; ++u) {
+      const int64_t i = vec_index<double>(tile, u);
+      if (i < n) ((io.v[u] = io.p[i]), ...);
+    }
+  }
+}
+__device__ __forceinline__ void vstore_tile(int64_t n, int64_t tile, double *a, const double *av) {
+  constexpr int VW = 2;
+  if (vtile_full(n, tile)) {
+#pragma unroll
+    for (int g = 0; g < VPT; g += VW) vst16<false>(a + vec_index<double>(tile, g), av + g);
+  } else {
+#pragma unroll
+    for (int u = 0; u < VPT; ++u) {
+      const int64_t i = vec_index<double>(tile, u);
+      if (i < n) a[i] = av[u];
+    }
+  }
+}
+// Whether slot u of the tile at `tile` is an element (< n).
+__device__ __forceinline__ bool vslot_ok(int64_t n, int64_t tile, int u) {
+  return vtile_full(n, tile) || vec_index<double>(tile, u) < n;
+}
+#define VEC_TILES(tile) \
+  for (int64_t tile = (int64_t)blockIdx.x * (BS * VPT); tile < n; tile += (int64_t)gridDim.x * (BS * VPT))
+
 // Whether every pointer is 16-byte aligned (the VEC form's precondition).
 inline bool aligned16(std::initializer_list<const void *> ps) {
   for (const void *p : ps)
@@ -349,14 +396,36 @@ void launch_cg_p(bool nt, bool vec, int g, hipStream_t s, int64_t n, T *p, const
 // r.r and rtilde.r.  The last workgroup checks the monitor on ||r||, sets
 // rho_1 = rtilde.r, raises breakdown when rho_1 == 0 or omega == 0
 // (:77-83) and forms beta (:88).
-template <bool UPDATE>
+template <bool UPDATE, bool VEC = false>
 __global__ __launch_bounds__(BS) void k_bicg_top(int64_t n, double *x, const double *shat, double *r,
                                                  const double *s, const double *t,
                                                  const double *rtilde, KState *st, GridRed red) {
   if (st->status != ST_RUNNING) return;
   const double omega = st->omega;
   dd v[2] = {{0.0, 0.0}, {0.0, 0.0}};
-  VLOOP(i) {
+  if (VEC) {
+    VEC_TILES(tile) {
+      double xv[VPT], hv[VPT], rv[VPT], tv[VPT], qv[VPT];
+      if (UPDATE) vload_tile(n, tile, VIo{x, xv}, VIo{shat, hv}, VIo{s, rv}, VIo{t, tv});
+      else vload_tile(n, tile, VIo{r, rv});
+      vload_tile(n, tile, VIo{rtilde, qv});
+      if (UPDATE && omega != 0) {
+#pragma unroll
+        for (int u = 0; u < VPT; ++u) {
+          xv[u] = xv[u] + omega * hv[u];
+          rv[u] = rv[u] + (-omega) * tv[u];
+        }
+        vstore_tile(n, tile, x, xv);
+      }
+      if (UPDATE) vstore_tile(n, tile, r, rv);
+#pragma unroll
+      for (int u = 0; u < VPT; ++u)
+        if (vslot_ok(n, tile, u)) {
+          dd_fma(v[0], rv[u], rv[u]);
+          dd_fma(v[1], qv[u], rv[u]);
+        }
+    }
+  } else VLOOP(i) {
     double ri;
     if (UPDATE) {
       if (omega != 0) x[i] = x[i] + omega * shat[i];
@@ -390,11 +459,33 @@ __global__ __launch_bounds__(BS) void k_bicg_top(int64_t n, double *x, const dou
 }
 
 // p = r (first) or p = r + beta * ((-omega) v + p) (S/BiCGstab.java:85-95).
+template <bool VEC = false>
 __global__ __launch_bounds__(BS) void k_bicg_p(int64_t n, double *p, const double *r, const double *v,
                                                const KState *st) {
   if (st->status != ST_RUNNING) return;
   const bool first = st->iter == 0;
   const double beta = st->beta, omega = st->omega;
+  if (VEC) {
+    VEC_TILES(tile) {
+      double pv[VPT], rv[VPT], vv[VPT];
+      if (first) {
+        vload_tile(n, tile, VIo{r, rv});
+        vstore_tile(n, tile, p, rv);
+        continue;
+      }
+      vload_tile(n, tile, VIo{r, rv}, VIo{v, vv}, VIo{p, pv});
+#pragma unroll
+      for (int u = 0; u < VPT; ++u) {
+        double temp = (-omega) * vv[u];
+        temp = temp + pv[u];
+        double pi = rv[u];
+        if (beta != 0) pi = pi + beta * temp;
+        pv[u] = pi;
+      }
+      vstore_tile(n, tile, p, pv);
+    }
+    return;
+  }
   VLOOP(i) {
     if (first) {
       p[i] = r[i];
@@ -410,12 +501,30 @@ __global__ __launch_bounds__(BS) void k_bicg_p(int64_t n, double *p, const doubl
 
 // s = r + (-alpha) v; x += alpha phat; then the half-step check on ||s||
 // (S/BiCGstab.java:100-104).
+template <bool VEC = false>
 __global__ __launch_bounds__(BS) void k_bicg_s(int64_t n, double *x, const double *phat, const double *r,
                                                const double *v, double *s, KState *st, GridRed red) {
   if (st->status != ST_RUNNING) return;
   const double alpha = st->alpha;
   dd acc[1] = {{0.0, 0.0}};
-  VLOOP(i) {
+  if (VEC) {
+    VEC_TILES(tile) {
+      double rv[VPT], vv[VPT], xv[VPT], hv[VPT];
+      vload_tile(n, tile, VIo{r, rv}, VIo{v, vv}, VIo{x, xv}, VIo{phat, hv});
+      if (alpha != 0) {
+#pragma unroll
+        for (int u = 0; u < VPT; ++u) {
+          rv[u] = rv[u] + (-alpha) * vv[u];
+          xv[u] = xv[u] + alpha * hv[u];
+        }
+        vstore_tile(n, tile, x, xv);
+      }
+      vstore_tile(n, tile, s, rv);
+#pragma unroll
+      for (int u = 0; u < VPT; ++u)
+        if (vslot_ok(n, tile, u)) dd_fma(acc[0], rv[u], rv[u]);
+    }
+  } else VLOOP(i) {
     double si = r[i];
     if (alpha != 0) {
       si = si + (-alpha) * v[i];
@@ -475,15 +584,27 @@ __global__ __launch_bounds__(BS) void k_gmres_start(int64_t n, const double *r, 
 }
 
 // v0 = (1/normr) r, with DenseVector.set(alpha, y)'s alpha == 0 zeroing.
+template <bool VEC = false>
 __global__ __launch_bounds__(BS) void k_gmres_v0(int64_t n, const double *r, double *v0, const KState *st) {
   if (st->status != ST_RUNNING) return;
   const double a = 1 / st->normr;
+  if (VEC) {
+    VEC_TILES(tile) {
+      double rv[VPT];
+      vload_tile(n, tile, VIo{r, rv});
+#pragma unroll
+      for (int u = 0; u < VPT; ++u) rv[u] = (a == 0) ? 0.0 : a * rv[u];
+      vstore_tile(n, tile, v0, rv);
+    }
+    return;
+  }
   VLOOP(i)
     v0[i] = (a == 0) ? 0.0 : a * r[i];
 }
 
 // Modified Gram-Schmidt step k of Arnoldi column i: w += (-H(k-1,i)) v_{k-1}
 // (S/GMRES.java:151) for k > 0, then H(k,i) = w.v_k (:150).
+template <bool VEC = false>
 __global__ __launch_bounds__(BS) void k_gmres_mgs(int64_t n, double *w, const double *vprev,
                                                   const double *vk, int k, int i, KState *st,
                                                   GmresDev g, GridRed red) {
@@ -491,7 +612,22 @@ __global__ __launch_bounds__(BS) void k_gmres_mgs(int64_t n, double *w, const do
   const int ld = g.m + 1;
   const double h = (k > 0) ? -g.H[(k - 1) + (size_t)i * ld] : 0.0;
   dd v[1] = {{0.0, 0.0}};
-  VLOOP(e) {
+  if (VEC) {
+    VEC_TILES(tile) {
+      double wv[VPT], pv[VPT], kv[VPT];
+      if (h != 0) {
+        vload_tile(n, tile, VIo{w, wv}, VIo{vprev, pv}, VIo{vk, kv});
+#pragma unroll
+        for (int u = 0; u < VPT; ++u) wv[u] = wv[u] + h * pv[u];
+        vstore_tile(n, tile, w, wv);
+      } else {
+        vload_tile(n, tile, VIo{w, wv}, VIo{vk, kv});
+      }
+#pragma unroll
+      for (int u = 0; u < VPT; ++u)
+        if (vslot_ok(n, tile, u)) dd_fma(v[0], wv[u], kv[u]);
+    }
+  } else VLOOP(e) {
     double we = w[e];
     if (h != 0) {
       we = we + h * vprev[e];
@@ -529,13 +665,29 @@ __device__ void givens_apply(double c, double s, double &a1, double &a2) {
 // (S/GMRES.java:153); the last workgroup applies the previous rotations to
 // column i, forms rotation i, rotates H and s (:157-162), advances i and the
 // iteration counter, and runs the next inner check (:144).
+template <bool VEC = false>
 __global__ __launch_bounds__(BS) void k_gmres_arnoldi_end(int64_t n, double *w, const double *vi, int i,
                                                           KState *st, GmresDev g, GridRed red) {
   if (st->status != ST_RUNNING) return;
   const int ld = g.m + 1;
   const double h = -g.H[i + (size_t)i * ld];
   dd v[1] = {{0.0, 0.0}};
-  VLOOP(e) {
+  if (VEC) {
+    VEC_TILES(tile) {
+      double wv[VPT], pv[VPT];
+      if (h != 0) {
+        vload_tile(n, tile, VIo{w, wv}, VIo{vi, pv});
+#pragma unroll
+        for (int u = 0; u < VPT; ++u) wv[u] = wv[u] + h * pv[u];
+        vstore_tile(n, tile, w, wv);
+      } else {
+        vload_tile(n, tile, VIo{w, wv});
+      }
+#pragma unroll
+      for (int u = 0; u < VPT; ++u)
+        if (vslot_ok(n, tile, u)) dd_fma(v[0], wv[u], wv[u]);
+    }
+  } else VLOOP(e) {
     double we = w[e];
     if (h != 0) {
       we = we + h * vi[e];
@@ -566,9 +718,20 @@ __global__ __launch_bounds__(BS) void k_gmres_arnoldi_end(int64_t n, double *w, 
 }
 
 // v_{i+1} = (1/H(i+1,i)) w (S/GMRES.java:154).
+template <bool VEC = false>
 __global__ __launch_bounds__(BS) void k_gmres_scale(int64_t n, const double *w, double *vnext, const KState *st) {
   if (st->status != ST_RUNNING) return;
   const double a = 1. / st->s[2];
+  if (VEC) {
+    VEC_TILES(tile) {
+      double wv[VPT];
+      vload_tile(n, tile, VIo{w, wv});
+#pragma unroll
+      for (int u = 0; u < VPT; ++u) wv[u] = (a == 0) ? 0.0 : a * wv[u];
+      vstore_tile(n, tile, vnext, wv);
+    }
+    return;
+  }
   VLOOP(e)
     vnext[e] = (a == 0) ? 0.0 : a * w[e];
 }
@@ -776,9 +939,9 @@ void ensure_red(mtjg_solver S, mtjg_csr A) {
   r.gcap = (need + RED_GROUP - 1) / RED_GROUP;
   HIPCHK(hipMalloc(&r.partials, sizeof(double) * RED_MAX_K * (size_t)r.cap));
   HIPCHK(hipMalloc(&r.gpart, sizeof(double) * RED_MAX_K * (size_t)r.gcap));
-  HIPCHK(hipMalloc(&r.gticket, sizeof(uint32_t) * (size_t)(r.gcap + 1)));
-  HIPCHK(hipMemsetAsync(r.gticket, 0, sizeof(uint32_t) * (size_t)(r.gcap + 1), S->ctx->stream));
-  r.fticket = r.gticket + r.gcap;
+  HIPCHK(hipMalloc(&r.gticket, sizeof(uint32_t) * red_ticket_words(r.gcap)));
+  HIPCHK(hipMemsetAsync(r.gticket, 0, sizeof(uint32_t) * red_ticket_words(r.gcap), S->ctx->stream));
+  r.fticket = r.gticket + (size_t)r.gcap * TICKET_STRIDE;
   S->red = r;
 }
 
@@ -913,7 +1076,9 @@ void solve_bicgstab(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   if (n) HIPCHK(hipMemcpyAsync(rtilde, r, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
   const int g = vgrid(n);
   ensure_red(S, A);
-  k_bicg_top<false><<<g, BS, 0, s>>>(n, x, shat, r, sv, t, rtilde, st, S->red);
+  const bool vec = aligned16({x, shat, r, sv, t, rtilde, p, phat, v});
+  if (vec) k_bicg_top<false, true><<<g, BS, 0, s>>>(n, x, shat, r, sv, t, rtilde, st, S->red);
+  else k_bicg_top<false><<<g, BS, 0, s>>>(n, x, shat, r, sv, t, rtilde, st, S->red);
   HIPCHK(hipGetLastError());
   SpmvArgs va;
   va.mode = SPMV_DOT;
@@ -925,14 +1090,17 @@ void solve_bicgstab(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   ta.w = sv;
   ta.fin = FIN_BICG_OMEGA;
   run_chunks(S, st, pk == PK_EXTERNAL ? 1 : chunk_for(A), [&]() {
-    k_bicg_p<<<g, BS, 0, s>>>(n, p, r, v, st);
+    if (vec) k_bicg_p<true><<<g, BS, 0, s>>>(n, p, r, v, st);
+    else k_bicg_p<<<g, BS, 0, s>>>(n, p, r, v, st);
     if (pk != PK_IDENTITY) prec_apply_dev(S->M, n, p, phat, s, st);
     spmv_f64(A, phat, v, va, s);
-    k_bicg_s<<<g, BS, 0, s>>>(n, x, phat, r, v, sv, st, S->red);
+    if (vec) k_bicg_s<true><<<g, BS, 0, s>>>(n, x, phat, r, v, sv, st, S->red);
+    else k_bicg_s<<<g, BS, 0, s>>>(n, x, phat, r, v, sv, st, S->red);
     if (pk != PK_IDENTITY) prec_apply_dev(S->M, n, sv, shat, s, st);
     spmv_f64(A, shat, t, ta, s);
     k_bicg_omega<<<1, 1, 0, s>>>(st);
-    k_bicg_top<true><<<g, BS, 0, s>>>(n, x, shat, r, sv, t, rtilde, st, S->red);
+    if (vec) k_bicg_top<true, true><<<g, BS, 0, s>>>(n, x, shat, r, sv, t, rtilde, st, S->red);
+    else k_bicg_top<true><<<g, BS, 0, s>>>(n, x, shat, r, sv, t, rtilde, st, S->red);
     HIPCHK(hipGetLastError());
   });
 }
@@ -971,6 +1139,8 @@ void solve_gmres(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   const int g = vgrid(n);
   ensure_red(S, A);
   auto Vk = [&](int k) { return S->V + (size_t)k * (size_t)n; };
+  // the basis vectors are 16-byte aligned when n is even (the basis is one allocation)
+  const bool vec = aligned16({w, u, r, S->V}) && n % 2 == 0;
   SpmvArgs ra;
   ra.mode = SPMV_MULTADD;
   ra.alpha = -1;
@@ -989,14 +1159,23 @@ void solve_gmres(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   for (;;) {
     state_read(S, st, k);
     if (k.status != ST_RUNNING) break;
-    k_gmres_v0<<<g, BS, 0, s>>>(n, r, Vk(0), st);
+    if (vec) k_gmres_v0<true><<<g, BS, 0, s>>>(n, r, Vk(0), st);
+    else k_gmres_v0<<<g, BS, 0, s>>>(n, r, Vk(0), st);
     for (int i = 0; i < S->restart && !k.cycle_done; ++i) {
       spmv_f64(A, Vk(i), u, ma, s);
       prec_apply_dev(S->M, n, u, w, s, nullptr);
       for (int kk = 0; kk <= i; ++kk)
-        k_gmres_mgs<<<g, BS, 0, s>>>(n, w, kk > 0 ? Vk(kk - 1) : nullptr, Vk(kk), kk, i, st, gd, S->red);
-      k_gmres_arnoldi_end<<<g, BS, 0, s>>>(n, w, Vk(i), i, st, gd, S->red);
-      k_gmres_scale<<<g, BS, 0, s>>>(n, w, Vk(i + 1), st);
+        if (vec)
+          k_gmres_mgs<true><<<g, BS, 0, s>>>(n, w, kk > 0 ? Vk(kk - 1) : nullptr, Vk(kk), kk, i, st, gd, S->red);
+        else
+          k_gmres_mgs<<<g, BS, 0, s>>>(n, w, kk > 0 ? Vk(kk - 1) : nullptr, Vk(kk), kk, i, st, gd, S->red);
+      if (vec) {
+        k_gmres_arnoldi_end<true><<<g, BS, 0, s>>>(n, w, Vk(i), i, st, gd, S->red);
+        k_gmres_scale<true><<<g, BS, 0, s>>>(n, w, Vk(i + 1), st);
+      } else {
+        k_gmres_arnoldi_end<<<g, BS, 0, s>>>(n, w, Vk(i), i, st, gd, S->red);
+        k_gmres_scale<<<g, BS, 0, s>>>(n, w, Vk(i + 1), st);
+      }
       HIPCHK(hipGetLastError());
       state_read(S, st, k);
       if (k.status != ST_RUNNING) break;
@@ -1499,7 +1678,7 @@ constexpr size_t NOALLOC_OFF_STATE = 5 * sizeof(double) * (size_t)NOALLOC_VEC;
 constexpr size_t NOALLOC_OFF_PART = NOALLOC_OFF_STATE + 256;
 constexpr size_t NOALLOC_OFF_GPART = NOALLOC_OFF_PART + sizeof(double) * RED_MAX_K * (size_t)NOALLOC_RED_CAP;
 constexpr size_t NOALLOC_OFF_TICKET = NOALLOC_OFF_GPART + sizeof(double) * RED_MAX_K * (size_t)NOALLOC_RED_GCAP;
-constexpr size_t NOALLOC_BYTES = NOALLOC_OFF_TICKET + sizeof(uint32_t) * (NOALLOC_RED_GCAP + 1);
+constexpr size_t NOALLOC_BYTES = NOALLOC_OFF_TICKET + sizeof(uint32_t) * (NOALLOC_RED_GCAP + 1) * TICKET_STRIDE;
 static_assert(sizeof(KState) <= 256, "solver state exceeds its slot");
 static_assert(NOALLOC_BYTES <= (size_t)MTJG_NOALLOC_SCRATCH_BYTES, "scratch layout exceeds the array");
 
@@ -1730,11 +1909,11 @@ extern "C" int mtjg_cg_solve_noalloc(mtjg_ctx ctx, mtjg_csr A, mtjg_prec M, mtjg
   w.red.partials = reinterpret_cast<double *>(base + NOALLOC_OFF_PART);
   w.red.gpart = reinterpret_cast<double *>(base + NOALLOC_OFF_GPART);
   w.red.gticket = reinterpret_cast<uint32_t *>(base + NOALLOC_OFF_TICKET);
-  w.red.fticket = w.red.gticket + NOALLOC_RED_GCAP;
+  w.red.fticket = w.red.gticket + (size_t)NOALLOC_RED_GCAP * TICKET_STRIDE;
   w.red.cap = NOALLOC_RED_CAP;
   w.red.gcap = NOALLOC_RED_GCAP;
   k_noalloc_init<<<1, BS, 0, s>>>(w.st, mon.max_iter, mon.rtol, mon.atol, mon.dtol, w.red.gticket,
-                                  NOALLOC_RED_GCAP + 1);
+                                  (int)red_ticket_words(NOALLOC_RED_GCAP));
   HIPCHK(hipGetLastError());
   // The status word (with the residual and iteration count beside it) lands
   // in the context's pinned scalars, allocated when the context was made.

@@ -207,7 +207,7 @@ __global__ __launch_bounds__(BS) void k_sell_part(int nslice, int n, const int32
   }
 }
 
-// Adds k_sell_part's np partials (RED_GROUP per workgroup, then the grid
+// Adds k_sell_part's np partials (BS per workgroup, then the grid
 // reduction, whose block partials sit below `part`) and runs the epilogue.
 __global__ __launch_bounds__(BS) void k_sell_fin(int np, const double *part, KState *st, int fin, GridRed red) {
   if (st->status != ST_RUNNING) return;
