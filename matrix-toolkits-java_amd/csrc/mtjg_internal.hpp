@@ -77,14 +77,6 @@ struct KState {
   double normr;
 };
 
-// Tiny deterministic-reduction workspace shared by the context's BLAS-1
-// calls.
-struct RedWS {
-  double *partials;  // [RED_MAX_K * RED_MAX_BLOCKS]
-  uint32_t *ticket;  // [8]
-  double *out;       // [8]
-};
-
 // Workspace of the two-level grid reduction (grid_sum_h in reduce.hpp) for
 // grids of up to `cap` workgroups in groups of RED_GROUP.  Atomic updates of
 // one memory line are serialised, so each ticket counter sits on a line of
@@ -103,6 +95,14 @@ struct GridRed {
 };
 // uint32 words of a GridRed's counters (group counters, then the final one).
 inline size_t red_ticket_words(int gcap) { return (size_t)(gcap + 1) * TICKET_STRIDE; }
+
+// Deterministic-reduction workspace of the context's BLAS-1 calls: a
+// two-level reduction workspace for RED_MAX_BLOCKS workgroups and the
+// device-side results.
+struct RedWS {
+  GridRed g;
+  double *out;  // [8]
+};
 
 // Grid size for an elementwise or reducing kernel over n items.
 inline int grid_for(int64_t n, int per_thread = 4, int cap = 2048) {
@@ -172,9 +172,6 @@ void launch_scale(hipStream_t s, int64_t n, double *x, double alpha);
 void launch_set_scaled(hipStream_t s, int64_t n, double *x, double alpha, const double *y);
 void launch_add_scaled(hipStream_t s, int64_t n, double *x, double alpha, const double *y);
 void launch_add(hipStream_t s, int64_t n, double *x, const double *y);
-// Deterministic two-stage dot / sum-of-squares; result left in red.out[slot].
-void launch_dot(hipStream_t s, int64_t n, const double *x, const double *y, const RedWS &red,
-                int slot);
 double read_scalar(mtjg_ctx ctx, const double *dev);
 
 // ---- SpMV (spmv.hip) ----

@@ -844,8 +844,8 @@ __global__ __launch_bounds__(FUSE_BS) void k_sweep_tiles(int32_t ntiles, const T
     if (threadIdx.x == 0) __hip_atomic_store(&flags[tile], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   // the last workgroup out leaves both counters at zero for the next launch
-  if (threadIdx.x == 0 && take_ticket(&ctr[1]) == gridDim.x - 1) {
-    __hip_atomic_store(&ctr[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0 && take_ticket(&ctr[TICKET_STRIDE]) == gridDim.x - 1) {
+    __hip_atomic_store(&ctr[TICKET_STRIDE], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&ctr[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
@@ -1342,8 +1342,9 @@ static bool ensure_pack(mtjg_prec M, Levels &L, int form, const std::vector<int3
       P.tdeps = to_device(tdeps, s);
       HIPCHK(hipMalloc(&P.tflags, sizeof(uint32_t) * (size_t)std::max(P.ntiles, 1)));
       HIPCHK(hipMemsetAsync(P.tflags, 0, sizeof(uint32_t) * (size_t)std::max(P.ntiles, 1), s));
-      HIPCHK(hipMalloc(&P.tctr, sizeof(uint32_t) * 2));
-      HIPCHK(hipMemsetAsync(P.tctr, 0, sizeof(uint32_t) * 2, s));
+      // the dispatch and exit counters, each on a memory line of its own
+      HIPCHK(hipMalloc(&P.tctr, sizeof(uint32_t) * 2 * TICKET_STRIDE));
+      HIPCHK(hipMemsetAsync(P.tctr, 0, sizeof(uint32_t) * 2 * TICKET_STRIDE, s));
       P.epoch = 0;
     }
     P.any_lds = false;

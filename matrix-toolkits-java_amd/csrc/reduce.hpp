@@ -61,40 +61,6 @@ __device__ __forceinline__ uint32_t take_ticket(uint32_t *t) {
   return __hip_atomic_fetch_add(t, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Grid-wide sum.  Every workgroup publishes its K partial sums to
-// partials[k * RED_MAX_BLOCKS + blockIdx.x] and takes a ticket; the
-// workgroup that takes the last ticket adds all partials in index order and
-// returns true with the grid totals in v (every thread).  Others return
-// false.  The ticket is reset by the last workgroup, so the counter is ready
-// for the next launch without a memset.
-template <int K>
-__device__ bool grid_sum(double (&v)[K], double *partials, uint32_t *ticket) {
-  __shared__ int last;
-  block_sum<K>(v);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) store_agent(&partials[k * RED_MAX_BLOCKS + blockIdx.x], v[k]);
-    stores_complete();
-    uint32_t t = take_ticket(ticket);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return false;
-  double acc[K];
-#pragma unroll
-  for (int k = 0; k < K; ++k) acc[k] = 0.0;
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += BS) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] += load_agent(&partials[k * RED_MAX_BLOCKS + i]);
-  }
-  block_sum<K>(acc);
-#pragma unroll
-  for (int k = 0; k < K; ++k) v[k] = acc[k];
-  if (threadIdx.x == 0)
-    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return true;
-}
-
 // Two-level grid-wide sum for grids of any size.  Workgroup b publishes its K
 // partials to partials[k * cap + b] and takes a ticket on its group of
 // RED_GROUP consecutive workgroups; the last workgroup of a group adds the
@@ -142,6 +108,73 @@ __device__ bool grid_sum_h(double (&v)[K], const GridRed &r) {
   block_sum<K>(acc);
 #pragma unroll
   for (int k = 0; k < K; ++k) v[k] = acc[k];
+  if (threadIdx.x == 0) __hip_atomic_store(r.fticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
+// Two-level grid-wide combination of per-workgroup values by an operation
+// that need not be a sum (maximum, scaled sum of squares), in a fixed order.
+// On entry thread 0 holds its workgroup's K values in v; the protocol is
+// grid_sum_h's, except that a finishing workgroup has thread 0 combine the
+// values left to right, comb(acc, next) with acc starting at the first.
+// Returns true in the last workgroup, with the grid's values in thread 0's v.
+template <int K, typename Comb>
+__device__ bool grid_combine_h(double (&v)[K], const GridRed &r, Comb comb) {
+  static_assert(K <= RED_MAX_K, "too many values");
+  __shared__ int last;
+  __shared__ double sh[K][RED_GROUP];
+  const int b = blockIdx.x, g = b / RED_GROUP;
+  const int ngroups = (gridDim.x + RED_GROUP - 1) / RED_GROUP;
+  const int gsize = min(RED_GROUP, (int)gridDim.x - g * RED_GROUP);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) store_agent(&r.partials[k * r.cap + b], v[k]);
+    stores_complete();
+    uint32_t t = take_ticket(&r.gticket[g * TICKET_STRIDE]);
+    last = (t == (uint32_t)gsize - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+  auto fold = [&](int cnt) {
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[k] = sh[k][0];
+      for (int j = 1; j < cnt; ++j) {
+        double nx[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) nx[k] = sh[k][j];
+        comb(v, nx);
+      }
+    }
+  };
+  if ((int)threadIdx.x < gsize) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) sh[k][threadIdx.x] = load_agent(&r.partials[k * r.cap + g * RED_GROUP + threadIdx.x]);
+  }
+  __syncthreads();
+  fold(gsize);
+  if (ngroups == 1) {
+    if (threadIdx.x == 0)
+      __hip_atomic_store(&r.gticket[g * TICKET_STRIDE], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) store_agent(&r.gpart[k * r.gcap + g], v[k]);
+    __hip_atomic_store(&r.gticket[g * TICKET_STRIDE], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    stores_complete();
+    uint32_t t = take_ticket(r.fticket);
+    last = (t == (uint32_t)ngroups - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+  // ngroups <= RED_GROUP: the workspace holds at most RED_GROUP^2 workgroups
+  if ((int)threadIdx.x < ngroups) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) sh[k][threadIdx.x] = load_agent(&r.gpart[k * r.gcap + threadIdx.x]);
+  }
+  __syncthreads();
+  fold(ngroups);
   if (threadIdx.x == 0) __hip_atomic_store(r.fticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return true;
 }

@@ -68,8 +68,7 @@ __global__ __launch_bounds__(BS) void k_add(int64_t n, T *x, const T *y) {
 enum RedOp : int { RED_DOT = 0, RED_SUMSQ = 1, RED_SUMABS = 2, RED_MAXABS = 3 };
 
 template <typename T, int OP>
-__global__ __launch_bounds__(BS) void k_reduce(int64_t n, const T *x, const T *y, double *partials,
-                                               uint32_t *ticket, double *out) {
+__global__ __launch_bounds__(BS) void k_reduce(int64_t n, const T *x, const T *y, GridRed red, double *out) {
   double v[1] = {0.0};
   for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
     double a = (double)x[i];
@@ -77,7 +76,7 @@ __global__ __launch_bounds__(BS) void k_reduce(int64_t n, const T *x, const T *y
     else if (OP == RED_SUMSQ) v[0] += a * a;
     else v[0] += fabs(a);
   }
-  if (grid_sum<1>(v, partials, ticket) && threadIdx.x == 0) *out = v[0];
+  if (grid_sum_h<1>(v, red) && threadIdx.x == 0) *out = v[0];
 }
 
 // Maximum of |x_i| with Math.max semantics (NaN propagates), fixed order.
@@ -88,10 +87,8 @@ __device__ __forceinline__ double nan_max(double a, double b) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(BS) void k_maxabs(int64_t n, const T *x, double *partials,
-                                               uint32_t *ticket, double *out) {
+__global__ __launch_bounds__(BS) void k_maxabs(int64_t n, const T *x, GridRed red, double *out) {
   __shared__ double sh[BS];
-  __shared__ int last;
   double m = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
     m = nan_max(fabs((double)x[i]), m);
@@ -101,20 +98,9 @@ __global__ __launch_bounds__(BS) void k_maxabs(int64_t n, const T *x, double *pa
     if ((int)threadIdx.x < s) sh[threadIdx.x] = nan_max(sh[threadIdx.x], sh[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    store_agent(&partials[blockIdx.x], sh[0]);
-    stores_complete();
-    uint32_t t = take_ticket(ticket);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  if (threadIdx.x == 0) {
-    double r = 0.0;
-    for (int i = 0; i < (int)gridDim.x; ++i) r = nan_max(load_agent(&partials[i]), r);
-    *out = r;
-    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  double v[1] = {sh[0]};
+  auto comb = [](double (&a)[1], const double (&b)[1]) { a[0] = nan_max(b[0], a[0]); };
+  if (grid_combine_h<1>(v, red, comb) && threadIdx.x == 0) *out = v[0];
 }
 
 // Scaled sum of squares (LAPACK dnrm2 style), as M/DenseVector.java:297-310
@@ -131,10 +117,8 @@ __device__ __forceinline__ void ssq_combine(double &sa, double &qa, double sb, d
 }
 
 template <typename T>
-__global__ __launch_bounds__(BS) void k_norm2_robust(int64_t n, const T *x, double *partials,
-                                                     uint32_t *ticket, double *out) {
+__global__ __launch_bounds__(BS) void k_norm2_robust(int64_t n, const T *x, GridRed red, double *out) {
   __shared__ double shs[BS], shq[BS];
-  __shared__ int last;
   double scale = 0, ssq = 1;
   bool any = false;
   for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
@@ -166,22 +150,9 @@ __global__ __launch_bounds__(BS) void k_norm2_robust(int64_t n, const T *x, doub
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    store_agent(&partials[blockIdx.x], shs[0]);
-    store_agent(&partials[RED_MAX_BLOCKS + blockIdx.x], shq[0]);
-    stores_complete();
-    uint32_t t = take_ticket(ticket);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  if (threadIdx.x == 0) {
-    double sa = 0, qa = 0;
-    for (int i = 0; i < (int)gridDim.x; ++i)
-      ssq_combine(sa, qa, load_agent(&partials[i]), load_agent(&partials[RED_MAX_BLOCKS + i]));
-    *out = sa == 0 ? 0.0 : sa * sqrt(qa);
-    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  double v[2] = {shs[0], shq[0]};
+  auto comb = [](double (&a)[2], const double (&b)[2]) { ssq_combine(a[0], a[1], b[0], b[1]); };
+  if (grid_combine_h<2>(v, red, comb) && threadIdx.x == 0) *out = v[0] == 0 ? 0.0 : v[0] * sqrt(v[1]);
 }
 
 // ---------------------------------------------------------------------------
@@ -212,13 +183,6 @@ void launch_add(hipStream_t s, int64_t n, double *x, const double *y) {
   k_add<double><<<grid_for(n), BS, 0, s>>>(n, x, y);
   HIPCHK(hipGetLastError());
 }
-void launch_dot(hipStream_t s, int64_t n, const double *x, const double *y, const RedWS &red,
-                int slot) {
-  k_reduce<double, RED_DOT><<<grid_for(n), BS, 0, s>>>(n, x, y, red.partials, red.ticket + slot,
-                                                      red.out + slot);
-  HIPCHK(hipGetLastError());
-}
-
 double read_scalar(mtjg_ctx ctx, const double *dev) {
   HIPCHK(hipMemcpyAsync(ctx->h_out, dev, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -280,10 +244,16 @@ extern "C" int mtjg_ctx_create(int device, mtjg_ctx *out) {
   auto *c = new mtjg_ctx_s();
   c->device = device;
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  HIPCHK(hipMalloc(&c->red.partials, sizeof(double) * RED_MAX_K * RED_MAX_BLOCKS));
-  HIPCHK(hipMalloc(&c->red.ticket, sizeof(uint32_t) * 8));
+  GridRed &g = c->red.g;
+  g.cap = RED_MAX_BLOCKS;
+  g.gcap = (RED_MAX_BLOCKS + RED_GROUP - 1) / RED_GROUP;
+  static_assert((RED_MAX_BLOCKS + RED_GROUP - 1) / RED_GROUP <= RED_GROUP, "grid_combine_h's second level");
+  HIPCHK(hipMalloc(&g.partials, sizeof(double) * RED_MAX_K * (size_t)g.cap));
+  HIPCHK(hipMalloc(&g.gpart, sizeof(double) * RED_MAX_K * (size_t)g.gcap));
+  HIPCHK(hipMalloc(&g.gticket, sizeof(uint32_t) * red_ticket_words(g.gcap)));
+  HIPCHK(hipMemset(g.gticket, 0, sizeof(uint32_t) * red_ticket_words(g.gcap)));
+  g.fticket = g.gticket + (size_t)g.gcap * TICKET_STRIDE;
   HIPCHK(hipMalloc(&c->red.out, sizeof(double) * 8));
-  HIPCHK(hipMemset(c->red.ticket, 0, sizeof(uint32_t) * 8));
   HIPCHK(hipHostMalloc(&c->h_out, sizeof(double) * 8, hipHostMallocDefault));
   *out = c;
   MTJG_API_END
@@ -294,8 +264,9 @@ extern "C" int mtjg_ctx_destroy(mtjg_ctx ctx) {
   if (!ctx) return MTJG_OK;
   HIPCHK(hipSetDevice(ctx->device));
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  (void)hipFree(ctx->red.partials);
-  (void)hipFree(ctx->red.ticket);
+  (void)hipFree(ctx->red.g.partials);
+  (void)hipFree(ctx->red.g.gpart);
+  (void)hipFree(ctx->red.g.gticket);
   (void)hipFree(ctx->red.out);
   (void)hipHostFree(ctx->h_out);
   (void)hipStreamDestroy(ctx->stream);
@@ -466,10 +437,10 @@ extern "C" int mtjg_vec_dot(mtjg_vec x, mtjg_vec y, double *out) {
   mtjg_ctx c = x->ctx;
   if (x->dtype == MTJG_F64)
     k_reduce<double, RED_DOT><<<grid_for(x->n), BS, 0, c->stream>>>(
-        x->n, (const double *)x->ptr, (const double *)y->ptr, c->red.partials, c->red.ticket, c->red.out);
+        x->n, (const double *)x->ptr, (const double *)y->ptr, c->red.g, c->red.out);
   else
     k_reduce<float, RED_DOT><<<grid_for(x->n), BS, 0, c->stream>>>(
-        x->n, (const float *)x->ptr, (const float *)y->ptr, c->red.partials, c->red.ticket, c->red.out);
+        x->n, (const float *)x->ptr, (const float *)y->ptr, c->red.g, c->red.out);
   HIPCHK(hipGetLastError());
   *out = read_scalar(c, c->red.out);
   MTJG_API_END
@@ -486,16 +457,16 @@ extern "C" int mtjg_vec_norm(mtjg_vec x, int norm_type, double *out) {
     const T *p = (const T *)x->ptr;
     switch (norm_type) {
       case MTJG_NORM_ONE:
-        k_reduce<T, RED_SUMABS><<<g, BS, 0, s>>>(x->n, p, p, c->red.partials, c->red.ticket, c->red.out);
+        k_reduce<T, RED_SUMABS><<<g, BS, 0, s>>>(x->n, p, p, c->red.g, c->red.out);
         break;
       case MTJG_NORM_TWO:
-        k_reduce<T, RED_SUMSQ><<<g, BS, 0, s>>>(x->n, p, p, c->red.partials, c->red.ticket, c->red.out);
+        k_reduce<T, RED_SUMSQ><<<g, BS, 0, s>>>(x->n, p, p, c->red.g, c->red.out);
         break;
       case MTJG_NORM_TWO_ROBUST:
-        k_norm2_robust<T><<<g, BS, 0, s>>>(x->n, p, c->red.partials, c->red.ticket, c->red.out);
+        k_norm2_robust<T><<<g, BS, 0, s>>>(x->n, p, c->red.g, c->red.out);
         break;
       case MTJG_NORM_INFINITY:
-        k_maxabs<T><<<g, BS, 0, s>>>(x->n, p, c->red.partials, c->red.ticket, c->red.out);
+        k_maxabs<T><<<g, BS, 0, s>>>(x->n, p, c->red.g, c->red.out);
         break;
       default:
         fail(MTJG_E_INVALID_ARGUMENT, "bad norm type");

@@ -38,6 +38,26 @@ def test_vector_ops(ctx):
         x.add(M.DenseVector(5, ctx))
 
 
+@pytest.mark.parametrize("n", [1, 255, 64 * 1024 + 3, 2 ** 21 + 7])
+def test_vector_reductions_grids(ctx, n):
+    """Reductions over one workgroup, one group of workgroups and the full
+    2048-workgroup grid (32 groups, both levels), with the NaN and the
+    maximum placed in the last workgroup."""
+    a = P.uniform(n, 3)
+    a[-1] = 7.5
+    x = M.DenseVector(a, ctx)
+    # summation order only: the oracle's sequential sum of n same-sign terms
+    # carries a relative rounding error of order sqrt(n) * 2^-53 (1.6e-13 at 2^21)
+    tol = 1e-14 if n < 4096 else 1e-12
+    assert x.dot(x) == pytest.approx(oracle.dv_dot(a, a), rel=tol)
+    assert x.norm(M.Norm.Two) == pytest.approx(oracle.dv_norm(a, "two"), rel=tol)
+    assert x.norm(M.Norm.One) == pytest.approx(oracle.dv_norm(a, "one"), rel=tol)
+    assert x.norm(M.Norm.Infinity) == 7.5
+    assert x.norm(M.Norm.TwoRobust) == pytest.approx(oracle.dv_norm(a, "robust"), rel=tol)
+    a[-1] = np.nan
+    assert np.isnan(M.DenseVector(a, ctx).norm(M.Norm.Infinity))
+
+
 def test_jacobi_bitwise_and_errors(ctx):
     sysm = P.varcoef_poisson(3, 20, seed=1)
     n, _, rp, ci, va = sysm
