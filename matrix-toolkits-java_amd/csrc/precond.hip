@@ -743,6 +743,7 @@ constexpr long long TILE_POLL_BUDGET = 1LL << 22;  // polls of one dependency (a
 constexpr size_t TILE_STATIC_LDS = 64;             // bytes of LDS the tile kernel declares itself
 constexpr int TILE_DEPTH = 8;                      // steps in flight in a tile (slots of at most four)
 constexpr int TILE_IMP_PER = 4;                    // import indices a lane loads before the wait
+constexpr int TILE_FLAG_STRIDE = 32;               // uint32 words between two tiles' flags (128 B)
 
 template <int KIND, int W>
 __global__ __launch_bounds__(FUSE_BS) void k_sweep_tiles(int32_t ntiles, const TileMeta *tiles, const int32_t *deps,
@@ -795,7 +796,8 @@ __global__ __launch_bounds__(FUSE_BS) void k_sweep_tiles(int32_t ntiles, const T
         const int32_t dt = mine ? deps[tm.dep0 + base + lane] : 0;
         long long budget = TILE_POLL_BUDGET;
         for (;;) {
-          const bool wait = mine && __hip_atomic_load(&flags[dt], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch;
+          const bool wait = mine && __hip_atomic_load(&flags[(size_t)dt * TILE_FLAG_STRIDE], __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT) != epoch;
           if (__builtin_amdgcn_ballot_w64(wait) == 0ULL) break;
           if (--budget < 0) {
             if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -841,7 +843,8 @@ __global__ __launch_bounds__(FUSE_BS) void k_sweep_tiles(int32_t ntiles, const T
     }
     stores_complete();
     __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(&flags[tile], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0)
+      __hip_atomic_store(&flags[(size_t)tile * TILE_FLAG_STRIDE], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   // the last workgroup out leaves both counters at zero for the next launch
   if (threadIdx.x == 0 && take_ticket(&ctr[TICKET_STRIDE]) == gridDim.x - 1) {
@@ -1340,8 +1343,8 @@ static bool ensure_pack(mtjg_prec M, Levels &L, int form, const std::vector<int3
       P.tile_lds = tile_lds;
       P.tiles = to_device(tiles, s);
       P.tdeps = to_device(tdeps, s);
-      HIPCHK(hipMalloc(&P.tflags, sizeof(uint32_t) * (size_t)std::max(P.ntiles, 1)));
-      HIPCHK(hipMemsetAsync(P.tflags, 0, sizeof(uint32_t) * (size_t)std::max(P.ntiles, 1), s));
+      HIPCHK(hipMalloc(&P.tflags, sizeof(uint32_t) * TILE_FLAG_STRIDE * (size_t)std::max(P.ntiles, 1)));
+      HIPCHK(hipMemsetAsync(P.tflags, 0, sizeof(uint32_t) * TILE_FLAG_STRIDE * (size_t)std::max(P.ntiles, 1), s));
       // the dispatch and exit counters, each on a memory line of its own
       HIPCHK(hipMalloc(&P.tctr, sizeof(uint32_t) * 2 * TICKET_STRIDE));
       HIPCHK(hipMemsetAsync(P.tctr, 0, sizeof(uint32_t) * 2 * TICKET_STRIDE, s));
@@ -1498,7 +1501,7 @@ static void launch_sweep_tiles_w(Packed &P, const double *vals, const double *b,
   const size_t bytes = P.tile_lds + (P.tile_grouped ? sizeof(double) * FUSE_BS * W : 0);
   const int grid = P.ntiles;
   if (++P.epoch == 0) {  // flags hold epochs; after 2^32 sweeps start again from clear flags
-    HIPCHK(hipMemsetAsync(P.tflags, 0, sizeof(uint32_t) * (size_t)std::max(P.ntiles, 1), s));
+    HIPCHK(hipMemsetAsync(P.tflags, 0, sizeof(uint32_t) * TILE_FLAG_STRIDE * (size_t)std::max(P.ntiles, 1), s));
     P.epoch = 1;
   }
   k_sweep_tiles<KIND, W><<<grid, FUSE_BS, bytes, s>>>(P.ntiles, P.tiles, P.tdeps, P.tflags, P.epoch, P.tctr, err,
