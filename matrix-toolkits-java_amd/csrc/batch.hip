@@ -76,7 +76,6 @@ struct BSlot {
   int32_t outer, inner, inner_total, max_outer, max_inner;
   double rtol, atol, dtol, init_r, rnorm;
   double rho, alpha, beta, inner_init, inner_rtol;
-  uint32_t ticket;
 };
 
 struct BWork {
@@ -89,6 +88,9 @@ struct BDev {
   const int32_t *nwork;
   double *partials;  // 4 arrays of `cap`: hi/lo of two sums
   int cap;
+  // per-slot tickets, slot s's at s * TICKET_STRIDE: each on a memory line
+  // of its own, apart from the slot record every workgroup of the slot reads
+  uint32_t *ticket;
 };
 
 __device__ __forceinline__ bool slot_live(const BSlot &S) { return S.active && S.status == ST_RUNNING; }
@@ -107,7 +109,7 @@ __device__ bool slot_sum(dd (&v)[K], BSlot &S, const BWork &w, const BDev &d, do
       store_agent(&d.partials[(2 * k + 1) * d.cap + base + w.tile], v[k].lo);
     }
     stores_complete();
-    uint32_t t = take_ticket(&S.ticket);
+    uint32_t t = take_ticket(&d.ticket[(size_t)w.slot * TICKET_STRIDE]);
     last = (t == (uint32_t)S.nb - 1);
   }
   __syncthreads();
@@ -124,7 +126,8 @@ __device__ bool slot_sum(dd (&v)[K], BSlot &S, const BWork &w, const BDev &d, do
   block_sum_dd<K>(acc);
 #pragma unroll
   for (int k = 0; k < K; ++k) out[k] = acc[k].hi + acc[k].lo;
-  if (threadIdx.x == 0) __hip_atomic_store(&S.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0)
+    __hip_atomic_store(&d.ticket[(size_t)w.slot * TICKET_STRIDE], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return true;
 }
 
@@ -906,6 +909,7 @@ struct mtjg_batch_s {
   std::unique_ptr<Workers> workers;
   HostPool pool;                // host blocks of queued requests and solutions (outlives both)
   BSlot *d_slots = nullptr;     // device slot table
+  uint32_t *d_ticket = nullptr;  // per-slot tickets (TICKET_STRIDE apart), zero between phases
   BWork *d_work = nullptr;
   int32_t *d_nwork = nullptr;
   double *d_partials = nullptr;
@@ -954,7 +958,9 @@ double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-BDev bdev(mtjg_batch B) { return BDev{B->d_slots, B->d_work, B->d_nwork, B->d_partials, B->max_work}; }
+BDev bdev(mtjg_batch B) {
+  return BDev{B->d_slots, B->d_work, B->d_nwork, B->d_partials, B->max_work, B->d_ticket};
+}
 
 // Admitted requests not yet reported finished: those in a slot and those
 // whose solution download is still to be drained.
@@ -1293,6 +1299,8 @@ extern "C" int mtjg_batch_create(mtjg_ctx ctx, int32_t max_slots, int32_t max_wo
   HIPCHK(hipStreamCreateWithFlags(&B->dl, hipStreamNonBlocking));
   HIPCHK(hipMalloc(&B->d_slots, sizeof(BSlot) * max_slots));
   HIPCHK(hipMemsetAsync(B->d_slots, 0, sizeof(BSlot) * max_slots, ctx->stream));
+  HIPCHK(hipMalloc(&B->d_ticket, sizeof(uint32_t) * TICKET_STRIDE * (size_t)max_slots));
+  HIPCHK(hipMemsetAsync(B->d_ticket, 0, sizeof(uint32_t) * TICKET_STRIDE * (size_t)max_slots, ctx->stream));
   HIPCHK(hipMalloc(&B->d_work, sizeof(BWork) * max_workgroups));
   HIPCHK(hipMalloc(&B->d_nwork, sizeof(int32_t)));
   HIPCHK(hipMemsetAsync(B->d_nwork, 0, sizeof(int32_t), ctx->stream));
@@ -1345,6 +1353,7 @@ extern "C" int mtjg_batch_destroy(mtjg_batch B) {
   B->up.release();
   B->down.release();
   (void)hipFree(B->d_slots);
+  (void)hipFree(B->d_ticket);
   (void)hipFree(B->d_work);
   (void)hipFree(B->d_nwork);
   (void)hipFree(B->d_partials);

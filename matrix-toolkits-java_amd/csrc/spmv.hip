@@ -207,15 +207,16 @@ __global__ __launch_bounds__(BS) void k_sell_part(int nslice, int n, const int32
   }
 }
 
-// Adds k_sell_part's np partials (BS per workgroup, then the grid
-// reduction, whose block partials sit below `part`) and runs the epilogue.
+// Adds k_sell_part's np partials (each thread those at its index and every
+// gridDim.x * BS after it, in order; then the grid reduction, whose block
+// partials sit below `part`) and runs the epilogue.  The grid is at most one
+// reduction group (sell_fin_grid), so the sum takes one ticket round trip.
 __global__ __launch_bounds__(BS) void k_sell_fin(int np, const double *part, KState *st, int fin, GridRed red) {
   if (st->status != ST_RUNNING) return;
-  const int i = blockIdx.x * BS + (int)threadIdx.x;
   dd v[2] = {{0.0, 0.0}, {0.0, 0.0}};
-  if (i < np) {
-    v[0] = dd{part[0 * (size_t)red.cap + i], part[1 * (size_t)red.cap + i]};
-    v[1] = dd{part[2 * (size_t)red.cap + i], part[3 * (size_t)red.cap + i]};
+  for (int i = blockIdx.x * BS + (int)threadIdx.x; i < np; i += gridDim.x * BS) {
+    v[0] = dd_add(v[0], dd{part[0 * (size_t)red.cap + i], part[1 * (size_t)red.cap + i]});
+    v[1] = dd_add(v[1], dd{part[2 * (size_t)red.cap + i], part[3 * (size_t)red.cap + i]});
   }
   double tot[2];
   if (grid_sum_dd<2>(v, red, tot) && threadIdx.x == 0) spmv_finish(st, fin, tot[0], tot[1]);
@@ -323,7 +324,7 @@ static void sell_ready(mtjg_csr A, bool fp32) {
 }
 
 // Workgroups of k_sell_fin for a k_sell_part grid.
-static int sell_fin_grid(int grid) { return (grid + BS - 1) / BS; }
+static int sell_fin_grid(int grid) { return std::min(RED_GROUP, (grid + BS - 1) / BS); }
 
 int64_t spmv_dot_red_cap(mtjg_csr A) {
   int64_t need = SPMV_DOT_GRID;
