@@ -590,7 +590,8 @@ def main():
     ctx = M.Context(0)
     A = M.CompRowMatrix(n, m, rp, ci, va, ctx=ctx)
     fmt, stored = A.format()
-    kernel = "k_sell<double,double,SPMV_MULT>" if fmt == "sell" else "k_spmv<double,double,SPMV_MULT>"
+    kernel = {"sell": "k_sell<double,double,SPMV_MULT>", "csr": "k_spmv<double,double,SPMV_MULT>",
+              "csr_xcd": "k_cb_prod + k_spmv<double,double,SPMV_MULT,PRE>"}[fmt]
     x = M.DenseVector(P.uniform(m, 1), ctx)
     y = M.DenseVector(n, ctx)
     for _ in range(args.warmup):

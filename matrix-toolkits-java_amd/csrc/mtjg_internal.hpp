@@ -157,6 +157,25 @@ struct mtjg_csr_s {
   int32_t *scol = nullptr;
   double *sval = nullptr;
   float *sval32 = nullptr;
+  // XCD column blocks, added for irregular patterns whose x gathers scatter
+  // over more than one XCD's L2 (spmv.hip, cb_build): the entries reordered
+  // by column block (block b, columns [b*cb_width, (b+1)*cb_width), is
+  // gathered on XCD b % 8), XCD k's blocks filling [cb_off[k], cb_off[k+1]);
+  // cb_perm[j] is CSR entry j's position in that order and cb_prod the
+  // products pass's output.  Row-block r's entries of block k sit at
+  // [cb_seg[r*nblk + k], cb_seg[(r+1)*nblk + k]) of that order, and cb_q[p]
+  // is the index within its row-block of the entry at position p of that
+  // order.  `cb_stale` marks values changed since cb_val was filled.
+  bool cb = false;
+  bool cb_stale = false;
+  int32_t cb_width = 0, cb_nblk = 0;
+  int32_t cb_off[9] = {};
+  int32_t *cb_perm = nullptr;
+  int32_t *cb_col = nullptr;
+  double *cb_val = nullptr;
+  double *cb_prod = nullptr;
+  int32_t *cb_seg = nullptr;
+  uint16_t *cb_q = nullptr;
   // Host copy of the row pointers (schedules, level analysis, validation).
   std::vector<int32_t> h_rp;
   // Transpose (CSR of A'), built on first transposed product.

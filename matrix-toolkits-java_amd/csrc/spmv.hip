@@ -7,7 +7,11 @@
 //    every load is coalesced and no LDS or barrier is needed; every row is
 //    summed sequentially, so every product is bitwise MTJ's;
 //  * row-block CSR (any pattern): one 256-thread workgroup per host-cut
-//    row-block (spmv_rowblock.hpp), bitwise MTJ's for blocks of >= 128 rows.
+//    row-block (spmv_rowblock.hpp), bitwise MTJ's for blocks of >= 128 rows;
+//  * for irregular patterns whose columns scatter over an x larger than one
+//    XCD's L2, XCD column blocks in front of the row-block kernel: a first
+//    pass forms every product with its x gather confined to one XCD's slice
+//    of x, and the row-block kernel then sums those products (cb_build).
 // The kernels are HBM-bound (12 bytes per stored entry, 20 per row), so MFMA
 // is not used.
 #include <algorithm>
@@ -89,12 +93,12 @@ __device__ __forceinline__ int xcd_run_block(int blk, int grid, int run) {
 // Plain products: one workgroup per row-block, under the XCD run mapping
 // (slabs = -run) or per slab-ordered ticket (slabs > 0).  NTY: y is stored
 // with the non-temporal hint.
-template <typename VT, typename XT, int MODE, bool NTY>
+template <typename VT, typename XT, int MODE, bool NTY, bool PRE>
 __global__ __launch_bounds__(BS) void k_spmv(int nrb, int slabs, const int2 *__restrict__ info,
                                              const int32_t *__restrict__ rp,
                                              const int32_t *__restrict__ ci,
                                              const VT *__restrict__ val, const XT *__restrict__ x,
-                                             XT *y, const XT *y0, double alpha, double inv_alpha) {
+                                             XT *y, const XT *y0, double alpha, double inv_alpha, CbView cb) {
   __shared__ XT prod[SPMV_CAP];
   int b;
   if (slabs < 0) {
@@ -106,8 +110,8 @@ __global__ __launch_bounds__(BS) void k_spmv(int nrb, int slabs, const int2 *__r
     if (b < 0) return;
   }
   dd acc0{0.0, 0.0}, acc1{0.0, 0.0};
-  spmv_rowblock<VT, XT, MODE, NTY>(b, prod, info, rp, ci, val, x, y, y0, alpha, inv_alpha, nullptr, acc0,
-                                   acc1);
+  spmv_rowblock<VT, XT, MODE, NTY, PRE>(b, prod, info, rp, ci, val, x, y, y0, alpha, inv_alpha, nullptr, acc0,
+                                        acc1, cb);
 }
 
 // Fused solver product q = A p with the grid sums w.q and q.q: at most
@@ -115,12 +119,12 @@ __global__ __launch_bounds__(BS) void k_spmv(int nrb, int slabs, const int2 *__r
 // that many take a reduction ticket; a two-level deterministic reduction
 // ends in the solver epilogue in the last workgroup.  Every workgroup returns
 // at once when the solve has left the running state.
-template <typename VT, typename XT>
+template <typename VT, typename XT, bool PRE>
 __global__ __launch_bounds__(BS) void k_spmv_dot(int nrb, int slabs, const int2 *__restrict__ info,
                                                  const int32_t *__restrict__ rp,
                                                  const int32_t *__restrict__ ci,
                                                  const VT *__restrict__ val, const XT *__restrict__ x,
-                                                 XT *y, const XT *w, KState *st, int fin, GridRed red) {
+                                                 XT *y, const XT *w, KState *st, int fin, GridRed red, CbView cb) {
   if (st->status != ST_RUNNING) return;
   __shared__ XT prod[SPMV_CAP];
   dd acc0{0.0, 0.0}, acc1{0.0, 0.0};
@@ -129,12 +133,82 @@ __global__ __launch_bounds__(BS) void k_spmv_dot(int nrb, int slabs, const int2 
   for (int t = blockIdx.x; t < ntickets; t += gridDim.x) {
     const int b = ticket_to_block(t, nrb, slabs, per);
     if (b >= 0)
-      spmv_rowblock<VT, XT, SPMV_DOT>(b, prod, info, rp, ci, val, x, y, nullptr, 1.0, 1.0, w, acc0, acc1);
+      spmv_rowblock<VT, XT, SPMV_DOT, false, PRE>(b, prod, info, rp, ci, val, x, y, nullptr, 1.0, 1.0, w, acc0,
+                                                  acc1, cb);
     __syncthreads();  // prod is reused by the next row-block
   }
   dd v[2] = {acc0, acc1};
   double tot[2];
   if (grid_sum_dd<2>(v, red, tot) && threadIdx.x == 0) spmv_finish(st, fin, tot[0], tot[1]);
+}
+
+// ---- XCD column blocks ----------------------------------------------------
+//
+// On a matrix whose columns scatter at random over a vector of 32 MB (the
+// 4 M-row power-law graph), nearly every x gather of the row-block kernel
+// misses L2 and fills a 128-byte line for 8 useful bytes (L2 hit rate 13%,
+// 2.0 GB of fills per product against 0.31 GB algorithmic).  Random 8-byte
+// gathers from 32 MB run at 63 G/s, against 182 G/s when the gathered slice
+// is 4 MB, the size of one XCD's L2.  So the products are formed in a pass of
+// their own: the entries are kept a second time ordered by column block
+// (CB_BLOCK_BYTES of x per block), the blocks dealt to the XCDs (block b to
+// XCD b % 8, which runs workgroups w with w % 8 == b % 8), so each XCD
+// gathers only from its own slices of x, which stay in its L2.  Within a
+// block the entries keep CSR order, and the products are stored with the
+// non-temporal hint so that they do not evict the slice.  The row-block
+// kernel then stages each row-block's products segment by segment (PRE,
+// spmv_rowblock.hpp) and sums the rows exactly as it sums products it forms
+// itself: the result is bitwise the row-block product's.
+constexpr int64_t CB_BLOCK_BYTES = 4ll << 20;  // x per column block: one XCD's L2
+constexpr int CB_PER = 8;                       // entries per thread of the products pass
+constexpr int CB_CHUNK = BS * CB_PER;           // entries per workgroup
+constexpr int CB_NXCD = 8;
+
+struct CbRanges {
+  int32_t off[CB_NXCD + 1];  // XCD k's entries: [off[k], off[k+1])
+};
+
+// prod[k] = val[k] * x[col[k]] over the reordered entries: workgroup w takes
+// the (w / 8)-th chunk of XCD (w % 8)'s range.  The entries are read once
+// (non-temporal loads, so they do not evict the XCD's slice of x).
+__global__ __launch_bounds__(BS) void k_cb_prod(CbRanges r, const int32_t *__restrict__ col,
+                                                const double *__restrict__ val, const double *__restrict__ x,
+                                                double *__restrict__ prod) {
+  const int xcd = blockIdx.x % CB_NXCD;
+  const int64_t k0 = r.off[xcd] + (int64_t)(blockIdx.x / CB_NXCD) * CB_CHUNK;
+  const int64_t end = r.off[xcd + 1];
+  if (k0 >= end) return;
+  int c[CB_PER];
+  double v[CB_PER], xv[CB_PER];
+#pragma unroll
+  for (int u = 0; u < CB_PER; ++u) {
+    const int64_t k = k0 + threadIdx.x + u * BS;
+    if (k < end) {
+      c[u] = __builtin_nontemporal_load(&col[k]);
+      v[u] = __builtin_nontemporal_load(&val[k]);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < CB_PER; ++u) {
+    const int64_t k = k0 + threadIdx.x + u * BS;
+    if (k < end) xv[u] = x[c[u]];
+  }
+#pragma unroll
+  for (int u = 0; u < CB_PER; ++u) {
+    const int64_t k = k0 + threadIdx.x + u * BS;
+    if (k < end) __builtin_nontemporal_store(v[u] * xv[u], &prod[k]);
+  }
+}
+
+// Copies CSR entry j's value (and, with col, its column) to its position
+// perm[j] in the column-block order.
+__global__ void k_cb_fill(int64_t nnz, const int32_t *__restrict__ perm, const int32_t *__restrict__ ci,
+                          const double *__restrict__ val, int32_t *col, double *cval) {
+  for (int64_t j = blockIdx.x * (int64_t)BS + threadIdx.x; j < nnz; j += (int64_t)gridDim.x * BS) {
+    const int32_t p = perm[j];
+    cval[p] = val[j];
+    if (col) col[p] = ci[j];
+  }
 }
 
 // Products from the SELL-64 copy: four slices (wavefronts) per workgroup.
@@ -308,8 +382,7 @@ __global__ void k_to_f32(int64_t n, const double *a, float *b) {
 }
 
 // Brings the SELL-64 values (and their FP32 copy when `fp32`) up to date.
-static void sell_ready(mtjg_csr A, bool fp32) {
-  hipStream_t s = A->ctx->stream;
+static void sell_ready(mtjg_csr A, bool fp32, hipStream_t s) {
   if (A->sell_stale) {
     k_sell_fill<<<(A->nr + BS - 1) / BS, BS, 0, s>>>(A->nr, A->rp, A->ci, A->sptr, A->val, nullptr, A->sval);
     HIPCHK(hipGetLastError());
@@ -380,9 +453,12 @@ static void sell_launch(mtjg_csr A, const VT *sval, const XT *x, XT *y, const Sp
   HIPCHK(hipGetLastError());
 }
 
-template <typename VT, typename XT>
+// With PRE the products come from the column-block products pass (cb); val
+// and x are not read.
+template <typename VT, typename XT, bool PRE = false>
 static void spmv_launch(mtjg_csr A, const VT *val, const XT *x, XT *y, const SpmvArgs &a,
-                        hipStream_t s) {
+                        hipStream_t s, const CbView cb = CbView{}) {
+  const int32_t *ci = A->ci;
   if (A->nr == 0) {
     if (a.mode == SPMV_DOT) fail(MTJG_E_INVALID_ARGUMENT, "empty matrix in fused product");
     return;
@@ -397,8 +473,8 @@ static void spmv_launch(mtjg_csr A, const VT *val, const XT *x, XT *y, const Spm
   const XT *w = (const XT *)a.w;
   const bool nt = spmv_nt_store(A->nr, sizeof(XT));
   auto plain = [&](auto mode, auto nty) {
-    k_spmv<VT, XT, decltype(mode)::value, decltype(nty)::value><<<grid, BS, 0, s>>>(
-        A->nrb, sl, A->rb, A->rp, A->ci, val, x, y, y0, a.alpha, a.inv_alpha);
+    k_spmv<VT, XT, decltype(mode)::value, decltype(nty)::value, PRE><<<grid, BS, 0, s>>>(
+        A->nrb, sl, A->rb, A->rp, ci, val, x, y, y0, a.alpha, a.inv_alpha, cb);
   };
   switch (a.mode) {
     case SPMV_MULT:
@@ -415,8 +491,8 @@ static void spmv_launch(mtjg_csr A, const VT *val, const XT *x, XT *y, const Spm
       break;
     case SPMV_DOT:
       if (grid > a.red.cap) fail(MTJG_E_INVALID_ARGUMENT, "reduction workspace smaller than the grid");
-      k_spmv_dot<VT, XT><<<grid, BS, 0, s>>>(A->nrb, sl, A->rb, A->rp, A->ci, val, x, y, w, a.st,
-                                             a.fin, a.red);
+      k_spmv_dot<VT, XT, PRE><<<grid, BS, 0, s>>>(A->nrb, sl, A->rb, A->rp, ci, val, x, y, w, a.st,
+                                                  a.fin, a.red, cb);
       break;
     default:
       fail(MTJG_E_INVALID_ARGUMENT, "bad spmv mode");
@@ -424,10 +500,39 @@ static void spmv_launch(mtjg_csr A, const VT *val, const XT *x, XT *y, const Spm
   HIPCHK(hipGetLastError());
 }
 
+// The column-block products pass (k_cb_prod) into A->cb_prod, the values
+// refreshed first if they changed.  Products of one matrix share cb_prod, so
+// they are ordered on one stream.
+static void cb_products(mtjg_csr A, const double *x, hipStream_t s) {
+  if (A->cb_stale) {
+    k_cb_fill<<<grid_for(A->nnz), BS, 0, s>>>(A->nnz, A->cb_perm, A->ci, A->val, nullptr, A->cb_val);
+    HIPCHK(hipGetLastError());
+    A->cb_stale = false;
+  }
+  CbRanges r;
+  int64_t chunks = 0;
+  for (int k = 0; k <= CB_NXCD; ++k) r.off[k] = A->cb_off[k];
+  for (int k = 0; k < CB_NXCD; ++k)
+    chunks = std::max<int64_t>(chunks, (r.off[k + 1] - r.off[k] + CB_CHUNK - 1) / CB_CHUNK);
+  if (chunks == 0) return;
+  k_cb_prod<<<(int)(chunks * CB_NXCD), BS, 0, s>>>(r, A->cb_col, A->cb_val, x, A->cb_prod);
+  HIPCHK(hipGetLastError());
+}
+
 void spmv_f64(mtjg_csr A, const double *x, double *y, const SpmvArgs &a, hipStream_t s) {
   if (A->sell && A->nr > 0) {
-    sell_ready(A, false);
+    sell_ready(A, false, s);
     sell_launch<double, double>(A, A->sval, x, y, a, s);
+    return;
+  }
+  if (A->cb && A->nr > 0) {
+    cb_products(A, x, s);
+    CbView cb;
+    cb.prod = A->cb_prod;
+    cb.seg = A->cb_seg;
+    cb.q = A->cb_q;
+    cb.nblk = A->cb_nblk;
+    spmv_launch<double, double, true>(A, nullptr, x, y, a, s, cb);
     return;
   }
   spmv_launch<double, double>(A, A->val, x, y, a, s);
@@ -445,7 +550,7 @@ bool spmv_cg_fused(mtjg_csr A, const double *p, const double *z, double *pn, dou
                    hipStream_t s) {
   if (!spmv_cg_fusable(A)) return false;
   const int grid = (A->nslice + BS / SELL_C - 1) / (BS / SELL_C);
-  sell_ready(A, false);
+  sell_ready(A, false, s);
   if (grid > a.red.cap) fail(MTJG_E_INVALID_ARGUMENT, "reduction workspace smaller than the grid");
   k_sell_dot_cgp<double><<<grid, BS, 0, s>>>(A->nslice, A->nr, A->sptr, A->scol, A->sval, p, z, pn, q, a.st,
                                              a.red);
@@ -455,7 +560,7 @@ bool spmv_cg_fused(mtjg_csr A, const double *p, const double *z, double *pn, dou
 
 void spmv_f32(mtjg_csr A, const float *x, float *y, const SpmvArgs &a, hipStream_t s) {
   if (A->sell && A->nr > 0) {
-    sell_ready(A, true);
+    sell_ready(A, true, s);
     sell_launch<float, float>(A, A->sval32, x, y, a, s);
     return;
   }
@@ -485,6 +590,12 @@ static void csr_free(mtjg_csr_s *A) {
   (void)hipFree(A->scol);
   (void)hipFree(A->sval);
   (void)hipFree(A->sval32);
+  (void)hipFree(A->cb_perm);
+  (void)hipFree(A->cb_col);
+  (void)hipFree(A->cb_val);
+  (void)hipFree(A->cb_prod);
+  (void)hipFree(A->cb_seg);
+  (void)hipFree(A->cb_q);
   delete A;
 }
 
@@ -498,7 +609,7 @@ static void sell_build(mtjg_csr_s *A) {
   if (nr == 0) return;
   int mode = 0;  // 0 auto, 1 never, 2 always
   if (const char *e = getenv("MTJG_SPMV_FORMAT")) {
-    if (!strcmp(e, "csr")) mode = 1;
+    if (!strcmp(e, "csr") || !strcmp(e, "xcd")) mode = 1;
     else if (!strcmp(e, "sell")) mode = 2;
   }
   if (mode == 1) return;
@@ -528,6 +639,105 @@ static void sell_build(mtjg_csr_s *A) {
   A->nslice = nslice;
   A->h_sptr_total = total;
   A->sell = true;
+}
+
+// Fraction of a row-block's entries that gather a 128-byte line of x that no
+// other entry of the block gathers, over up to 512 row-blocks spread over the
+// matrix (the first 4096 entries of each): about 1 when the columns scatter
+// at random, a few hundredths for stencils and other local patterns.
+static double gather_scatter(const int32_t *ci, const std::vector<int2> &rb) {
+  const size_t nrb = rb.size() - 1;
+  const size_t step = std::max<size_t>(1, nrb / 512);
+  std::vector<int32_t> lines;
+  int64_t uniq = 0, tot = 0;
+  for (size_t b = 0; b < nrb; b += step) {
+    const int32_t j0 = rb[b].y, j1 = std::min(rb[b + 1].y, j0 + 4096);
+    lines.assign(ci + j0, ci + j1);
+    for (auto &l : lines) l >>= 4;
+    std::sort(lines.begin(), lines.end());
+    uniq += std::unique(lines.begin(), lines.end()) - lines.begin();
+    tot += j1 - j0;
+  }
+  return tot ? (double)uniq / (double)tot : 0.0;
+}
+
+// Adds the XCD column blocks (see k_cb_prod) to a matrix without a SELL-64
+// copy when x is larger than CB_MIN_X_BYTES and at least CB_SCATTER_MIN of
+// the gathers fetch a line no neighbouring entry fetches (auto), or always
+// with MTJG_SPMV_FORMAT=xcd.  The blocks are the fewest multiple of eight
+// whose x slices are at most CB_BLOCK_BYTES (at most CB_MAX_BLK blocks, so x
+// of at most 128 MB); XCD k's blocks k, k + 8, ... are laid out one after the
+// other, each block's entries in CSR order.  Alongside: each row-block's
+// segment starts per block and, in the products' order, each product's
+// entry index within its row-block.  ci is the host copy of the (sorted)
+// columns and rb the row-block schedule.
+constexpr int64_t CB_MIN_X_BYTES = 16ll << 20;
+constexpr double CB_SCATTER_MIN = 0.5;
+
+static void cb_build(mtjg_csr_s *A, const int32_t *ci, const std::vector<int2> &rb) {
+  if (A->sell || A->nnz == 0 || A->nc == 0) return;
+  int mode = 0;  // 0 auto, 1 never, 2 always
+  if (const char *e = getenv("MTJG_SPMV_FORMAT")) {
+    if (!strcmp(e, "csr") || !strcmp(e, "sell")) mode = 1;
+    else if (!strcmp(e, "xcd")) mode = 2;
+  }
+  if (mode == 1) return;
+  const int64_t xbytes = (int64_t)A->nc * (int64_t)sizeof(double);
+  if (mode == 0 && (xbytes <= CB_MIN_X_BYTES || gather_scatter(ci, rb) < CB_SCATTER_MIN)) return;
+  int64_t blk_bytes = CB_BLOCK_BYTES;  // MTJG_CB_BLOCK_KB overrides it for experiments
+  if (const char *e = getenv("MTJG_CB_BLOCK_KB")) blk_bytes = std::max<int64_t>(8, atoll(e)) << 10;
+  int64_t nblk = (xbytes + blk_bytes - 1) / blk_bytes;
+  nblk = std::max<int64_t>(CB_NXCD, (nblk + CB_NXCD - 1) / CB_NXCD * CB_NXCD);
+  if (nblk > CB_MAX_BLK) {
+    if (mode == 2) fail(MTJG_E_UNSUPPORTED, "x too large for the XCD column blocks");
+    return;
+  }
+  nblk = std::min<int64_t>(nblk, std::max<int64_t>(CB_NXCD, (int64_t)A->nc / CB_NXCD * CB_NXCD));
+  const int32_t width = (int32_t)(((int64_t)A->nc + nblk - 1) / nblk);
+  nblk = ((int64_t)A->nc + width - 1) / width;  // blocks past the last column stay empty
+  const size_t nnz = (size_t)A->nnz;
+  std::vector<int64_t> start((size_t)nblk, 0);
+  for (size_t j = 0; j < nnz; ++j) start[(size_t)(ci[j] / width)]++;
+  int64_t pos = 0;
+  for (int k = 0; k < CB_NXCD; ++k) {
+    A->cb_off[k] = (int32_t)pos;
+    for (int64_t b = k; b < nblk; b += CB_NXCD) {
+      const int64_t c = start[(size_t)b];
+      start[(size_t)b] = pos;
+      pos += c;
+    }
+  }
+  A->cb_off[CB_NXCD] = (int32_t)pos;
+  const size_t nrb = rb.size() - 1, nb = (size_t)nblk;
+  std::vector<int32_t> perm(nnz), seg((nrb + 1) * nb);
+  std::vector<uint16_t> q(nnz, 0);
+  for (size_t r = 0; r < nrb; ++r) {
+    const int32_t j0 = rb[r].y, j1 = rb[r + 1].y;
+    for (size_t k = 0; k < nb; ++k) seg[r * nb + k] = (int32_t)start[k];
+    const bool staged = j1 - j0 <= SPMV_CAP;  // a longer row reads its segments in place
+    for (int32_t j = j0; j < j1; ++j) {
+      const int64_t p = start[(size_t)(ci[j] / width)]++;
+      perm[(size_t)j] = (int32_t)p;
+      if (staged) q[(size_t)p] = (uint16_t)(j - j0);
+    }
+  }
+  for (size_t k = 0; k < nb; ++k) seg[nrb * nb + k] = (int32_t)start[k];
+  hipStream_t s = A->ctx->stream;
+  HIPCHK(hipMalloc(&A->cb_perm, sizeof(int32_t) * nnz));
+  HIPCHK(hipMalloc(&A->cb_col, sizeof(int32_t) * nnz));
+  HIPCHK(hipMalloc(&A->cb_val, sizeof(double) * nnz));
+  HIPCHK(hipMalloc(&A->cb_prod, sizeof(double) * nnz));
+  HIPCHK(hipMalloc(&A->cb_seg, sizeof(int32_t) * seg.size()));
+  HIPCHK(hipMalloc(&A->cb_q, sizeof(uint16_t) * nnz));
+  HIPCHK(hipMemcpyAsync(A->cb_perm, perm.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(A->cb_seg, seg.data(), sizeof(int32_t) * seg.size(), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(A->cb_q, q.data(), sizeof(uint16_t) * nnz, hipMemcpyHostToDevice, s));
+  k_cb_fill<<<grid_for(A->nnz), BS, 0, s>>>(A->nnz, A->cb_perm, A->ci, A->val, A->cb_col, A->cb_val);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  A->cb_width = width;
+  A->cb_nblk = (int32_t)nblk;
+  A->cb = true;
 }
 
 // Uploads a validated host CSR (sorted unique columns) and builds its
@@ -563,6 +773,7 @@ static mtjg_csr_s *csr_upload(mtjg_ctx ctx, int32_t nr, int32_t nc, std::vector<
                           ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     sell_build(A);
+    cb_build(A, ci, rb);
   } catch (...) {
     csr_free(A);
     throw;
@@ -674,7 +885,7 @@ extern "C" int mtjg_csr_destroy(mtjg_csr A) {
 extern "C" int mtjg_csr_format(mtjg_csr A, int32_t *format, int64_t *stored) {
   MTJG_API_BEGIN
   if (!A) fail(MTJG_E_NULL, "null matrix");
-  if (format) *format = A->sell ? MTJG_FORMAT_SELL : MTJG_FORMAT_CSR;
+  if (format) *format = A->sell ? MTJG_FORMAT_SELL : A->cb ? MTJG_FORMAT_CSR_XCD : MTJG_FORMAT_CSR;
   if (stored) *stored = A->sell ? A->h_sptr_total : A->nnz;
   MTJG_API_END
 }
@@ -705,6 +916,7 @@ extern "C" int mtjg_csr_download(mtjg_csr A, int32_t *row_pointer, int32_t *colu
 static void csr_values_changed(mtjg_csr A) {
   // derived copies of the values are rebuilt on next use
   if (A->sell) A->sell_stale = true;
+  if (A->cb) A->cb_stale = true;
   if (A->val32 || A->sval32) {
     HIPCHK(hipStreamSynchronize(A->ctx->stream));
     (void)hipFree(A->val32);

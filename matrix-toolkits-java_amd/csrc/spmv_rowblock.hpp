@@ -8,17 +8,30 @@
 //      any use), gathers x at those columns, and stages the products
 //      val[j] * x[col[j]] in LDS;
 //   2. reduces each row from LDS.  When the block holds at least 128 rows a
-//      thread owns a row and adds its products sequentially in storage
-//      order, exactly as MTJ's loop (S/CompRowMatrix.java:304-312) does;
-//      blocks with fewer, longer rows give each row a power-of-two group of
-//      lanes that add strided partials and combine them with an xor
-//      butterfly.
+//      thread owns a row of at most SPMV_LONG_ROW entries and adds its
+//      products sequentially in storage order, exactly as MTJ's loop
+//      (S/CompRowMatrix.java:304-312) does, and a longer row is then summed
+//      by a whole wavefront; blocks with fewer, longer rows give each row a
+//      power-of-two group of lanes.  Lane groups and wavefronts add strided
+//      partials and combine them with an xor butterfly.
 // A row longer than SPMV_CAP gets a row-block of its own, reduced by the
 // whole workgroup straight from global memory.  Blocks are cut at 256 rows
 // (SPMV_CUT_ROWS) so that on short-row matrices every thread sums one row in
 // a single pass: on the 3D Poisson 256^3 product 335.2 -> 315.6 us against
 // blocks of up to 512 rows, both with the XCD run mapping of spmv.hip (either
 // change alone: 333-335 us, plain order and 512 rows: 331-334 us).
+//
+// With PRE the products were formed by an earlier pass (the XCD column-block
+// products of spmv.hip), which writes them grouped by column block: a
+// row-block's products of block k are one contiguous segment of its output.
+// Step 1 then reads the block's segments one after the other with coalesced
+// loads (thread slot s of the segments' order reads the segment that holds
+// s), together with each product's entry index within the row-block (q,
+// stored alongside the products), and stores each product at its entry's
+// place in LDS: step 2 then runs unchanged.  Each product is the value
+// val[j] * x[col[j]] the multiplication here would give, so the result is
+// bitwise the plain form's.  A row longer than SPMV_CAP is one row, whose
+// entries' column order is the segments' order.
 #pragma once
 
 #include <vector>
@@ -32,6 +45,7 @@ constexpr int SPMV_CAP = 2048;           // products staged per row-block
 constexpr int SPMV_PER = SPMV_CAP / BS;  // entries per thread per row-block
 constexpr int SPMV_MAX_ROWS = 512;       // rows a row-block may hold (2 per thread)
 constexpr int SPMV_CUT_ROWS = 256;       // rows the cutter puts in a row-block
+constexpr int SPMV_LONG_ROW = 64;        // longer rows of one-lane blocks are summed by a wavefront
 
 // Row-block schedule: greedy cut so every block holds <= SPMV_CAP entries
 // and <= max_rows (<= SPMV_MAX_ROWS) rows, except single rows longer than
@@ -64,6 +78,49 @@ inline std::vector<int2> build_row_blocks(const std::vector<int32_t> &rp, int32_
 }
 
 __device__ __forceinline__ int pow2_floor(int v) { return 1 << (31 - __clz(v)); }
+
+// The products pass's output as the PRE form reads it: row-block r's
+// products of column block k at [seg[r * nblk + k], seg[(r + 1) * nblk + k])
+// of prod, and q[p] the index within its row-block of the entry whose
+// product is prod[p] (row-blocks of at most SPMV_CAP entries).
+constexpr int CB_MAX_BLK = 32;
+struct CbView {
+  const double *prod = nullptr;
+  const int32_t *seg = nullptr;
+  const uint16_t *q = nullptr;
+  int nblk = 0;
+};
+
+// Segment starts and staging offsets of row-block b (lo[k] = entries of the
+// segments before k, lo[nblk] = all of them) into LDS, by wavefront 0.
+__device__ __forceinline__ void cb_segments(int b, const CbView &cb, int32_t *st, int32_t *lo) {
+  if (threadIdx.x < 64) {
+    const int k = threadIdx.x;
+    int32_t s0 = 0, len = 0;
+    if (k < cb.nblk) {
+      s0 = cb.seg[(size_t)b * cb.nblk + k];
+      len = cb.seg[(size_t)(b + 1) * cb.nblk + k] - s0;
+    }
+    int32_t incl = len;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int32_t t = __shfl_up(incl, off, 64);
+      if (k >= off) incl += t;
+    }
+    if (k < cb.nblk) {
+      st[k] = s0;
+      lo[k] = incl - len;
+    }
+    if (k == cb.nblk - 1) lo[cb.nblk] = incl;
+  }
+}
+
+// Position in prod of staged slot i: the segment k holding it found by a
+// forward walk from the caller's k (slots visited in increasing order).
+__device__ __forceinline__ int32_t cb_slot_pos(int i, int &k, const int32_t *st, const int32_t *lo) {
+  while (lo[k + 1] <= i) ++k;
+  return st[k] + (i - lo[k]);
+}
 
 // Store of a result entry; NTY: with the non-temporal hint (y is written
 // once and is too large to stay cached for the next kernel).
@@ -103,25 +160,35 @@ __device__ __forceinline__ XT spmv_init(int row, const XT *y0, double inv_alpha)
 }
 
 // One row-block of y = A x (and the mode's epilogue) by one workgroup.
-template <typename VT, typename XT, int MODE, bool NTY = false>
+template <typename VT, typename XT, int MODE, bool NTY = false, bool PRE = false>
 __device__ __forceinline__ void spmv_rowblock(int b, XT *prod, const int2 *__restrict__ info,
                                               const int32_t *__restrict__ rp,
                                               const int32_t *__restrict__ ci,
                                               const VT *__restrict__ val, const XT *__restrict__ x,
                                               XT *y, const XT *y0, double alpha, double inv_alpha,
-                                              const XT *w, dd &acc0, dd &acc1) {
+                                              const XT *w, dd &acc0, dd &acc1, const CbView cb = CbView{}) {
   constexpr int RPER = SPMV_MAX_ROWS / BS;
+  __shared__ int32_t seg_st[PRE ? CB_MAX_BLK : 1], seg_lo[PRE ? CB_MAX_BLK + 1 : 1];
   {
     const int2 s0 = info[b], s1 = info[b + 1];
     const int r0 = s0.x, r1 = s1.x;
     const int j0 = s0.y;
     const int nnzb = s1.y - j0;
     const int nrows = r1 - r0;
+    if (PRE) {
+      cb_segments(b, cb, seg_st, seg_lo);
+      __syncthreads();
+    }
     if (nnzb > SPMV_CAP) {
       // one long row: strided partials from global memory, fixed tree
       double v[1] = {0.0};
       XT s = 0;
-      for (int k = threadIdx.x; k < nnzb; k += BS) s += (XT)val[j0 + k] * x[ci[j0 + k]];
+      if (PRE) {
+        int kseg = 0;
+        for (int k = threadIdx.x; k < nnzb; k += BS) s += (XT)cb.prod[cb_slot_pos(k, kseg, seg_st, seg_lo)];
+      } else {
+        for (int k = threadIdx.x; k < nnzb; k += BS) s += (XT)val[j0 + k] * x[ci[j0 + k]];
+      }
       v[0] = (double)s;
       block_sum<1>(v);
       if (threadIdx.x == 0) {
@@ -136,12 +203,23 @@ __device__ __forceinline__ void spmv_rowblock(int b, XT *prod, const int2 *__res
     int c[SPMV_PER];
     VT a[SPMV_PER];
     XT xv[SPMV_PER];
+    int qv[SPMV_PER];
+    if (PRE) {
+      // the position in prod of slot k of the segments' order
+      int kseg = 0;
 #pragma unroll
-    for (int u = 0; u < SPMV_PER; ++u) {
-      const int k = threadIdx.x + u * BS;
-      if (k < nnzb) {  // (non-temporal loads here measured 5.22 -> 4.88 TB/s on 256^3)
-        c[u] = ci[j0 + k];
-        a[u] = val[j0 + k];
+      for (int u = 0; u < SPMV_PER; ++u) {
+        const int k = threadIdx.x + u * BS;
+        if (k < nnzb) c[u] = cb_slot_pos(k, kseg, seg_st, seg_lo);
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < SPMV_PER; ++u) {
+        const int k = threadIdx.x + u * BS;
+        if (k < nnzb) {  // (non-temporal loads here measured 5.22 -> 4.88 TB/s on 256^3)
+          c[u] = ci[j0 + k];
+          a[u] = val[j0 + k];
+        }
       }
     }
     const bool one_lane = nrows * 2 > BS;  // uniform over the workgroup
@@ -159,25 +237,70 @@ __device__ __forceinline__ void spmv_rowblock(int b, XT *prod, const int2 *__res
 #pragma unroll
     for (int u = 0; u < SPMV_PER; ++u) {
       const int k = threadIdx.x + u * BS;
-      if (k < nnzb) xv[u] = x[c[u]];
+      if (k < nnzb) {
+        if (PRE) {
+          xv[u] = (XT)cb.prod[c[u]];
+          qv[u] = cb.q[c[u]];
+        } else {
+          xv[u] = x[c[u]];
+        }
+      }
     }
 #pragma unroll
     for (int u = 0; u < SPMV_PER; ++u) {
       const int k = threadIdx.x + u * BS;
-      if (k < nnzb) prod[k] = (XT)a[u] * xv[u];
+      if (k < nnzb) prod[PRE ? qv[u] : k] = PRE ? xv[u] : (XT)a[u] * xv[u];
     }
     __syncthreads();
     // 2. per-row reduction from LDS
     if (one_lane) {
-      // a thread owns a row: sequential sum in storage order (MTJ's order)
+      // a thread owns a row: sequential sum in storage order (MTJ's order).
+      // A row of more than SPMV_LONG_ROW entries would hold the whole
+      // workgroup while one lane walks it, so those rows are listed (in
+      // (slot, wavefront, lane) order, from ballots) and summed afterwards
+      // by a wavefront each: strided partials, then the xor butterfly.
+      __shared__ int32_t lcnt[RPER * (BS / 64)];
+      __shared__ int16_t lbeg[SPMV_MAX_ROWS], lend[SPMV_MAX_ROWS], lrow[SPMV_MAX_ROWS];
+      const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+      uint64_t lmask[RPER];
 #pragma unroll
       for (int u = 0; u < RPER; ++u) {
         const int g = threadIdx.x + u * BS;
-        if (g < nrows) {
+        const bool lng = g < nrows && re[u] - rs[u] > SPMV_LONG_ROW;
+        lmask[u] = __ballot(lng);
+        if (lane == 0) lcnt[u * (BS / 64) + wv] = __popcll(lmask[u]);
+        if (g < nrows && !lng) {
           const int row = r0 + g;
           XT s = spmv_init<XT, MODE>(row, y0, inv_alpha);
           for (int k = rs[u]; k < re[u]; ++k) s += prod[k];
           spmv_store<XT, MODE, NTY>(row, s, y, y0, alpha, inv_alpha, w, acc0, acc1);
+        }
+      }
+      __syncthreads();
+      int nlong = 0;
+#pragma unroll
+      for (int u = 0; u < RPER; ++u) {
+        int base = nlong;
+        for (int v = 0; v < BS / 64; ++v) {
+          const int c = lcnt[u * (BS / 64) + v];
+          if (v < wv) base += c;
+          nlong += c;
+        }
+        if ((lmask[u] >> lane) & 1ull) {
+          const int pos = base + __popcll(lmask[u] & ((1ull << lane) - 1ull));
+          lbeg[pos] = (int16_t)rs[u];  // < SPMV_CAP
+          lend[pos] = (int16_t)re[u];
+          lrow[pos] = (int16_t)(threadIdx.x + u * BS);
+        }
+      }
+      if (nlong > 0) {
+        __syncthreads();
+        for (int i = wv; i < nlong; i += BS / 64) {
+          const int row = r0 + lrow[i];
+          XT s = (lane == 0) ? spmv_init<XT, MODE>(row, y0, inv_alpha) : XT(0);
+          for (int k = lbeg[i] + lane; k < lend[i]; k += 64) s += prod[k];
+          for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+          if (lane == 0) spmv_store<XT, MODE, NTY>(row, s, y, y0, alpha, inv_alpha, w, acc0, acc1);
         }
       }
     } else {

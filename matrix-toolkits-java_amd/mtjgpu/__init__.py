@@ -321,10 +321,11 @@ class CompRowMatrix:
         return r.value, c.value, z.value
 
     def format(self):
-        """("csr" | "sell", entries streamed by a product incl. padding)."""
+        """("csr" | "sell" | "csr_xcd", entries streamed by a product incl.
+        padding)."""
         f, z = C.c_int32(), C.c_int64()
         _ck(self._lib.mtjg_csr_format(self.handle, C.byref(f), C.byref(z)))
-        return ("sell" if f.value == 1 else "csr"), z.value
+        return {0: "csr", 1: "sell", 2: "csr_xcd"}[f.value], z.value
 
     def numRows(self):
         return self.shape()[0]

@@ -1,7 +1,8 @@
 """GPU parity of the CompRowMatrix products (S/CompRowMatrix.java:294-388)
 through the C ABI against the oracle, for each device format: the automatic
-choice, the row-block CSR kernels only (MTJG_SPMV_FORMAT=csr) and the SELL-64
-copy for every matrix (MTJG_SPMV_FORMAT=sell).
+choice, the row-block CSR kernels only (MTJG_SPMV_FORMAT=csr), the SELL-64
+copy for every matrix (MTJG_SPMV_FORMAT=sell) and the XCD column blocks in
+front of the row-block kernels for every matrix (MTJG_SPMV_FORMAT=xcd).
 
 Bar: bitwise equality wherever the kernel sums a row sequentially (every row
 of the SELL-64 kernels; rows in row-blocks of >= 128 rows for the CSR
@@ -24,7 +25,7 @@ GOLD = os.path.join(os.path.dirname(__file__), "golden")
 TOL = 1e-14
 
 
-@pytest.fixture(params=["auto", "csr", "sell"], autouse=True)
+@pytest.fixture(params=["auto", "csr", "sell", "xcd"], autouse=True)
 def spmv_format(request, monkeypatch):
     # read when a matrix is created
     monkeypatch.setenv("MTJG_SPMV_FORMAT", request.param)
@@ -214,3 +215,46 @@ def test_value_updates_reach_every_copy(ctx):
     va3[rp[5]] = -3.0
     _check(A.mult(x, y).getData(), oracle.Csr(n, m, rp, ci, va3).mult(xh), rp, ci, va3, xh,
            exact=A.format()[0] == "sell")
+
+
+@pytest.mark.parametrize("gen", ["powerlaw_20k", "long_rows", "rect_3x700"])
+def test_xcd_blocks_bitwise_row_block(ctx, monkeypatch, gen):
+    """The XCD column-block products form each product val[j] * x[col[j]] in a
+    pass of their own and the row-block kernels sum them in the order they sum
+    the products they form: every result is bitwise the row-block one's."""
+    rng = np.random.default_rng(9)
+    if gen == "powerlaw_20k":
+        sysm = P.powerlaw(20000, seed=4)
+    elif gen == "long_rows":
+        n = 5000
+        lens = rng.integers(0, 12, size=n)
+        lens[17], lens[900] = 4100, 2049
+        rows = [np.sort(rng.choice(n, size=L, replace=False)) for L in lens]
+        rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+        sysm = (n, n, rp, np.concatenate(rows).astype(np.int32), rng.standard_normal(int(rp[-1])))
+    else:
+        m = sp.random(3, 700, density=0.4, random_state=5, format="csr")
+        m.sort_indices()
+        sysm = (3, 700, m.indptr, m.indices, m.data)
+    nr, nc = sysm[0], sysm[1]
+    mats = {}
+    for fmt in ("csr", "xcd"):
+        monkeypatch.setenv("MTJG_SPMV_FORMAT", fmt)
+        mats[fmt] = M.CompRowMatrix(*sysm, ctx=ctx)
+    assert mats["csr"].format()[0] == "csr" and mats["xcd"].format()[0] == "csr_xcd"
+    xh, xt = rng.standard_normal(nc), rng.standard_normal(nr)
+    yh, yt = rng.standard_normal(nr), rng.standard_normal(nc)
+    out = {}
+    for fmt, A in mats.items():
+        x, y = M.DenseVector(xh, ctx), M.DenseVector(nr, ctx)
+        r = [A.mult(x, y).getData()]
+        y.upload(yh)
+        r.append(A.multAdd(-0.5, x, y).getData())
+        xT, yT = M.DenseVector(xt, ctx), M.DenseVector(yt, ctx)
+        r.append(A.transMultAdd(1.25, xT, yT).getData())
+        r.append(A.transMult(xT, yT).getData())
+        out[fmt] = r
+        for o in (x, y, xT, yT, A):
+            o.close()
+    for a, b in zip(out["csr"], out["xcd"]):
+        np.testing.assert_array_equal(a, b)
