@@ -150,10 +150,10 @@ int mtjg_csr_download(mtjg_csr A, int32_t *row_pointer, int32_t *column_index,
  * CSR kernels (MTJG_FORMAT_CSR); for patterns whose rows in each group of
  * 64 have similar lengths, an added SELL-64 copy (MTJG_FORMAT_SELL; padding
  * at most a quarter of the stored entries); or, for irregular patterns whose
- * columns scatter over an x of more than 16 MB, an added copy of the entries
- * ordered by column blocks, one XCD's L2 of x each, whose products a first
- * pass forms before the row-block kernels sum them (MTJG_FORMAT_CSR_XCD;
- * results bitwise those of MTJG_FORMAT_CSR).  *stored counts the entries the
+ * columns scatter over an x of 16 to 128 MB, an added copy of the entries
+ * ordered by column blocks of 2 to 4 MB of x, each gathered on one XCD, whose
+ * products a first pass forms before the row-block kernels sum them
+ * (MTJG_FORMAT_CSR_XCD; results bitwise those of MTJG_FORMAT_CSR).  *stored counts the entries the
  * products stream, padding included.  MTJG_SPMV_FORMAT=csr|sell|xcd in the
  * environment at creation forces the choice. */
 enum { MTJG_FORMAT_CSR = 0, MTJG_FORMAT_SELL = 1, MTJG_FORMAT_CSR_XCD = 2 };

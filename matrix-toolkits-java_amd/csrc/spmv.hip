@@ -153,14 +153,18 @@ __global__ __launch_bounds__(BS) void k_spmv_dot(int nrb, int slabs, const int2 
 // their own: the entries are kept a second time ordered by column block
 // (CB_BLOCK_BYTES of x per block), the blocks dealt to the XCDs (block b to
 // XCD b % 8, which runs workgroups w with w % 8 == b % 8), so each XCD
-// gathers only from its own slices of x, which stay in its L2.  Within a
+// gathers only from its own slices of x, which stay in its L2.  On the
+// power-law graph (products pass + sums pass): 4 MB blocks and 8 entries per
+// thread 163 + 95 us, 16 entries 147 + 94 us (32: 152 us), 2 MB blocks with
+// 16 entries 133 + 98 us (1 MB: 130 + 110 us), against 298 us for the
+// row-block kernel alone.  Within a
 // block the entries keep CSR order, and the products are stored with the
 // non-temporal hint so that they do not evict the slice.  The row-block
 // kernel then stages each row-block's products segment by segment (PRE,
 // spmv_rowblock.hpp) and sums the rows exactly as it sums products it forms
 // itself: the result is bitwise the row-block product's.
-constexpr int64_t CB_BLOCK_BYTES = 4ll << 20;  // x per column block: one XCD's L2
-constexpr int CB_PER = 8;                       // entries per thread of the products pass
+constexpr int64_t CB_BLOCK_BYTES = 2ll << 20;  // x per column block: half of one XCD's L2
+constexpr int CB_PER = 16;                      // entries per thread of the products pass
 constexpr int CB_CHUNK = BS * CB_PER;           // entries per workgroup
 constexpr int CB_NXCD = 8;
 
@@ -665,9 +669,11 @@ static double gather_scatter(const int32_t *ci, const std::vector<int2> &rb) {
 // copy when x is larger than CB_MIN_X_BYTES and at least CB_SCATTER_MIN of
 // the gathers fetch a line no neighbouring entry fetches (auto), or always
 // with MTJG_SPMV_FORMAT=xcd.  The blocks are the fewest multiple of eight
-// whose x slices are at most CB_BLOCK_BYTES (at most CB_MAX_BLK blocks, so x
-// of at most 128 MB); XCD k's blocks k, k + 8, ... are laid out one after the
-// other, each block's entries in CSR order.  Alongside: each row-block's
+// whose x slices are at most CB_BLOCK_BYTES, or CB_MAX_BLK blocks; the
+// automatic choice leaves x of more than CB_MAX_BLK slices of twice that
+// (128 MB) alone, since larger slices no longer stay in an XCD's L2.  XCD
+// k's blocks k, k + 8, ... are laid out one after the other, each block's
+// entries in CSR order.  Alongside: each row-block's
 // segment starts per block and, in the products' order, each product's
 // entry index within its row-block.  ci is the host copy of the (sorted)
 // columns and rb the row-block schedule.
@@ -689,8 +695,8 @@ static void cb_build(mtjg_csr_s *A, const int32_t *ci, const std::vector<int2> &
   int64_t nblk = (xbytes + blk_bytes - 1) / blk_bytes;
   nblk = std::max<int64_t>(CB_NXCD, (nblk + CB_NXCD - 1) / CB_NXCD * CB_NXCD);
   if (nblk > CB_MAX_BLK) {
-    if (mode == 2) fail(MTJG_E_UNSUPPORTED, "x too large for the XCD column blocks");
-    return;
+    if (mode == 0 && xbytes > CB_MAX_BLK * 2 * CB_BLOCK_BYTES) return;
+    nblk = CB_MAX_BLK;
   }
   nblk = std::min<int64_t>(nblk, std::max<int64_t>(CB_NXCD, (int64_t)A->nc / CB_NXCD * CB_NXCD));
   const int32_t width = (int32_t)(((int64_t)A->nc + nblk - 1) / nblk);

@@ -138,7 +138,7 @@ def test_mixed_precision_vs_fp64_oracle(ctx, scheme):
              {"inner_iterations": inner, "oracle_is": "FP64 Jacobi-CG to 1e-10"})
 
 
-@pytest.mark.parametrize("fmt", ["auto", "csr"])
+@pytest.mark.parametrize("fmt", ["auto", "csr", "xcd"])
 def test_spmv_poisson3d_256_bitwise(ctx, monkeypatch, fmt):
     monkeypatch.setenv("MTJG_SPMV_FORMAT", fmt)
     sysm = P.poisson3d(256)
@@ -157,7 +157,7 @@ def test_spmv_poisson3d_256_bitwise(ctx, monkeypatch, fmt):
 
 
 @pytest.mark.parametrize("gen", ["powerlaw_4m", "powerlaw_spd_4m"])
-def test_spmv_powerlaw_4m_row_bound(ctx, gen):
+def test_spmv_powerlaw_4m_row_bound(ctx, monkeypatch, gen):
     sysm = P.powerlaw(4_000_000, seed=0) if gen == "powerlaw_4m" else P.powerlaw_spd(4_000_000, seed=0)
     n, _, rp, ci, va = sysm
     xh = P.uniform(n, 1)
@@ -166,11 +166,20 @@ def test_spmv_powerlaw_4m_row_bound(ctx, gen):
     x = M.DenseVector(xh, ctx)
     y = M.DenseVector(n, ctx)
     yg = A.mult(x, y).getData()
-    for o in (y, x, A):
+    used = A.format()[0]
+    A.close()
+    # the XCD column blocks (chosen for these scattered columns) sum the same
+    # products in the same order as the row-block kernel
+    monkeypatch.setenv("MTJG_SPMV_FORMAT", "csr")
+    Ac = M.CompRowMatrix(*sysm, ctx=ctx)
+    yc = Ac.mult(x, y).getData()
+    for o in (y, x, Ac):
         o.close()
     Aabs = sp.csr_matrix((np.abs(va), ci, rp), shape=(n, n))
     scale = Aabs @ np.abs(xh)
     ratio = np.abs(yg - ref) / (1e-14 * scale + 1e-300)
-    _report(workload="spmv_" + gen, nnz=int(len(va)), max_err_over_bound=float(ratio.max()),
-            rows_bitwise=float(np.mean(yg == ref)))
+    _report(workload="spmv_" + gen, nnz=int(len(va)), format=used, max_err_over_bound=float(ratio.max()),
+            rows_bitwise=float(np.mean(yg == ref)), bitwise_row_block_kernel=bool(np.array_equal(yg, yc)))
+    assert used == "csr_xcd"
     assert np.all(ratio <= 1.0)
+    np.testing.assert_array_equal(yg, yc)
