@@ -15,13 +15,84 @@
 
 namespace mtjg {
 
+// The value x holds in lane (lane ^ OFF) of the wavefront, OFF one of 1, 2,
+// 4, 8, 16, 32: the partner __shfl_xor(x, OFF) reads.  Inside each row of 16
+// lanes DPP moves it as a VALU operand permutation instead of a trip through
+// the LDS crossbar (ds_bpermute_b32, at LDS latency): quad_perm [1,0,3,2] for
+// 1, [2,3,0,1] for 2, the half-row mirror (lane ^ 7) then quad_perm
+// [3,2,1,0] (lane ^ 3) for 4, the row mirror (lane ^ 15) then the half-row
+// mirror for 8.  16 is a ds_swizzle (xor 0x10 within each 32 lanes, no
+// address operand) and 32 stays a ds_bpermute.  (The gfx950 permlane swaps
+// gave wrong partners in reductions as this compiler schedules them, though
+// right ones in an isolated lane test, so they are not used.)  Every lane of
+// the partner pairs used must be active.
+template <int OFF>
+__device__ __forceinline__ uint32_t lane_xor_u32(uint32_t x) {
+  static_assert(OFF == 1 || OFF == 2 || OFF == 4 || OFF == 8 || OFF == 16 || OFF == 32, "lane offset");
+  if constexpr (OFF == 1) {
+    return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);
+  } else if constexpr (OFF == 2) {
+    return __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);
+  } else if constexpr (OFF == 4) {
+    return __builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(x, 0x141, 0xF, 0xF, false), 0x1B, 0xF, 0xF, false);
+  } else if constexpr (OFF == 8) {
+    return __builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(x, 0x140, 0xF, 0xF, false), 0x141, 0xF, 0xF, false);
+  } else if constexpr (OFF == 16) {
+    return __builtin_amdgcn_ds_swizzle(x, 0x401F);
+  } else {
+    return __shfl_xor(x, 32, 64);
+  }
+}
+
+template <int OFF>
+__device__ __forceinline__ double lane_xor(double v) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint64_t lo = lane_xor_u32<OFF>((uint32_t)b), hi = lane_xor_u32<OFF>((uint32_t)(b >> 32));
+  return __longlong_as_double((long long)((hi << 32) | lo));
+}
+
+template <int OFF>
+__device__ __forceinline__ float lane_xor(float v) {
+  return __uint_as_float(lane_xor_u32<OFF>(__float_as_uint(v)));
+}
+
+// lane_xor for an offset known only at run time (uniform over the wavefront).
+__device__ __forceinline__ float lane_xor_rt(float v, int off) {
+  switch (off) {
+    case 1: return lane_xor<1>(v);
+    case 2: return lane_xor<2>(v);
+    case 4: return lane_xor<4>(v);
+    case 8: return lane_xor<8>(v);
+    case 16: return lane_xor<16>(v);
+    default: return lane_xor<32>(v);
+  }
+}
+__device__ __forceinline__ double lane_xor_rt(double v, int off) {
+  switch (off) {
+    case 1: return lane_xor<1>(v);
+    case 2: return lane_xor<2>(v);
+    case 4: return lane_xor<4>(v);
+    case 8: return lane_xor<8>(v);
+    case 16: return lane_xor<16>(v);
+    default: return lane_xor<32>(v);
+  }
+}
+
+// One step of the xor butterfly: v += v of lane (lane ^ OFF).
+template <int OFF, int K>
+__device__ __forceinline__ void xor_step(double (&v)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] += lane_xor<OFF>(v[k]);
+}
+
 template <int K>
 __device__ __forceinline__ void wave_sum(double (&v)[K]) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) v[k] += __shfl_xor(v[k], off, 64);
-  }
+  xor_step<32>(v);
+  xor_step<16>(v);
+  xor_step<8>(v);
+  xor_step<4>(v);
+  xor_step<2>(v);
+  xor_step<1>(v);
 }
 
 // Sum over the workgroup; the total is returned in every thread.
@@ -211,17 +282,21 @@ __device__ __forceinline__ void dd_fma(dd &acc, double a, double b) {
   acc.lo += e + pe;
 }
 
+template <int OFF, int K>
+__device__ __forceinline__ void xor_step_dd(dd (&v)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = dd_add(v[k], dd{lane_xor<OFF>(v[k].hi), lane_xor<OFF>(v[k].lo)});
+}
+
 template <int K>
 __device__ __forceinline__ void block_sum_dd(dd (&v)[K]) {
   __shared__ double sh[2 * K][BS / 64];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      dd o{__shfl_xor(v[k].hi, off, 64), __shfl_xor(v[k].lo, off, 64)};
-      v[k] = dd_add(v[k], o);
-    }
-  }
+  xor_step_dd<32>(v);
+  xor_step_dd<16>(v);
+  xor_step_dd<8>(v);
+  xor_step_dd<4>(v);
+  xor_step_dd<2>(v);
+  xor_step_dd<1>(v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0) {
 #pragma unroll

@@ -299,7 +299,8 @@ __device__ __forceinline__ void spmv_rowblock(int b, XT *prod, const int2 *__res
           const int row = r0 + lrow[i];
           XT s = (lane == 0) ? spmv_init<XT, MODE>(row, y0, inv_alpha) : XT(0);
           for (int k = lbeg[i] + lane; k < lend[i]; k += 64) s += prod[k];
-          for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+#pragma unroll
+          for (int off = 32; off > 0; off >>= 1) s += lane_xor_rt(s, off);
           if (lane == 0) spmv_store<XT, MODE, NTY>(row, s, y, y0, alpha, inv_alpha, w, acc0, acc1);
         }
       }
@@ -312,7 +313,7 @@ __device__ __forceinline__ void spmv_rowblock(int b, XT *prod, const int2 *__res
         const int beg = rp[row] - j0, end = rp[row + 1] - j0;
         XT s = (lane == 0) ? spmv_init<XT, MODE>(row, y0, inv_alpha) : XT(0);
         for (int k = beg + lane; k < end; k += tpr) s += prod[k];
-        for (int off = tpr >> 1; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        for (int off = tpr >> 1; off > 0; off >>= 1) s += lane_xor_rt(s, off);
         if (lane == 0) spmv_store<XT, MODE, NTY>(row, s, y, y0, alpha, inv_alpha, w, acc0, acc1);
       }
     }
