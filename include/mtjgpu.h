@@ -158,6 +158,14 @@ int mtjg_csr_download(mtjg_csr A, int32_t *row_pointer, int32_t *column_index,
  * environment at creation forces the choice. */
 enum { MTJG_FORMAT_CSR = 0, MTJG_FORMAT_SELL = 1, MTJG_FORMAT_CSR_XCD = 2 };
 int mtjg_csr_format(mtjg_csr A, int32_t *format, int64_t *stored);
+/* Diagonals of the FP32 stencil copy (no MTJ counterpart): the FP32 products
+ * of the mixed-precision solvers (MTJG_MIXED_CG, MTJG_PREC_CG32) read a copy
+ * that stores only values, diagonal by diagonal, when every entry lies on one
+ * of at most 8 diagonals (column - row offsets) and the pattern got a SELL-64
+ * copy; *count is their number, or 0 when the FP32 products use SELL-64 or
+ * CSR.  Results are bitwise those of the SELL-64 FP32 product.
+ * MTJG_SPMV_STENCIL=0 in the environment at creation turns the copy off. */
+int mtjg_csr_fp32_diagonals(mtjg_csr A, int32_t *count);
 /* set(Matrix) with an identical pattern (S/CompRowMatrix.java:449-471). */
 int mtjg_csr_set_values(mtjg_csr A, const double *data);
 /* get(row, column): 0 outside the pattern (S/CompRowMatrix.java:406-417). */

@@ -176,6 +176,18 @@ struct mtjg_csr_s {
   double *cb_prod = nullptr;
   int32_t *cb_seg = nullptr;
   uint16_t *cb_q = nullptr;
+  // FP32 stencil copy for the FP32 products (mixed precision), added when
+  // every entry lies on one of at most 8 diagonals (column - row offsets
+  // st_off[0..st_m), ascending; spmv.hip, stencil_detect): st_val32[k*nr + i]
+  // is row i's value on diagonal k (0 where the row has none) and bit k of
+  // st_mask[i] says whether it has one.  Columns are implicit, so an entry
+  // costs 4 bytes instead of 8.  Filled on first use; `st_stale` marks values
+  // changed since.
+  int32_t st_m = 0;
+  int32_t st_off[8] = {};
+  bool st_stale = true;
+  float *st_val32 = nullptr;
+  uint8_t *st_mask = nullptr;
   // Host copy of the row pointers (schedules, level analysis, validation).
   std::vector<int32_t> h_rp;
   // Transpose (CSR of A'), built on first transposed product.

@@ -300,6 +300,103 @@ __global__ __launch_bounds__(BS) void k_sell_fin(int np, const double *part, KSt
   if (grid_sum_dd<2>(v, red, tot) && threadIdx.x == 0) spmv_finish(st, fin, tot[0], tot[1]);
 }
 
+// ---- FP32 stencil copy --------------------------------------------------------
+//
+// The mixed-precision inner solves stream the FP32 matrix once per iteration;
+// in SELL-64 an entry costs a 4-byte value and a 4-byte column.  When every
+// entry lies on one of at most ST_MAX_OFF diagonals (stencil matrices:
+// column - row in {0, +-1, +-nx, +-nx*ny}), row i's columns are i + off[k]
+// for the diagonals its mask byte names, so the FP32 copy keeps only the
+// values, diagonal-major (a lane per row, each diagonal's values of 64
+// consecutive rows one contiguous 256-byte line, as in SELL-64), and the
+// gathers x[i + off[k]] of neighbouring lanes are contiguous.  A row's
+// products are added in diagonal order, which is ascending column order,
+// the order of the SELL-64 row and of MTJ's loop: every sum is bitwise the
+// SELL-64 FP32 product's.
+constexpr int ST_MAX_OFF = 8;
+struct StOff {
+  int32_t off[ST_MAX_OFF];
+};
+
+template <int MODE>
+__device__ __forceinline__ void st_row(int row, int n, int m, const StOff &o, const float *__restrict__ val,
+                                       const uint8_t *__restrict__ mask, const float *__restrict__ x, float *y,
+                                       const float *w, dd &acc0, dd &acc1) {
+  if (row >= n) return;
+  const uint32_t mk = mask[row];
+  float v[ST_MAX_OFF], xv[ST_MAX_OFF];
+#pragma unroll
+  for (int k = 0; k < ST_MAX_OFF; ++k) {
+    if (k < m && ((mk >> k) & 1u)) {
+      v[k] = __builtin_nontemporal_load(&val[(size_t)k * n + row]);
+      xv[k] = x[row + o.off[k]];
+    }
+  }
+  float s = 0.0f;
+#pragma unroll
+  for (int k = 0; k < ST_MAX_OFF; ++k)
+    if (k < m && ((mk >> k) & 1u)) s += v[k] * xv[k];
+  spmv_store<float, MODE>(row, s, y, nullptr, 1.0, 1.0, w, acc0, acc1);
+}
+
+// Split fused product (k_sell_part's protocol): a workgroup per 256 rows
+// under the XCD run mapping, compensated partials stored without a ticket.
+__global__ __launch_bounds__(BS) void k_st_part(int n, int m, StOff o, const float *__restrict__ val,
+                                                const uint8_t *__restrict__ mask, const float *__restrict__ x,
+                                                float *y, const float *w, const KState *st, double *part, int cap) {
+  if (st->status != ST_RUNNING) return;
+  const int blk = xcd_run_block(blockIdx.x, gridDim.x, SELL_XCD_RUN);
+  dd v[2] = {{0.0, 0.0}, {0.0, 0.0}};
+  st_row<SPMV_DOT>(blk * BS + (int)threadIdx.x, n, m, o, val, mask, x, y, w, v[0], v[1]);
+  block_sum_dd<2>(v);
+  if (threadIdx.x == 0) {
+    part[0 * (size_t)cap + blockIdx.x] = v[0].hi;
+    part[1 * (size_t)cap + blockIdx.x] = v[0].lo;
+    part[2 * (size_t)cap + blockIdx.x] = v[1].hi;
+    part[3 * (size_t)cap + blockIdx.x] = v[1].lo;
+  }
+}
+
+// One-kernel fused product for grids of at most SPMV_DOT_GRID workgroups.
+__global__ __launch_bounds__(BS) void k_st_dot(int n, int m, StOff o, const float *__restrict__ val,
+                                               const uint8_t *__restrict__ mask, const float *__restrict__ x,
+                                               float *y, const float *w, KState *st, int fin, GridRed red) {
+  if (st->status != ST_RUNNING) return;
+  dd acc0{0.0, 0.0}, acc1{0.0, 0.0};
+  for (int row = blockIdx.x * BS + (int)threadIdx.x; row < n; row += gridDim.x * BS)
+    st_row<SPMV_DOT>(row, n, m, o, val, mask, x, y, w, acc0, acc1);
+  dd v[2] = {acc0, acc1};
+  double tot[2];
+  if (grid_sum_dd<2>(v, red, tot) && threadIdx.x == 0) spmv_finish(st, fin, tot[0], tot[1]);
+}
+
+// Fills the stencil copy from the CSR values: a thread per row, each entry
+// on the diagonal of its offset (as (float)value, the SELL-64 FP32 copy's
+// conversion), the other diagonals of the row 0 with their mask bit clear.
+__global__ void k_st_fill32(int32_t n, int m, StOff o, const int32_t *__restrict__ rp,
+                            const int32_t *__restrict__ ci, const double *__restrict__ val, float *sv,
+                            uint8_t *mask) {
+  const int32_t row = blockIdx.x * BS + threadIdx.x;
+  if (row >= n) return;
+  float out[ST_MAX_OFF];
+  uint32_t mk = 0;
+#pragma unroll
+  for (int k = 0; k < ST_MAX_OFF; ++k) out[k] = 0.0f;
+  for (int32_t j = rp[row]; j < rp[row + 1]; ++j) {
+    const int32_t d = ci[j] - row;
+#pragma unroll
+    for (int k = 0; k < ST_MAX_OFF; ++k)
+      if (k < m && o.off[k] == d) {
+        out[k] = (float)val[j];
+        mk |= 1u << k;
+      }
+  }
+#pragma unroll
+  for (int k = 0; k < ST_MAX_OFF; ++k)
+    if (k < m) sv[(size_t)k * n + row] = out[k];
+  mask[row] = (uint8_t)mk;
+}
+
 // CG's direction update folded into its fused product (S/CG.java:71-79):
 // p' = z on the first iteration and p' = (p * beta) + z after it, formed at
 // every gather from p and z (the operations of the separate update, so p' is
@@ -562,7 +659,48 @@ bool spmv_cg_fused(mtjg_csr A, const double *p, const double *z, double *pn, dou
   return true;
 }
 
+// The FP32 stencil copy's fused product (SPMV_DOT), filled first if the
+// values changed.
+static bool st_use(mtjg_csr A, const SpmvArgs &a) {
+  return A->st_m > 0 && A->sell && A->nr > 0 && a.mode == SPMV_DOT;
+}
+
+static void st_launch(mtjg_csr A, const float *x, float *y, const SpmvArgs &a, hipStream_t s) {
+  StOff o;
+  for (int k = 0; k < ST_MAX_OFF; ++k) o.off[k] = k < A->st_m ? A->st_off[k] : 0;
+  if (!A->st_val32) {
+    HIPCHK(hipMalloc(&A->st_val32, sizeof(float) * (size_t)A->st_m * (size_t)A->nr));
+    HIPCHK(hipMalloc(&A->st_mask, (size_t)A->nr));
+    A->st_stale = true;
+  }
+  if (A->st_stale) {
+    k_st_fill32<<<(A->nr + BS - 1) / BS, BS, 0, s>>>(A->nr, A->st_m, o, A->rp, A->ci, A->val, A->st_val32,
+                                                    A->st_mask);
+    HIPCHK(hipGetLastError());
+    A->st_stale = false;
+  }
+  const int grid = (A->nr + BS - 1) / BS;  // equals the SELL-64 product's grid (four slices per workgroup)
+  if (grid > SPMV_DOT_GRID) {
+    const int f = sell_fin_grid(grid);
+    if ((int64_t)f + grid > a.red.cap) fail(MTJG_E_INVALID_ARGUMENT, "reduction workspace smaller than the grid");
+    double *part = a.red.partials + f;
+    k_st_part<<<grid, BS, 0, s>>>(A->nr, A->st_m, o, A->st_val32, A->st_mask, x, y, (const float *)a.w, a.st,
+                                  part, a.red.cap);
+    HIPCHK(hipGetLastError());
+    k_sell_fin<<<f, BS, 0, s>>>(grid, part, a.st, a.fin, a.red);
+  } else {
+    if (grid > a.red.cap) fail(MTJG_E_INVALID_ARGUMENT, "reduction workspace smaller than the grid");
+    k_st_dot<<<grid, BS, 0, s>>>(A->nr, A->st_m, o, A->st_val32, A->st_mask, x, y, (const float *)a.w, a.st,
+                                 a.fin, a.red);
+  }
+  HIPCHK(hipGetLastError());
+}
+
 void spmv_f32(mtjg_csr A, const float *x, float *y, const SpmvArgs &a, hipStream_t s) {
+  if (st_use(A, a)) {
+    st_launch(A, x, y, a, s);
+    return;
+  }
   if (A->sell && A->nr > 0) {
     sell_ready(A, true, s);
     sell_launch<float, float>(A, A->sval32, x, y, a, s);
@@ -600,6 +738,8 @@ static void csr_free(mtjg_csr_s *A) {
   (void)hipFree(A->cb_prod);
   (void)hipFree(A->cb_seg);
   (void)hipFree(A->cb_q);
+  (void)hipFree(A->st_val32);
+  (void)hipFree(A->st_mask);
   delete A;
 }
 
@@ -746,6 +886,33 @@ static void cb_build(mtjg_csr_s *A, const int32_t *ci, const std::vector<int2> &
   A->cb = true;
 }
 
+// Records the diagonals of a pattern whose entries all lie on at most
+// ST_MAX_OFF of them (column - row offsets, ascending), for the FP32 stencil
+// copy; only patterns that got a SELL-64 copy (rows of similar length) are
+// examined.  MTJG_SPMV_STENCIL=0 at creation keeps the SELL-64 FP32 product.
+// ci is the host copy of the (sorted) columns.
+static void stencil_detect(mtjg_csr_s *A, const int32_t *ci) {
+  if (!A->sell || A->nnz == 0) return;
+  if (const char *e = getenv("MTJG_SPMV_STENCIL"))
+    if (atoi(e) == 0) return;
+  int32_t off[ST_MAX_OFF];
+  int m = 0;
+  for (int32_t i = 0; i < A->nr; ++i) {
+    for (int32_t j = A->h_rp[i]; j < A->h_rp[i + 1]; ++j) {
+      const int32_t d = ci[j] - i;
+      int k = 0;
+      while (k < m && off[k] < d) ++k;
+      if (k < m && off[k] == d) continue;
+      if (m == ST_MAX_OFF) return;
+      for (int t = m; t > k; --t) off[t] = off[t - 1];
+      off[k] = d;
+      ++m;
+    }
+  }
+  A->st_m = m;
+  for (int k = 0; k < m; ++k) A->st_off[k] = off[k];
+}
+
 // Uploads a validated host CSR (sorted unique columns) and builds its
 // schedule.
 static mtjg_csr_s *csr_upload(mtjg_ctx ctx, int32_t nr, int32_t nc, std::vector<int32_t> &&rp,
@@ -779,6 +946,7 @@ static mtjg_csr_s *csr_upload(mtjg_ctx ctx, int32_t nr, int32_t nc, std::vector<
                           ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     sell_build(A);
+    stencil_detect(A, ci);
     cb_build(A, ci, rb);
   } catch (...) {
     csr_free(A);
@@ -896,6 +1064,13 @@ extern "C" int mtjg_csr_format(mtjg_csr A, int32_t *format, int64_t *stored) {
   MTJG_API_END
 }
 
+extern "C" int mtjg_csr_fp32_diagonals(mtjg_csr A, int32_t *count) {
+  MTJG_API_BEGIN
+  if (!A || !count) fail(MTJG_E_NULL, "null argument");
+  *count = A->sell ? A->st_m : 0;
+  MTJG_API_END
+}
+
 extern "C" int mtjg_csr_shape(mtjg_csr A, int32_t *n_rows, int32_t *n_cols, int64_t *nnz) {
   MTJG_API_BEGIN
   if (!A) fail(MTJG_E_NULL, "null matrix");
@@ -923,6 +1098,7 @@ static void csr_values_changed(mtjg_csr A) {
   // derived copies of the values are rebuilt on next use
   if (A->sell) A->sell_stale = true;
   if (A->cb) A->cb_stale = true;
+  A->st_stale = true;
   if (A->val32 || A->sval32) {
     HIPCHK(hipStreamSynchronize(A->ctx->stream));
     (void)hipFree(A->val32);

@@ -327,6 +327,13 @@ class CompRowMatrix:
         _ck(self._lib.mtjg_csr_format(self.handle, C.byref(f), C.byref(z)))
         return {0: "csr", 1: "sell", 2: "csr_xcd"}[f.value], z.value
 
+    def fp32_diagonals(self):
+        """Diagonals of the FP32 stencil copy the mixed-precision products
+        read (0: they use SELL-64 or CSR)."""
+        c = C.c_int32()
+        _ck(self._lib.mtjg_csr_fp32_diagonals(self.handle, C.byref(c)))
+        return c.value
+
     def numRows(self):
         return self.shape()[0]
 

@@ -80,6 +80,7 @@ PROTOTYPES = [
     ("mtjg_csr_destroy", C.c_int, [P]),
     ("mtjg_csr_shape", C.c_int, [P, PI32, PI32, PI64]),
     ("mtjg_csr_format", C.c_int, [P, PI32, PI64]),
+    ("mtjg_csr_fp32_diagonals", C.c_int, [P, PI32]),
     ("mtjg_csr_download", C.c_int, [P, P, P, P]),
     ("mtjg_csr_set_values", C.c_int, [P, P]),
     ("mtjg_csr_get", C.c_int, [P, I32, I32, PD]),

@@ -262,3 +262,53 @@ def test_fp32_cg_preconditioner_errors(ctx):
                                [1.0, 0.5, 2.0], rtol=1e-7)
     with pytest.raises(M.IllegalArgumentException):
         M.FP32CGPreconditioner(3, inner_rtol=0.0, ctx=ctx)
+
+
+@pytest.mark.parametrize("gen,diags", [("varcoef3d_24", 7), ("poisson2d_96", 5), ("poisson3d_20x17x9", 7)])
+def test_fp32_stencil_copy_bitwise_sell(ctx, monkeypatch, gen, diags):
+    """The FP32 stencil copy (values only, diagonal by diagonal) sums each row
+    in the SELL-64 FP32 product's order: the mixed-precision solve and the
+    FP32-CG preconditioner give bitwise the same results with the copy as
+    without it (MTJG_SPMV_STENCIL=0)."""
+    sysm = {"varcoef3d_24": lambda: P.varcoef_poisson(3, 24, seed=3),
+            "poisson2d_96": lambda: P.poisson2d(96),
+            "poisson3d_20x17x9": lambda: P.varcoef_poisson(3, 20, 17, 9, seed=4)}[gen]()
+    n = sysm[0]
+    bh = oracle.Csr(*sysm).mult(P.uniform(n, 2))
+    out = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("MTJG_SPMV_STENCIL", flag)
+        A = M.CompRowMatrix(*sysm, ctx=ctx)
+        assert A.fp32_diagonals() == (diags if flag == "1" else 0)
+        b = M.DenseVector(bh, ctx)
+        x = M.DenseVector(n, ctx)
+        S = M.MixedCG(x, inner_rtol=1e-4)
+        S.setIterationMonitor(M.DefaultIterationMonitor(100, 1e-10))
+        S.solve(A, b, x)
+        Mp = M.FP32CGPreconditioner(n, inner_rtol=1e-4, ctx=ctx)
+        Mp.setMatrix(A)
+        z = M.DenseVector(n, ctx)
+        out[flag] = (x.getData(), S.inner_iterations(), Mp.apply(b, z).getData(), Mp.inner_iterations())
+        # values changed after the copy was filled reach it before the next product
+        A.setValues(sysm[4] * 2.0)
+        Mp.setMatrix(A)
+        out[flag] += (Mp.apply(b, z).getData(),)
+        for o in (Mp, S, z, x, b, A):
+            o.close()
+    a, c = out["1"], out["0"]
+    np.testing.assert_array_equal(a[0], c[0])
+    assert a[1] == c[1] and a[3] == c[3]
+    np.testing.assert_array_equal(a[2], c[2])
+    np.testing.assert_array_equal(a[4], c[4])
+
+
+def test_fp32_stencil_copy_not_for_wide_patterns(ctx):
+    # nine distinct diagonals: the FP32 products keep SELL-64
+    n = 200
+    rows = [sorted({i, (i + 1) % n, (i + 2) % n, (i + 3) % n, (i + 5) % n, (i + 7) % n, (i + 11) % n,
+                    (i + 13) % n, (i + 17) % n}) for i in range(n)]
+    rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    ci = np.concatenate(rows).astype(np.int32)
+    A = M.CompRowMatrix(n, n, rp, ci, np.ones(len(ci)), ctx=ctx)
+    assert A.format()[0] == "sell" and A.fp32_diagonals() == 0
+    A.close()
