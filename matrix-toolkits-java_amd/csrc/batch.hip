@@ -66,7 +66,11 @@ struct BSlot {
   const int32_t *sptr;     // SELL-64 copy: slice offsets, columns, values
   const int32_t *scol;
   const double *sv64;
-  const float *sv32;
+  const float *sv32;       // FP32 values, or null when the system has a stencil copy:
+  const float *st32;       // the FP32 values diagonal by diagonal (st_row),
+  const uint8_t *stmask;   // a mask byte per row,
+  int32_t st_m;            // the number of diagonals (0: no stencil copy)
+  int32_t st_off[ST_MAX_OFF];  // and their column - row offsets
   double *x, *r;
   float *rr, *d, *p, *q, *z;
   int64_t rid;             // request id (host bookkeeping)
@@ -180,10 +184,17 @@ __device__ __forceinline__ void b1_work(const BDev &d, int wi) {
   const int wave = threadIdx.x / SELL_C, lane = threadIdx.x % SELL_C;
   dd acc0{0.0, 0.0}, acc1{0.0, 0.0};
   if (phase == PH_INNER) {
-    // q = A32 p with p.q (SELL-64 rows, FP32 values and vectors)
-    for (int sl = w.tile * per_tile + wave; sl < sl_end; sl += BS / SELL_C)
-      sell_row<float, float, SPMV_DOT>(sl, lane, S.n, S.sptr, S.scol, S.sv32, S.p, S.q, nullptr, 1.0, 1.0, S.p,
-                                       acc0, acc1);
+    // q = A32 p with p.q (FP32 values and vectors): from the stencil copy
+    // when the system has one, else the SELL-64 rows; either adds a row's
+    // products in column order, so both give the same bits
+    if (S.st_m > 0) {
+      for (int sl = w.tile * per_tile + wave; sl < sl_end; sl += BS / SELL_C)
+        st_row<SPMV_DOT>(sl * SELL_C + lane, S.n, S.st_m, S.st_off, S.st32, S.stmask, S.p, S.q, S.p, acc0, acc1);
+    } else {
+      for (int sl = w.tile * per_tile + wave; sl < sl_end; sl += BS / SELL_C)
+        sell_row<float, float, SPMV_DOT>(sl, lane, S.n, S.sptr, S.scol, S.sv32, S.p, S.q, nullptr, 1.0, 1.0, S.p,
+                                         acc0, acc1);
+    }
   } else {
     // r = b + (-1) * (A64 x): MTJ's multAdd order, with r.r
     for (int sl = w.tile * per_tile + wave; sl < sl_end; sl += BS / SELL_C)
@@ -386,18 +397,47 @@ __global__ __launch_bounds__(BS) void k_batch_persist(BDev d, int iters, GridBar
   }
 }
 
+// Diagonals of a system's FP32 stencil copy (m = 0: none).
+struct StDiag {
+  int32_t m;
+  int32_t off[ST_MAX_OFF];
+};
+
 // Admission: the FP32 inverse diagonal and the row's lane of the SELL-64
-// copy (columns, FP64 and FP32 values; padding was preset to -1 / 0).
+// copy (columns, FP64 values, and FP32 values unless the system has a
+// stencil copy; padding was preset to -1 / 0), or the row's FP32 values on
+// its diagonals and its mask byte.
 __global__ void k_batch_setup(int32_t n, const int32_t *rp, const int32_t *ci, const double *v,
-                              const int32_t *sptr, float *inv, int32_t *scol, double *sv64, float *sv32) {
+                              const int32_t *sptr, float *inv, int32_t *scol, double *sv64, float *sv32,
+                              StDiag sd, float *st32, uint8_t *stmask) {
   for (int32_t i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
     const int32_t off = sptr[i / SELL_C] + i % SELL_C;
     double dg = 0;
+    float out[ST_MAX_OFF];
+    uint32_t mk = 0;
+#pragma unroll
+    for (int k = 0; k < ST_MAX_OFF; ++k) out[k] = 0.0f;
     for (int32_t k = 0, j = rp[i]; j < rp[i + 1]; ++k, ++j) {
       if (ci[j] == i) dg = v[j];
       scol[off + k * SELL_C] = ci[j];
       sv64[off + k * SELL_C] = v[j];
-      sv32[off + k * SELL_C] = (float)v[j];
+      if (sd.m == 0) {
+        sv32[off + k * SELL_C] = (float)v[j];
+      } else {
+        const int32_t dd_ = ci[j] - i;
+#pragma unroll
+        for (int t = 0; t < ST_MAX_OFF; ++t)
+          if (t < sd.m && sd.off[t] == dd_) {
+            out[t] = (float)v[j];
+            mk |= 1u << t;
+          }
+      }
+    }
+    if (sd.m > 0) {
+#pragma unroll
+      for (int t = 0; t < ST_MAX_OFF; ++t)
+        if (t < sd.m) st32[(size_t)t * n + i] = out[t];
+      stmask[i] = (uint8_t)mk;
     }
     inv[i] = (float)(1.0 / dg);
   }
@@ -736,6 +776,7 @@ struct Request {
   int32_t *rp, *ci;
   double *val, *b;
   std::vector<int32_t> sptr;    // SELL-64 slice offsets
+  StDiag st{};                  // diagonals of the FP32 stencil copy (st.m = 0: none)
   int32_t nb = 0;               // tiles
   double rtol;
   int32_t max_outer, max_inner;
@@ -794,6 +835,19 @@ int host_workers() {
   return (int)std::max(0u, std::min(hw, 8u) - (hw > 0 ? 1u : 0u));
 }
 
+// Adds the column - row offset d to the ascending set off[0..m) of at most
+// ST_MAX_OFF; false when it does not fit.
+bool diag_insert(int32_t *off, int32_t &m, int32_t d) {
+  int k = 0;
+  while (k < m && off[k] < d) ++k;
+  if (k < m && off[k] == d) return true;
+  if (m == ST_MAX_OFF) return false;
+  for (int t = m; t > k; --t) off[t] = off[t - 1];
+  off[k] = d;
+  ++m;
+  return true;
+}
+
 // Checks a system's CSR structure and copies it into R (whose block is
 // sized for it), in parallel parts of rows on the workers.  Each row is
 // checked in the order a serial pass would: its pointers (0 <= rp[i] <=
@@ -812,9 +866,16 @@ void check_and_copy(Workers &W, Request &R, const int32_t *rp, const int32_t *ci
   const int parts = nnz < (1 << 16) ? 1 : std::min(4 * W.size(), std::max(1, n / 256));
   const int32_t per = (int32_t)(((int64_t)n + parts - 1) / parts);
   std::vector<Fail> fails((size_t)parts);
+  std::vector<StDiag> diags((size_t)parts);
   W.run(parts, [&](int p) {
     const int32_t r0 = (int32_t)std::min<int64_t>(n, (int64_t)p * per);
     const int32_t r1 = (int32_t)std::min<int64_t>(n, (int64_t)r0 + per);
+    // the part's diagonals (column - row offsets) for the FP32 stencil
+    // copy: a row whose offsets repeat the previous row's adds nothing
+    StDiag &sd = diags[(size_t)p];
+    sd.m = 0;
+    int32_t last[ST_MAX_OFF];
+    int32_t lastn = -1;
     for (int32_t i = r0; i < r1; ++i) {
       const int32_t a = rp[i], e = rp[i + 1];
       if (a < 0 || e < a || e > nnz) {
@@ -840,6 +901,21 @@ void check_and_copy(Workers &W, Request &R, const int32_t *rp, const int32_t *ci
         fails[p] = Fail{i, NO_DIAG};
         return;
       }
+      if (sd.m >= 0) {
+        bool same = e - a == lastn;
+        for (int32_t k = a; same && k < e; ++k) same = ci[k] - i == last[k - a];
+        if (!same) {
+          if (e - a > ST_MAX_OFF) {
+            sd.m = -1;
+          } else {
+            for (int32_t k = a; k < e && sd.m >= 0; ++k) {
+              last[k - a] = ci[k] - i;
+              if (!diag_insert(sd.off, sd.m, ci[k] - i)) sd.m = -1;
+            }
+            lastn = e - a;
+          }
+        }
+      }
     }
     // the part's rows hold the entries rp[r0] .. rp[r1] (checked above)
     if (r1 > r0) {
@@ -859,12 +935,30 @@ void check_and_copy(Workers &W, Request &R, const int32_t *rp, const int32_t *ci
     fail(MTJG_E_MISSING_DIAGONAL, "missing diagonal in row " + row);
   }
   R.rp[n] = rp[n];
+  // the union of the parts' diagonals (MTJG_SPMV_STENCIL=0 at submission
+  // keeps the SELL-64 FP32 values)
+  R.st.m = 0;
+  const char *env = getenv("MTJG_SPMV_STENCIL");
+  bool fits = !(env && atoi(env) == 0);
+  for (size_t p = 0; fits && p < diags.size(); ++p) {
+    fits = diags[p].m >= 0;
+    for (int k = 0; fits && k < diags[p].m; ++k) fits = diag_insert(R.st.off, R.st.m, diags[p].off[k]);
+  }
+  if (!fits) R.st.m = 0;
+}
+
+// Bytes of the FP32 values: the stencil copy (values per diagonal and a mask
+// byte per row) or the SELL-64 FP32 values.
+size_t req_fp32_bytes(const Request &R) {
+  const size_t n = (size_t)R.n;
+  if (R.st.m > 0) return al256(4 * (size_t)R.st.m * n) + al256(n);
+  return al256(4 * (size_t)R.sptr.back());
 }
 
 size_t req_bytes(const Request &R) {
   const size_t n = (size_t)R.n, nnz = (size_t)R.nnz, sell = (size_t)R.sptr.back();
   return al256(4 * (n + 1)) + al256(4 * nnz) + al256(8 * nnz) + al256(4 * n) + 3 * al256(8 * n) +
-         5 * al256(4 * n) + al256(4 * R.sptr.size()) + al256(4 * sell) + al256(8 * sell) + al256(4 * sell);
+         5 * al256(4 * n) + al256(4 * R.sptr.size()) + al256(4 * sell) + al256(8 * sell) + req_fp32_bytes(R);
 }
 
 // Slice offsets of the system's SELL-64 copy (each slice as wide as its
@@ -1146,7 +1240,10 @@ int admit(mtjg_batch B, std::vector<int> &new_slots) {
     int32_t *sptr = (int32_t *)take(4 * nsp);
     int32_t *scol = (int32_t *)take(4 * sell);
     double *sv64 = (double *)take(8 * sell);
-    float *sv32 = (float *)take(4 * sell);
+    const bool stencil = R.st.m > 0;
+    float *sv32 = stencil ? nullptr : (float *)take(4 * sell);
+    float *st32 = stencil ? (float *)take(4 * (size_t)R.st.m * n) : nullptr;
+    uint8_t *stmask = stencil ? (uint8_t *)take(n) : nullptr;
     if ((size_t)(p - base) != al256(bytes)) fail(MTJG_E_DEVICE, "batch: system layout does not match its size");
     // host side: the five uploaded arrays, packed into the pinned staging
     const size_t hb = al256(4 * (n + 1)) + al256(4 * nnz) + al256(8 * nnz) + al256(8 * n) + al256(4 * nsp);
@@ -1171,10 +1268,10 @@ int admit(mtjg_batch B, std::vector<int> &new_slots) {
     if (sell) {
       HIPCHK(hipMemsetAsync(scol, 0xFF, 4 * sell, cs));  // padding column -1
       HIPCHK(hipMemsetAsync(sv64, 0, 8 * sell, cs));
-      HIPCHK(hipMemsetAsync(sv32, 0, 4 * sell, cs));
+      if (sv32) HIPCHK(hipMemsetAsync(sv32, 0, 4 * sell, cs));
     }
     k_zero_f64<<<grid_for(R.n), BS, 0, cs>>>(R.n, x);
-    k_batch_setup<<<grid_for(R.n), BS, 0, cs>>>(R.n, rp, ci, v64, sptr, inv, scol, sv64, sv32);
+    k_batch_setup<<<grid_for(R.n), BS, 0, cs>>>(R.n, rp, ci, v64, sptr, inv, scol, sv64, sv32, R.st, st32, stmask);
     HIPCHK(hipGetLastError());
     BSlot S{};
     S.rp = rp;
@@ -1184,6 +1281,10 @@ int admit(mtjg_batch B, std::vector<int> &new_slots) {
     S.scol = scol;
     S.sv64 = sv64;
     S.sv32 = sv32;
+    S.st32 = st32;
+    S.stmask = stmask;
+    S.st_m = R.st.m;
+    for (int k = 0; k < ST_MAX_OFF; ++k) S.st_off[k] = R.st.off[k];
     S.inv = inv;
     S.b = b;
     S.x = x;

@@ -313,31 +313,9 @@ __global__ __launch_bounds__(BS) void k_sell_fin(int np, const double *part, KSt
 // products are added in diagonal order, which is ascending column order,
 // the order of the SELL-64 row and of MTJ's loop: every sum is bitwise the
 // SELL-64 FP32 product's.
-constexpr int ST_MAX_OFF = 8;
 struct StOff {
   int32_t off[ST_MAX_OFF];
 };
-
-template <int MODE>
-__device__ __forceinline__ void st_row(int row, int n, int m, const StOff &o, const float *__restrict__ val,
-                                       const uint8_t *__restrict__ mask, const float *__restrict__ x, float *y,
-                                       const float *w, dd &acc0, dd &acc1) {
-  if (row >= n) return;
-  const uint32_t mk = mask[row];
-  float v[ST_MAX_OFF], xv[ST_MAX_OFF];
-#pragma unroll
-  for (int k = 0; k < ST_MAX_OFF; ++k) {
-    if (k < m && ((mk >> k) & 1u)) {
-      v[k] = __builtin_nontemporal_load(&val[(size_t)k * n + row]);
-      xv[k] = x[row + o.off[k]];
-    }
-  }
-  float s = 0.0f;
-#pragma unroll
-  for (int k = 0; k < ST_MAX_OFF; ++k)
-    if (k < m && ((mk >> k) & 1u)) s += v[k] * xv[k];
-  spmv_store<float, MODE>(row, s, y, nullptr, 1.0, 1.0, w, acc0, acc1);
-}
 
 // Split fused product (k_sell_part's protocol): a workgroup per 256 rows
 // under the XCD run mapping, compensated partials stored without a ticket.
@@ -347,7 +325,7 @@ __global__ __launch_bounds__(BS) void k_st_part(int n, int m, StOff o, const flo
   if (st->status != ST_RUNNING) return;
   const int blk = xcd_run_block(blockIdx.x, gridDim.x, SELL_XCD_RUN);
   dd v[2] = {{0.0, 0.0}, {0.0, 0.0}};
-  st_row<SPMV_DOT>(blk * BS + (int)threadIdx.x, n, m, o, val, mask, x, y, w, v[0], v[1]);
+  st_row<SPMV_DOT>(blk * BS + (int)threadIdx.x, n, m, o.off, val, mask, x, y, w, v[0], v[1]);
   block_sum_dd<2>(v);
   if (threadIdx.x == 0) {
     part[0 * (size_t)cap + blockIdx.x] = v[0].hi;
@@ -364,7 +342,7 @@ __global__ __launch_bounds__(BS) void k_st_dot(int n, int m, StOff o, const floa
   if (st->status != ST_RUNNING) return;
   dd acc0{0.0, 0.0}, acc1{0.0, 0.0};
   for (int row = blockIdx.x * BS + (int)threadIdx.x; row < n; row += gridDim.x * BS)
-    st_row<SPMV_DOT>(row, n, m, o, val, mask, x, y, w, acc0, acc1);
+    st_row<SPMV_DOT>(row, n, m, o.off, val, mask, x, y, w, acc0, acc1);
   dd v[2] = {acc0, acc1};
   double tot[2];
   if (grid_sum_dd<2>(v, red, tot) && threadIdx.x == 0) spmv_finish(st, fin, tot[0], tot[1]);

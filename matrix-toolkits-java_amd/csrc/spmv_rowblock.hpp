@@ -320,6 +320,37 @@ __device__ __forceinline__ void spmv_rowblock(int b, XT *prod, const int2 *__res
   }
 }
 
+// ---- FP32 stencil copy --------------------------------------------------------
+
+// Row `row` of y = A x (and the mode's epilogue) from an FP32 stencil copy:
+// the values of an n-row matrix whose entries all lie on the m <= ST_MAX_OFF
+// diagonals off[0..m) (ascending), diagonal-major (row i's value on
+// diagonal k at val[k * n + i]), and one mask byte per row whose bit k says
+// that row i has an entry on diagonal k (column i + off[k]).  The products
+// are added in diagonal order, which is ascending column order: the order
+// sell_row adds them in, so the sums are bitwise the SELL-64 FP32 ones.
+constexpr int ST_MAX_OFF = 8;
+template <int MODE>
+__device__ __forceinline__ void st_row(int row, int n, int m, const int32_t *off, const float *__restrict__ val,
+                                       const uint8_t *__restrict__ mask, const float *__restrict__ x, float *y,
+                                       const float *w, dd &acc0, dd &acc1) {
+  if (row >= n) return;
+  const uint32_t mk = mask[row];
+  float v[ST_MAX_OFF], xv[ST_MAX_OFF];
+#pragma unroll
+  for (int k = 0; k < ST_MAX_OFF; ++k) {
+    if (k < m && ((mk >> k) & 1u)) {
+      v[k] = __builtin_nontemporal_load(&val[(size_t)k * n + row]);
+      xv[k] = x[row + off[k]];
+    }
+  }
+  float s = 0.0f;
+#pragma unroll
+  for (int k = 0; k < ST_MAX_OFF; ++k)
+    if (k < m && ((mk >> k) & 1u)) s += v[k] * xv[k];
+  spmv_store<float, MODE>(row, s, y, nullptr, 1.0, 1.0, w, acc0, acc1);
+}
+
 // ---- SELL-64 -----------------------------------------------------------------
 
 constexpr int SELL_C = 64;       // rows per slice: one wavefront, a lane per row

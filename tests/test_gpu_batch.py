@@ -87,16 +87,20 @@ def test_admission_waits_for_memory(ctx):
 
 def S_bytes(sysm):
     """Arena bytes of one admitted system: CSR, FP32 inverse diagonal, FP64
-    b x r, FP32 rr d p q z, and the SELL-64 copy (slice offsets, columns,
-    FP64 and FP32 values, each slice as wide as its longest row)."""
+    b x r, FP32 rr d p q z, the SELL-64 copy (slice offsets, columns, FP64
+    values, each slice as wide as its longest row) and the FP32 values: the
+    stencil copy (a value per row and diagonal, a mask byte per row) when the
+    entries lie on at most 8 diagonals, else the SELL-64 FP32 values."""
     n, rp, ci, va, b = sysm
     nnz = len(ci)
     al = lambda v: (v + 255) // 256 * 256
     lens = np.diff(rp)
     nslice = (n + 63) // 64
     sell = sum(int(lens[64 * s:64 * s + 64].max()) * 64 for s in range(nslice))
+    diags = len(np.unique(ci - np.repeat(np.arange(n), lens)))
+    fp32 = al(4 * diags * n) + al(n) if diags <= 8 else al(4 * sell)
     return (al(4 * (n + 1)) + al(4 * nnz) + al(8 * nnz) + al(4 * n) + 3 * al(8 * n) + 5 * al(4 * n)
-            + al(4 * (nslice + 1)) + al(4 * sell) + al(8 * sell) + al(4 * sell))
+            + al(4 * (nslice + 1)) + al(4 * sell) + al(8 * sell) + fp32)
 
 
 def test_nan_system_is_isolated(ctx):
@@ -245,3 +249,18 @@ def test_autotune_picks_a_measured_shape(ctx):
     for k in range(len(reqs)):
         assert res[k]["code"] == 0
         assert S.true_residual(systems[k], res[k]["x"]) < 1e-8
+
+
+def test_stencil_copy_same_bits_as_sell(ctx, monkeypatch):
+    """The engine's FP32 inner products read each system's stencil copy
+    (values per diagonal, the stream's 2-D and 3-D Poisson systems have 5 and
+    7); with MTJG_SPMV_STENCIL=0 at submission they read the SELL-64 FP32
+    values instead.  Both add a row's products in column order: the
+    solutions and iteration counts are bitwise equal."""
+    reqs, systems = _systems(6, seed=17, n_max=30000)
+    with_st = _run(ctx, systems, max_slots=16, max_workgroups=2048, arena_bytes=1 << 29)
+    monkeypatch.setenv("MTJG_SPMV_STENCIL", "0")
+    without = _run(ctx, systems, max_slots=16, max_workgroups=2048, arena_bytes=1 << 29)
+    for k in range(len(systems)):
+        np.testing.assert_array_equal(with_st[k][0], without[k][0])
+        assert with_st[k][1:4] == without[k][1:4]
