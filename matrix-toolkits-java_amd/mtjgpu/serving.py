@@ -131,16 +131,20 @@ def _stats(lat, t_first, t_last, count):
 
 def batch_bytes(system, outer, inner):
     """Algorithmic HBM bytes the batch engine moves for one solved system:
-    per inner FP32 CG iteration 8 nnz + 52 n (q = A32 p over the FP32 values
-    and columns, p gathered once, q written; d p rr q invdiag read and d rr z
-    written; p z read and p written), per refinement 12 nnz + 76 n (the FP64
-    residual r = b - A x: values, columns, x and b read, r written; the
-    restart rr = r/||r||, d = 0, z = D^-1 rr; p = z; x += ||r|| d), plus the
-    final residual check."""
+    per inner FP32 CG iteration the product q = A32 p (4 m n + n on the FP32
+    stencil copy of a system whose entries lie on m <= 8 diagonals: a value
+    per row and diagonal and a mask byte per row; otherwise 8 nnz, the
+    SELL-64 FP32 values and columns) plus 52 n (p gathered once, q written;
+    d p rr q invdiag read and d rr z written; p z read and p written), per
+    refinement 12 nnz + 76 n (the FP64 residual r = b - A x: values, columns,
+    x and b read, r written; the restart rr = r/||r||, d = 0, z = D^-1 rr;
+    p = z; x += ||r|| d), plus the final residual check."""
     n, rp, ci, va, b = system
     nnz = len(ci)
+    diags = np.unique(np.asarray(ci, dtype=np.int64) - np.repeat(np.arange(n), np.diff(rp))).size
+    prod32 = 4 * diags * n + n if diags <= 8 else 8 * nnz
     resid = 12 * nnz + 24 * n
-    return inner * (8 * nnz + 52 * n) + outer * (resid + 52 * n) + resid
+    return inner * (prod32 + 52 * n) + outer * (resid + 52 * n) + resid
 
 
 def serve_batched(ctx, reqs, systems, rtol=1e-8, **kw):
