@@ -264,15 +264,18 @@ def test_fp32_cg_preconditioner_errors(ctx):
         M.FP32CGPreconditioner(3, inner_rtol=0.0, ctx=ctx)
 
 
-@pytest.mark.parametrize("gen,diags", [("varcoef3d_24", 7), ("poisson2d_96", 5), ("poisson3d_20x17x9", 7)])
+@pytest.mark.parametrize("gen,diags", [("varcoef3d_24", 7), ("poisson2d_96", 5), ("poisson3d_20x17x9", 7),
+                                       ("varcoef3d_84", 7)])
 def test_fp32_stencil_copy_bitwise_sell(ctx, monkeypatch, gen, diags):
     """The FP32 stencil copy (values only, diagonal by diagonal) sums each row
     in the SELL-64 FP32 product's order: the mixed-precision solve and the
     FP32-CG preconditioner give bitwise the same results with the copy as
-    without it (MTJG_SPMV_STENCIL=0)."""
+    without it (MTJG_SPMV_STENCIL=0).  84^3 (592,704 rows) runs the split
+    product (more than 2048 workgroups: partials, then k_sell_fin)."""
     sysm = {"varcoef3d_24": lambda: P.varcoef_poisson(3, 24, seed=3),
             "poisson2d_96": lambda: P.poisson2d(96),
-            "poisson3d_20x17x9": lambda: P.varcoef_poisson(3, 20, 17, 9, seed=4)}[gen]()
+            "poisson3d_20x17x9": lambda: P.varcoef_poisson(3, 20, 17, 9, seed=4),
+            "varcoef3d_84": lambda: P.varcoef_poisson(3, 84, seed=6)}[gen]()
     n = sysm[0]
     bh = oracle.Csr(*sysm).mult(P.uniform(n, 2))
     out = {}
