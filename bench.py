@@ -378,6 +378,7 @@ PRECOND_WORKLOADS = {
     "p2d_256": ("2D 5-point Poisson 256^2", lambda P: P.poisson2d(256)),
     "p3d_64": ("3D 7-point Poisson 64^3", lambda P: P.poisson3d(64)),
     "cd_64": ("3D upwind convection-diffusion 64^3 (Pe 10)", lambda P: P.convdiff3d(64, peclet=10.0, seed=1)),
+    "cd_128": ("3D upwind convection-diffusion 128^3 (Pe 10)", lambda P: P.convdiff3d(128, peclet=10.0, seed=1)),
 }
 
 

@@ -494,6 +494,13 @@ class ILU(_Preconditioner):
         _ck(self._lib.mtjg_prec_ilu_create(self.ctx.handle, LU.handle, C.byref(h)))
         self.handle = h
 
+    def grid(self):
+        """(nx, ny, nz) of the grid sweeps apply runs, or None when the
+        pattern is not a natural-order grid stencil (level schedule)."""
+        d = (C.c_int32 * 3)()
+        _ck(self._lib.mtjg_prec_ilu_grid(self.handle, d))
+        return tuple(d) if d[0] else None
+
 
 class ICC(_Preconditioner):
     """S/ICC.java: incomplete Cholesky without fill-in, in the pattern of the

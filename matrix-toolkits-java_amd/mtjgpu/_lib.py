@@ -94,6 +94,7 @@ PROTOTYPES = [
     ("mtjg_csr_row_histogram", C.c_int, [P, PI64, C.c_int]),
     ("mtjg_prec_diagonal_create", C.c_int, [P, I32, PP]),
     ("mtjg_prec_ilu_create", C.c_int, [P, P, PP]),
+    ("mtjg_prec_ilu_grid", C.c_int, [P, PI32]),
     ("mtjg_prec_cg32_create", C.c_int, [P, I32, D, I32, PP]),
     ("mtjg_prec_inner_iterations", C.c_int, [P, C.POINTER(C.c_int64)]),
     ("mtjg_prec_icc_create", C.c_int, [P, P, PP]),

@@ -57,6 +57,7 @@ SHAPES = [(64, 1), (700, 3), (4096, 8), (None, None)]
 @pytest.mark.parametrize("name", sorted(SYSTEMS))
 def test_tiled_ilu_and_icc_bitwise(ctx, monkeypatch, name, shape):
     monkeypatch.setenv("MTJG_SWEEP_TILES", "1")
+    monkeypatch.setenv("MTJG_ILU_GRID", "0")  # stencil patterns would take grid sweeps
     if shape[0] is not None:
         monkeypatch.setenv("MTJG_TILE_ROWS", str(shape[0]))
         monkeypatch.setenv("MTJG_TILE_LEVELS", str(shape[1]))
@@ -97,6 +98,7 @@ def test_tiled_bicgstab_matches_untiled(ctx, monkeypatch):
     n = sysm[0]
     bh = P.uniform(n, 1)
     out = []
+    monkeypatch.setenv("MTJG_ILU_GRID", "0")
     for mode in ("0", "1"):
         monkeypatch.setenv("MTJG_SWEEP_TILES", mode)
         A = M.CompRowMatrix(*sysm, ctx=ctx)
