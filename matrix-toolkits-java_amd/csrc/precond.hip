@@ -158,6 +158,12 @@ struct GridPlan {
 enum { GRID_B = 7, GRID_Y = 8, GRID_X = 9, GRID_MASK = 10 };
 __host__ __device__ inline int64_t grid_off(const GridPlan &g, int k) { return (int64_t)k * 8 * g.nslot; }
 __host__ __device__ inline int64_t grid_bytes(const GridPlan &g) { return 81 * g.nslot; }
+// The transposed copy (U' and L' solves: ILU's transApply, ICC's R' solve),
+// a second allocation: U(k-nx ny, k), U(k-nx, k), U(k-1, k), the diagonal,
+// L(k+nx ny, k), L(k+nx, k), L(k+1, k) (doubles, nslot each), then the
+// GM_T* mask bytes.
+enum { GRIDT_U = 0, GRIDT_D = 3, GRIDT_L = 4, GRIDT_MASK = 7 };
+__host__ __device__ inline int64_t gridt_bytes(const GridPlan &g) { return 57 * g.nslot; }
 
 struct mtjg_prec_s {
   mtjg_ctx ctx = nullptr;
@@ -195,6 +201,8 @@ struct mtjg_prec_s {
   char *g_buf = nullptr;
   uint32_t *g_ctr = nullptr;
   uint64_t grid_vals_version = ~0ull;
+  char *g_tbuf = nullptr;  // transposed copy (gridt_bytes), built on first use
+  uint64_t grid_t_version = ~0ull;
   // SSOR: relaxations, reverse sweep flag, half iterate (values in lu)
   double omega_f = 1, omega_r = 1;
   bool reverse = true;
@@ -1021,6 +1029,9 @@ constexpr int GRID_DEPTH = 4;                  // steps whose operands are in fl
 constexpr long long GRID_POLL_BUDGET = 1LL << 22;
 // mask bits: present L terms (z-1, y-1, x-1), present U terms (x+1, y+1, z+1)
 constexpr uint32_t GM_ZL = 1, GM_YL = 2, GM_XL = 4, GM_XU = 8, GM_YU = 16, GM_ZU = 32;
+// mask bits of the transposed copy: present U(k-nx ny, k), U(k-nx, k),
+// U(k-1, k) (U' solve) and L(k+nx ny, k), L(k+nx, k), L(k+1, k) (L' solve)
+constexpr uint32_t GM_TUZ = 1, GM_TUY = 2, GM_TUX = 4, GM_TLZ = 8, GM_TLY = 16, GM_TLX = 32;
 
 // Decodes the slot of lane l in plane-row q = (z YB + yb) TW + t.
 struct GridSlot {
@@ -1076,6 +1087,47 @@ __global__ __launch_bounds__(BS) void k_grid_fill(const GridPlan g, const int32_
   for (int k = 0; k < 7; ++k) val[k * g.nslot + s] = v[k];
 }
 
+// Value of entry (row, col) of the factor (CSR, at most seven entries a
+// row), and whether the pattern holds it.
+__device__ __forceinline__ bool grid_entry(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                           const double *__restrict__ lu, int64_t row, int64_t col, double &v) {
+  for (int32_t j = rp[row]; j < rp[row + 1]; ++j)
+    if (ci[j] == col) {
+      v = lu[j];
+      return true;
+    }
+  return false;
+}
+
+// The transposed copy: per slot of row k the U entries of column k above the
+// diagonal (rows k - nx ny, k - nx, k - 1), the diagonal and the L entries of
+// column k below it (rows k + nx ny, k + nx, k + 1), with their mask bits.
+__global__ __launch_bounds__(BS) void k_grid_fill_t(const GridPlan g, const int32_t *__restrict__ rp,
+                                                    const int32_t *__restrict__ ci, const double *__restrict__ lu,
+                                                    char *tbuf) {
+  const int64_t q = (int64_t)blockIdx.x * (BS / 64) + (threadIdx.x >> 6);
+  const int32_t l = threadIdx.x & 63;
+  if (q >= (int64_t)g.nz * g.YB * g.TW) return;
+  const GridSlot c = grid_slot(g, q, l);
+  const int64_t s = q * 64 + l;
+  uint32_t m = 0;
+  double v[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (c.valid) {
+    const int64_t nxny = (int64_t)g.nx * g.ny;
+    const int64_t k = c.x + (int64_t)g.nx * (c.y + (int64_t)g.ny * c.z);
+    if (c.z > 0 && grid_entry(rp, ci, lu, k - nxny, k, v[0])) m |= GM_TUZ;
+    if (c.y > 0 && grid_entry(rp, ci, lu, k - g.nx, k, v[1])) m |= GM_TUY;
+    if (c.x > 0 && grid_entry(rp, ci, lu, k - 1, k, v[2])) m |= GM_TUX;
+    grid_entry(rp, ci, lu, k, k, v[3]);
+    if (c.z < g.nz - 1 && grid_entry(rp, ci, lu, k + nxny, k, v[4])) m |= GM_TLZ;
+    if (c.y < g.ny - 1 && grid_entry(rp, ci, lu, k + g.nx, k, v[5])) m |= GM_TLY;
+    if (c.x < g.nx - 1 && grid_entry(rp, ci, lu, k + 1, k, v[6])) m |= GM_TLX;
+  }
+  ((uint8_t *)(tbuf + 8 * GRIDT_MASK * g.nslot))[s] = (uint8_t)m;
+#pragma unroll
+  for (int j = 0; j < 7; ++j) ((double *)tbuf)[j * g.nslot + s] = v[j];
+}
+
 // b into the skewed slots, and TRSV_PENDING into the imported planes of the
 // two sweeps: the top plane of every group but the last in y (lower sweep),
 // the bottom plane of every group but the first in x (upper sweep).
@@ -1125,9 +1177,18 @@ __device__ __forceinline__ double grid_ld(__amdgpu_buffer_rsrc_t r, uint32_t vo,
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
 }
 
-// DIR = +1: L.solve (unit lower, y from b); DIR = -1: U.solve (x from y).
-// blockDim = 64 P YB, grid = G; buf is the grid allocation (grid_off), rhs
-// and out the byte offsets of the right-hand side and the result in it.
+// One triangular solve of the grid copy, in one of the four forms of the
+// level-scheduled solves (TRSV_*): ROW_UNIT L.solve and ROW_DIV U.solve
+// (S/UnitLowerCompRowMatrix.java:56-64, S/UpperCompRowMatrix.java:56-64:
+// s = sum of a_ij x_j in ascending column, then b - s, divided by the
+// diagonal for U), COL_DIV U'.solve and COL_UNIT_REV L'.solve (their
+// transSolve: s = b, minus a_ji x_j in the column algorithm's order, divided
+// by the diagonal for U').  ROW_UNIT and COL_DIV walk the anti-diagonals up
+// (dependencies at x-1, y-1, z-1), the other two down.  vbuf holds the three
+// values of each row in summation order at v0, v0 + 8 nslot, v0 + 16 nslot,
+// the diagonal at vd and the mask bytes at vm; xbuf holds the right-hand
+// side at rhs and receives the result at out (byte offsets).
+// blockDim = 64 P YB, grid = G.
 // Wavefront w = zl YB + yb; its lanes' ring index is the thread index.  The
 // skew column t of a step is the same for all lanes of a wavefront, so every
 // load and store is a buffer access at a wavefront-uniform (scalar) offset
@@ -1140,9 +1201,19 @@ __device__ __forceinline__ double grid_ld(__amdgpu_buffer_rsrc_t r, uint32_t vo,
 // lane produced while active.  The step has no divergent branch, whose
 // exec-mask bookkeeping costs scalar instructions that the sixteen
 // wavefronts of a compute unit issue through one scalar unit.
-template <int DIR>
-__global__ __launch_bounds__(GRID_WAVES * 64) void k_grid_sweep(const GridPlan g, char *buf, uint32_t rhs,
-                                                                uint32_t out, uint32_t *ctr, int32_t *err) {
+template <int FORM>
+__global__ __launch_bounds__(GRID_WAVES * 64) void k_grid_sweep(const GridPlan g, const char *vbuf, uint32_t vbytes,
+                                                                uint32_t v0, uint32_t vd, uint32_t vm, char *xbuf,
+                                                                uint32_t rhs, uint32_t out, uint32_t *ctr,
+                                                                int32_t *err) {
+  constexpr int DIR = (FORM == TRSV_ROW_UNIT || FORM == TRSV_COL_DIV) ? 1 : -1;
+  constexpr bool ROW = FORM == TRSV_ROW_UNIT || FORM == TRSV_ROW_DIV;
+  constexpr bool DIV = FORM == TRSV_ROW_DIV || FORM == TRSV_COL_DIV;
+  constexpr bool XFIRST = FORM == TRSV_ROW_DIV;  // terms x, y, z (else z, y, x)
+  // mask bits of the z-, y- and x-terms
+  constexpr uint32_t mz = FORM == TRSV_ROW_DIV ? GM_ZU : FORM == TRSV_COL_UNIT_REV ? GM_TLZ : FORM == TRSV_COL_DIV ? GM_TUZ : GM_ZL;
+  constexpr uint32_t my = FORM == TRSV_ROW_DIV ? GM_YU : FORM == TRSV_COL_UNIT_REV ? GM_TLY : FORM == TRSV_COL_DIV ? GM_TUY : GM_YL;
+  constexpr uint32_t mx = FORM == TRSV_ROW_DIV ? GM_XU : FORM == TRSV_COL_UNIT_REV ? GM_TLX : FORM == TRSV_COL_DIV ? GM_TUX : GM_XL;
   __shared__ double ring[2][GRID_WAVES * 64];
   __shared__ int32_t s_ticket;
   const int32_t tid = threadIdx.x, l = tid & 63;
@@ -1163,13 +1234,14 @@ __global__ __launch_bounds__(GRID_WAVES * 64) void k_grid_sweep(const GridPlan g
   const bool exporter = DIR > 0 ? zl == g.P - 1 : zl == 0;
   const bool imports = importer && (DIR > 0 ? gi > 0 : gi < g.G - 1);  // the imported plane exists
   const uint32_t ioff = (uint32_t)(DIR > 0 ? -g.slab : g.slab) * 8u;  // the imported plane (bytes, mod 2^32)
-  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(buf, (short)0, (int)(uint32_t)grid_bytes(g),
+  const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(vbuf), (short)0,
+                                                                      (int)vbytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(xbuf, (short)0, (int)(uint32_t)grid_bytes(g),
                                                                       0x00020000);
   // byte offsets of the operand arrays (scalar) and of the lane
   const uint32_t ns8 = (uint32_t)g.nslot * 8u, l8 = (uint32_t)l * 8u;
-  const uint32_t k0 = DIR > 0 ? 0 : 3;
-  const uint32_t a0 = k0 * ns8, a1 = a0 + ns8, a2 = a1 + ns8, a3 = 6 * ns8, am = (uint32_t)grid_off(g, GRID_MASK);
-  double *const outp = (double *)(buf + out);
+  const uint32_t a0 = v0, a1 = a0 + ns8, a2 = a1 + ns8, a3 = vd, am = vm;
+  double *const outp = (double *)(xbuf + out);
   // ring indices of the y- and z-neighbours (clamped: used only when present)
   const int32_t iy = DIR > 0 ? max(tid - 1, 0) : min(tid + 1, GRID_WAVES * 64 - 1);
   const int32_t iz = DIR > 0 ? max(tid - 64 * g.YB, 0) : min(tid + 64 * g.YB, GRID_WAVES * 64 - 1);
@@ -1222,14 +1294,14 @@ __global__ __launch_bounds__(GRID_WAVES * 64) void k_grid_sweep(const GridPlan g
   auto lane_off = [&](int32_t p) -> uint32_t { return (p >= pa && p <= pb) ? l8 : 0x80000000u; };
   auto load_op = [&](int32_t p, GridOp &o) {
     const uint32_t so = step_off(p), vo = lane_off(p);
-    o.m = __builtin_amdgcn_raw_buffer_load_b8(rs, vo >> 3, am + (so >> 3), 0);
-    o.v0 = grid_ld(rs, vo, a0 + so);
-    o.v1 = grid_ld(rs, vo, a1 + so);
-    o.v2 = grid_ld(rs, vo, a2 + so);
-    if (DIR < 0) o.v3 = grid_ld(rs, vo, a3 + so);
-    o.b = grid_ld(rs, vo, rhs + so);
+    o.m = __builtin_amdgcn_raw_buffer_load_b8(rv, vo >> 3, am + (so >> 3), 0);
+    o.v0 = grid_ld(rv, vo, a0 + so);
+    o.v1 = grid_ld(rv, vo, a1 + so);
+    o.v2 = grid_ld(rv, vo, a2 + so);
+    if (DIV) o.v3 = grid_ld(rv, vo, a3 + so);
+    o.b = grid_ld(rx, vo, rhs + so);
     if (imports)
-      o.imp = __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(rs, vo, out + so + ioff, GRID_SC1));
+      o.imp = __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(rx, vo, out + so + ioff, GRID_SC1));
   };
 
   GridOp op[GRID_DEPTH];
@@ -1246,7 +1318,6 @@ __global__ __launch_bounds__(GRID_WAVES * 64) void k_grid_sweep(const GridPlan g
       GridOp &o = op[u];
       const uint32_t so = step_off(p);
       const double *rg = ring[(p + 1) & 1];  // the previous step's values
-      const uint32_t mz = DIR > 0 ? GM_ZL : GM_ZU, my = DIR > 0 ? GM_YL : GM_YU, mx = DIR > 0 ? GM_XL : GM_XU;
       double zn = rg[iz];
       const double yn = rg[iy];
       if (importer) {
@@ -1265,27 +1336,31 @@ __global__ __launch_bounds__(GRID_WAVES * 64) void k_grid_sweep(const GridPlan g
         }
         zn = __longlong_as_double((long long)bits);
       }
-      // absent terms add nothing: a present term adds v * neighbour; an
-      // absent one (v = 0) adds 0 * 0, which leaves every sum unchanged
-      // (sums start at +0.0 and never become -0.0)
+      // absent terms change nothing: a present term adds (subtracts)
+      // v * neighbour; an absent one (v = 0) adds 0 * 0 = +0, which leaves
+      // every value unchanged (row sums start at +0.0 and never become
+      // -0.0; -0.0 - +0.0 is -0.0)
       const double zx = (o.m & mz) ? zn : 0.0, yx = (o.m & my) ? yn : 0.0, xx = (o.m & mx) ? prev : 0.0;
-      double sum = 0.0, r;
-      if (DIR > 0) {
-        sum += o.v0 * zx;
+      const double n0 = XFIRST ? xx : zx, n2 = XFIRST ? zx : xx;
+      double r;
+      if (ROW) {
+        double sum = 0.0;
+        sum += o.v0 * n0;
         sum += o.v1 * yx;
-        sum += o.v2 * xx;
-        r = o.b - sum;
+        sum += o.v2 * n2;
+        r = DIV ? (o.b - sum) / o.v3 : o.b - sum;
       } else {
-        sum += o.v0 * xx;
-        sum += o.v1 * yx;
-        sum += o.v2 * zx;
-        r = (o.b - sum) / o.v3;
+        double sum = o.b;
+        sum -= o.v0 * n0;
+        sum -= o.v1 * yx;
+        sum -= o.v2 * n2;
+        r = DIV ? sum / o.v3 : sum;
       }
       if (exporter)
         __hip_atomic_store((uint64_t *)outp + (so >> 3) + l, finished_bits(r), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
       else
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(grid_u32x2, r), rs, lane_off(p), out + so, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(grid_u32x2, r), rx, lane_off(p), out + so, 0);
       ring[p & 1][tid] = r;
       prev = r;
       load_op(p + GRID_DEPTH, o);
@@ -1982,9 +2057,11 @@ static bool detect_grid(int32_t n, const std::vector<int32_t> &rp, const std::ve
 
 static void grid_free(mtjg_prec M) {
   (void)hipFree(M->g_buf);
+  (void)hipFree(M->g_tbuf);
   (void)hipFree(M->g_ctr);
-  M->g_buf = nullptr;
+  M->g_buf = M->g_tbuf = nullptr;
   M->g_ctr = nullptr;
+  M->grid_t_version = ~0ull;
   M->grid = false;
   M->grid_vals_version = ~0ull;
 }
@@ -2001,21 +2078,47 @@ static void grid_setup(mtjg_prec M) {
   M->grid_vals_version = ~0ull;
 }
 
-// L.solve(b, y); U.solve(y, x) as grid sweeps (see k_grid_sweep).
-static void grid_apply(mtjg_prec M, hipStream_t s, const double *b, double *x) {
+// Two triangular solves as grid sweeps (see k_grid_sweep): a forward solve
+// into y, then a backward solve into x.  FWD / BWD: TRSV_ROW_UNIT /
+// TRSV_ROW_DIV for L.solve(b, y); U.solve(y, x) (ILU's apply), TRSV_COL_DIV /
+// TRSV_ROW_DIV for R'.solve, R.solve (ICC's apply), TRSV_COL_DIV /
+// TRSV_COL_UNIT_REV for U'.solve, L'.solve (ILU's transApply).
+template <int FWD, int BWD>
+static void grid_solves(mtjg_prec M, hipStream_t s, const double *b, double *x) {
   const GridPlan &g = M->gp;
   const int64_t nq = (int64_t)g.nz * g.YB * g.TW;
   const int blocks = (int)((nq + BS / 64 - 1) / (BS / 64));
+  constexpr bool TRANS = FWD == TRSV_COL_DIV || BWD == TRSV_COL_UNIT_REV;
   if (M->grid_vals_version != M->vals_version) {
     k_grid_fill<<<blocks, BS, 0, s>>>(g, M->rp, M->ci, M->lu, M->g_buf);
     M->grid_vals_version = M->vals_version;
   }
+  if (TRANS) {
+    if (!M->g_tbuf) HIPCHK(hipMalloc(&M->g_tbuf, (size_t)gridt_bytes(g)));
+    if (M->grid_t_version != M->vals_version) {
+      k_grid_fill_t<<<blocks, BS, 0, s>>>(g, M->rp, M->ci, M->lu, M->g_tbuf);
+      M->grid_t_version = M->vals_version;
+    }
+  }
   k_grid_gather<<<blocks, BS, 0, s>>>(g, b, M->g_buf);
   const int threads = 64 * g.P * g.YB;
+  const uint32_t ns8 = (uint32_t)g.nslot * 8u;
+  const uint32_t rb = (uint32_t)grid_bytes(g), tb = (uint32_t)gridt_bytes(g);
   const uint32_t ob = (uint32_t)grid_off(g, GRID_B), oy = (uint32_t)grid_off(g, GRID_Y),
-                 ox = (uint32_t)grid_off(g, GRID_X);
-  k_grid_sweep<1><<<g.G, threads, 0, s>>>(g, M->g_buf, ob, oy, M->g_ctr, M->sf_err);
-  k_grid_sweep<-1><<<g.G, threads, 0, s>>>(g, M->g_buf, oy, ox, M->g_ctr, M->sf_err);
+                 ox = (uint32_t)grid_off(g, GRID_X), om = (uint32_t)grid_off(g, GRID_MASK);
+  const uint32_t tm = GRIDT_MASK * ns8;
+  if (FWD == TRSV_ROW_UNIT)
+    k_grid_sweep<TRSV_ROW_UNIT><<<g.G, threads, 0, s>>>(g, M->g_buf, rb, 0, 0, om, M->g_buf, ob, oy, M->g_ctr,
+                                                         M->sf_err);
+  else
+    k_grid_sweep<TRSV_COL_DIV><<<g.G, threads, 0, s>>>(g, M->g_tbuf, tb, GRIDT_U * ns8, GRIDT_D * ns8, tm, M->g_buf,
+                                                        ob, oy, M->g_ctr, M->sf_err);
+  if (BWD == TRSV_ROW_DIV)
+    k_grid_sweep<TRSV_ROW_DIV><<<g.G, threads, 0, s>>>(g, M->g_buf, rb, 3 * ns8, 6 * ns8, om, M->g_buf, oy, ox,
+                                                        M->g_ctr, M->sf_err);
+  else
+    k_grid_sweep<TRSV_COL_UNIT_REV><<<g.G, threads, 0, s>>>(g, M->g_tbuf, tb, GRIDT_L * ns8, 0, tm, M->g_buf, oy,
+                                                             ox, M->g_ctr, M->sf_err);
   k_grid_scatter<<<blocks, BS, 0, s>>>(g, M->g_buf, x);
   HIPCHK(hipGetLastError());
 }
@@ -2178,7 +2281,7 @@ static void take_pattern(mtjg_prec M, mtjg_csr A, int32_t *bad_row, const char *
       end[(size_t)i] = M->h_rp[i + 1];
     }
     build_levels(ctx, n, M->h_rp, M->h_ci, beg, end, false, M->lev_u);
-    if (M->kind == PREC_ILU) grid_setup(M);
+    if (M->kind == PREC_ILU || M->kind == PREC_ICC) grid_setup(M);
   }
   if (A->nnz)
     HIPCHK(hipMemcpyAsync(M->lu, A->val, sizeof(double) * (size_t)A->nnz, hipMemcpyDeviceToDevice, s));
@@ -2314,7 +2417,9 @@ void prec_apply_dev(mtjg_prec M, int64_t n, const double *b, double *x, hipStrea
   } else if (M->kind == PREC_ICC) {
     if (!M->factored) fail(MTJG_E_INVALID_ARGUMENT, "ICC.setMatrix must precede apply");
     // Rt.transSolve(b, y); Rt.solve(y, x) (S/ICC.java:65-71)
-    if (M->level_launch) {
+    if (M->level_launch && M->grid) {
+      grid_solves<TRSV_COL_DIV, TRSV_ROW_DIV>(M, s, b, x);
+    } else if (M->level_launch) {
       run_levels<TRSV_COL_DIV>(M, M->lev_ut, s, M->h_trp, M->h_tci, M->h_tdiag, M->tval, b, M->y);
       run_levels<TRSV_ROW_DIV>(M, M->lev_u, s, M->h_rp, M->h_ci, M->h_diag, M->lu, M->y, x);
     } else {
@@ -2325,7 +2430,7 @@ void prec_apply_dev(mtjg_prec M, int64_t n, const double *b, double *x, hipStrea
     if (!M->factored) fail(MTJG_E_INVALID_ARGUMENT, "ILU.setMatrix must precede apply");
     // L.solve(b, y); U.solve(y, x) (S/ILU.java:64-70)
     if (M->level_launch && M->grid) {
-      grid_apply(M, s, b, x);
+      grid_solves<TRSV_ROW_UNIT, TRSV_ROW_DIV>(M, s, b, x);
     } else if (M->level_launch) {
       run_levels<TRSV_ROW_UNIT>(M, M->lev_l, s, M->h_rp, M->h_ci, M->h_diag, M->lu, b, M->y);
       run_levels<TRSV_ROW_DIV>(M, M->lev_u, s, M->h_rp, M->h_ci, M->h_diag, M->lu, M->y, x);
@@ -2350,7 +2455,9 @@ void prec_trans_apply_dev(mtjg_prec M, const double *b, double *x, hipStream_t s
   if (!M->factored) fail(MTJG_E_INVALID_ARGUMENT, "ILU.setMatrix must precede transApply");
   if (!M->trans_ready) build_transpose(M, true);
   // U.transSolve(b, y); L.transSolve(y, x) (S/ILU.java:72-78)
-  if (M->level_launch) {
+  if (M->level_launch && M->grid) {
+    grid_solves<TRSV_COL_DIV, TRSV_COL_UNIT_REV>(M, s, b, x);
+  } else if (M->level_launch) {
     run_levels<TRSV_COL_DIV>(M, M->lev_ut, s, M->h_trp, M->h_tci, M->h_tdiag, M->tval, b, M->y);
     run_levels<TRSV_COL_UNIT_REV>(M, M->lev_lt, s, M->h_trp, M->h_tci, M->h_tdiag, M->tval, M->y, x);
   } else {
@@ -2452,7 +2559,7 @@ extern "C" int mtjg_prec_ilu_create(mtjg_ctx ctx, mtjg_csr LU, mtjg_prec *out) {
 extern "C" int mtjg_prec_ilu_grid(mtjg_prec M, int32_t dims[3]) {
   MTJG_API_BEGIN
   if (!M || !dims) fail(MTJG_E_NULL, "null argument");
-  if (M->kind != PREC_ILU) fail(MTJG_E_INVALID_ARGUMENT, "not an ILU preconditioner");
+  if (M->kind != PREC_ILU && M->kind != PREC_ICC) fail(MTJG_E_INVALID_ARGUMENT, "not an ILU or ICC preconditioner");
   dims[0] = M->grid ? M->gp.nx : 0;
   dims[1] = M->grid ? M->gp.ny : 0;
   dims[2] = M->grid ? M->gp.nz : 0;
@@ -2476,6 +2583,7 @@ extern "C" int mtjg_prec_icc_create(mtjg_ctx ctx, mtjg_csr R, mtjg_prec *out) {
   HIPCHK(hipMemsetAsync(M->sf_err, 0, sizeof(int32_t), ctx->stream));
   M->level_launch = true;
   if (const char *e = getenv("MTJG_ILU_SYNCFREE")) M->level_launch = atoi(e) == 0;
+  if (const char *e = getenv("MTJG_ILU_GRID")) M->grid_allowed = atoi(e) != 0;
   *out = M;
   MTJG_API_END
 }

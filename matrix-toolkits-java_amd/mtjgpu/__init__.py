@@ -513,6 +513,13 @@ class ICC(_Preconditioner):
         _ck(self._lib.mtjg_prec_icc_create(self.ctx.handle, R.handle, C.byref(h)))
         self.handle = h
 
+    def grid(self):
+        """(nx, ny, nz) of the grid sweeps apply runs, or None when the
+        pattern is not a natural-order grid stencil (level schedule)."""
+        d = (C.c_int32 * 3)()
+        _ck(self._lib.mtjg_prec_ilu_grid(self.handle, d))
+        return tuple(d) if d[0] else None
+
 
 class AMG(_Preconditioner):
     """S/AMG.java: smoothed-aggregation algebraic multigrid with SSOR smoothing.

@@ -1,7 +1,8 @@
-"""ILU(0) grid sweeps (precond.hip: detect_grid, k_grid_sweep): when the
-factor's pattern is a natural-order 5- or 7-point grid stencil, apply's two
-triangular solves (S/ILU.java:64-70, S/UnitLowerCompRowMatrix.java:48-66,
-S/UpperCompRowMatrix.java:48-68) run as sweeps over the grid's
+"""ILU(0) and ICC grid sweeps (precond.hip: detect_grid, k_grid_sweep): when
+the factor's pattern is a natural-order 5- or 7-point grid stencil, the
+triangular solves of ILU's apply and transApply (S/ILU.java:64-78,
+S/UnitLowerCompRowMatrix.java:48-86, S/UpperCompRowMatrix.java:48-90) and
+ICC's apply (S/ICC.java:65-71) run as sweeps over the grid's
 anti-diagonals, plane groups pipelined across workgroups.  The results must
 be bitwise the oracle's, which solves row by row in MTJ's order; that covers
 grids of one and several 64-row blocks, partial last plane groups, one plane
@@ -79,15 +80,53 @@ def test_grid_sweeps_bitwise_oracle(ctx, monkeypatch, name):
     assert Oi.code == oracle.OK
     for seed in (4, 5, 6):
         bh = P.uniform(n, seed)
-        got = ilu.apply(M.DenseVector(bh, ctx), M.DenseVector(n, ctx)).getData()
-        np.testing.assert_array_equal(got, Oi.apply(bh))
-    # new values on the same pattern: the skewed copy of the factor follows
+        b = M.DenseVector(bh, ctx)
+        np.testing.assert_array_equal(ilu.apply(b, M.DenseVector(n, ctx)).getData(), Oi.apply(bh))
+        np.testing.assert_array_equal(ilu.transApply(b, M.DenseVector(n, ctx)).getData(), Oi.trans_apply(bh))
+    # new values on the same pattern: the skewed copies of the factor follow
     sysm2 = (sysm[0], sysm[1], sysm[2], sysm[3], np.asarray(sysm[4]) * 1.5)
     ilu.setMatrix(M.CompRowMatrix(*sysm2, ctx=ctx))
     Oi2 = oracle.Precond(oracle.PREC_ILU, oracle.Csr(*sysm2))
     bh = P.uniform(n, 9)
     b = M.DenseVector(bh, ctx)
     np.testing.assert_array_equal(ilu.apply(b, M.DenseVector(n, ctx)).getData(), Oi2.apply(bh))
+    np.testing.assert_array_equal(ilu.transApply(b, M.DenseVector(n, ctx)).getData(), Oi2.trans_apply(bh))
+
+
+ICC_CASES = {
+    "varcoef_20x70x9": (lambda: P.varcoef_poisson(3, 20, 70, 9, seed=3), (20, 70, 9), {}),
+    "varcoef_8x8x100_one_plane": (lambda: P.varcoef_poisson(3, 8, 8, 100, seed=5), (8, 8, 100),
+                                  {"MTJG_ILU_GRID_PLANES": "1"}),
+    "poisson3d_12x40x30_no_lead": (lambda: P.poisson3d(12, 40, 30), (12, 40, 30), {"MTJG_ILU_GRID_LEAD": "0"}),
+    "poisson2d_200x400": (lambda: P.poisson2d(200, 400), (200, 400, 1), {}),
+    "varcoef2d_130x77": (lambda: P.varcoef_poisson(2, 130, 77, seed=9), (130, 77, 1), {}),
+}
+
+
+@pytest.mark.parametrize("name", sorted(ICC_CASES))
+def test_icc_grid_sweeps_bitwise_oracle(ctx, monkeypatch, name):
+    """ICC's apply (R'y = b, then R x = y) as grid sweeps, bitwise the
+    oracle's, also after setMatrix with new values on the same pattern."""
+    make, grid, env = ICC_CASES[name]
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    sysm = make()
+    n = sysm[0]
+    icc = M.ICC(M.CompRowMatrix(*sysm, ctx=ctx))
+    icc.setMatrix(M.CompRowMatrix(*sysm, ctx=ctx))
+    assert icc.grid() == grid
+    Oc = oracle.Precond(oracle.PREC_ICC, oracle.Csr(*sysm))
+    assert Oc.code == oracle.OK
+    for seed in (4, 5):
+        bh = P.uniform(n, seed)
+        np.testing.assert_array_equal(icc.apply(M.DenseVector(bh, ctx), M.DenseVector(n, ctx)).getData(),
+                                      Oc.apply(bh))
+    sysm2 = (sysm[0], sysm[1], sysm[2], sysm[3], np.asarray(sysm[4]) * 1.5)
+    icc.setMatrix(M.CompRowMatrix(*sysm2, ctx=ctx))
+    Oc2 = oracle.Precond(oracle.PREC_ICC, oracle.Csr(*sysm2))
+    bh = P.uniform(n, 9)
+    np.testing.assert_array_equal(icc.apply(M.DenseVector(bh, ctx), M.DenseVector(n, ctx)).getData(),
+                                  Oc2.apply(bh))
 
 
 def _not_grid_cases():
@@ -137,22 +176,24 @@ def test_grid_switch_off(ctx, monkeypatch):
     assert ilu.grid() is None
 
 
-@pytest.mark.parametrize("method", ["bicgstab", "gmres"])
+@pytest.mark.parametrize("method", ["bicgstab", "gmres", "bicg", "qmr", "cg_icc"])
 def test_solvers_same_bits_as_level_schedule(ctx, monkeypatch, method):
-    """BiCGstab and GMRES(30) with ILU on 3-D convection-diffusion 24^3:
-    grid sweeps and the level schedule give the same iterates."""
-    sysm = P.convdiff3d(24, peclet=10.0, seed=1)
+    """BiCGstab, GMRES(30), BiCG and QMR (these two through transApply) with
+    ILU on 3-D convection-diffusion 24^3, and CG with ICC on variable-kappa
+    Poisson 24^3: grid sweeps and the level schedule give the same iterates."""
+    sysm = P.varcoef_poisson(3, 24, seed=2) if method == "cg_icc" else P.convdiff3d(24, peclet=10.0, seed=1)
     n = sysm[0]
     A = M.CompRowMatrix(*sysm, ctx=ctx)
     bh = A.mult(M.DenseVector(P.uniform(n, 1), ctx), M.DenseVector(n, ctx)).getData()
     out = []
     for grid in ("1", "0"):
         monkeypatch.setenv("MTJG_ILU_GRID", grid)
-        ilu = M.ILU(A)
+        ilu = M.ICC(A) if method == "cg_icc" else M.ILU(A)
         ilu.setMatrix(A)
         assert (ilu.grid() is not None) == (grid == "1")
         x = M.DenseVector(n, ctx)
-        S = M.GMRES(x, 30) if method == "gmres" else M.BiCGstab(x)
+        S = {"gmres": lambda: M.GMRES(x, 30), "bicgstab": lambda: M.BiCGstab(x), "bicg": lambda: M.BiCG(x),
+             "qmr": lambda: M.QMR(x), "cg_icc": lambda: M.CG(x)}[method]()
         S.setPreconditioner(ilu)
         S.setIterationMonitor(M.DefaultIterationMonitor(10000, 1e-10))
         S.solve(A, M.DenseVector(bh, ctx), x)
