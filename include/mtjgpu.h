@@ -199,14 +199,14 @@ int mtjg_prec_diagonal_create(mtjg_ctx ctx, int32_t n, mtjg_prec *out);
 /* new ILU(CompRowMatrix LU): LU supplies the pattern of the factor
  * (S/ILU.java:54-62). */
 int mtjg_prec_ilu_create(mtjg_ctx ctx, mtjg_csr LU, mtjg_prec *out);
-/* Grid of an ILU's or ICC's sweeps (no MTJ counterpart): when the pattern
- * given to setMatrix is a 5- or 7-point stencil of an nx x ny (x nz) grid in
- * natural order (row x + nx (y + ny z), off-diagonal columns among its grid
- * neighbours, ny <= 1024), the triangular solves of apply (and of ILU's
- * transApply) run as grid sweeps over the anti-diagonals x + y + z, bitwise
- * equal to the level-scheduled solves; dims receives nx, ny, nz, or zeros
- * when the level schedule is used.  MTJG_ILU_GRID=0 in the environment at
- * creation turns grid sweeps off. */
+/* Grid of an ILU's, ICC's or SSOR's sweeps (no MTJ counterpart): when the
+ * pattern given to setMatrix is a 5- or 7-point stencil of an nx x ny (x nz)
+ * grid in natural order (row x + nx (y + ny z), off-diagonal columns among
+ * its grid neighbours, ny <= 1024), the triangular solves of apply (and of
+ * ILU's transApply) and SSOR's two sweeps run as grid sweeps over the
+ * anti-diagonals x + y + z, bitwise equal to the level-scheduled ones; dims
+ * receives nx, ny, nz, or zeros when the level schedule is used.
+ * MTJG_ILU_GRID=0 in the environment at creation turns grid sweeps off. */
 int mtjg_prec_ilu_grid(mtjg_prec M, int32_t dims[3]);
 /* new SSOR(F, reverse, omegaF, omegaR) (S/SSOR.java:81-93): symmetric
  * successive over-relaxation; F supplies the size (square, else

@@ -327,6 +327,11 @@ void amg_apply_device(AmgDevice *H, const double *b, double *x, bool transpose, 
   if (L0.n > 0) HIPCHK(hipMemcpyAsync(x, L0.u, bytes, hipMemcpyDeviceToDevice, s));
 }
 
+void amg_check_device(AmgDevice *H) {
+  for (AmgLevel &L : H->L)
+    if (L.ssor) prec_check(L.ssor);
+}
+
 int32_t amg_level_sizes(const AmgDevice *H, int32_t *sizes, int32_t cap) {
   const int32_t m = (int32_t)H->L.size();
   for (int32_t k = 0; k < m && k < cap; ++k) sizes[k] = H->L[(size_t)k].n;

@@ -284,6 +284,9 @@ AmgDevice *amg_create_device(mtjg_ctx ctx, mtjg_csr A, const amg::Params &p, int
 // transposed coarse solve (transApply).
 void amg_apply_device(AmgDevice *H, const double *b, double *x, bool transpose, hipStream_t s);
 void amg_destroy_device(AmgDevice *H);
+// Raises MTJG_E_DEVICE if a smoother's grid sweep ran out of its poll budget
+// since the last check (prec_check of every level's SSOR).
+void amg_check_device(AmgDevice *H);
 // Level sizes (finest first); returns the number of levels.
 int32_t amg_level_sizes(const AmgDevice *H, int32_t *sizes, int32_t cap);
 const double *prec_invdiag(mtjg_prec M);

@@ -158,6 +158,11 @@ struct GridPlan {
 enum { GRID_B = 7, GRID_Y = 8, GRID_X = 9, GRID_MASK = 10 };
 __host__ __device__ inline int64_t grid_off(const GridPlan &g, int k) { return (int64_t)k * 8 * g.nslot; }
 __host__ __device__ inline int64_t grid_bytes(const GridPlan &g) { return 81 * g.nslot; }
+// SSOR's grid copy appends, per slot, a sweep's three fixed-side products and
+// the value it relaxes from (doubles, nslot each, from byte grid_bytes on).
+__host__ __device__ inline int64_t grid_ssor_f(const GridPlan &g) { return 81 * g.nslot; }
+__host__ __device__ inline int64_t grid_ssor_own(const GridPlan &g) { return 105 * g.nslot; }
+__host__ __device__ inline int64_t grid_ssor_bytes(const GridPlan &g) { return 113 * g.nslot; }
 // The transposed copy (U' and L' solves: ILU's transApply, ICC's R' solve),
 // a second allocation: U(k-nx ny, k), U(k-nx, k), U(k-1, k), the diagonal,
 // L(k+nx ny, k), L(k+nx, k), L(k+1, k) (doubles, nslot each), then the
@@ -1146,14 +1151,77 @@ __global__ __launch_bounds__(BS) void k_grid_gather(const GridPlan g, const doub
   if (c.z % g.P == 0 && c.z > 0) *(uint64_t *)&xs[s] = TRSV_PENDING;
 }
 
-// x from the skewed slots.
-__global__ __launch_bounds__(BS) void k_grid_scatter(const GridPlan g, const char *__restrict__ buf, double *x) {
-  const double *xs = (const double *)(buf + grid_off(g, GRID_X));
+// x from the skewed slots (of array src).
+__global__ __launch_bounds__(BS) void k_grid_scatter(const GridPlan g, const char *__restrict__ buf, double *x,
+                                                     int src = GRID_X) {
+  const double *xs = (const double *)(buf + grid_off(g, src));
   const int64_t q = (int64_t)blockIdx.x * (BS / 64) + (threadIdx.x >> 6);
   const int32_t l = threadIdx.x & 63;
   if (q >= (int64_t)g.nz * g.YB * g.TW) return;
   const GridSlot c = grid_slot(g, q, l);
   if (c.valid) x[c.x + (int64_t)g.nx * (c.y + (int64_t)g.ny * c.z)] = xs[q * 64 + l];
+}
+
+// Slot of grid row (x, y, z).
+__device__ __forceinline__ int64_t grid_slot_of(const GridPlan &g, int32_t x, int32_t y, int32_t z) {
+  const int32_t yb = y >> 6, l = y & 63;
+  return (((int64_t)z * g.YB + yb) * g.TW + x + l) * 64 + l;
+}
+
+// SSOR's forward sweep: b and x (the value relaxed from) into the slots, the
+// fixed-side products of the upper entries (x+1, y+1, z+1 in storage order)
+// with x, and TRSV_PENDING into the sweeps' imported planes (as
+// k_grid_gather).  An absent entry's product is +0.0, which leaves the
+// row's sum unchanged.
+__global__ __launch_bounds__(BS) void k_grid_ssor_prep_fwd(const GridPlan g, const double *__restrict__ b,
+                                                           const double *__restrict__ x, char *buf) {
+  const int64_t q = (int64_t)blockIdx.x * (BS / 64) + (threadIdx.x >> 6);
+  const int32_t l = threadIdx.x & 63;
+  if (q >= (int64_t)g.nz * g.YB * g.TW) return;
+  const GridSlot c = grid_slot(g, q, l);
+  const int64_t s = q * 64 + l;
+  const double *val = (const double *)buf;
+  double *f = (double *)(buf + grid_ssor_f(g)), *own = (double *)(buf + grid_ssor_own(g));
+  double bv = 0.0, xv = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  if (c.valid) {
+    const uint32_t m = ((const uint8_t *)(buf + grid_off(g, GRID_MASK)))[s];
+    const int64_t nxny = (int64_t)g.nx * g.ny;
+    const int64_t i = c.x + (int64_t)g.nx * (c.y + (int64_t)g.ny * c.z);
+    bv = b[i];
+    xv = x[i];
+    if (m & GM_XU) f0 = val[3 * g.nslot + s] * x[i + 1];
+    if (m & GM_YU) f1 = val[4 * g.nslot + s] * x[i + g.nx];
+    if (m & GM_ZU) f2 = val[5 * g.nslot + s] * x[i + nxny];
+  }
+  ((double *)(buf + grid_off(g, GRID_B)))[s] = bv;
+  own[s] = xv;
+  f[s] = f0;
+  f[g.nslot + s] = f1;
+  f[2 * g.nslot + s] = f2;
+  if (c.z % g.P == g.P - 1 && c.z + 1 < g.nz) ((uint64_t *)(buf + grid_off(g, GRID_Y)))[s] = TRSV_PENDING;
+  if (c.z % g.P == 0 && c.z > 0) ((uint64_t *)(buf + grid_off(g, GRID_X)))[s] = TRSV_PENDING;
+}
+
+// SSOR's backward sweep: the fixed-side products of the lower entries (z-1,
+// y-1, x-1 in storage order) with the forward sweep's result (y slots).
+__global__ __launch_bounds__(BS) void k_grid_ssor_prep_bwd(const GridPlan g, char *buf) {
+  const int64_t q = (int64_t)blockIdx.x * (BS / 64) + (threadIdx.x >> 6);
+  const int32_t l = threadIdx.x & 63;
+  if (q >= (int64_t)g.nz * g.YB * g.TW) return;
+  const GridSlot c = grid_slot(g, q, l);
+  const int64_t s = q * 64 + l;
+  const double *val = (const double *)buf, *xx = (const double *)(buf + grid_off(g, GRID_Y));
+  double *f = (double *)(buf + grid_ssor_f(g));
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  if (c.valid) {
+    const uint32_t m = ((const uint8_t *)(buf + grid_off(g, GRID_MASK)))[s];
+    if (m & GM_ZL) f0 = val[s] * xx[grid_slot_of(g, c.x, c.y, c.z - 1)];
+    if (m & GM_YL) f1 = val[g.nslot + s] * xx[grid_slot_of(g, c.x, c.y - 1, c.z)];
+    if (m & GM_XL) f2 = val[2 * g.nslot + s] * xx[grid_slot_of(g, c.x - 1, c.y, c.z)];
+  }
+  f[s] = f0;
+  f[g.nslot + s] = f1;
+  f[2 * g.nslot + s] = f2;
 }
 
 // A barrier that orders LDS only: global loads stay in flight across it.
@@ -1166,6 +1234,7 @@ __device__ __forceinline__ void lds_barrier() {
 // Operands of one step of one lane.
 struct GridOp {
   double v0, v1, v2, v3, b;
+  double f0, f1, f2, own;  // SSOR: the fixed-side products and the value relaxed from
   uint64_t imp;  // imported z-neighbour (bits), importing wavefronts only
   uint32_t m;
 };
@@ -1204,16 +1273,19 @@ __device__ __forceinline__ double grid_ld(__amdgpu_buffer_rsrc_t r, uint32_t vo,
 template <int FORM>
 __global__ __launch_bounds__(GRID_WAVES * 64) void k_grid_sweep(const GridPlan g, const char *vbuf, uint32_t vbytes,
                                                                 uint32_t v0, uint32_t vd, uint32_t vm, char *xbuf,
-                                                                uint32_t rhs, uint32_t out, uint32_t *ctr,
-                                                                int32_t *err) {
-  constexpr int DIR = (FORM == TRSV_ROW_UNIT || FORM == TRSV_COL_DIV) ? 1 : -1;
+                                                                uint32_t xbytes, uint32_t rhs, uint32_t out,
+                                                                uint32_t fo, uint32_t owo, double omega,
+                                                                uint32_t *ctr, int32_t *err) {
+  constexpr bool SSOR = FORM == PACK_SSOR || FORM == PACK_SSOR_BWD;
+  constexpr int DIR = (FORM == TRSV_ROW_UNIT || FORM == TRSV_COL_DIV || FORM == PACK_SSOR) ? 1 : -1;
   constexpr bool ROW = FORM == TRSV_ROW_UNIT || FORM == TRSV_ROW_DIV;
-  constexpr bool DIV = FORM == TRSV_ROW_DIV || FORM == TRSV_COL_DIV;
-  constexpr bool XFIRST = FORM == TRSV_ROW_DIV;  // terms x, y, z (else z, y, x)
+  constexpr bool DIV = FORM == TRSV_ROW_DIV || FORM == TRSV_COL_DIV || SSOR;
+  constexpr bool XFIRST = FORM == TRSV_ROW_DIV || FORM == PACK_SSOR_BWD;  // terms x, y, z (else z, y, x)
+  constexpr bool UPPER = FORM == TRSV_ROW_DIV || FORM == PACK_SSOR_BWD;   // row copy's U terms and bits
   // mask bits of the z-, y- and x-terms
-  constexpr uint32_t mz = FORM == TRSV_ROW_DIV ? GM_ZU : FORM == TRSV_COL_UNIT_REV ? GM_TLZ : FORM == TRSV_COL_DIV ? GM_TUZ : GM_ZL;
-  constexpr uint32_t my = FORM == TRSV_ROW_DIV ? GM_YU : FORM == TRSV_COL_UNIT_REV ? GM_TLY : FORM == TRSV_COL_DIV ? GM_TUY : GM_YL;
-  constexpr uint32_t mx = FORM == TRSV_ROW_DIV ? GM_XU : FORM == TRSV_COL_UNIT_REV ? GM_TLX : FORM == TRSV_COL_DIV ? GM_TUX : GM_XL;
+  constexpr uint32_t mz = UPPER ? GM_ZU : FORM == TRSV_COL_UNIT_REV ? GM_TLZ : FORM == TRSV_COL_DIV ? GM_TUZ : GM_ZL;
+  constexpr uint32_t my = UPPER ? GM_YU : FORM == TRSV_COL_UNIT_REV ? GM_TLY : FORM == TRSV_COL_DIV ? GM_TUY : GM_YL;
+  constexpr uint32_t mx = UPPER ? GM_XU : FORM == TRSV_COL_UNIT_REV ? GM_TLX : FORM == TRSV_COL_DIV ? GM_TUX : GM_XL;
   __shared__ double ring[2][GRID_WAVES * 64];
   __shared__ int32_t s_ticket;
   const int32_t tid = threadIdx.x, l = tid & 63;
@@ -1236,8 +1308,7 @@ __global__ __launch_bounds__(GRID_WAVES * 64) void k_grid_sweep(const GridPlan g
   const uint32_t ioff = (uint32_t)(DIR > 0 ? -g.slab : g.slab) * 8u;  // the imported plane (bytes, mod 2^32)
   const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(vbuf), (short)0,
                                                                       (int)vbytes, 0x00020000);
-  const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(xbuf, (short)0, (int)(uint32_t)grid_bytes(g),
-                                                                      0x00020000);
+  const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(xbuf, (short)0, (int)xbytes, 0x00020000);
   // byte offsets of the operand arrays (scalar) and of the lane
   const uint32_t ns8 = (uint32_t)g.nslot * 8u, l8 = (uint32_t)l * 8u;
   const uint32_t a0 = v0, a1 = a0 + ns8, a2 = a1 + ns8, a3 = vd, am = vm;
@@ -1300,6 +1371,12 @@ __global__ __launch_bounds__(GRID_WAVES * 64) void k_grid_sweep(const GridPlan g
     o.v2 = grid_ld(rv, vo, a2 + so);
     if (DIV) o.v3 = grid_ld(rv, vo, a3 + so);
     o.b = grid_ld(rx, vo, rhs + so);
+    if (SSOR) {
+      o.f0 = grid_ld(rx, vo, fo + so);
+      o.f1 = grid_ld(rx, vo, fo + ns8 + so);
+      o.f2 = grid_ld(rx, vo, fo + 2 * ns8 + so);
+      o.own = grid_ld(rx, vo, owo + so);
+    }
     if (imports)
       o.imp = __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(rx, vo, out + so + ioff, GRID_SC1));
   };
@@ -1343,7 +1420,28 @@ __global__ __launch_bounds__(GRID_WAVES * 64) void k_grid_sweep(const GridPlan g
       const double zx = (o.m & mz) ? zn : 0.0, yx = (o.m & my) ? yn : 0.0, xx = (o.m & mx) ? prev : 0.0;
       const double n0 = XFIRST ? xx : zx, n2 = XFIRST ? zx : xx;
       double r;
-      if (ROW) {
+      if (SSOR) {
+        // the row's entries in storage order (lower, then upper): the
+        // forward sweep's lower terms and the backward sweep's upper terms
+        // are this sweep's values, the others the fixed-side products; then
+        // MTJ's relaxation of the value the sweep starts from
+        double sum = 0.0;
+        if (FORM == PACK_SSOR_BWD) {
+          sum += o.f0;
+          sum += o.f1;
+          sum += o.f2;
+        }
+        sum += o.v0 * n0;
+        sum += o.v1 * yx;
+        sum += o.v2 * n2;
+        if (FORM == PACK_SSOR) {
+          sum += o.f0;
+          sum += o.f1;
+          sum += o.f2;
+        }
+        const double sigma = (o.b - sum) / o.v3;
+        r = o.own + omega * (sigma - o.own);
+      } else if (ROW) {
         double sum = 0.0;
         sum += o.v0 * n0;
         sum += o.v1 * yx;
@@ -2070,7 +2168,9 @@ static void grid_setup(mtjg_prec M) {
   GridPlan g;
   if (!M->grid_allowed || !detect_grid(M->n, M->h_rp, M->h_ci, g)) return;
   hipStream_t s = M->ctx->stream;
-  HIPCHK(hipMalloc(&M->g_buf, (size_t)grid_bytes(g)));
+  const int64_t bytes = M->kind == PREC_SSOR ? grid_ssor_bytes(g) : grid_bytes(g);
+  if (bytes >= (int64_t)1 << 31) return;  // one buffer descriptor with 32-bit offsets
+  HIPCHK(hipMalloc(&M->g_buf, (size_t)bytes));
   HIPCHK(hipMalloc(&M->g_ctr, sizeof(uint32_t) * 2));
   HIPCHK(hipMemsetAsync(M->g_ctr, 0, sizeof(uint32_t) * 2, s));
   M->gp = g;
@@ -2108,18 +2208,46 @@ static void grid_solves(mtjg_prec M, hipStream_t s, const double *b, double *x) 
                  ox = (uint32_t)grid_off(g, GRID_X), om = (uint32_t)grid_off(g, GRID_MASK);
   const uint32_t tm = GRIDT_MASK * ns8;
   if (FWD == TRSV_ROW_UNIT)
-    k_grid_sweep<TRSV_ROW_UNIT><<<g.G, threads, 0, s>>>(g, M->g_buf, rb, 0, 0, om, M->g_buf, ob, oy, M->g_ctr,
-                                                         M->sf_err);
+    k_grid_sweep<TRSV_ROW_UNIT><<<g.G, threads, 0, s>>>(g, M->g_buf, rb, 0, 0, om, M->g_buf, rb, ob, oy, 0, 0, 0.0,
+                                                         M->g_ctr, M->sf_err);
   else
     k_grid_sweep<TRSV_COL_DIV><<<g.G, threads, 0, s>>>(g, M->g_tbuf, tb, GRIDT_U * ns8, GRIDT_D * ns8, tm, M->g_buf,
-                                                        ob, oy, M->g_ctr, M->sf_err);
+                                                        rb, ob, oy, 0, 0, 0.0, M->g_ctr, M->sf_err);
   if (BWD == TRSV_ROW_DIV)
-    k_grid_sweep<TRSV_ROW_DIV><<<g.G, threads, 0, s>>>(g, M->g_buf, rb, 3 * ns8, 6 * ns8, om, M->g_buf, oy, ox,
-                                                        M->g_ctr, M->sf_err);
+    k_grid_sweep<TRSV_ROW_DIV><<<g.G, threads, 0, s>>>(g, M->g_buf, rb, 3 * ns8, 6 * ns8, om, M->g_buf, rb, oy, ox,
+                                                        0, 0, 0.0, M->g_ctr, M->sf_err);
   else
-    k_grid_sweep<TRSV_COL_UNIT_REV><<<g.G, threads, 0, s>>>(g, M->g_tbuf, tb, GRIDT_L * ns8, 0, tm, M->g_buf, oy,
-                                                             ox, M->g_ctr, M->sf_err);
+    k_grid_sweep<TRSV_COL_UNIT_REV><<<g.G, threads, 0, s>>>(g, M->g_tbuf, tb, GRIDT_L * ns8, 0, tm, M->g_buf, rb,
+                                                             oy, ox, 0, 0, 0.0, M->g_ctr, M->sf_err);
   k_grid_scatter<<<blocks, BS, 0, s>>>(g, M->g_buf, x);
+  HIPCHK(hipGetLastError());
+}
+
+// SSOR.apply as grid sweeps (see k_grid_sweep): the forward sweep from x
+// into the y slots, then (reverse) the backward sweep from them into the x
+// slots; each sweep's fixed-side products are formed first.
+static void grid_ssor(mtjg_prec M, hipStream_t s, const double *b, double *x) {
+  const GridPlan &g = M->gp;
+  const int64_t nq = (int64_t)g.nz * g.YB * g.TW;
+  const int blocks = (int)((nq + BS / 64 - 1) / (BS / 64));
+  if (M->grid_vals_version != M->vals_version) {
+    k_grid_fill<<<blocks, BS, 0, s>>>(g, M->rp, M->ci, M->lu, M->g_buf);
+    M->grid_vals_version = M->vals_version;
+  }
+  k_grid_ssor_prep_fwd<<<blocks, BS, 0, s>>>(g, b, x, M->g_buf);
+  const int threads = 64 * g.P * g.YB;
+  const uint32_t ns8 = (uint32_t)g.nslot * 8u, rb = (uint32_t)grid_ssor_bytes(g);
+  const uint32_t ob = (uint32_t)grid_off(g, GRID_B), oy = (uint32_t)grid_off(g, GRID_Y),
+                 ox = (uint32_t)grid_off(g, GRID_X), om = (uint32_t)grid_off(g, GRID_MASK);
+  const uint32_t of = (uint32_t)grid_ssor_f(g), oo = (uint32_t)grid_ssor_own(g);
+  k_grid_sweep<PACK_SSOR><<<g.G, threads, 0, s>>>(g, M->g_buf, rb, 0, 6 * ns8, om, M->g_buf, rb, ob, oy, of, oo,
+                                                   M->omega_f, M->g_ctr, M->sf_err);
+  if (M->reverse) {
+    k_grid_ssor_prep_bwd<<<blocks, BS, 0, s>>>(g, M->g_buf);
+    k_grid_sweep<PACK_SSOR_BWD><<<g.G, threads, 0, s>>>(g, M->g_buf, rb, 3 * ns8, 6 * ns8, om, M->g_buf, rb, ob, ox,
+                                                         of, oy, M->omega_r, M->g_ctr, M->sf_err);
+  }
+  k_grid_scatter<<<blocks, BS, 0, s>>>(g, M->g_buf, x, M->reverse ? GRID_X : GRID_Y);
   HIPCHK(hipGetLastError());
 }
 
@@ -2281,7 +2409,7 @@ static void take_pattern(mtjg_prec M, mtjg_csr A, int32_t *bad_row, const char *
       end[(size_t)i] = M->h_rp[i + 1];
     }
     build_levels(ctx, n, M->h_rp, M->h_ci, beg, end, false, M->lev_u);
-    if (M->kind == PREC_ILU || M->kind == PREC_ICC) grid_setup(M);
+    if (M->kind == PREC_ILU || M->kind == PREC_ICC || M->kind == PREC_SSOR) grid_setup(M);
   }
   if (A->nnz)
     HIPCHK(hipMemcpyAsync(M->lu, A->val, sizeof(double) * (size_t)A->nnz, hipMemcpyDeviceToDevice, s));
@@ -2407,7 +2535,8 @@ void prec_apply_dev(mtjg_prec M, int64_t n, const double *b, double *x, hipStrea
     if (n > 0) k_jacobi_apply<<<grid_for(n), BS, 0, s>>>(n, b, M->invdiag, x, st);
   } else if (M->kind == PREC_SSOR) {
     if (!M->factored) fail(MTJG_E_INVALID_ARGUMENT, "SSOR.setMatrix must precede apply");
-    run_ssor(M, s, b, x);
+    if (M->grid) grid_ssor(M, s, b, x);
+    else run_ssor(M, s, b, x);
   } else if (M->kind == PREC_AMG) {
     if (!M->amg) fail(MTJG_E_INVALID_ARGUMENT, "AMG.setMatrix must precede apply");
     amg_apply_device(M->amg, b, x, false, s);
@@ -2469,7 +2598,12 @@ void prec_trans_apply_dev(mtjg_prec M, const double *b, double *x, hipStream_t s
 // Raises if a single-launch triangular solve ran out of its poll budget
 // since the last check (synchronises the stream).
 void prec_check(mtjg_prec M) {
-  if (!M || (M->kind != PREC_ILU && M->kind != PREC_ICC) || !M->sf_err) return;
+  if (M && M->kind == PREC_AMG && M->amg) {
+    amg_check_device(M->amg);  // its smoothers' grid sweeps
+    return;
+  }
+  if (!M || !M->sf_err) return;
+  if (M->kind != PREC_ILU && M->kind != PREC_ICC && !(M->kind == PREC_SSOR && M->grid)) return;
   int32_t e = 0;
   HIPCHK(hipMemcpyAsync(&e, M->sf_err, sizeof(int32_t), hipMemcpyDeviceToHost, M->ctx->stream));
   HIPCHK(hipStreamSynchronize(M->ctx->stream));
@@ -2559,7 +2693,8 @@ extern "C" int mtjg_prec_ilu_create(mtjg_ctx ctx, mtjg_csr LU, mtjg_prec *out) {
 extern "C" int mtjg_prec_ilu_grid(mtjg_prec M, int32_t dims[3]) {
   MTJG_API_BEGIN
   if (!M || !dims) fail(MTJG_E_NULL, "null argument");
-  if (M->kind != PREC_ILU && M->kind != PREC_ICC) fail(MTJG_E_INVALID_ARGUMENT, "not an ILU or ICC preconditioner");
+  if (M->kind != PREC_ILU && M->kind != PREC_ICC && M->kind != PREC_SSOR)
+    fail(MTJG_E_INVALID_ARGUMENT, "not an ILU, ICC or SSOR preconditioner");
   dims[0] = M->grid ? M->gp.nx : 0;
   dims[1] = M->grid ? M->gp.ny : 0;
   dims[2] = M->grid ? M->gp.nz : 0;
@@ -2638,6 +2773,9 @@ extern "C" int mtjg_prec_ssor_create(mtjg_ctx ctx, mtjg_csr F, int32_t reverse, 
   HIPCHK(hipMalloc(&M->xx, sizeof(double) * (size_t)std::max(M->n, 1)));
   HIPCHK(hipMemsetAsync(M->xx, 0, sizeof(double) * (size_t)std::max(M->n, 1), ctx->stream));
   HIPCHK(hipMalloc(&M->bad, sizeof(int32_t)));
+  HIPCHK(hipMalloc(&M->sf_err, sizeof(int32_t)));  // grid sweeps' poll-budget flag
+  HIPCHK(hipMemsetAsync(M->sf_err, 0, sizeof(int32_t), ctx->stream));
+  if (const char *e = getenv("MTJG_ILU_GRID")) M->grid_allowed = atoi(e) != 0;
   *out = M;
   MTJG_API_END
 }

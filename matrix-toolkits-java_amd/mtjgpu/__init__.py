@@ -428,6 +428,15 @@ class CompRowMatrix:
 # ---------------------------------------------------------------------------
 # preconditioners
 
+def _grid_dims(self):
+    """(nx, ny, nz) of the grid sweeps the preconditioner runs (ILU, ICC and
+    SSOR on a natural-order 5- or 7-point stencil), or None when the pattern
+    is not such a stencil (level schedule)."""
+    d = (C.c_int32 * 3)()
+    _ck(self._lib.mtjg_prec_ilu_grid(self.handle, d))
+    return tuple(d) if d[0] else None
+
+
 class _Preconditioner:
     def setMatrix(self, A):
         bad = C.c_int32(-1)
@@ -494,12 +503,7 @@ class ILU(_Preconditioner):
         _ck(self._lib.mtjg_prec_ilu_create(self.ctx.handle, LU.handle, C.byref(h)))
         self.handle = h
 
-    def grid(self):
-        """(nx, ny, nz) of the grid sweeps apply runs, or None when the
-        pattern is not a natural-order grid stencil (level schedule)."""
-        d = (C.c_int32 * 3)()
-        _ck(self._lib.mtjg_prec_ilu_grid(self.handle, d))
-        return tuple(d) if d[0] else None
+    grid = _grid_dims
 
 
 class ICC(_Preconditioner):
@@ -513,12 +517,7 @@ class ICC(_Preconditioner):
         _ck(self._lib.mtjg_prec_icc_create(self.ctx.handle, R.handle, C.byref(h)))
         self.handle = h
 
-    def grid(self):
-        """(nx, ny, nz) of the grid sweeps apply runs, or None when the
-        pattern is not a natural-order grid stencil (level schedule)."""
-        d = (C.c_int32 * 3)()
-        _ck(self._lib.mtjg_prec_ilu_grid(self.handle, d))
-        return tuple(d) if d[0] else None
+    grid = _grid_dims
 
 
 class AMG(_Preconditioner):
@@ -572,6 +571,8 @@ class SSOR(_Preconditioner):
 
     def setOmega(self, omegaF, omegaR):
         _ck(self._lib.mtjg_prec_ssor_set_omega(self.handle, float(omegaF), float(omegaR)))
+
+    grid = _grid_dims
 
 
 # ---------------------------------------------------------------------------

@@ -1,9 +1,10 @@
-"""ILU(0) and ICC grid sweeps (precond.hip: detect_grid, k_grid_sweep): when
-the factor's pattern is a natural-order 5- or 7-point grid stencil, the
+"""ILU(0), ICC and SSOR grid sweeps (precond.hip: detect_grid, k_grid_sweep):
+when the pattern is a natural-order 5- or 7-point grid stencil, the
 triangular solves of ILU's apply and transApply (S/ILU.java:64-78,
-S/UnitLowerCompRowMatrix.java:48-86, S/UpperCompRowMatrix.java:48-90) and
-ICC's apply (S/ICC.java:65-71) run as sweeps over the grid's
-anti-diagonals, plane groups pipelined across workgroups.  The results must
+S/UnitLowerCompRowMatrix.java:48-86, S/UpperCompRowMatrix.java:48-90),
+ICC's apply (S/ICC.java:65-71) and SSOR's two sweeps (S/SSOR.java:143-194)
+run as sweeps over the grid's anti-diagonals, plane groups pipelined across
+workgroups.  The results must
 be bitwise the oracle's, which solves row by row in MTJ's order; that covers
 grids of one and several 64-row blocks, partial last plane groups, one plane
 per workgroup (every plane imported across workgroups), no head start
@@ -129,6 +130,40 @@ def test_icc_grid_sweeps_bitwise_oracle(ctx, monkeypatch, name):
                                   Oc2.apply(bh))
 
 
+SSOR_CASES = ["convdiff_20x70x9", "convdiff_9x130x11", "varcoef_8x8x100_one_plane", "convdiff_12x40x30_no_lead",
+              "poisson2d_300x1000", "dropped_3d", "dropped_2d"]
+
+
+@pytest.mark.parametrize("wf,wr,reverse", [(1.0, 1.0, True), (1.3, 0.7, True), (1.7, 1.7, False)])
+@pytest.mark.parametrize("name", SSOR_CASES)
+def test_ssor_grid_sweeps_bitwise_oracle(ctx, monkeypatch, name, wf, wr, reverse):
+    """SSOR's forward and (reverse) backward sweeps as grid sweeps, bitwise
+    the oracle's from a zero and a nonzero incoming x, and after setMatrix
+    with new values on the same pattern."""
+    make, grid, env = CASES[name]
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    sysm = make()
+    n = sysm[0]
+    A = M.CompRowMatrix(*sysm, ctx=ctx)
+    S = M.SSOR(A, reverse=reverse, omegaF=wf, omegaR=wr)
+    S.setMatrix(A)
+    assert S.grid() == grid
+    Op = oracle.Precond(oracle.PREC_SSOR, oracle.Csr(*sysm), reverse=reverse, omega_f=wf, omega_r=wr)
+    b = P.uniform(n, 7)
+    for x0 in (np.zeros(n), P.uniform(n, 8)):
+        x = M.DenseVector(x0, ctx)
+        S.apply(M.DenseVector(b, ctx), x)
+        assert x.getData().tobytes() == Op.apply(b, x0).tobytes()
+    sysm2 = (sysm[0], sysm[1], sysm[2], sysm[3], np.asarray(sysm[4]) * 1.5)
+    S.setMatrix(M.CompRowMatrix(*sysm2, ctx=ctx))
+    Op2 = oracle.Precond(oracle.PREC_SSOR, oracle.Csr(*sysm2), reverse=reverse, omega_f=wf, omega_r=wr)
+    x0 = P.uniform(n, 9)
+    x = M.DenseVector(x0, ctx)
+    S.apply(M.DenseVector(b, ctx), x)
+    assert x.getData().tobytes() == Op2.apply(b, x0).tobytes()
+
+
 def _not_grid_cases():
     m = 12
     T = sp.diags([np.ones(m - 1), np.ones(m), np.ones(m - 1)], [-1, 0, 1])
@@ -176,27 +211,36 @@ def test_grid_switch_off(ctx, monkeypatch):
     assert ilu.grid() is None
 
 
-@pytest.mark.parametrize("method", ["bicgstab", "gmres", "bicg", "qmr", "cg_icc"])
+@pytest.mark.parametrize("method", ["bicgstab", "gmres", "bicg", "qmr", "cg_icc", "cg_ssor"])
 def test_solvers_same_bits_as_level_schedule(ctx, monkeypatch, method):
     """BiCGstab, GMRES(30), BiCG and QMR (these two through transApply) with
-    ILU on 3-D convection-diffusion 24^3, and CG with ICC on variable-kappa
-    Poisson 24^3: grid sweeps and the level schedule give the same iterates."""
-    sysm = P.varcoef_poisson(3, 24, seed=2) if method == "cg_icc" else P.convdiff3d(24, peclet=10.0, seed=1)
+    ILU on 3-D convection-diffusion 24^3, and CG with ICC or SSOR on
+    variable-kappa Poisson 24^3: grid sweeps and the level schedule give the
+    same iterates.  (SSOR's apply starts from x's incoming values, so inside
+    CG it is not a fixed operator and CG need not converge: that case runs 30
+    iterations and compares the iterates.)"""
+    spd = method in ("cg_icc", "cg_ssor")
+    sysm = P.varcoef_poisson(3, 24, seed=2) if spd else P.convdiff3d(24, peclet=10.0, seed=1)
     n = sysm[0]
     A = M.CompRowMatrix(*sysm, ctx=ctx)
     bh = A.mult(M.DenseVector(P.uniform(n, 1), ctx), M.DenseVector(n, ctx)).getData()
     out = []
     for grid in ("1", "0"):
         monkeypatch.setenv("MTJG_ILU_GRID", grid)
-        ilu = M.ICC(A) if method == "cg_icc" else M.ILU(A)
+        ilu = M.ICC(A) if method == "cg_icc" else M.SSOR(A) if method == "cg_ssor" else M.ILU(A)
         ilu.setMatrix(A)
         assert (ilu.grid() is not None) == (grid == "1")
         x = M.DenseVector(n, ctx)
         S = {"gmres": lambda: M.GMRES(x, 30), "bicgstab": lambda: M.BiCGstab(x), "bicg": lambda: M.BiCG(x),
-             "qmr": lambda: M.QMR(x), "cg_icc": lambda: M.CG(x)}[method]()
+             "qmr": lambda: M.QMR(x), "cg_icc": lambda: M.CG(x), "cg_ssor": lambda: M.CG(x)}[method]()
         S.setPreconditioner(ilu)
-        S.setIterationMonitor(M.DefaultIterationMonitor(10000, 1e-10))
-        S.solve(A, M.DenseVector(bh, ctx), x)
+        if method == "cg_ssor":
+            S.setIterationMonitor(M.DefaultIterationMonitor(30, 1e-30))
+            with pytest.raises(M.IterativeSolverNotConvergedException):
+                S.solve(A, M.DenseVector(bh, ctx), x)
+        else:
+            S.setIterationMonitor(M.DefaultIterationMonitor(10000, 1e-10))
+            S.solve(A, M.DenseVector(bh, ctx), x)
         out.append((S.getIterationMonitor().iterations(), x.getData()))
     assert out[0][0] == out[1][0]
     np.testing.assert_array_equal(out[0][1], out[1][1])
