@@ -149,6 +149,7 @@ struct GridPlan {
   int32_t TW = 0;    // skew columns: nx + 64 (the last holds no grid row in any lane)
   int32_t lead = 0;  // steps a workgroup starts behind the group it imports from
   int64_t slab = 0, nslot = 0;  // slots per plane, slots in all
+  int64_t poll_budget = 0;      // polls of one pending value before the error flag (<= 0: none)
 };
 // One allocation holds a factor's grid copy and the sweeps' vectors, at these
 // byte offsets (nslot slots each; all below 2^32, so one buffer descriptor
@@ -1339,8 +1340,10 @@ __global__ __launch_bounds__(GRID_WAVES * 64) void k_grid_sweep(const GridPlan g
       const int32_t xw = max(g.nx - 1 - g.lead, 0), yw = g.ny - 1, ybw = yw >> 6, lw = yw & 63;
       ws = (int64_t)((z0 + g.P) * g.YB + ybw) * g.TW * 64 + (int64_t)(xw + lw) * 64 + lw;
     }
-    long long budget = GRID_POLL_BUDGET;
-    while (load_bits(&outp[ws]) == TRSV_PENDING) {
+    // a budget of 0 or less admits no wait at all (the host forces the
+    // error path that way)
+    long long budget = g.poll_budget;
+    while (budget <= 0 || load_bits(&outp[ws]) == TRSV_PENDING) {
       if (--budget < 0) {
         atomicExch(err, 1);
         break;
@@ -1401,7 +1404,7 @@ __global__ __launch_bounds__(GRID_WAVES * 64) void k_grid_sweep(const GridPlan g
         uint64_t bits = o.imp;
         if (p >= pa && p <= pb && (o.m & mz) && bits == TRSV_PENDING) {
           const double *src = outp + ((so + ioff) >> 3) + l;  // the plane below (above)
-          long long budget = GRID_POLL_BUDGET;
+          long long budget = g.poll_budget;
           do {
             if (--budget < 0) {
               atomicExch(err, 1);
@@ -2146,6 +2149,8 @@ static bool detect_grid(int32_t n, const std::vector<int32_t> &rp, const std::ve
   // a head start of 3 steps: a larger one idles every workgroup longer than
   // the polls it saves (128^3 sweeps 242 / 261 us at 3, 277 / 306 at 5)
   g.lead = std::max(0, env_int("MTJG_ILU_GRID_LEAD", 3));
+  g.poll_budget = GRID_POLL_BUDGET;
+  if (const char *e = getenv("MTJG_ILU_GRID_POLL_BUDGET")) g.poll_budget = atoll(e);
   g.slab = (int64_t)g.YB * g.TW * 64;
   g.nslot = g.slab * g.nz;
   // one buffer descriptor with 32-bit offsets; below 2^31 bytes, an offset of

@@ -17,6 +17,7 @@ import scipy.sparse as sp
 
 import mtjgpu as M
 import oracle
+from mtjgpu import _lib as L
 from mtjgpu import problems as P
 
 pytestmark = pytest.mark.gpu
@@ -200,6 +201,31 @@ def test_not_grid_patterns_keep_levels(ctx, name):
     bh = P.uniform(n, 4)
     np.testing.assert_array_equal(ilu.apply(M.DenseVector(bh, ctx), M.DenseVector(n, ctx)).getData(),
                                   Oi.apply(bh))
+
+
+def test_poll_budget_error_path(ctx, monkeypatch):
+    """A poll budget of 0, set from the host, admits no wait: every plane
+    group that imports raises the error flag at its start, and the sweeps
+    still run to their end.  apply then fails with MTJG_E_DEVICE (and clears
+    the flag); a preconditioner with the default budget is bitwise the
+    oracle's afterwards.  No kernel waits longer than its own budget."""
+    sysm = P.convdiff3d(12, 40, 30, seed=6)
+    n = sysm[0]
+    A = M.CompRowMatrix(*sysm, ctx=ctx)
+    bh = P.uniform(n, 4)
+    monkeypatch.setenv("MTJG_ILU_GRID_POLL_BUDGET", "0")
+    for make in (lambda: M.ILU(A), lambda: M.SSOR(A)):
+        pc = make()
+        pc.setMatrix(A)
+        assert pc.grid() == (12, 40, 30)
+        with pytest.raises(M.MtjgError) as e:
+            pc.apply(M.DenseVector(bh, ctx), M.DenseVector(n, ctx))
+        assert e.value.code == L.E_DEVICE
+    monkeypatch.delenv("MTJG_ILU_GRID_POLL_BUDGET")
+    ilu = M.ILU(A)
+    ilu.setMatrix(A)
+    Oi = oracle.Precond(oracle.PREC_ILU, oracle.Csr(*sysm))
+    np.testing.assert_array_equal(ilu.apply(M.DenseVector(bh, ctx), M.DenseVector(n, ctx)).getData(), Oi.apply(bh))
 
 
 def test_grid_switch_off(ctx, monkeypatch):
