@@ -50,6 +50,7 @@ enum : int {
   ST_DIVERGENCE = 3,
   ST_BREAKDOWN = 4,
   ST_SINGULAR = 5,
+  ST_DEVICE = 6,  // a device-side wait ran out of its poll budget
 };
 
 // Device-resident scalar state of one Krylov solve.  All fields are written

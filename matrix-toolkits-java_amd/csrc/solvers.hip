@@ -50,6 +50,12 @@ struct mtjg_solver_s {
   double *V = nullptr;   // (restart+1) x n basis, vector-major
   double *H = nullptr;   // (restart+1) x restart, column-major
   double *gs = nullptr;  // s, rotation c and s: 3 * (restart+1)
+  // cooperative Gram-Schmidt (k_gmres_mgs_all): published-step counter,
+  // poll-budget flag, poll budget; mgs_coop: 0 undecided, 1 usable, -1 not
+  uint32_t *mgs_bar = nullptr;
+  int32_t *mgs_err = nullptr;
+  int64_t mgs_budget = 0;
+  int mgs_coop = 0;
   int32_t inner_total = 0;
 };
 
@@ -639,6 +645,98 @@ __global__ __launch_bounds__(BS) void k_gmres_mgs(int64_t n, double *w, const do
   if (grid_sum_dd<1>(v, red, tot) && threadIdx.x == 0) g.H[k + (size_t)i * ld] = tot[0];
 }
 
+// The i + 1 Gram-Schmidt steps of Arnoldi step i (k_gmres_mgs for
+// k = 0 .. i) in one cooperative launch: each workgroup keeps its NT tiles
+// of w and of the previous basis vector in registers across the steps, so a
+// step reads one basis vector instead of w, two basis vectors and a write of
+// w.  Every workgroup walks the same tiles in the same order as k_gmres_mgs
+// with the same grid, and the sums take the same reduction tree, so H and w
+// are bitwise those of the per-step launches.
+//
+// Step barrier: *bar counts published steps.  Thread 0 of each workgroup
+// reads it at the start (no step of this launch can be published before
+// every workgroup has taken its first ticket, so all read the same base).
+// The workgroup that completes step k's sum publishes H(k, i) (agent-scope
+// store, waited for), then base + k + 1 in *bar; the others wait for that
+// number before reading H(k, i), polling at most `budget` times (<= 0 admits
+// no wait).  A workgroup that runs out sets *err and the status ST_DEVICE
+// and leaves; the others then run out at their next wait, so the grid always
+// drains (the reduction's tickets are left part-taken and the host resets
+// them).  The reduction's counters are reset by their last users before the
+// step is published, so the next step's sum starts clean.  The last step
+// needs no barrier (nothing here reads H(i, i)).  All workgroups must be
+// resident at once: the host launches this only cooperatively.
+// 8 wavefronts per SIMD (at most 64 VGPRs): the 2048-workgroup grid of
+// 256-thread workgroups is then resident at once on 256 CUs.
+template <int NT>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_gmres_mgs_all(int64_t n, double *w, const double *V, int i, KState *st,
+                                                      GmresDev g, GridRed red, uint32_t *bar, int64_t budget,
+                                                      int32_t *err) {
+  __shared__ int quit;
+  if (st->status != ST_RUNNING) return;
+  const int ld = g.m + 1;
+  const int64_t t0 = (int64_t)blockIdx.x * (BS * VPT), stride = (int64_t)gridDim.x * (BS * VPT);
+  uint32_t base = 0;
+  if (threadIdx.x == 0) base = __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  double wv[NT][VPT], pv[NT][VPT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j)
+    if (t0 + j * stride < n) vload_tile(n, t0 + j * stride, VIo{w, wv[j]});
+  for (int k = 0; k <= i; ++k) {
+    const double h = (k > 0) ? -load_agent(&g.H[(k - 1) + (size_t)i * ld]) : 0.0;
+    const double *vk = V + (size_t)k * (size_t)n;
+    dd v[1] = {{0.0, 0.0}};
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int64_t tile = t0 + j * stride;
+      if (tile < n) {
+        double kv[VPT];
+        vload_tile(n, tile, VIo{vk, kv});
+        if (h != 0) {
+#pragma unroll
+          for (int u = 0; u < VPT; ++u) wv[j][u] = wv[j][u] + h * pv[j][u];
+        }
+#pragma unroll
+        for (int u = 0; u < VPT; ++u) {
+          if (vslot_ok(n, tile, u)) dd_fma(v[0], wv[j][u], kv[u]);
+          pv[j][u] = kv[u];
+        }
+      }
+    }
+    double tot[1];
+    const bool fin = grid_sum_dd<1>(v, red, tot);
+    if (k == i) {
+      if (fin && threadIdx.x == 0) g.H[k + (size_t)i * ld] = tot[0];
+      break;
+    }
+    if (threadIdx.x == 0) {
+      const uint32_t step = base + (uint32_t)k + 1u;
+      quit = 0;
+      if (fin) {
+        store_agent(&g.H[k + (size_t)i * ld], tot[0]);
+        stores_complete();
+        __hip_atomic_store(bar, step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        int64_t left = budget;
+        while ((int32_t)(__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - step) < 0) {
+          if (--left < 0) {
+            atomicExch(err, 1);
+            st->status = ST_DEVICE;
+            quit = 1;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+    }
+    __syncthreads();
+    if (quit) return;
+  }
+#pragma unroll
+  for (int j = 0; j < NT; ++j)
+    if (t0 + j * stride < n) vstore_tile(n, t0 + j * stride, w, wv[j]);
+}
+
 // GivensRotation constructor and apply (M/GivensRotation.java:42-92).
 __device__ void givens(double x, double y, double &c, double &s) {
   double roe = fabs(x) > fabs(y) ? x : y;
@@ -1125,7 +1223,66 @@ void gmres_free(mtjg_solver S) {
   (void)hipFree(S->V);
   (void)hipFree(S->H);
   (void)hipFree(S->gs);
+  (void)hipFree(S->mgs_bar);
+  (void)hipFree(S->mgs_err);
   S->V = S->H = S->gs = nullptr;
+  S->mgs_bar = nullptr;
+  S->mgs_err = nullptr;
+  S->mgs_coop = 0;
+}
+
+// k_gmres_mgs_all for nt tiles per workgroup (1 or 2), else null (more
+// tiles would not fit the 64 registers the resident grid allows).
+static const void *mgs_all_kernel(int64_t nt) {
+  switch (nt) {
+    case 1: return (const void *)k_gmres_mgs_all<1>;
+    case 2: return (const void *)k_gmres_mgs_all<2>;
+    default: return nullptr;
+  }
+}
+
+// Whether kernel kf (a k_gmres_mgs_all) can run on grid g: cooperative
+// launches supported and g workgroups resident at once.  Decided once per
+// solver (its size is fixed); allocates the step counter and flag then.
+// MTJG_GMRES_COOP=0 keeps the per-step launches; MTJG_GMRES_POLL_BUDGET sets
+// the barrier's poll budget (default 1 << 22).
+static bool mgs_coop_ok(mtjg_solver S, const void *kf, int g) {
+  if (S->mgs_coop == 0) {
+    S->mgs_coop = -1;
+    if (const char *e = getenv("MTJG_GMRES_COOP"))
+      if (atoi(e) == 0) return false;
+    if (!kf) return false;
+    int dev = 0, coop = 0, cus = 0, per_cu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kf, BS, 0));
+    if (coop && (int64_t)cus * per_cu >= g) {
+      S->mgs_budget = 1LL << 22;
+      if (const char *e = getenv("MTJG_GMRES_POLL_BUDGET")) S->mgs_budget = atoll(e);
+      HIPCHK(hipMalloc(&S->mgs_bar, sizeof(uint32_t)));
+      HIPCHK(hipMalloc(&S->mgs_err, sizeof(int32_t)));
+      HIPCHK(hipMemsetAsync(S->mgs_bar, 0, sizeof(uint32_t), S->ctx->stream));
+      HIPCHK(hipMemsetAsync(S->mgs_err, 0, sizeof(int32_t), S->ctx->stream));
+      S->mgs_coop = 1;
+    }
+  }
+  return S->mgs_coop > 0;
+}
+
+// Raises if a cooperative Gram-Schmidt barrier ran out of its poll budget
+// during the last solve (synchronises the stream).  The workgroups that left
+// early abandoned the reduction's tickets part-taken: they are cleared here.
+static void gmres_check(mtjg_solver S) {
+  if (!S->mgs_err) return;
+  int32_t e = 0;
+  HIPCHK(hipMemcpyAsync(&e, S->mgs_err, sizeof(int32_t), hipMemcpyDeviceToHost, S->ctx->stream));
+  HIPCHK(hipStreamSynchronize(S->ctx->stream));
+  if (!e) return;
+  HIPCHK(hipMemsetAsync(S->mgs_err, 0, sizeof(int32_t), S->ctx->stream));
+  HIPCHK(hipMemsetAsync(S->red.gticket, 0, sizeof(uint32_t) * red_ticket_words(S->red.gcap), S->ctx->stream));
+  HIPCHK(hipStreamSynchronize(S->ctx->stream));
+  fail(MTJG_E_DEVICE, "GMRES Gram-Schmidt grid barrier exceeded its poll budget");
 }
 
 void solve_gmres(mtjg_solver S, mtjg_csr A, double *b, double *x) {
@@ -1141,6 +1298,10 @@ void solve_gmres(mtjg_solver S, mtjg_csr A, double *b, double *x) {
   auto Vk = [&](int k) { return S->V + (size_t)k * (size_t)n; };
   // the basis vectors are 16-byte aligned when n is even (the basis is one allocation)
   const bool vec = aligned16({w, u, r, S->V}) && n % 2 == 0;
+  // the whole Gram-Schmidt loop per cooperative launch when every workgroup
+  // holds one tile (the grid's tile order is then the per-step kernels')
+  const void *mgs_kf = mgs_all_kernel((n + (int64_t)g * (BS * VPT) - 1) / ((int64_t)g * (BS * VPT)));
+  const bool coop = vec && mgs_coop_ok(S, mgs_kf, g);
   SpmvArgs ra;
   ra.mode = SPMV_MULTADD;
   ra.alpha = -1;
@@ -1164,11 +1325,21 @@ void solve_gmres(mtjg_solver S, mtjg_csr A, double *b, double *x) {
     for (int i = 0; i < S->restart && !k.cycle_done; ++i) {
       spmv_f64(A, Vk(i), u, ma, s);
       prec_apply_dev(S->M, n, u, w, s, nullptr);
-      for (int kk = 0; kk <= i; ++kk)
-        if (vec)
-          k_gmres_mgs<true><<<g, BS, 0, s>>>(n, w, kk > 0 ? Vk(kk - 1) : nullptr, Vk(kk), kk, i, st, gd, S->red);
-        else
-          k_gmres_mgs<<<g, BS, 0, s>>>(n, w, kk > 0 ? Vk(kk - 1) : nullptr, Vk(kk), kk, i, st, gd, S->red);
+      if (coop) {
+        int64_t nn = n;
+        const double *V0 = S->V;
+        void *args[] = {(void *)&nn,     (void *)&w,          (void *)&V0,         (void *)&i,
+                        (void *)&st,     (void *)&gd,         (void *)&S->red,     (void *)&S->mgs_bar,
+                        (void *)&S->mgs_budget, (void *)&S->mgs_err};
+        HIPCHK(hipLaunchCooperativeKernel(mgs_kf, dim3(g), dim3(BS), args, 0, s));
+      } else {
+        for (int kk = 0; kk <= i; ++kk)
+          if (vec)
+            k_gmres_mgs<true><<<g, BS, 0, s>>>(n, w, kk > 0 ? Vk(kk - 1) : nullptr, Vk(kk), kk, i, st, gd,
+                                                S->red);
+          else
+            k_gmres_mgs<<<g, BS, 0, s>>>(n, w, kk > 0 ? Vk(kk - 1) : nullptr, Vk(kk), kk, i, st, gd, S->red);
+      }
       if (vec) {
         k_gmres_arnoldi_end<true><<<g, BS, 0, s>>>(n, w, Vk(i), i, st, gd, S->red);
         k_gmres_scale<true><<<g, BS, 0, s>>>(n, w, Vk(i + 1), st);
@@ -1843,7 +2014,10 @@ extern "C" int mtjg_solver_solve(mtjg_solver S, mtjg_csr A, mtjg_vec b, mtjg_vec
   switch (S->kind) {
     case MTJG_CG: solve_cg(S, A, bp, xp); break;
     case MTJG_BICGSTAB: solve_bicgstab(S, A, bp, xp); break;
-    case MTJG_GMRES: solve_gmres(S, A, bp, xp); break;
+    case MTJG_GMRES:
+      solve_gmres(S, A, bp, xp);
+      gmres_check(S);
+      break;
     case MTJG_IR: solve_ir(S, A, bp, xp); break;
     case MTJG_MIXED_CG: solve_mixed(S, A, bp, xp); break;
     case MTJG_BICG: solve_bicg(S, A, b, x); break;
